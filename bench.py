@@ -1,0 +1,153 @@
+"""Benchmark: Lloyd site-iterations/s of the HIP surface-CVT path on one MI355X.
+
+Workload (default): displaced cube-sphere with 1,997,568 triangles (998,786
+vertices), 1,000,000 area-uniform sites, K = 24, freezing off, FP32 -- the
+large-job shape the headline metric is quoted on.  A "step" is one Lloyd
+iteration over all sites (site-grid rebuild + fused KNN / frame / cell /
+projection kernel), launched as HIP graphs; inputs are resident in HBM before
+the timed region.
+
+Prints one JSON line (rank 0).  Also measures, in the same run:
+  * roofline: algorithmic bytes of the fused Lloyd kernel per launch divided by
+    its mean launch duration (HIP events on the job's stream);
+  * cpu_baseline: the FP64 CPU oracle's Lloyd step on this host, on a bounded
+    sample (one iteration of a sub-sampled site set on the same mesh).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Lloyd site-iters/s at 1M sites (% HBM roofline); p50/p99 remesh latency"
+
+
+def algorithmic_bytes(nS: int, nV: int, nF: int, K: int) -> int:
+    """Compulsory bytes of one Lloyd iteration with freezing off: each site read
+    (12 B) and written (12 B), its K-list written (4K B), and the mesh read once
+    (positions 12 nV, normals 12 nV, faces 12 nF, vertex->face CSR 4(nV+1) + 12 nF)."""
+    return nS * (24 + 4 * K) + 28 * nV + 24 * nF + 4
+
+
+def cpu_baseline(V, F, S, K: int, budget_s: float = 20.0):
+    """Time the FP64 oracle's full Lloyd step on host threads (bounded sample)."""
+    from oracle import oracle as orc
+
+    orc.build()
+    threads = min(16, orc.hw_threads())
+    orc.set_threads(threads)
+    mesh = orc.Mesh(V, F)
+    R = float(np.float32(mesh.bbox_max_edge))
+    # calibrate on a small subsample, then size the sample to the budget
+    rng = np.random.Generator(np.random.PCG64(123))
+    n0 = min(len(S), 20000)
+    sub = S[np.sort(rng.choice(len(S), n0, replace=False))]
+    st = orc.LloydState(sub, K)
+    t0 = time.perf_counter()
+    orc.lloyd_step(mesh, st, K=K, R=R)
+    dt0 = time.perf_counter() - t0
+    rate0 = n0 / dt0
+    n = int(min(len(S), max(n0, rate0 * budget_s * 0.5)))
+    if n > n0:
+        sub = S[np.sort(rng.choice(len(S), n, replace=False))]
+        st = orc.LloydState(sub, K)
+        t0 = time.perf_counter()
+        orc.lloyd_step(mesh, st, K=K, R=R)
+        dt = time.perf_counter() - t0
+    else:
+        n, dt = n0, dt0
+    return {"value": n / dt, "unit": "site-iters/s", "cores": threads, "kind": "port",
+            "sample": f"1 FP64 oracle Lloyd iteration over {n} of the {len(S)} sites on the same mesh "
+                      f"({dt:.1f} s, {threads} threads)"}
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dsphere_2m_1m")
+    ap.add_argument("--freeze", action="store_true", help="freezing on (reported alongside)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--profile-iters", type=int, default=5)
+    args = ap.parse_args(argv)
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this path has no multi-GPU form in this build "
+              "(jobs shard across GPUs with no communication; see the design notes).", file=sys.stderr)
+        return 2
+
+    from gpu_cvt_amd import cvt, meshgen as mg
+
+    t_gen = time.perf_counter()
+    V, F, S, K, _iters, _fz = mg.workload(args.workload)
+    t_gen = time.perf_counter() - t_gen
+    params = cvt.default_params(K=K, freeze_mode=1 if args.freeze else 0, freeze_need=5, freeze_disp_rel=1e-5)
+    job = cvt.Job(V, F, S, params)
+    info = job.info()
+    print(f"[bench] workload={args.workload} nV={len(V)} nF={len(F)} nS={len(S)} K={K} gen={t_gen:.1f}s "
+          f"site_grid={info['site_grid_dims']} cell={info['site_cell']:.4g} vert_grid={info['vert_grid_dims']} "
+          f"dev_bytes={info['device_bytes'] / 2**20:.0f}MiB", file=sys.stderr, flush=True)
+
+    job.step(args.warmup)
+    stats = job.step(1, stats=True)[0]
+    print(f"[bench] warm stats {stats}", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    ms = job.step_timed(args.steps)
+    wall = time.perf_counter() - t0
+    ms_per_step = ms / args.steps
+    value = len(S) * args.steps / (ms / 1e3)
+
+    prof = job.profile(args.profile_iters)
+    alg = algorithmic_bytes(len(S), len(V), len(F), K)
+    achieved = alg / (prof["lloyd_ms"] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
+    whole_iter_gbs = alg / (ms_per_step * 1e-3) / 1e9
+    print(f"[bench] timed {args.steps} steps: {ms:.3f} ms device ({wall * 1e3:.1f} ms wall); "
+          f"profile {prof}; kernel {achieved:.1f} GB/s; whole iteration {whole_iter_gbs:.1f} GB/s",
+          file=sys.stderr, flush=True)
+    job.close()
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(V, F, S, K, args.cpu_budget)
+        except Exception as e:  # the CPU leg must not hide the GPU result
+            print(f"[bench] cpu baseline failed: {e!r}", file=sys.stderr)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "site-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded displaced sphere, area-uniform sites)",
+        "config": {"workload": args.workload, "triangles": int(len(F)), "vertices": int(len(V)),
+                   "sites": int(len(S)), "K": K, "freezing": bool(args.freeze), "kproj": 5},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "whole_iteration_hbm_frac": round(whole_iter_gbs / HBM_PEAK_GBS, 5),
+        "stage_ms": prof,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,642 @@
+// cvt_api.hip -- host side of the C ABI declared in include/cvt.h.
+//
+// A cvt_job owns every device buffer of one remeshing job (mesh, static
+// vertex grid, sites, lists, freeze state, site-grid work buffers) and one HIP
+// stream.  Iterations are captured once per buffer parity into HIP graphs and
+// replayed; no host round trip happens inside an iteration (the reference
+// synchronises and copies after almost every stage, main.cu:669-909).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cvt.h"
+#include "cvt_kernels.h"
+
+using namespace cvt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(CVT_EHIP, "%s: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                                          __FILE__, __LINE__);                                     \
+    } while (0)
+
+unsigned next_pow2(unsigned long v) {
+    unsigned p = 1;
+    while (p < v && p < (1u << 30)) p <<= 1;
+    return p;
+}
+
+constexpr int kHistCap = 4096;
+
+}  // namespace
+
+struct cvt_job {
+    cvt_params p{};
+    int nV = 0, nF = 0, nS = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    unsigned long long bytes = 0;
+
+    // mesh
+    float4 *V = nullptr, *N = nullptr, *tri = nullptr;
+    int *F = nullptr, *vf_off = nullptr, *vf_faces = nullptr;
+    GridP vg{};
+    GridBufs vgb{};
+    // sites
+    GridP sgp{};
+    GridBufs sgb{};
+    float4* S[2] = {nullptr, nullptr};
+    int cur = 0;
+    int *knn = nullptr, *knnV = nullptr;
+    unsigned char *frozen = nullptr, *streak = nullptr, *tier = nullptr;
+    int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr;
+    // debug
+    int* dbg_nn = nullptr;
+    float4 *dbg_U = nullptr, *dbg_V = nullptr, *dbg_cent = nullptr;
+    float2* dbg_am = nullptr;
+    // host-side facts
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double diag = 0, R = 0, area = 0;
+    int iters_done = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};
+
+    template <class T>
+    int alloc(T** ptr, size_t count) {
+        size_t nbytes = std::max<size_t>(count, 1) * sizeof(T);
+        void* d = nullptr;
+        hipError_t e = hipMalloc(&d, nbytes);
+        if (e != hipSuccess) return fail(CVT_ENOMEM, "hipMalloc(%zu bytes): %s", nbytes, hipGetErrorString(e));
+        allocs.push_back(d);
+        bytes += nbytes;
+        *ptr = (T*)d;
+        return 0;
+    }
+    ~cvt_job() {
+        for (auto& g : graph)
+            if (g) (void)hipGraphExecDestroy(g);
+        for (void* d : allocs) (void)hipFree(d);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+GridP make_grid(const double lo[3], const double hi[3], double h, unsigned table) {
+    GridP g{};
+    double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    if (!(ext > 0)) ext = 1.0;
+    if (!(h > 0)) h = ext;
+    int dims[3];
+    for (;;) {
+        bool ok = true;
+        for (int a = 0; a < 3; ++a) {
+            double n = std::floor((hi[a] - lo[a]) / h) + 1;
+            if (n > 1023) ok = false;
+            dims[a] = (int)std::max(1.0, std::min(n, 1023.0));
+        }
+        if (ok) break;
+        h *= 1.1;
+    }
+    g.ox = (float)lo[0];
+    g.oy = (float)lo[1];
+    g.oz = (float)lo[2];
+    g.h = (float)h;
+    g.inv_h = (float)(1.0 / h);
+    double amax = 0;
+    for (int a = 0; a < 3; ++a) amax = std::max(amax, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+    g.eps_abs = (float)(4e-7 * (amax + ext) + 1e-30);
+    g.dx = dims[0];
+    g.dy = dims[1];
+    g.dz = dims[2];
+    g.mask = table - 1;
+    return g;
+}
+
+int alloc_grid(cvt_job* j, GridBufs& b, int n, unsigned table) {
+    const long T2 = (long)table + 2;
+    const int nb = (int)((T2 + 4095) / 4096);
+    int rc;
+    if ((rc = j->alloc(&b.count, T2))) return rc;
+    if ((rc = j->alloc(&b.start, T2))) return rc;
+    if ((rc = j->alloc(&b.bsum, nb + 1))) return rc;
+    if ((rc = j->alloc(&b.bucket, n))) return rc;
+    if ((rc = j->alloc(&b.rank, n))) return rc;
+    if ((rc = j->alloc(&b.pts, n))) return rc;
+    if ((rc = j->alloc(&b.ids, n))) return rc;
+    return 0;
+}
+
+StepArgs make_args(cvt_job* j, int parity, int eval_only) {
+    StepArgs a{};
+    a.m.V = j->V;
+    a.m.N = j->N;
+    a.m.tri = j->tri;
+    a.m.F = j->F;
+    a.m.vf_off = j->vf_off;
+    a.m.vf_faces = j->vf_faces;
+    a.m.nV = j->nV;
+    a.m.nF = j->nF;
+    a.m.vg = j->vg;
+    a.m.vg_start = j->vgb.start;
+    a.m.vg_pts = j->vgb.pts;
+    a.m.vg_ids = j->vgb.ids;
+    a.sg.g = j->sgp;
+    a.sg.start = j->sgb.start;
+    a.sg.pts = j->sgb.pts;
+    a.sg.ids = j->sgb.ids;
+    a.S = j->S[parity];
+    a.Snew = j->S[1 - parity];
+    a.knn = j->knn;
+    a.knnV = j->knnV;
+    a.frozen = j->frozen;
+    a.streak = j->streak;
+    a.tier = j->tier;
+    a.has_prev = j->has_prev;
+    a.stats = j->stats;
+    a.nS = j->nS;
+    a.R = (float)j->R;
+    a.freeze_mode = j->p.freeze_mode;
+    const double base2 = j->p.freeze_disp_abs2 > 0 ? j->p.freeze_disp_abs2
+                                                   : (j->p.freeze_disp_rel * j->diag) * (j->p.freeze_disp_rel * j->diag);
+    for (int t = 0; t < 6; ++t) {
+        if (j->p.freeze_mode == 2) {
+            a.thr2[t] = (float)(j->p.tier_disp_abs2[t] > 0 ? j->p.tier_disp_abs2[t] : base2);
+            a.need[t] = j->p.tier_streak[t];
+        } else {
+            a.thr2[t] = (float)base2;
+            a.need[t] = j->p.freeze_need;
+        }
+    }
+    a.nan_policy = j->p.nan_policy;
+    a.write_knnv = (j->p.freeze_mode != 0 || j->p.debug) ? 1 : 0;
+    a.eval_only = eval_only;
+    if (j->p.debug) {
+        a.dbg_nn = j->dbg_nn;
+        a.dbg_U = j->dbg_U;
+        a.dbg_V = j->dbg_V;
+        a.dbg_cent = j->dbg_cent;
+        a.dbg_am = j->dbg_am;
+    }
+    return a;
+}
+
+// One iteration from S[parity] into S[1-parity].
+void enqueue_iteration(cvt_job* j, int parity) {
+    hipStream_t s = j->stream;
+    launch_fill(j->stats, kStatWords, 0, s);
+    launch_grid_build(j->S[parity], j->nS, j->sgp, j->sgb, s);
+    StepArgs a = make_args(j, parity, 0);
+    launch_lloyd(a, j->p.K, j->p.kproj, s);
+    launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, s);
+}
+
+int ensure_graph(cvt_job* j, int parity) {
+    if (j->graph[parity]) return 0;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_iteration(j, parity);
+    HIPCHK(hipStreamEndCapture(j->stream, &g));
+    hipError_t e = hipGraphInstantiate(&j->graph[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(CVT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    return 0;
+}
+
+int check_finite(const double* a, long n) {
+    for (long i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return 0;
+    return 1;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cvt_version(void) { return 1; }
+
+const char* cvt_last_error(void) { return g_err.c_str(); }
+
+void cvt_params_default(cvt_params* p) {
+    std::memset(p, 0, sizeof *p);
+    p->K = 24;
+    p->kproj = 5;
+    p->freeze_mode = 0;
+    p->freeze_need = 5;
+    p->freeze_disp_rel = 1e-5;
+    p->freeze_disp_abs2 = 0;
+    const int st[6] = {5, 10, 15, 20, 25, 10};
+    for (int t = 0; t < 6; ++t) {
+        p->tier_streak[t] = st[t];
+        p->tier_disp_abs2[t] = 0;
+    }
+    p->nan_policy = 0;
+    p->clip_R_mode = 0;
+    p->clip_R = 0;
+    p->site_cell_scale = 0;
+    p->vert_cell_scale = 0;
+    p->debug = 0;
+}
+
+int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double* S0, int nS, const cvt_params* pin,
+                   cvt_job** out) {
+    if (!out) return fail(CVT_EINVAL, "out is null");
+    *out = nullptr;
+    cvt_params p;
+    if (pin) p = *pin;
+    else cvt_params_default(&p);
+    if (!Vh || !Fh || !S0) return fail(CVT_EINVAL, "null input array");
+    if (nV < 3 || nF < 1) return fail(CVT_EINVAL, "empty mesh (nV=%d nF=%d)", nV, nF);
+    if (!lloyd_supported(p.K, p.kproj))
+        return fail(CVT_EINVAL, "unsupported K=%d / kproj=%d (K in {4,6,8,12,16,20,24,32}, kproj=5)", p.K, p.kproj);
+    if (nS < p.K + 1) return fail(CVT_EINVAL, "need nS >= K+1 (nS=%d K=%d)", nS, p.K);
+    if (nV < p.kproj) return fail(CVT_EINVAL, "need nV >= kproj");
+    if (p.freeze_mode < 0 || p.freeze_mode > 2) return fail(CVT_EINVAL, "freeze_mode must be 0, 1 or 2");
+    if (!check_finite(Vh, 3L * nV)) return fail(CVT_ENONFINITE, "non-finite vertex coordinate");
+    if (!check_finite(S0, 3L * nS)) return fail(CVT_ENONFINITE, "non-finite site coordinate");
+    for (long f = 0; f < 3L * nF; ++f)
+        if (Fh[f] < 0 || Fh[f] >= nV) return fail(CVT_EINVAL, "face index out of range at %ld", f);
+    if ((long)nS * p.K > (1L << 31) - 1) return fail(CVT_EINVAL, "nS*K too large");
+
+    cvt_job* j = new cvt_job();
+    j->p = p;
+    j->nV = nV;
+    j->nF = nF;
+    j->nS = nS;
+    int rc;
+    auto bail = [&](int code) {
+        delete j;
+        return code;
+    };
+
+    // host-side mesh facts (FP32-rounded values, FP64 arithmetic)
+    std::vector<float4> V4(nV);
+    for (int a = 0; a < 3; ++a) { j->lo[a] = 1e300; j->hi[a] = -1e300; }
+    for (int v = 0; v < nV; ++v) {
+        float x = (float)Vh[3 * v], y = (float)Vh[3 * v + 1], z = (float)Vh[3 * v + 2];
+        V4[v] = make_float4(x, y, z, 0.f);
+        double c[3] = {x, y, z};
+        for (int a = 0; a < 3; ++a) { j->lo[a] = std::min(j->lo[a], c[a]); j->hi[a] = std::max(j->hi[a], c[a]); }
+    }
+    double e[3], emax = 0;
+    for (int a = 0; a < 3; ++a) { e[a] = j->hi[a] - j->lo[a]; emax = std::max(emax, e[a]); }
+    j->diag = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    j->R = p.clip_R_mode == 1 ? p.clip_R : (double)(float)emax;  // main.cu:226 (FP32 max bbox edge)
+    if (!(j->R > 0)) return bail(fail(CVT_EINVAL, "clip half-size R must be > 0"));
+    std::vector<float4> tri((size_t)3 * nF);
+    double area = 0;
+    for (int f = 0; f < nF; ++f) {
+        const float4 A = V4[Fh[3 * f]], B = V4[Fh[3 * f + 1]], C = V4[Fh[3 * f + 2]];
+        for (int k = 0; k < 3; ++k) {
+            const float4 P = V4[Fh[3 * f + k]];
+            int id = Fh[3 * f + k];
+            float w;
+            std::memcpy(&w, &id, 4);
+            tri[(size_t)3 * f + k] = make_float4(P.x, P.y, P.z, w);
+        }
+        double ux = (double)B.x - A.x, uy = (double)B.y - A.y, uz = (double)B.z - A.z;
+        double vx = (double)C.x - A.x, vy = (double)C.y - A.y, vz = (double)C.z - A.z;
+        double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+        area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+    }
+    j->area = area > 0 ? area : emax * emax;
+    // vertex -> incident-face CSR in face order (main.cu:259-285)
+    std::vector<int> vf_off(nV + 1, 0), vf_faces((size_t)3 * nF);
+    for (long f = 0; f < 3L * nF; ++f) vf_off[Fh[f] + 1]++;
+    for (int v = 0; v < nV; ++v) vf_off[v + 1] += vf_off[v];
+    {
+        std::vector<int> cur(vf_off.begin(), vf_off.end() - 1);
+        for (int f = 0; f < nF; ++f)
+            for (int k = 0; k < 3; ++k) vf_faces[cur[Fh[3 * f + k]]++] = f;
+    }
+    // padded grid bbox
+    double glo[3], ghi[3];
+    for (int a = 0; a < 3; ++a) {
+        const double pad = 1e-3 * emax + 1e-6;
+        glo[a] = j->lo[a] - pad;
+        ghi[a] = j->hi[a] + pad;
+    }
+    const double kPi = 3.14159265358979323846;
+    const double ss = p.site_cell_scale > 0 ? p.site_cell_scale : 1.0;
+    const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.0;
+    const double rK = std::sqrt((p.K + 1) / (kPi * (nS / j->area)));
+    const double rV = std::sqrt((p.kproj + 1) / (kPi * (nV / j->area)));
+    j->sgp = make_grid(glo, ghi, ss * rK, next_pow2(std::max(1024L, 2L * nS)));
+    j->vg = make_grid(glo, ghi, vs * rV, next_pow2(std::max(1024L, 2L * nV)));
+
+    if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
+    if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
+        (rc = j->alloc(&j->F, (size_t)3 * nF)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
+        (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.mask + 1)) ||
+        (rc = alloc_grid(j, j->sgb, nS, j->sgp.mask + 1)) || (rc = j->alloc(&j->S[0], nS)) ||
+        (rc = j->alloc(&j->S[1], nS)) || (rc = j->alloc(&j->knn, (size_t)nS * p.K)) ||
+        (rc = j->alloc(&j->knnV, (size_t)nS * p.kproj)) || (rc = j->alloc(&j->frozen, nS)) ||
+        (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
+        (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
+        (rc = j->alloc(&j->hist_count, 1)))
+        return bail(rc);
+    if (p.debug) {
+        if ((rc = j->alloc(&j->dbg_nn, nS)) || (rc = j->alloc(&j->dbg_U, nS)) || (rc = j->alloc(&j->dbg_V, nS)) ||
+            (rc = j->alloc(&j->dbg_cent, nS)) || (rc = j->alloc(&j->dbg_am, nS)))
+            return bail(rc);
+    }
+    std::vector<float4> S4(nS);
+    for (int i = 0; i < nS; ++i) S4[i] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
+    std::vector<int> Fv(Fh, Fh + 3L * nF);
+    hipStream_t s = j->stream;
+    auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
+    if (up(j->V, V4.data(), sizeof(float4) * nV) || up(j->tri, tri.data(), sizeof(float4) * tri.size()) ||
+        up(j->F, Fv.data(), sizeof(int) * Fv.size()) || up(j->vf_off, vf_off.data(), sizeof(int) * vf_off.size()) ||
+        up(j->vf_faces, vf_faces.data(), sizeof(int) * vf_faces.size()) ||
+        up(j->S[0], S4.data(), sizeof(float4) * nS))
+        return bail(fail(CVT_EHIP, "upload failed"));
+    launch_fill(j->knn, (long)nS * p.K, 0, s);
+    launch_fill(j->knnV, (long)nS * p.kproj, -1, s);
+    if (hipMemsetAsync(j->frozen, 0, nS, s) || hipMemsetAsync(j->streak, 0, nS, s) ||
+        hipMemsetAsync(j->tier, 0, nS, s))
+        return bail(fail(CVT_EHIP, "memset failed"));
+    launch_fill(j->has_prev, 1, 0, s);
+    launch_fill(j->hist_count, 1, 0, s);
+    launch_fill(j->stats, kStatWords, 0, s);
+    launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
+    launch_grid_build(j->V, nV, j->vg, j->vgb, s);
+    hipError_t e2 = hipStreamSynchronize(s);
+    if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup: %s", hipGetErrorString(e2)));
+    e2 = hipGetLastError();
+    if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup launch: %s", hipGetErrorString(e2)));
+    *out = j;
+    return CVT_OK;
+}
+
+void cvt_job_destroy(cvt_job* job) {
+    if (!job) return;
+    (void)hipStreamSynchronize(job->stream);
+    delete job;
+}
+
+int cvt_job_info_get(cvt_job* j, cvt_job_info* o) {
+    if (!j || !o) return fail(CVT_EINVAL, "null argument");
+    std::memset(o, 0, sizeof *o);
+    o->nV = j->nV; o->nF = j->nF; o->nS = j->nS; o->K = j->p.K; o->kproj = j->p.kproj;
+    o->site_grid_dims[0] = j->sgp.dx; o->site_grid_dims[1] = j->sgp.dy; o->site_grid_dims[2] = j->sgp.dz;
+    o->vert_grid_dims[0] = j->vg.dx; o->vert_grid_dims[1] = j->vg.dy; o->vert_grid_dims[2] = j->vg.dz;
+    o->site_table = (int)j->sgp.mask + 1;
+    o->vert_table = (int)j->vg.mask + 1;
+    o->site_cell = j->sgp.h;
+    o->vert_cell = j->vg.h;
+    for (int a = 0; a < 3; ++a) { o->bbox_lo[a] = j->lo[a]; o->bbox_hi[a] = j->hi[a]; }
+    o->bbox_diag = j->diag;
+    o->clip_R = j->R;
+    o->area = j->area;
+    o->device_bytes = j->bytes;
+    o->iterations_done = j->iters_done;
+    return CVT_OK;
+}
+
+static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
+    std::vector<int> h((size_t)iters * kStatWords);
+    const int lo = std::min(first, kHistCap - 1);
+    const int cnt = std::min(iters, kHistCap - lo);
+    if (cnt > 0)
+        HIPCHK(hipMemcpy(h.data(), j->hist + (size_t)lo * kStatWords, sizeof(int) * cnt * kStatWords,
+                         hipMemcpyDeviceToHost));
+    for (int it = 0; it < iters; ++it) {
+        const int* w = h.data() + (size_t)std::min(it, std::max(cnt - 1, 0)) * kStatWords;
+        cvt_iter_stats& o = stats[it];
+        std::memset(&o, 0, sizeof o);
+        o.count_same = w[0];
+        o.count_low_disp = w[1];
+        o.count_both = w[2];
+        o.count_newly_frozen = w[3];
+        o.frozen_total = w[4];
+        o.knn_exact_rescans = w[5];
+        o.knn_reranks = w[6];
+        o.cell_failures = w[7];
+        o.nonfinite_sites = w[8];
+    }
+    return 0;
+}
+
+static int run_iters(cvt_job* j, int iters, double* ms) {
+    int rc;
+    if ((rc = ensure_graph(j, 0)) || (rc = ensure_graph(j, 1))) return rc;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, j->stream));
+    }
+    for (int it = 0; it < iters; ++it) {
+        HIPCHK(hipGraphLaunch(j->graph[j->cur], j->stream));
+        j->cur ^= 1;
+    }
+    if (ms) HIPCHK(hipEventRecord(e1, j->stream));
+    HIPCHK(hipStreamSynchronize(j->stream));
+    if (ms) {
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    j->iters_done += iters;
+    return 0;
+}
+
+int cvt_job_step(cvt_job* j, int iters, cvt_iter_stats* stats) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    if (iters < 0) return fail(CVT_EINVAL, "iters < 0");
+    if (iters == 0) return 0;
+    int first = 0;
+    if (stats) {
+        HIPCHK(hipMemcpy(&first, j->hist_count, sizeof(int), hipMemcpyDeviceToHost));
+    }
+    int rc = run_iters(j, iters, nullptr);
+    if (rc) return rc;
+    if (stats) return read_stats(j, first, iters, stats);
+    return 0;
+}
+
+int cvt_job_step_timed(cvt_job* j, int iters, double* ms) {
+    if (!j || !ms) return fail(CVT_EINVAL, "null argument");
+    if (iters <= 0) return fail(CVT_EINVAL, "iters must be > 0");
+    return run_iters(j, iters, ms);
+}
+
+int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, double* iter_ms) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    if (iters <= 0) return fail(CVT_EINVAL, "iters must be > 0");
+    hipStream_t s = j->stream;
+    hipEvent_t ev[4];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double g = 0, l = 0, t = 0;
+    for (int it = 0; it < iters; ++it) {
+        const int par = j->cur;
+        HIPCHK(hipEventRecord(ev[0], s));
+        launch_fill(j->stats, kStatWords, 0, s);
+        launch_grid_build(j->S[par], j->nS, j->sgp, j->sgb, s);
+        HIPCHK(hipEventRecord(ev[1], s));
+        StepArgs a = make_args(j, par, 0);
+        launch_lloyd(a, j->p.K, j->p.kproj, s);
+        HIPCHK(hipEventRecord(ev[2], s));
+        launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, s);
+        HIPCHK(hipEventRecord(ev[3], s));
+        HIPCHK(hipEventSynchronize(ev[3]));
+        float f01 = 0, f12 = 0, f03 = 0;
+        HIPCHK(hipEventElapsedTime(&f01, ev[0], ev[1]));
+        HIPCHK(hipEventElapsedTime(&f12, ev[1], ev[2]));
+        HIPCHK(hipEventElapsedTime(&f03, ev[0], ev[3]));
+        g += f01;
+        l += f12;
+        t += f03;
+        j->cur ^= 1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    HIPCHK(hipGetLastError());
+    j->iters_done += iters;
+    if (grid_ms) *grid_ms = g / iters;
+    if (lloyd_ms) *lloyd_ms = l / iters;
+    if (iter_ms) *iter_ms = t / iters;
+    return 0;
+}
+
+int cvt_job_get_sites(cvt_job* j, double* S) {
+    if (!j || !S) return fail(CVT_EINVAL, "null argument");
+    std::vector<float4> h(j->nS);
+    HIPCHK(hipMemcpy(h.data(), j->S[j->cur], sizeof(float4) * j->nS, hipMemcpyDeviceToHost));
+    for (int i = 0; i < j->nS; ++i) { S[3 * i] = h[i].x; S[3 * i + 1] = h[i].y; S[3 * i + 2] = h[i].z; }
+    return 0;
+}
+
+int cvt_job_get_knn(cvt_job* j, int* out) {
+    if (!j || !out) return fail(CVT_EINVAL, "null argument");
+    HIPCHK(hipMemcpy(out, j->knn, sizeof(int) * (size_t)j->nS * j->p.K, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cvt_job_get_knnv(cvt_job* j, int* out) {
+    if (!j || !out) return fail(CVT_EINVAL, "null argument");
+    HIPCHK(hipMemcpy(out, j->knnV, sizeof(int) * (size_t)j->nS * j->p.kproj, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cvt_job_get_freeze(cvt_job* j, unsigned char* fz, unsigned char* st) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    if (fz) HIPCHK(hipMemcpy(fz, j->frozen, j->nS, hipMemcpyDeviceToHost));
+    if (st) HIPCHK(hipMemcpy(st, j->streak, j->nS, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int cvt_job_get_normals(cvt_job* j, double* N) {
+    if (!j || !N) return fail(CVT_EINVAL, "null argument");
+    std::vector<float4> h(j->nV);
+    HIPCHK(hipMemcpy(h.data(), j->N, sizeof(float4) * j->nV, hipMemcpyDeviceToHost));
+    for (int i = 0; i < j->nV; ++i) { N[3 * i] = h[i].x; N[3 * i + 1] = h[i].y; N[3 * i + 2] = h[i].z; }
+    return 0;
+}
+
+int cvt_job_set_state(cvt_job* j, const double* S, const int* knn_prev, const int* knnv_prev,
+                      const unsigned char* frozen, const unsigned char* streak, int has_prev) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    const int nS = j->nS;
+    if (S) {
+        if (!check_finite(S, 3L * nS)) return fail(CVT_ENONFINITE, "non-finite site coordinate");
+        std::vector<float4> h(nS);
+        for (int i = 0; i < nS; ++i) h[i] = make_float4((float)S[3 * i], (float)S[3 * i + 1], (float)S[3 * i + 2], 0.f);
+        HIPCHK(hipMemcpy(j->S[j->cur], h.data(), sizeof(float4) * nS, hipMemcpyHostToDevice));
+    }
+    if (knn_prev) {
+        for (long k = 0; k < (long)nS * j->p.K; ++k)
+            if (knn_prev[k] < 0 || knn_prev[k] >= nS) return fail(CVT_EINVAL, "knn_prev entry out of range");
+        HIPCHK(hipMemcpy(j->knn, knn_prev, sizeof(int) * (size_t)nS * j->p.K, hipMemcpyHostToDevice));
+    }
+    if (knnv_prev) {
+        for (long k = 0; k < (long)nS * j->p.kproj; ++k)
+            if (knnv_prev[k] < -1 || knnv_prev[k] >= j->nV) return fail(CVT_EINVAL, "knnv_prev entry out of range");
+        HIPCHK(hipMemcpy(j->knnV, knnv_prev, sizeof(int) * (size_t)nS * j->p.kproj, hipMemcpyHostToDevice));
+    }
+    if (frozen) HIPCHK(hipMemcpy(j->frozen, frozen, nS, hipMemcpyHostToDevice));
+    if (streak) HIPCHK(hipMemcpy(j->streak, streak, nS, hipMemcpyHostToDevice));
+    if (has_prev >= 0) {
+        int hp = has_prev ? 1 : 0;
+        HIPCHK(hipMemcpy(j->has_prev, &hp, sizeof(int), hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int cvt_job_set_tiers(cvt_job* j, const unsigned char* tier) {
+    if (!j || !tier) return fail(CVT_EINVAL, "null argument");
+    for (int i = 0; i < j->nS; ++i)
+        if (tier[i] > 5) return fail(CVT_EINVAL, "tier id > 5 at site %d", i);
+    HIPCHK(hipMemcpy(j->tier, tier, j->nS, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int cvt_job_get_debug(cvt_job* j, int* nn, double* U, double* Vf, double* cent, double* area, double* moment) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    if (!j->p.debug) return fail(CVT_ESTATE, "job was created without params.debug");
+    const int n = j->nS;
+    if (nn) HIPCHK(hipMemcpy(nn, j->dbg_nn, sizeof(int) * n, hipMemcpyDeviceToHost));
+    std::vector<float4> h(n);
+    auto get3 = [&](float4* d, double* o) -> int {
+        if (!o) return 0;
+        HIPCHK(hipMemcpy(h.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) { o[3 * i] = h[i].x; o[3 * i + 1] = h[i].y; o[3 * i + 2] = h[i].z; }
+        return 0;
+    };
+    int rc;
+    if ((rc = get3(j->dbg_U, U)) || (rc = get3(j->dbg_V, Vf)) || (rc = get3(j->dbg_cent, cent))) return rc;
+    if (area || moment) {
+        std::vector<float2> am(n);
+        HIPCHK(hipMemcpy(am.data(), j->dbg_am, sizeof(float2) * n, hipMemcpyDeviceToHost));
+        for (int i = 0; i < n; ++i) {
+            if (area) area[i] = am[i].x;
+            if (moment) moment[i] = am[i].y;
+        }
+    }
+    return 0;
+}
+
+int cvt_job_energy(cvt_job* j, double* E) {
+    if (!j || !E) return fail(CVT_EINVAL, "null argument");
+    if (!j->p.debug) return fail(CVT_ESTATE, "energy evaluation needs params.debug (per-site moment buffer)");
+    hipStream_t s = j->stream;
+    launch_grid_build(j->S[j->cur], j->nS, j->sgp, j->sgb, s);
+    StepArgs a = make_args(j, j->cur, 1);
+    launch_lloyd(a, j->p.K, j->p.kproj, s);
+    HIPCHK(hipStreamSynchronize(s));
+    std::vector<float2> am(j->nS);
+    HIPCHK(hipMemcpy(am.data(), j->dbg_am, sizeof(float2) * j->nS, hipMemcpyDeviceToHost));
+    double sum = 0;
+    for (int i = 0; i < j->nS; ++i) sum += am[i].y;
+    *E = sum;
+    return 0;
+}
+
+int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
+    (void)j; (void)P; (void)n; (void)d2;
+    return fail(CVT_EINVAL, "cvt_job_point_mesh_d2: not built yet");
+}
+
+}  // extern "C"

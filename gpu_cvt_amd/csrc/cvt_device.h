@@ -1,0 +1,426 @@
+// cvt_device.h -- device-side building blocks of the gfx950 Lloyd step.
+//
+// Hash grid + exact K-nearest search, tangent frames, the tangent-plane cell
+// as a dual convex hull, closest point on a triangle.  Written for wave64
+// CDNA: one lane per query, fixed-size register lists (no dynamic indexing,
+// no LDS, no scratch), and FP64 only where an FP32 comparison cannot be
+// proven (near-ties in KNN order, near-zero hull orientation tests,
+// ill-conditioned frame switches).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cvt {
+
+// ------------------------------------------------------------------ hash grid
+// Uniform cells of size h over a padded bbox; cell (x,y,z) -> key
+// x | y<<10 | z<<20 (each < 1024) -> bucket hash(key) & mask.  Points are
+// stored bucket-sorted as float4 (x, y, z, key bits); a bucket may hold
+// points of several colliding cells, so scans compare the key.
+struct GridP {
+    float ox, oy, oz;
+    float h, inv_h;
+    float eps_abs;  // absolute rounding allowance of coordinate differences
+    int dx, dy, dz;
+    unsigned mask;  // table size - 1 (power of two); bucket `mask+1` holds non-finite points
+};
+
+constexpr unsigned kInvalidKey = 0xffffffffu;
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ __forceinline__ bool finite3(float x, float y, float z) {
+    return isfinite(x) && isfinite(y) && isfinite(z);
+}
+
+__device__ __forceinline__ void cell_of(const GridP& g, float x, float y, float z, int& cx, int& cy, int& cz) {
+    cx = clampi((int)floorf((x - g.ox) * g.inv_h), 0, g.dx - 1);
+    cy = clampi((int)floorf((y - g.oy) * g.inv_h), 0, g.dy - 1);
+    cz = clampi((int)floorf((z - g.oz) * g.inv_h), 0, g.dz - 1);
+}
+
+__device__ __forceinline__ unsigned cell_key(int cx, int cy, int cz) {
+    return (unsigned)cx | ((unsigned)cy << 10) | ((unsigned)cz << 20);
+}
+
+__device__ __forceinline__ unsigned cell_hash(const GridP& g, int cx, int cy, int cz) {
+    return (((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u)) & g.mask;
+}
+
+// Canonical ordering key: FP64 ((dx*dx + dy*dy) + dz*dz), dx = p - q,
+// no contraction.  Bit-identical to the oracle's key.
+__device__ __forceinline__ double key64(float qx, float qy, float qz, float px, float py, float pz) {
+#pragma clang fp contract(off)
+    double dx = (double)px - (double)qx;
+    double dy = (double)py - (double)qy;
+    double dz = (double)pz - (double)qz;
+    double s = dx * dx;
+    s = s + dy * dy;
+    s = s + dz * dz;
+    return s;
+}
+
+__device__ __forceinline__ float d2f(float qx, float qy, float qz, float4 p) {
+    float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
+    return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+}
+
+// Relative error allowance of d2f against the exact squared distance
+// (|err| <= ~5 u d2 with u = 2^-24); comparisons use 2e-6.
+constexpr float kRel = 2e-6f;
+
+// Sorted (ascending) top-L list of (FP32 d2, slot) in registers.
+template <int L>
+struct TopL {
+    float d[L];
+    int s[L];
+    __device__ __forceinline__ void init() {
+#pragma unroll
+        for (int t = 0; t < L; ++t) { d[t] = __builtin_huge_valf(); s[t] = -1; }
+    }
+    __device__ __forceinline__ float worst() const { return d[L - 1]; }
+    // Branch-free sorted insertion; equal keys keep the earlier entry.
+    __device__ __forceinline__ void insert(float x, int slot) {
+#pragma unroll
+        for (int t = L - 1; t >= 1; --t) {
+            bool ct = x < d[t], cp = x < d[t - 1];
+            d[t] = cp ? d[t - 1] : (ct ? x : d[t]);
+            s[t] = cp ? s[t - 1] : (ct ? slot : s[t]);
+        }
+        bool c0 = x < d[0];
+        d[0] = c0 ? x : d[0];
+        s[0] = c0 ? slot : s[0];
+    }
+};
+
+// Ring search over cells around q.  For each visited, non-culled cell calls
+// visit(bucket_begin, bucket_end, key).  cull_r2() returns the current
+// squared radius beyond which cells may be skipped; the search stops once the
+// block boundary is provably farther than that radius.
+template <class Visit, class CullR2>
+__device__ __forceinline__ void ring_search(const GridP& g, const int* __restrict__ start, float qx, float qy, float qz,
+                                            Visit&& visit, CullR2&& cull_r2) {
+    int cx, cy, cz;
+    cell_of(g, qx, qy, qz, cx, cy, cz);
+    const float h = g.h;
+    const int maxr = max(g.dx, max(g.dy, g.dz));
+    for (int r = 0; r <= maxr; ++r) {
+        for (int dz = -r; dz <= r; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= g.dz) continue;
+            const float bz = dz < 0 ? fmaxf(0.f, qz - (g.oz + (z + 1) * h) - g.eps_abs)
+                             : (dz > 0 ? fmaxf(0.f, (g.oz + z * h) - qz - g.eps_abs) : 0.f);
+            for (int dy = -r; dy <= r; ++dy) {
+                const int y = cy + dy;
+                if (y < 0 || y >= g.dy) continue;
+                const float by = dy < 0 ? fmaxf(0.f, qy - (g.oy + (y + 1) * h) - g.eps_abs)
+                                 : (dy > 0 ? fmaxf(0.f, (g.oy + y * h) - qy - g.eps_abs) : 0.f);
+                const bool shell = (dz == -r) || (dz == r) || (dy == -r) || (dy == r);
+                const int step = (shell || r == 0) ? 1 : 2 * r;
+                for (int dxx = -r; dxx <= r; dxx += step) {
+                    const int x = cx + dxx;
+                    if (x < 0 || x >= g.dx) continue;
+                    const float bx = dxx < 0 ? fmaxf(0.f, qx - (g.ox + (x + 1) * h) - g.eps_abs)
+                                     : (dxx > 0 ? fmaxf(0.f, (g.ox + x * h) - qx - g.eps_abs) : 0.f);
+                    const float bd2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
+                    if (bd2 > cull_r2() * (1.f + 1e-5f)) continue;
+                    const unsigned key = cell_key(x, y, z);
+                    const unsigned b = cell_hash(g, x, y, z);
+                    visit(start[b], start[b + 1], key);
+                }
+            }
+        }
+        // distance from q to the outside of block [c-r, c+r] (inf where the block reaches the grid edge)
+        float bnd = __builtin_huge_valf();
+        if (cx - r > 0) bnd = fminf(bnd, qx - (g.ox + (cx - r) * h));
+        if (cx + r + 1 < g.dx) bnd = fminf(bnd, (g.ox + (cx + r + 1) * h) - qx);
+        if (cy - r > 0) bnd = fminf(bnd, qy - (g.oy + (cy - r) * h));
+        if (cy + r + 1 < g.dy) bnd = fminf(bnd, (g.oy + (cy + r + 1) * h) - qy);
+        if (cz - r > 0) bnd = fminf(bnd, qz - (g.oz + (cz - r) * h));
+        if (cz + r + 1 < g.dz) bnd = fminf(bnd, (g.oz + (cz + r + 1) * h) - qz);
+        if (bnd == __builtin_huge_valf()) break;
+        bnd -= g.eps_abs;
+        if (bnd > 0.f && cull_r2() < bnd * bnd * (1.f - 1e-5f)) break;
+    }
+}
+
+// FP32 top-L over a grid (self_slot excluded).  Exact: the returned list is
+// the FP32 top-L of the whole point set.
+template <int L>
+__device__ __forceinline__ void grid_topl(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+                                          float qx, float qy, float qz, int self_slot, TopL<L>& T) {
+    T.init();
+    ring_search(
+        g, start, qx, qy, qz,
+        [&](int s0, int s1, unsigned key) {
+            for (int s = s0; s < s1; ++s) {
+                const float4 p = pts[s];
+                if (__float_as_uint(p.w) != key || s == self_slot) continue;
+                const float d = d2f(qx, qy, qz, p);
+                if (d < T.worst()) T.insert(d, s);
+            }
+        },
+        [&]() { return T.worst(); });
+}
+
+// Exact K nearest under (key64, id): FP64 insertion over every point with
+// FP32 distance <= tau.  Used only when the FP32 filter is inconclusive.
+template <int K>
+__device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+                                          const int* __restrict__ ids, float qx, float qy, float qz, int self_slot,
+                                          float tau, int* __restrict__ out_slot) {
+    double kd[K];
+    int kid[K], ks[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) { kd[t] = __builtin_huge_val(); kid[t] = 0x7fffffff; ks[t] = -1; }
+    ring_search(
+        g, start, qx, qy, qz,
+        [&](int s0, int s1, unsigned key) {
+            for (int s = s0; s < s1; ++s) {
+                const float4 p = pts[s];
+                if (__float_as_uint(p.w) != key || s == self_slot) continue;
+                if (d2f(qx, qy, qz, p) > tau) continue;
+                const double d = key64(qx, qy, qz, p.x, p.y, p.z);
+                const int id = ids[s];
+                if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) continue;
+#pragma unroll
+                for (int t = K - 1; t >= 1; --t) {
+                    const bool ct = d < kd[t] || (d == kd[t] && id < kid[t]);
+                    const bool cp = d < kd[t - 1] || (d == kd[t - 1] && id < kid[t - 1]);
+                    kd[t] = cp ? kd[t - 1] : (ct ? d : kd[t]);
+                    kid[t] = cp ? kid[t - 1] : (ct ? id : kid[t]);
+                    ks[t] = cp ? ks[t - 1] : (ct ? s : ks[t]);
+                }
+                const bool c0 = d < kd[0] || (d == kd[0] && id < kid[0]);
+                kd[0] = c0 ? d : kd[0];
+                kid[0] = c0 ? id : kid[0];
+                ks[0] = c0 ? s : ks[0];
+            }
+        },
+        [&]() { return tau; });
+#pragma unroll
+    for (int t = 0; t < K; ++t) out_slot[t] = ks[t];
+}
+
+// K nearest (K < L) in exact (key64, id) order, as slots.  Returns
+// 0 = FP32 order proven, 1 = FP64 rerank inside the list, 2 = exact rescan.
+template <int K, int L>
+__device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+                                         const int* __restrict__ ids, float qx, float qy, float qz, int self_slot,
+                                         int* __restrict__ slot_out) {
+    static_assert(L > K, "need at least one extra entry");
+    TopL<L> T;
+    grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, T);
+    const bool margin = T.d[L - 1] > T.d[K - 1] * (1.f + kRel) && T.d[K - 1] < __builtin_huge_valf();
+    if (!margin) {
+        const float tau = T.d[L - 1] * (1.f + 1e-5f);
+        grid_exact64<K>(g, start, pts, ids, qx, qy, qz, self_slot, tau, slot_out);
+        return 2;
+    }
+    bool order = true;
+#pragma unroll
+    for (int t = 0; t + 1 < K; ++t) order = order && (T.d[t + 1] > T.d[t] * (1.f + kRel));
+#pragma unroll
+    for (int t = 0; t < K; ++t) slot_out[t] = T.s[t];
+    if (order) return 0;
+    // FP64 rerank: odd-even transposition passes on (key64, id) until sorted.
+    double kd[K];
+    int kid[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        const float4 p = pts[slot_out[t]];
+        kd[t] = key64(qx, qy, qz, p.x, p.y, p.z);
+        kid[t] = ids[slot_out[t]];
+    }
+    for (int pass = 0; pass < K; ++pass) {
+        bool swapped = false;
+#pragma unroll
+        for (int par = 0; par < 2; ++par) {
+#pragma unroll
+            for (int t = par; t + 1 < K; t += 2) {
+                const bool sw = kd[t + 1] < kd[t] || (kd[t + 1] == kd[t] && kid[t + 1] < kid[t]);
+                if (sw) {
+                    double td = kd[t]; kd[t] = kd[t + 1]; kd[t + 1] = td;
+                    int ti = kid[t]; kid[t] = kid[t + 1]; kid[t + 1] = ti;
+                    int ts = slot_out[t]; slot_out[t] = slot_out[t + 1]; slot_out[t + 1] = ts;
+                    swapped = true;
+                }
+            }
+        }
+        if (!swapped) break;
+    }
+    return 1;
+}
+
+// ------------------------------------------------------------------- frames
+// site_normals.cuh:79-109: h = (1,0,0) if |n_z| > 0.9 else (0,0,1);
+// u = normalize(h x n), v = normalize(n x u).
+__device__ __forceinline__ void frame_f32(float nx, float ny, float nz, float3& u, float3& v) {
+    float inv = rsqrtf(nx * nx + ny * ny + nz * nz + 1e-30f);
+    nx *= inv; ny *= inv; nz *= inv;
+    const bool hx = fabsf(nz) > 0.9f;
+    // h x n
+    float ux = hx ? 0.f : -ny;
+    float uy = hx ? -nz : nx;
+    float uz = hx ? ny : 0.f;
+    float iu = rsqrtf(ux * ux + uy * uy + uz * uz + 1e-30f);
+    ux *= iu; uy *= iu; uz *= iu;
+    float vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
+    float iv = rsqrtf(vx * vx + vy * vy + vz * vz + 1e-30f);
+    u = make_float3(ux, uy, uz);
+    v = make_float3(vx * iv, vy * iv, vz * iv);
+}
+
+__device__ __forceinline__ void frame_f64(double nx, double ny, double nz, float3& u, float3& v) {
+#pragma clang fp contract(off)
+    double inv = 1.0 / sqrt(nx * nx + ny * ny + nz * nz + 1e-30);
+    nx *= inv; ny *= inv; nz *= inv;
+    const bool hx = fabs(nz) > 0.9;
+    double ux = hx ? 0.0 : -ny, uy = hx ? -nz : nx, uz = hx ? ny : 0.0;
+    double iu = 1.0 / sqrt(ux * ux + uy * uy + uz * uz + 1e-30);
+    ux *= iu; uy *= iu; uz *= iu;
+    double vx = ny * uz - nz * uy, vy = nz * ux - nx * uz, vz = nx * uy - ny * ux;
+    double iv = 1.0 / sqrt(vx * vx + vy * vy + vz * vz + 1e-30);
+    u = make_float3((float)ux, (float)uy, (float)uz);
+    v = make_float3((float)(vx * iv), (float)(vy * iv), (float)(vz * iv));
+}
+
+// ------------------------------------------------------- compensated sums
+struct Kahan {
+    float s, c;
+    __device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+    __device__ __forceinline__ void add(float x) {
+        float y = x - c;
+        float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+// -------------------------------------------------------- tangent-plane cell
+// The cell {x : a_j x + b_j y <= c_j} clipped to the square |x|,|y| <= R
+// (2dClip.cuh:82-135) is the polar dual of the convex hull of the points
+// p_j = (a_j, b_j)/c_j (plus (+-1/R,0), (0,+-1/R) for the square).  Consecutive
+// hull points P, Q give the cell vertex P.x = 1, Q.x = 1.  The hull is walked
+// counter-clockwise by gift wrapping over registers; orientation tests that
+// FP32 cannot decide are redone in FP64 on the same FP32 inputs.
+struct CellOut {
+    float cx, cy, area, moment;
+    int nverts;
+    bool ok;
+};
+
+template <int M>
+__device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
+    CellOut o;
+    o.ok = false;
+    o.nverts = 0;
+    // start: max x, then max y -- always a hull vertex
+    float sx = px[0], sy = py[0];
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        const bool b = px[j] > sx || (px[j] == sx && py[j] > sy);
+        sx = b ? px[j] : sx;
+        sy = b ? py[j] : sy;
+    }
+    float curx = sx, cury = sy;
+    Kahan A2, Cx, Cy, I12;
+    A2.init(); Cx.init(); Cy.init(); I12.init();
+    float fvx = 0.f, fvy = 0.f, pvx = 0.f, pvy = 0.f;
+    for (int step = 0; step <= M; ++step) {
+        float bx = curx, by = cury;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const float wx = px[j] - curx, wy = py[j] - cury;
+            const float ex = bx - curx, ey = by - cury;
+            const bool isnew = (wx != 0.f) || (wy != 0.f);
+            const bool none = (ex == 0.f) && (ey == 0.f);
+            const float t1 = ex * wy, t2 = ey * wx;
+            float cr = t1 - t2;
+            if (fabsf(cr) <= 1e-6f * (fabsf(t1) + fabsf(t2)) && !none && isnew) {
+                const double dex = (double)bx - (double)curx, dey = (double)by - (double)cury;
+                const double dwx = (double)px[j] - (double)curx, dwy = (double)py[j] - (double)cury;
+                const double c64 = dex * dwy - dey * dwx;
+                cr = c64 < 0.0 ? -1.f : (c64 > 0.0 ? 1.f : 0.f);
+            }
+            const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
+            const bool take = isnew && (none || cr < 0.f || (cr == 0.f && farther));
+            bx = take ? px[j] : bx;
+            by = take ? py[j] : by;
+        }
+        if (bx == curx && by == cury) return o;  // degenerate: all dual points coincide
+        const float det = curx * by - cury * bx;
+        const float inv = 1.0f / det;
+        const float vx = (by - cury) * inv, vy = (curx - bx) * inv;
+        if (step == 0) {
+            fvx = vx; fvy = vy;
+        } else {
+            const float cr = pvx * vy - vx * pvy;
+            A2.add(cr);
+            Cx.add((pvx + vx) * cr);
+            Cy.add((pvy + vy) * cr);
+            I12.add(cr * (pvx * pvx + pvx * vx + vx * vx + pvy * pvy + pvy * vy + vy * vy));
+        }
+        pvx = vx; pvy = vy;
+        o.nverts++;
+        curx = bx; cury = by;
+        if (curx == sx && cury == sy) {
+            const float cr = pvx * fvy - fvx * pvy;
+            A2.add(cr);
+            Cx.add((pvx + fvx) * cr);
+            Cy.add((pvy + fvy) * cr);
+            I12.add(cr * (pvx * pvx + pvx * fvx + fvx * fvx + pvy * pvy + pvy * fvy + fvy * fvy));
+            o.ok = true;
+            break;
+        }
+    }
+    if (!o.ok) return o;
+    const float A = 0.5f * A2.s;
+    o.area = A;
+    o.moment = fabsf(I12.s) * (1.0f / 12.0f);
+    if (o.nverts < 3) { o.ok = false; return o; }
+    if (fabsf(A) < 1e-18f) { o.cx = 0.f; o.cy = 0.f; return o; }
+    const float inv6A = 1.0f / (3.0f * A2.s);
+    o.cx = Cx.s * inv6A;
+    o.cy = Cy.s * inv6A;
+    return o;
+}
+
+// ---------------------------------------------------- closest point (Ericson)
+// near_project.cuh:12-53, including the reference's +1e-30 guards.
+__device__ __forceinline__ float3 closest_point_tri(float3 P, float3 A, float3 B, float3 C) {
+    const float3 AB = make_float3(B.x - A.x, B.y - A.y, B.z - A.z);
+    const float3 AC = make_float3(C.x - A.x, C.y - A.y, C.z - A.z);
+    const float3 AP = make_float3(P.x - A.x, P.y - A.y, P.z - A.z);
+    const float d1 = AB.x * AP.x + AB.y * AP.y + AB.z * AP.z;
+    const float d2 = AC.x * AP.x + AC.y * AP.y + AC.z * AP.z;
+    if (d1 <= 0.f && d2 <= 0.f) return A;
+    const float3 BP = make_float3(P.x - B.x, P.y - B.y, P.z - B.z);
+    const float d3 = AB.x * BP.x + AB.y * BP.y + AB.z * BP.z;
+    const float d4 = AC.x * BP.x + AC.y * BP.y + AC.z * BP.z;
+    if (d3 >= 0.f && d4 <= d3) return B;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+        const float t = d1 / (d1 - d3 + 1e-30f);
+        return make_float3(A.x + AB.x * t, A.y + AB.y * t, A.z + AB.z * t);
+    }
+    const float3 CP = make_float3(P.x - C.x, P.y - C.y, P.z - C.z);
+    const float d5 = AB.x * CP.x + AB.y * CP.y + AB.z * CP.z;
+    const float d6 = AC.x * CP.x + AC.y * CP.y + AC.z * CP.z;
+    if (d6 >= 0.f && d5 <= d6) return C;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+        const float w = d2 / (d2 - d6 + 1e-30f);
+        return make_float3(A.x + AC.x * w, A.y + AC.y * w, A.z + AC.z * w);
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6) + 1e-30f);
+        return make_float3(B.x + (C.x - B.x) * w, B.y + (C.y - B.y) * w, B.z + (C.z - B.z) * w);
+    }
+    const float denom = 1.0f / (va + vb + vc + 1e-30f);
+    const float vv = vb * denom, ww = vc * denom;
+    return make_float3(A.x + AB.x * vv + AC.x * ww, A.y + AB.y * vv + AC.y * ww, A.z + AB.z * vv + AC.z * ww);
+}
+
+}  // namespace cvt

@@ -1,0 +1,459 @@
+// cvt_kernels.hip -- gfx950 kernels of the surface Lloyd iteration.
+//
+// One iteration (the body of the reference's main.cu:669-881) is:
+//   site hash-grid rebuild  : k_fill_int, k_grid_count, k_scan_* x3, k_grid_scatter
+//   fused per-site step      : k_lloyd  (site KNN -> frame -> cell/centroid ->
+//                              centroid->mesh kproj-NN -> reprojection -> freeze)
+//   iteration epilogue       : k_iter_end (stats history, has_prev flag)
+// all captured in one HIP graph per buffer parity.
+#include <hip/hip_runtime.h>
+
+#include "cvt_device.h"
+#include "cvt_kernels.h"
+
+namespace cvt {
+
+__device__ __forceinline__ int xcd_block(int b, int nb8) {
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of (spatially sorted) work so its L2 sees neighbours.
+    // nb8 (the launched grid) is a multiple of 8, so this is a bijection.
+    const int per = nb8 >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// ------------------------------------------------------------- small kernels
+__global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_grid_count(const float4* __restrict__ P, int n, GridP g, int* __restrict__ count,
+                             int* __restrict__ bucket, int* __restrict__ rank) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = P[i];
+    unsigned b;
+    if (finite3(p.x, p.y, p.z)) {
+        int cx, cy, cz;
+        cell_of(g, p.x, p.y, p.z, cx, cy, cz);
+        b = cell_hash(g, cx, cy, cz);
+    } else {
+        b = g.mask + 1;
+    }
+    bucket[i] = (int)b;
+    rank[i] = atomicAdd(&count[b], 1);
+}
+
+__global__ void k_grid_scatter(const float4* __restrict__ P, int n, GridP g, const int* __restrict__ bucket,
+                               const int* __restrict__ rank, const int* __restrict__ start, float4* __restrict__ out,
+                               int* __restrict__ ids) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 p = P[i];
+    unsigned key = kInvalidKey;
+    if (finite3(p.x, p.y, p.z)) {
+        int cx, cy, cz;
+        cell_of(g, p.x, p.y, p.z, cx, cy, cz);
+        key = cell_key(cx, cy, cz);
+    }
+    const int pos = start[bucket[i]] + rank[i];
+    out[pos] = make_float4(p.x, p.y, p.z, __uint_as_float(key));
+    ids[pos] = i;
+}
+
+// Exclusive scan of n ints (3 kernels).  Block = 256 threads x 16 items.
+constexpr int kScanItems = 16;
+constexpr int kScanTile = 256 * kScanItems;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(v, o, 64);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns the
+// exclusive prefix and writes the block total.
+__device__ __forceinline__ int block_excl_scan(int v, int& total) {
+    __shared__ int wsum[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int inc = wave_incl_scan(v);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    int off = 0;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) off += (k < w) ? wsum[k] : 0;
+    total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    __syncthreads();
+    return off + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const int* __restrict__ in, int n, int* __restrict__ bsum) {
+    const int base = blockIdx.x * kScanTile + threadIdx.x;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k * 256;
+        s += (j < n) ? in[j] : 0;
+    }
+    int total;
+    block_excl_scan(s, total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// single block: exclusive scan of nb block sums in place; out_total = sum
+__global__ __launch_bounds__(256) void k_scan_bsum(int* __restrict__ bsum, int nb, int* __restrict__ out_total) {
+    int carry = 0;
+    for (int c = 0; c < nb; c += 256) {
+        const int j = c + threadIdx.x;
+        const int v = (j < nb) ? bsum[j] : 0;
+        int total;
+        const int ex = block_excl_scan(v, total);
+        if (j < nb) bsum[j] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) *out_total = carry;
+}
+
+__global__ __launch_bounds__(256) void k_scan_down(const int* __restrict__ in, int n, const int* __restrict__ bsum,
+                                                   int* __restrict__ out) {
+    // thread t owns items [tile + t*16, tile + t*16 + 16) (blocked layout)
+    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k;
+        v[k] = (j < n) ? in[j] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_excl_scan(s, total) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k;
+        if (j < n) out[j] = run;
+        run += v[k];
+    }
+}
+
+// Vertex normals by CSR gather in face order (deterministic; replaces the
+// float-atomic accumulate of site_normals.cuh:19-30), normalised as :32-41.
+__global__ void k_vertex_normals(const float4* __restrict__ V, const int* __restrict__ F, const int* __restrict__ vf_off,
+                                 const int* __restrict__ vf_faces, int nV, float4* __restrict__ N) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    float sx = 0.f, sy = 0.f, sz = 0.f;
+    for (int ii = vf_off[v]; ii < vf_off[v + 1]; ++ii) {
+        const int f = vf_faces[ii];
+        const float4 a = V[F[3 * f]], b = V[F[3 * f + 1]], c = V[F[3 * f + 2]];
+        const float ex = b.x - a.x, ey = b.y - a.y, ez = b.z - a.z;
+        const float gx = c.x - a.x, gy = c.y - a.y, gz = c.z - a.z;
+        sx += ey * gz - ez * gy;
+        sy += ez * gx - ex * gz;
+        sz += ex * gy - ey * gx;
+    }
+    const float l2 = sx * sx + sy * sy + sz * sz;
+    if (l2 > 0.f) {
+        const float inv = rsqrtf(l2);
+        sx *= inv; sy *= inv; sz *= inv;
+    }
+    N[v] = make_float4(sx, sy, sz, 0.f);
+}
+
+// --------------------------------------------------------------- Lloyd step
+// FP64 vertex normal (same CSR order and operations as the oracle), for
+// frames whose axis switch |n_z| > 0.9 FP32 cannot decide.
+__device__ __noinline__ void frame_fp64_vertex(const MeshDev& m, int vtx, float3& u, float3& v) {
+#pragma clang fp contract(off)
+    double sx = 0, sy = 0, sz = 0;
+    for (int ii = m.vf_off[vtx]; ii < m.vf_off[vtx + 1]; ++ii) {
+        const int f = m.vf_faces[ii];
+        const float4 a = m.V[m.F[3 * f]], b = m.V[m.F[3 * f + 1]], c = m.V[m.F[3 * f + 2]];
+        const double ex = (double)b.x - a.x, ey = (double)b.y - a.y, ez = (double)b.z - a.z;
+        const double gx = (double)c.x - a.x, gy = (double)c.y - a.y, gz = (double)c.z - a.z;
+        sx = sx + (ey * gz - ez * gy);
+        sy = sy + (ez * gx - ex * gz);
+        sz = sz + (ex * gy - ey * gx);
+    }
+    const double l2 = sx * sx + sy * sy + sz * sz;
+    if (l2 > 0) {
+        const double inv = 1.0 / sqrt(l2);
+        sx *= inv; sy *= inv; sz *= inv;
+    }
+    frame_f64(sx, sy, sz, u, v);
+}
+
+__device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, bool pred) {
+    const unsigned long long b = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&stats[idx], (int)__popcll(b));
+}
+
+template <int K, int KP>
+__global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int t = lb * 256 + (int)threadIdx.x;
+    const bool live = t < a.nS;
+    const bool eval_only = a.eval_only != 0;
+    const int has_prev = eval_only ? 0 : *a.has_prev;
+
+    int i = 0;
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+        q = a.sg.pts[t];
+        i = a.sg.ids[t];
+    }
+    const bool valid = live && __float_as_uint(q.w) != kInvalidKey;
+    const bool fz = valid && !eval_only && a.frozen[i] != 0;
+
+    // ---- 1. site KNN (active) or pinned list (frozen)
+    int nb[K];
+    bool same = true;
+    int kind = 0;
+    if (valid && !fz) {
+        int slots[K];
+        kind = knn_exact<K, K + 1>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
+#pragma unroll
+        for (int k = 0; k < K; ++k) nb[k] = slots[k] >= 0 ? a.sg.ids[slots[k]] : i;
+        if (!eval_only) {
+            int* row = a.knn + (size_t)i * K;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+                same = same && (row[k] == nb[k]);
+            }
+#pragma unroll
+            for (int k = 0; k < K; ++k) row[k] = nb[k];
+        }
+    } else if (valid) {
+        const int* row = a.knn + (size_t)i * K;
+#pragma unroll
+        for (int k = 0; k < K; ++k) nb[k] = row[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) nb[k] = i;
+    }
+
+    // ---- 2. frame from the nearest mesh vertex's normal
+    int nn = -1;
+    float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
+    if (valid) {
+        if (!fz) {
+            int s1[1];
+            knn_exact<1, 2>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
+            nn = s1[0] >= 0 ? a.m.vg_ids[s1[0]] : -1;
+        } else {
+            nn = a.knnV[(size_t)i * KP];
+        }
+        if (nn >= 0 && nn < a.m.nV) {
+            const float4 nv = a.m.N[nn];
+            const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
+            const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
+            if (l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f) frame_fp64_vertex(a.m, nn, u, w);
+            else frame_f32(nv.x, nv.y, nv.z, u, w);
+        }
+    }
+
+    // ---- 3. tangent-plane cell (dual hull) and centroid
+    float3 c3 = make_float3(q.x, q.y, q.z);
+    bool cell_ok = false;
+    float area = 0.f, moment = 0.f;
+    if (valid) {
+        float px[K + 4], py[K + 4];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const float4 pj = a.S[nb[k]];
+            const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
+            const float aa = dx * u.x + dy * u.y + dz * u.z;
+            const float bb = dx * w.x + dy * w.y + dz * w.z;
+            const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
+            const float ic = cc > 0.f ? 1.0f / cc : 0.f;
+            px[k] = aa * ic;
+            py[k] = bb * ic;
+        }
+        const float iR = 1.0f / a.R;
+        px[K] = iR;   py[K] = 0.f;
+        px[K + 1] = 0.f; py[K + 1] = iR;
+        px[K + 2] = -iR; py[K + 2] = 0.f;
+        px[K + 3] = 0.f; py[K + 3] = -iR;
+        const CellOut co = dual_hull_cell<K + 4>(px, py);
+        cell_ok = co.ok;
+        area = co.area;
+        moment = co.moment;
+        if (co.ok) {
+            c3 = make_float3(q.x + co.cx * u.x + co.cy * w.x, q.y + co.cx * u.y + co.cy * w.y,
+                             q.z + co.cx * u.z + co.cy * w.z);
+        } else {
+            const float nan = __builtin_nanf("");
+            c3 = make_float3(nan, nan, nan);
+        }
+    }
+
+    // ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection
+    float3 sp = c3;
+    int vl[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) vl[k] = -1;
+    if (valid && !eval_only) {
+        if (!fz) {
+            if (finite3(c3.x, c3.y, c3.z)) {
+                int sl[KP];
+                knn_exact<KP, KP + 1>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, c3.x, c3.y, c3.z, -1, sl);
+#pragma unroll
+                for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? a.m.vg_ids[sl[k]] : -1;
+            }
+            if (a.write_knnv) {
+                int* row = a.knnV + (size_t)i * KP;
+#pragma unroll
+                for (int k = 0; k < KP; ++k) row[k] = vl[k];
+            }
+        } else {
+            const int* row = a.knnV + (size_t)i * KP;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) vl[k] = row[k];
+        }
+        float best = 1e30f;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int v = vl[k];
+            if ((unsigned)v >= (unsigned)a.m.nV) continue;
+            const int e = a.m.vf_off[v + 1];
+            for (int ii = a.m.vf_off[v]; ii < e; ++ii) {
+                const int f = a.m.vf_faces[ii];
+                const float4 A = a.m.tri[3 * f], B = a.m.tri[3 * f + 1], C = a.m.tri[3 * f + 2];
+                // a face incident to an earlier list vertex was already visited
+                bool seen = false;
+#pragma unroll
+                for (int s = 0; s < k; ++s) {
+                    const float vs = __int_as_float(vl[s]);
+                    seen = seen || (A.w == vs) || (B.w == vs) || (C.w == vs);
+                }
+                if (seen) continue;
+                const float3 p = closest_point_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                                   make_float3(C.x, C.y, C.z));
+                const float ddx = p.x - c3.x, ddy = p.y - c3.y, ddz = p.z - c3.z;
+                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 < best) { best = d2; sp = p; }
+            }
+        }
+        if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
+    }
+
+    // ---- 6. write the new site and the freeze state machine
+    bool low = false, both = false, newly = false, fz_after = fz;
+    if (valid && !eval_only) {
+        a.Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
+        if (has_prev) {
+            const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
+            const float ddx = sp.x - q.x, ddy = sp.y - q.y, ddz = sp.z - q.z;
+            low = (ddx * ddx + ddy * ddy + ddz * ddz) <= a.thr2[tr];
+            both = same && low;
+            if (a.freeze_mode != 0) {
+                int s = a.streak[i];
+                s = both ? min(s + 1, 255) : 0;
+                a.streak[i] = (unsigned char)s;
+                const int need = max(a.need[tr] - 1, 1);
+                if (s >= need && !fz) {
+                    a.frozen[i] = 1;
+                    newly = true;
+                    fz_after = true;
+                }
+            }
+        } else if (a.freeze_mode != 0) {
+            a.streak[i] = 0;
+        }
+    } else if (live && !valid && !eval_only) {
+        // non-finite site: keep it (it is excluded from every neighbour list)
+        a.Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
+    }
+    if (live && a.dbg_nn) {
+        a.dbg_nn[i] = nn;
+        a.dbg_U[i] = make_float4(u.x, u.y, u.z, 0.f);
+        a.dbg_V[i] = make_float4(w.x, w.y, w.z, 0.f);
+        a.dbg_cent[i] = make_float4(c3.x, c3.y, c3.z, 0.f);
+        a.dbg_am[i] = make_float2(area, moment);
+    }
+    if (!eval_only) {
+        const bool hp = has_prev != 0;
+        wave_count(a.stats, 0, valid && hp && same);
+        wave_count(a.stats, 1, valid && hp && low);
+        wave_count(a.stats, 2, valid && hp && both);
+        wave_count(a.stats, 3, newly);
+        wave_count(a.stats, 4, valid && fz_after);
+        wave_count(a.stats, 5, kind == 2);
+        wave_count(a.stats, 6, kind == 1);
+        wave_count(a.stats, 7, valid && !cell_ok);
+        wave_count(a.stats, 8, live && !finite3(sp.x, sp.y, sp.z));
+    }
+}
+
+__global__ void k_iter_end(int* __restrict__ stats, int* __restrict__ hist, int* __restrict__ hist_count, int cap,
+                           int* __restrict__ has_prev) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int c = *hist_count;
+    const int slot = c < cap ? c : cap - 1;
+    for (int k = 0; k < kStatWords; ++k) hist[slot * kStatWords + k] = stats[k];
+    *hist_count = c + 1;
+    *has_prev = 1;
+}
+
+// ------------------------------------------------------------------ launches
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+void launch_fill(int* p, long n, int v, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, s, p, (int)n, v);
+}
+
+void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipStream_t s) {
+    const int T1 = (int)g.mask + 2;  // buckets incl. the non-finite bucket
+    launch_fill(b.count, T1, 0, s);
+    hipLaunchKernelGGL(k_grid_count, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.count, b.bucket, b.rank);
+    const int nb = nblk(T1, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, b.count, T1, b.bsum);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, s, b.bsum, nb, b.start + T1);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, b.count, T1, b.bsum, b.start);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.bucket, b.rank, b.start,
+                       b.pts, b.ids);
+}
+
+void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nV, 256)), dim3(256), 0, s, V, F, vf_off, vf_faces, nV, N);
+}
+
+template <int K>
+static void launch_lloyd_k(const StepArgs& a, int kproj, hipStream_t s) {
+    const int nb8 = ((nblk(a.nS, 256) + 7) / 8) * 8;
+    (void)kproj;
+    hipLaunchKernelGGL((k_lloyd<K, 5>), dim3(nb8), dim3(256), 0, s, a);
+}
+
+bool lloyd_supported(int K, int kproj) {
+    if (kproj != 5) return false;
+    switch (K) {
+        case 4: case 6: case 8: case 12: case 16: case 20: case 24: case 32: return true;
+        default: return false;
+    }
+}
+
+void launch_lloyd(const StepArgs& a, int K, int kproj, hipStream_t s) {
+    switch (K) {
+        case 4: launch_lloyd_k<4>(a, kproj, s); break;
+        case 6: launch_lloyd_k<6>(a, kproj, s); break;
+        case 8: launch_lloyd_k<8>(a, kproj, s); break;
+        case 12: launch_lloyd_k<12>(a, kproj, s); break;
+        case 16: launch_lloyd_k<16>(a, kproj, s); break;
+        case 20: launch_lloyd_k<20>(a, kproj, s); break;
+        case 24: launch_lloyd_k<24>(a, kproj, s); break;
+        case 32: launch_lloyd_k<32>(a, kproj, s); break;
+        default: break;
+    }
+}
+
+void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, stats, hist, hist_count, cap, has_prev);
+}
+
+}  // namespace cvt

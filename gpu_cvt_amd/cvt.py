@@ -1,0 +1,277 @@
+"""Python host mirror of the reference's Lloyd loop over the C ABI (ctypes).
+
+The reference drives its loop from ``main()`` (main.cu:371-1073) with
+positional arguments ``gpu_cvt <mesh.obj> [mode] [iters]``; this module
+exposes the same loop as a :class:`Job` object over ``libcvt_hip.so``.  It
+loads the in-tree library and fails loudly if it is missing -- there is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcvt_hip.so")
+
+CVT_OK = 0
+STATUS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
+
+# Every entry point include/cvt.h declares (checked by the CPU tests).
+EXPORTS = [
+    "cvt_version", "cvt_last_error", "cvt_params_default", "cvt_job_create", "cvt_job_destroy",
+    "cvt_job_info_get", "cvt_job_step", "cvt_job_step_timed", "cvt_job_get_sites", "cvt_job_get_knn",
+    "cvt_job_get_knnv", "cvt_job_get_freeze", "cvt_job_get_normals", "cvt_job_set_state", "cvt_job_set_tiers",
+    "cvt_job_get_debug", "cvt_job_energy", "cvt_job_point_mesh_d2", "cvt_job_profile",
+]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("K", C.c_int), ("kproj", C.c_int), ("freeze_mode", C.c_int), ("freeze_need", C.c_int),
+        ("freeze_disp_rel", C.c_double), ("freeze_disp_abs2", C.c_double),
+        ("tier_streak", C.c_int * 6), ("tier_disp_abs2", C.c_double * 6),
+        ("nan_policy", C.c_int), ("clip_R_mode", C.c_int), ("clip_R", C.c_double),
+        ("site_cell_scale", C.c_double), ("vert_cell_scale", C.c_double),
+        ("debug", C.c_int), ("reserved", C.c_int * 7),
+    ]
+
+
+class IterStats(C.Structure):
+    _fields_ = [
+        ("count_same", C.c_int), ("count_low_disp", C.c_int), ("count_both", C.c_int),
+        ("count_newly_frozen", C.c_int), ("frozen_total", C.c_int), ("knn_exact_rescans", C.c_int),
+        ("knn_reranks", C.c_int), ("cell_failures", C.c_int), ("nonfinite_sites", C.c_int),
+        ("reserved", C.c_int * 3),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class JobInfo(C.Structure):
+    _fields_ = [
+        ("nV", C.c_int), ("nF", C.c_int), ("nS", C.c_int), ("K", C.c_int), ("kproj", C.c_int),
+        ("site_grid_dims", C.c_int * 3), ("vert_grid_dims", C.c_int * 3),
+        ("site_table", C.c_int), ("vert_table", C.c_int),
+        ("site_cell", C.c_double), ("vert_cell", C.c_double),
+        ("bbox_lo", C.c_double * 3), ("bbox_hi", C.c_double * 3),
+        ("bbox_diag", C.c_double), ("clip_R", C.c_double), ("area", C.c_double),
+        ("device_bytes", C.c_ulonglong), ("iterations_done", C.c_int), ("reserved", C.c_int * 7),
+    ]
+
+
+class CVTError(RuntimeError):
+    def __init__(self, fn: str, code: int, msg: str):
+        super().__init__(f"{fn} failed ({STATUS.get(code, code)}): {msg}")
+        self.code = code
+
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_up = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
+
+
+def _nullable(nd):
+    """ndpointer that also accepts None."""
+
+    class _P(nd):  # type: ignore[misc, valid-type]
+        @classmethod
+        def from_param(cls, obj):
+            if obj is None:
+                return None
+            return super().from_param(obj)
+
+    return _P
+
+
+_dpn, _ipn, _upn = _nullable(_dp), _nullable(_ip), _nullable(_up)
+
+
+def lib():
+    """Load libcvt_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m gpu_cvt_amd.build` "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    L.cvt_version.restype = C.c_int
+    L.cvt_last_error.restype = C.c_char_p
+    L.cvt_params_default.argtypes = [C.POINTER(Params)]
+    L.cvt_params_default.restype = None
+    L.cvt_job_create.argtypes = [_dp, C.c_int, _ip, C.c_int, _dp, C.c_int, C.POINTER(Params), C.POINTER(C.c_void_p)]
+    L.cvt_job_destroy.argtypes = [C.c_void_p]
+    L.cvt_job_destroy.restype = None
+    L.cvt_job_info_get.argtypes = [C.c_void_p, C.POINTER(JobInfo)]
+    L.cvt_job_step.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.cvt_job_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.cvt_job_get_sites.argtypes = [C.c_void_p, _dp]
+    L.cvt_job_get_knn.argtypes = [C.c_void_p, _ip]
+    L.cvt_job_get_knnv.argtypes = [C.c_void_p, _ip]
+    L.cvt_job_get_freeze.argtypes = [C.c_void_p, _upn, _upn]
+    L.cvt_job_get_normals.argtypes = [C.c_void_p, _dp]
+    L.cvt_job_set_state.argtypes = [C.c_void_p, _dpn, _ipn, _ipn, _upn, _upn, C.c_int]
+    L.cvt_job_set_tiers.argtypes = [C.c_void_p, _up]
+    L.cvt_job_get_debug.argtypes = [C.c_void_p, _ipn, _dpn, _dpn, _dpn, _dpn, _dpn]
+    L.cvt_job_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
+    L.cvt_job_point_mesh_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
+    L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_double)]
+    missing = [n for n in EXPORTS if not hasattr(L, n)]
+    if missing:
+        raise RuntimeError(f"{LIB_PATH} lacks exports {missing}: rebuild it")
+    _lib = L
+    return L
+
+
+def _check(fn: str, rc: int) -> None:
+    if rc != CVT_OK:
+        msg = lib().cvt_last_error()
+        raise CVTError(fn, rc, msg.decode() if msg else "")
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().cvt_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k in ("tier_streak", "tier_disp_abs2"):
+            arr = getattr(p, k)
+            for t, x in enumerate(v):
+                arr[t] = x
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Job:
+    """One device-resident remeshing job (mesh + sites + Lloyd state)."""
+
+    def __init__(self, V, F, S0, params: Params | None = None, **kw):
+        L = lib()
+        self.params = params if params is not None else default_params(**kw)
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        S0 = np.ascontiguousarray(S0, dtype=np.float64).reshape(-1, 3)
+        self.nV, self.nF, self.nS = len(V), len(F), len(S0)
+        self.K = self.params.K
+        self.kproj = self.params.kproj
+        h = C.c_void_p()
+        _check("cvt_job_create", L.cvt_job_create(V.reshape(-1), len(V), F.reshape(-1), len(F), S0.reshape(-1),
+                                                   len(S0), C.byref(self.params), C.byref(h)))
+        self._h = h
+
+    # ------------------------------------------------------------- lifecycle
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cvt_job_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        if not self._h:
+            raise CVTError("job", -4, "job is closed")
+        return self._h
+
+    def info(self) -> dict:
+        o = JobInfo()
+        _check("cvt_job_info_get", lib().cvt_job_info_get(self.handle, C.byref(o)))
+        d = {}
+        for k, _ in o._fields_:
+            if k == "reserved":
+                continue
+            v = getattr(o, k)
+            d[k] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+    # --------------------------------------------------------------- running
+    def step(self, iters: int = 1, stats: bool = False):
+        if stats:
+            arr = (IterStats * max(iters, 1))()
+            _check("cvt_job_step", lib().cvt_job_step(self.handle, iters, C.cast(arr, C.c_void_p)))
+            return [arr[i].as_dict() for i in range(iters)]
+        _check("cvt_job_step", lib().cvt_job_step(self.handle, iters, None))
+        return None
+
+    def step_timed(self, iters: int) -> float:
+        ms = C.c_double()
+        _check("cvt_job_step_timed", lib().cvt_job_step_timed(self.handle, iters, C.byref(ms)))
+        return ms.value
+
+    def profile(self, iters: int) -> dict:
+        """Mean device ms per iteration: site-grid rebuild, Lloyd kernel, whole iteration."""
+        g, l, t = C.c_double(), C.c_double(), C.c_double()
+        _check("cvt_job_profile", lib().cvt_job_profile(self.handle, iters, C.byref(g), C.byref(l), C.byref(t)))
+        return {"grid_ms": g.value, "lloyd_ms": l.value, "iter_ms": t.value}
+
+    # ----------------------------------------------------------------- state
+    def sites(self) -> np.ndarray:
+        out = np.zeros((self.nS, 3))
+        _check("cvt_job_get_sites", lib().cvt_job_get_sites(self.handle, out.reshape(-1)))
+        return out
+
+    def knn(self) -> np.ndarray:
+        out = np.zeros((self.nS, self.K), np.int32)
+        _check("cvt_job_get_knn", lib().cvt_job_get_knn(self.handle, out.reshape(-1)))
+        return out
+
+    def knnv(self) -> np.ndarray:
+        out = np.zeros((self.nS, self.kproj), np.int32)
+        _check("cvt_job_get_knnv", lib().cvt_job_get_knnv(self.handle, out.reshape(-1)))
+        return out
+
+    def freeze_state(self):
+        fz = np.zeros(self.nS, np.uint8)
+        st = np.zeros(self.nS, np.uint8)
+        _check("cvt_job_get_freeze", lib().cvt_job_get_freeze(self.handle, fz, st))
+        return fz, st
+
+    def normals(self) -> np.ndarray:
+        out = np.zeros((self.nV, 3))
+        _check("cvt_job_get_normals", lib().cvt_job_get_normals(self.handle, out.reshape(-1)))
+        return out
+
+    def set_state(self, S=None, knn_prev=None, knnv_prev=None, frozen=None, streak=None, has_prev: int = -1):
+        c = lambda a, t: None if a is None else np.ascontiguousarray(a, dtype=t).reshape(-1)  # noqa: E731
+        _check("cvt_job_set_state", lib().cvt_job_set_state(
+            self.handle, c(S, np.float64), c(knn_prev, np.int32), c(knnv_prev, np.int32), c(frozen, np.uint8),
+            c(streak, np.uint8), int(has_prev)))
+
+    def set_tiers(self, tier) -> None:
+        _check("cvt_job_set_tiers", lib().cvt_job_set_tiers(self.handle, np.ascontiguousarray(tier, np.uint8)))
+
+    def debug(self) -> dict:
+        n = self.nS
+        d = dict(nn_site=np.zeros(n, np.int32), U=np.zeros((n, 3)), W=np.zeros((n, 3)), cent=np.zeros((n, 3)),
+                 area=np.zeros(n), moment=np.zeros(n))
+        _check("cvt_job_get_debug", lib().cvt_job_get_debug(
+            self.handle, d["nn_site"], d["U"].reshape(-1), d["W"].reshape(-1), d["cent"].reshape(-1), d["area"],
+            d["moment"]))
+        return d
+
+    def energy(self) -> float:
+        e = C.c_double()
+        _check("cvt_job_energy", lib().cvt_job_energy(self.handle, C.byref(e)))
+        return e.value
+
+    def point_mesh_d2(self, P) -> np.ndarray:
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(P))
+        _check("cvt_job_point_mesh_d2", lib().cvt_job_point_mesh_d2(self.handle, P.reshape(-1), len(P), out))
+        return out

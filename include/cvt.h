@@ -1,0 +1,144 @@
+/* cvt.h -- C ABI of libcvt_hip.so, the MI355X (gfx950) surface Lloyd / CVT path.
+ *
+ * The reference (Harper-Yan/GPU_CVT) has no library boundary: its Lloyd loop
+ * lives in main() (main.cu:669-1073) and calls header-only CUDA helpers.  Each
+ * entry point below states which reference interface it replaces.  All
+ * pointers are plain host pointers unless the name says otherwise; sizes are
+ * element counts.  Every function returns 0 on success and a negative status
+ * on error (never exits); cvt_last_error() returns a thread-local message.
+ *
+ * Geometry on the host side is FP64, xyz interleaved (n*3 doubles).  On the
+ * device it is stored in FP32.  Site ids are the caller's indices 0..nS-1.
+ */
+#ifndef CVT_H_
+#define CVT_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVT_OK 0
+#define CVT_EINVAL (-1)    /* bad argument / unsupported parameter value */
+#define CVT_EHIP (-2)      /* HIP runtime error */
+#define CVT_ENOMEM (-3)    /* device allocation failed or over budget */
+#define CVT_ESTATE (-4)    /* call not valid in the current job state */
+#define CVT_ENONFINITE (-5) /* non-finite input geometry */
+
+/* Parameters of one job.  Replaces the constexpr block and argv parsing of
+ * main.cu:381-432 and the tier tables of main.cu:19-39. */
+typedef struct cvt_params {
+    int K;              /* site neighbours per cell (main.cu:398 K_NEIGH); one of 4,6,8,12,16,20,24,32 */
+    int kproj;          /* vertex list length for reprojection (main.cu:394 KPROJ); 5 */
+    int freeze_mode;    /* 0 off (main.cu mode 0), 1 single-tier streak, 2 per-site tiers (modes 1/2) */
+    int freeze_need;    /* mode 1: streak length (main.cu:407 freeze_monitor_iters) */
+    double freeze_disp_rel;   /* displacement threshold, fraction of the mesh bbox diagonal (linear) */
+    double freeze_disp_abs2;  /* if > 0: absolute squared threshold (main.cu:431 uses (0.01 R)^2) */
+    int tier_streak[6];       /* mode 2: streak per tier (main.cu:22,39) */
+    double tier_disp_abs2[6]; /* mode 2: squared threshold per tier; <= 0 means use the mode-1 value */
+    int nan_policy;     /* 0 keep the old site (scripts/testfreeze.py:413-425), 1 propagate NaN (2dClip.cuh:129-132) */
+    int clip_R_mode;    /* 0: R = mesh bbox max edge (main.cu:226,711), 1: R = clip_R */
+    double clip_R;
+    double site_cell_scale;  /* site grid cell size in units of the expected K-NN radius (0 = default) */
+    double vert_cell_scale;  /* vertex grid cell size in units of the expected kproj-NN radius (0 = default) */
+    int debug;          /* 1: record per-site intermediates of the last step (cvt_job_get_debug) */
+    int reserved[7];
+} cvt_params;
+
+/* Per-iteration counters; replaces the printf/CSV counters of main.cu:809-834
+ * (freeze columns of eval_iters.csv). */
+typedef struct cvt_iter_stats {
+    int count_same;        /* ordered KNN list equal to the previous one (freezetest.cuh:180-184) */
+    int count_low_disp;    /* |S'-S|^2 <= threshold */
+    int count_both;
+    int count_newly_frozen;
+    int frozen_total;      /* count_frozen_kernel (freezetest.cuh:50-54) */
+    int knn_exact_rescans; /* queries whose FP32 filter was inconclusive at the K-th boundary */
+    int knn_reranks;       /* queries whose in-list order needed FP64 keys */
+    int cell_failures;     /* cells whose hull walk did not close (site kept) */
+    int nonfinite_sites;   /* sites with non-finite coordinates after the step */
+    int reserved[3];
+} cvt_iter_stats;
+
+typedef struct cvt_job_info {
+    int nV, nF, nS, K, kproj;
+    int site_grid_dims[3];
+    int vert_grid_dims[3];
+    int site_table, vert_table;
+    double site_cell, vert_cell;
+    double bbox_lo[3], bbox_hi[3];
+    double bbox_diag, clip_R, area;
+    unsigned long long device_bytes;
+    int iterations_done;
+    int reserved[7];
+} cvt_job_info;
+
+typedef struct cvt_job cvt_job;
+
+int cvt_version(void);
+const char* cvt_last_error(void);
+void cvt_params_default(cvt_params* p);
+
+/* Create a job: uploads the mesh (V nV*3, F nF*3 zero-based), builds vertex
+ * normals (site_normals.cuh:14-41, as a deterministic CSR gather), the
+ * vertex->face CSR (main.cu:259-285), the face table and the static vertex
+ * grid, and uploads the sites S0 (nS*3).  Replaces the setup half of main()
+ * (main.cu:419-610).  Requires nS >= K+1, nV >= kproj. */
+int cvt_job_create(const double* V, int nV, const int* F, int nF, const double* S0, int nS, const cvt_params* p,
+                   cvt_job** out);
+void cvt_job_destroy(cvt_job* job);
+int cvt_job_info_get(cvt_job* job, cvt_job_info* out);
+
+/* Run `iters` Lloyd iterations (one iteration = the body of main.cu:669-881:
+ * site KNN, site->vertex frame, tangent-plane cell + centroid, centroid->mesh
+ * kproj-NN, reprojection, freeze test).  stats (nullable) receives one record
+ * per iteration.  Iterations are launched as HIP graphs on the job's stream. */
+int cvt_job_step(cvt_job* job, int iters, cvt_iter_stats* stats);
+
+/* Same, timed: device milliseconds between HIP events around the `iters`
+ * graph launches (no host synchronisation inside).  For benchmarking. */
+int cvt_job_step_timed(cvt_job* job, int iters, double* device_ms);
+
+/* Per-stage device timing: runs `iters` iterations launched eagerly on the
+ * job's stream with HIP events around each stage; returns the mean device
+ * milliseconds per iteration of the site-grid rebuild (grid_ms), of the fused
+ * Lloyd kernel (lloyd_ms) and of the whole iteration (iter_ms).  Replaces the
+ * per-stage cudaEvent pairs of main.cu:692-803. */
+int cvt_job_profile(cvt_job* job, int iters, double* grid_ms, double* lloyd_ms, double* iter_ms);
+
+/* State access (host arrays).  knn rows are in (d^2, index) order; knnv are
+ * the pinned/last vertex lists (nS*kproj); frozen/streak one byte per site. */
+int cvt_job_get_sites(cvt_job* job, double* S_out);
+int cvt_job_get_knn(cvt_job* job, int* knn_out);
+int cvt_job_get_knnv(cvt_job* job, int* knnv_out);
+int cvt_job_get_freeze(cvt_job* job, unsigned char* frozen_out, unsigned char* streak_out);
+/* Vertex normals as computed on the device (nV*3). */
+int cvt_job_get_normals(cvt_job* job, double* N_out);
+
+/* Teacher forcing for per-step parity: overwrite the job state.  Any pointer
+ * may be null (keeps that part).  has_prev: whether knn_prev is a valid
+ * previous list (main.cu:769). */
+int cvt_job_set_state(cvt_job* job, const double* S, const int* knn_prev, const int* knnv_prev,
+                      const unsigned char* frozen, const unsigned char* streak, int has_prev);
+/* Per-site tier ids 0..5 for freeze_mode 2 (main.cu:640-667 computes them on the host). */
+int cvt_job_set_tiers(cvt_job* job, const unsigned char* tier);
+
+/* Intermediates of the last step (requires params.debug): nearest vertex used
+ * for the frame, frame U/V (nS*3), pre-projection centroid (nS*3), cell area
+ * and polar second moment about the site.  Any pointer may be null. */
+int cvt_job_get_debug(cvt_job* job, int* nn_site, double* U, double* Vf, double* cent, double* area,
+                      double* moment);
+
+/* CVT energy of the current sites: sum over sites of the polar second moment
+ * of the tangent-plane cell about its site (the reference computes no energy;
+ * closest analogue scripts/testfreeze2.py:285-309).  All sites, fresh KNN. */
+int cvt_job_energy(cvt_job* job, double* E_out);
+
+/* Squared distance from each query point (n*3, host) to the job's input mesh,
+ * via a device LBVH (replaces bvh_closest_d2_kernel, evaluate.cuh:206-250, and
+ * the CPU BVH of evaluate.cuh:114-201).  out: n doubles. */
+int cvt_job_point_mesh_d2(cvt_job* job, const double* P, int n, double* d2_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CVT_H_ */

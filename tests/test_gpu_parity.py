@@ -1,0 +1,192 @@
+"""GPU parity: every stage of the HIP Lloyd step against the FP64 oracle.
+
+Per-step parity is teacher-forced: the oracle recomputes each stage from the
+GPU's own inputs for that stage (its FP32 sites, its centroids), so stages are
+compared one at a time.  Bars (written here, not derived from the code):
+  * KNN lists (site-site, site->vertex, centroid->vertex): bit-exact under the
+    (FP64 d^2, index) order;
+  * centroids and projected sites: within 1e-5 of the mesh bbox diagonal;
+  * frames: within 1e-5 (unit vectors);
+  * freeze state: exact.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gpu_cvt_amd import cvt, meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+TOL_REL_DIAG = 1e-5
+
+
+def _case(name):
+    if name == "ico3_300":
+        V, F = mg.icosphere(3)
+        S = mg.sample_surface(V, F, 300, seed=11)
+        return V, F, S, 8
+    if name == "ico5_2k_k20":  # the CPU reference-path configuration (first config)
+        V, F = mg.icosphere(5)
+        S = mg.sample_surface(V, F, 2000, seed=0)
+        return V, F, S, 20
+    if name == "torus_20k_k24":
+        V, F = mg.torus(1.0, 0.3, 150, 60, jitter=1e-3, seed=1)
+        S = mg.sample_surface(V, F, 8000, seed=2)
+        return V, F, S, 24
+    if name == "dsphere_k32":
+        V, F = mg.displaced_sphere(40, seed=4, crease_seed=5)
+        S = mg.sample_surface(V, F, 6000, seed=6)
+        return V, F, S, 32
+    raise KeyError(name)
+
+
+def _stage_check(orc, mesh, job, S32, K, prev_state=None):
+    """Run one GPU step from S32 and compare every stage with the oracle."""
+    diag = mesh.bbox_diag
+    knn_g = job.knn()
+    dbg = job.debug()
+    knnv_g = job.knnv()
+    Snew_g = job.sites()
+    # 1. site KNN, bit-exact
+    knn_o = orc.knn(S32, S32, K, exclude_self=True)
+    assert np.array_equal(knn_g, knn_o), f"site KNN mismatch in {np.sum(np.any(knn_g != knn_o, 1))} rows"
+    # 2. site -> nearest vertex (frame source), bit-exact
+    nn_o = orc.knn(mesh.V, S32, 1, exclude_self=False)[:, 0]
+    assert np.array_equal(dbg["nn_site"], nn_o)
+    U_o, W_o = orc.frames(mesh.N[nn_o])
+    assert np.abs(dbg["U"] - U_o).max() < 1e-5
+    assert np.abs(dbg["W"] - W_o).max() < 1e-5
+    # 3. cells and centroids (oracle: Sutherland-Hodgman in FP64; GPU: dual hull in FP32)
+    R = mesh.bbox_max_edge
+    cent_o, area_o, mom_o, _ = orc.cells(S32, U_o, W_o, knn_o, np.float32(R))
+    err_c = np.linalg.norm(dbg["cent"] - cent_o, axis=1).max()
+    assert err_c <= TOL_REL_DIAG * diag, f"centroid error {err_c:.3e} > {TOL_REL_DIAG * diag:.3e}"
+    np.testing.assert_allclose(dbg["area"], area_o, rtol=1e-4, atol=1e-12)
+    np.testing.assert_allclose(dbg["moment"], mom_o, rtol=1e-3, atol=1e-16)
+    # 4. centroid -> mesh kproj-NN on the GPU's FP32 centroids, bit-exact
+    cent_g32 = orc.f32r(dbg["cent"])
+    knnv_o = orc.knn(mesh.V, cent_g32, 5, exclude_self=False)
+    assert np.array_equal(knnv_g, knnv_o), f"vertex KNN mismatch in {np.sum(np.any(knnv_g != knnv_o, 1))} rows"
+    # 5. projection from the same centroids and lists
+    proj_o = orc.project(cent_g32, knnv_o, mesh.V, mesh.F, mesh.vf)
+    err_p = np.linalg.norm(Snew_g - proj_o, axis=1).max()
+    assert err_p <= TOL_REL_DIAG * diag, f"projection error {err_p:.3e}"
+    return dict(knn=knn_g, cent=dbg["cent"], Snew=Snew_g, err_c=err_c, err_p=err_p)
+
+
+@pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32"])
+def test_stages_teacher_forced(orc, name):
+    V, F, S, K = _case(name)
+    print(f"[parity] case={name} nV={len(V)} nF={len(F)} nS={len(S)} K={K}", flush=True)
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=K, debug=1) as job:
+        for it in range(3):
+            S32 = orc.f32r(job.sites())
+            st = job.step(1, stats=True)[0]
+            r = _stage_check(orc, mesh, job, S32, K)
+            print(f"  iter {it}: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e} stats {st}", flush=True)
+            assert st["cell_failures"] == 0
+
+
+def test_vertex_normals(orc):
+    V, F = mg.torus(1.0, 0.3, 60, 24, jitter=1e-3, seed=3)
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, mg.sample_surface(V, F, 100, 1), K=8) as job:
+        Ng = job.normals()
+    assert np.abs(Ng - mesh.N).max() < 1e-5
+
+
+def test_freeze_state_machine(orc):
+    V, F, S, K = _case("ico3_300")
+    mesh = orc.Mesh(V, F)
+    thr = 2e-3
+    with cvt.Job(V, F, S, K=K, debug=1, freeze_mode=1, freeze_need=3, freeze_disp_abs2=thr * thr) as job:
+        prev_knn = None
+        fz = np.zeros(len(S), np.uint8)
+        st = np.zeros(len(S), np.uint8)
+        for it in range(12):
+            S32 = orc.f32r(job.sites())
+            stats = job.step(1, stats=True)[0]
+            knn_g = job.knn()
+            Snew = orc.f32r(job.sites())
+            fz_g, st_g = job.freeze_state()
+            if prev_knn is None:
+                exp_st, exp_fz = np.zeros_like(st), fz
+                counts = np.zeros(4, np.int32)
+            else:
+                exp_st, exp_fz, counts = orc.freeze(S32, Snew, knn_g, prev_knn, None, np.full(6, thr * thr),
+                                                    np.full(6, 3, np.int32), 1, 1, st, fz)
+            assert np.array_equal(st_g, exp_st), f"iter {it}: streak mismatch"
+            assert np.array_equal(fz_g, exp_fz), f"iter {it}: frozen mismatch"
+            if prev_knn is not None:
+                assert stats["count_same"] == counts[0] and stats["count_low_disp"] == counts[1]
+                assert stats["count_both"] == counts[2] and stats["count_newly_frozen"] == counts[3]
+            assert stats["frozen_total"] == int(fz_g.sum())
+            fz, st, prev_knn = fz_g, st_g, knn_g
+        print(f"[freeze] frozen after 12 iters: {int(fz.sum())}/{len(S)}", flush=True)
+        assert fz.sum() > 0
+
+
+def test_frozen_sites_keep_lists(orc):
+    V, F, S, K = _case("ico3_300")
+    with cvt.Job(V, F, S, K=K, debug=1, freeze_mode=1) as job:
+        job.step(1)
+        knn0, knnv0 = job.knn(), job.knnv()
+        fz = np.zeros(len(S), np.uint8)
+        fz[::3] = 1
+        job.set_state(frozen=fz)
+        job.step(2)
+        knn1, knnv1 = job.knn(), job.knnv()
+    assert np.array_equal(knn1[fz == 1], knn0[fz == 1])
+    assert np.array_equal(knnv1[fz == 1], knnv0[fz == 1])
+
+
+def test_exact_ties_lattice(orc):
+    """Planar square lattice: equal distances everywhere, index tie-break."""
+    n = 12
+    xs, ys = np.meshgrid(np.arange(n, dtype=np.float64), np.arange(n, dtype=np.float64), indexing="ij")
+    V = np.stack([xs.ravel(), ys.ravel(), np.zeros(n * n)], 1) * 0.25
+    idx = np.arange(n * n).reshape(n, n)
+    a, b, c, d = idx[:-1, :-1].ravel(), idx[1:, :-1].ravel(), idx[1:, 1:].ravel(), idx[:-1, 1:].ravel()
+    F = np.concatenate([np.stack([a, b, c], 1), np.stack([a, c, d], 1)]).astype(np.int32)
+    S = V[idx[2:-2, 2:-2].ravel()] + np.array([0.0, 0.0, 0.0])
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=8, debug=1) as job:
+        S32 = orc.f32r(job.sites())
+        st = job.step(1, stats=True)[0]
+        knn_g = job.knn()
+        knn_o = orc.knn(S32, S32, 8, exclude_self=True)
+        assert np.array_equal(knn_g, knn_o)
+        assert st["knn_exact_rescans"] + st["knn_reranks"] > 0  # ties forced the FP64 paths
+        dbg = job.debug()
+    # interior cells of a square lattice are squares centred on their sites
+    inner = np.array([i for i in range(len(S)) if 0.5 < S[i, 0] < 2.0 and 0.5 < S[i, 1] < 2.0])
+    np.testing.assert_allclose(dbg["cent"][inner], S32[inner], atol=1e-6)
+    np.testing.assert_allclose(dbg["area"][inner], 0.25 ** 2, rtol=1e-5)
+
+
+def test_energy_decreases_and_matches_oracle(orc):
+    """Free-running: GPU and oracle trajectories from the same sites; the CVT
+    energy after 10 iterations agrees within 0.1%."""
+    V, F, S, K = _case("ico5_2k_k20")
+    mesh = orc.Mesh(V, F)
+    R = np.float32(mesh.bbox_max_edge)
+    st = orc.LloydState(S, K)
+    with cvt.Job(V, F, S, K=K, debug=1) as job:
+        e0 = job.energy()
+        job.step(10)
+        e_gpu = job.energy()
+        Sg = job.sites()
+    for _ in range(10):
+        out = orc.lloyd_step(mesh, st, K=K, R=R)
+        st.S = orc.f32r(out["Snew"])
+    knn_o = orc.knn(st.S, st.S, K, exclude_self=True)
+    nn_o = orc.knn(mesh.V, st.S, 1, exclude_self=False)[:, 0]
+    U, W = orc.frames(mesh.N[nn_o])
+    _, _, mom, _ = orc.cells(st.S, U, W, knn_o, R)
+    e_orc = mom.sum()
+    print(f"[energy] start {e0:.6e} gpu {e_gpu:.6e} oracle {e_orc:.6e}", flush=True)
+    assert e_gpu < e0
+    assert abs(e_gpu - e_orc) <= 1e-3 * e_orc
+    assert np.isfinite(Sg).all()
