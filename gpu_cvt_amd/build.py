@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcvt_hip.so")
-SOURCES = ["cvt_kernels.hip", "cvt_api.hip"]
+SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVT_OFFLOAD_ARCH", "gfx950")
 

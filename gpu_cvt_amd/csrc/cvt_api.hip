@@ -11,11 +11,13 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "../../include/cvt.h"
+#include "cvt_bvh.h"
 #include "cvt_kernels.h"
 
 using namespace cvt;
@@ -70,7 +72,7 @@ struct cvt_job {
     int cur = 0;
     int *knn = nullptr, *knnV = nullptr;
     unsigned char *frozen = nullptr, *streak = nullptr, *tier = nullptr;
-    int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr;
+    int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
     // debug
     int* dbg_nn = nullptr;
     float4 *dbg_U = nullptr, *dbg_V = nullptr, *dbg_cent = nullptr;
@@ -80,6 +82,7 @@ struct cvt_job {
     double diag = 0, R = 0, area = 0;
     int iters_done = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};
+    cvt_bvh* bvh = nullptr;  // LBVH of the input mesh, built on first use
 
     template <class T>
     int alloc(T** ptr, size_t count) {
@@ -92,13 +95,121 @@ struct cvt_job {
         *ptr = (T*)d;
         return 0;
     }
-    ~cvt_job() {
+    ~cvt_job();
+    void release() {
         for (auto& g : graph)
             if (g) (void)hipGraphExecDestroy(g);
         for (void* d : allocs) (void)hipFree(d);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
+
+// A device LBVH over one triangle mesh (owned buffers).
+struct cvt_bvh {
+    BVHDev d;
+    float4* tri = nullptr;
+    std::vector<void*> allocs;
+    hipStream_t stream = nullptr;
+    int nF = 0;
+    ~cvt_bvh() {
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+cvt_job::~cvt_job() {
+    if (bvh) delete bvh;
+    release();
+}
+
+namespace {
+
+int bvh_alloc(cvt_bvh* b, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
+    if (e != hipSuccess) return fail(CVT_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    b->allocs.push_back(*p);
+    return 0;
+}
+
+// Build an LBVH over (V, F) host arrays (FP32 on the device).
+int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_tri, cvt_bvh** out) {
+    cvt_bvh* b = new cvt_bvh();
+    b->nF = nF;
+    auto bail = [&](int rc) { delete b; return rc; };
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    std::vector<float4> tri;
+    if (!dev_tri) tri.resize((size_t)3 * nF);
+    for (int f = 0; f < nF; ++f)
+        for (int k = 0; k < 3; ++k) {
+            const int v = Fh[3 * f + k];
+            const float x = (float)Vh[3 * v], y = (float)Vh[3 * v + 1], z = (float)Vh[3 * v + 2];
+            lo[0] = std::min(lo[0], x); lo[1] = std::min(lo[1], y); lo[2] = std::min(lo[2], z);
+            hi[0] = std::max(hi[0], x); hi[1] = std::max(hi[1], y); hi[2] = std::max(hi[2], z);
+            if (!dev_tri) {
+                float w;
+                std::memcpy(&w, &v, 4);
+                tri[(size_t)3 * f + k] = make_float4(x, y, z, w);
+            }
+        }
+    (void)nV;
+    int rc;
+    void* p;
+    if (dev_tri) {
+        b->d.tri = dev_tri;
+    } else {
+        if ((rc = bvh_alloc(b, &p, sizeof(float4) * tri.size()))) return bail(rc);
+        b->tri = (float4*)p;
+        b->d.tri = b->tri;
+        if (hipMemcpyAsync(b->tri, tri.data(), sizeof(float4) * tri.size(), hipMemcpyHostToDevice, b->stream))
+            return bail(fail(CVT_EHIP, "upload failed"));
+    }
+    const long nn = 2L * nF - 1;
+    if ((rc = bvh_alloc(b, &p, sizeof(float4) * nn))) return bail(rc);
+    b->d.lo = (float4*)p;
+    if ((rc = bvh_alloc(b, &p, sizeof(float4) * nn))) return bail(rc);
+    b->d.hi = (float4*)p;
+    if ((rc = bvh_alloc(b, &p, sizeof(int2) * std::max(1, nF - 1)))) return bail(rc);
+    b->d.child = (int2*)p;
+    if ((rc = bvh_alloc(b, &p, sizeof(int) * nn))) return bail(rc);
+    b->d.parent = (int*)p;
+    if ((rc = bvh_alloc(b, &p, sizeof(int) * nF))) return bail(rc);
+    b->d.leaf_tri = (int*)p;
+    void* work = nullptr;
+    if (hipMalloc(&work, bvh_work_bytes(nF)) != hipSuccess) return bail(fail(CVT_ENOMEM, "bvh work alloc"));
+    bvh_build(b->d, b->d.tri, nF, lo, hi, work, b->stream);
+    hipError_t e = hipStreamSynchronize(b->stream);
+    (void)hipFree(work);
+    if (e != hipSuccess) return bail(fail(CVT_EHIP, "bvh build: %s", hipGetErrorString(e)));
+    if ((e = hipGetLastError()) != hipSuccess) return bail(fail(CVT_EHIP, "bvh build launch: %s", hipGetErrorString(e)));
+    *out = b;
+    return 0;
+}
+
+int bvh_query(cvt_bvh* b, const double* P, int n, double* d2) {
+    if (n <= 0) return 0;
+    std::vector<float4> h(n);
+    for (int i = 0; i < n; ++i) h[i] = make_float4((float)P[3 * i], (float)P[3 * i + 1], (float)P[3 * i + 2], 0.f);
+    float4* dP = nullptr;
+    float* dD = nullptr;
+    if (hipMalloc(&dP, sizeof(float4) * n) != hipSuccess) return fail(CVT_ENOMEM, "query alloc");
+    if (hipMalloc(&dD, sizeof(float) * n) != hipSuccess) { (void)hipFree(dP); return fail(CVT_ENOMEM, "query alloc"); }
+    std::vector<float> o(n);
+    hipError_t e = hipMemcpyAsync(dP, h.data(), sizeof(float4) * n, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+        bvh_point_d2(b->d, dP, n, dD, b->stream);
+        e = hipMemcpyAsync(o.data(), dD, sizeof(float) * n, hipMemcpyDeviceToHost, b->stream);
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+    (void)hipFree(dP);
+    (void)hipFree(dD);
+    if (e != hipSuccess) return fail(CVT_EHIP, "bvh query: %s", hipGetErrorString(e));
+    for (int i = 0; i < n; ++i) d2[i] = o[i];
+    return 0;
+}
+
+}  // namespace
 
 namespace {
 
@@ -174,6 +285,7 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     a.tier = j->tier;
     a.has_prev = j->has_prev;
     a.stats = j->stats;
+    a.defer_list = j->defer_list;
     a.nS = j->nS;
     a.R = (float)j->R;
     a.freeze_mode = j->p.freeze_mode;
@@ -354,7 +466,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->knnV, (size_t)nS * p.kproj)) || (rc = j->alloc(&j->frozen, nS)) ||
         (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
         (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
-        (rc = j->alloc(&j->hist_count, 1)))
+        (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)))
         return bail(rc);
     if (p.debug) {
         if ((rc = j->alloc(&j->dbg_nn, nS)) || (rc = j->alloc(&j->dbg_U, nS)) || (rc = j->alloc(&j->dbg_V, nS)) ||
@@ -434,6 +546,7 @@ static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
         o.knn_reranks = w[6];
         o.cell_failures = w[7];
         o.nonfinite_sites = w[8];
+        o.deferred_sites = w[9];
     }
     return 0;
 }
@@ -622,6 +735,7 @@ int cvt_job_energy(cvt_job* j, double* E) {
     if (!j || !E) return fail(CVT_EINVAL, "null argument");
     if (!j->p.debug) return fail(CVT_ESTATE, "energy evaluation needs params.debug (per-site moment buffer)");
     hipStream_t s = j->stream;
+    launch_fill(j->stats, kStatWords, 0, s);
     launch_grid_build(j->S[j->cur], j->nS, j->sgp, j->sgb, s);
     StepArgs a = make_args(j, j->cur, 1);
     launch_lloyd(a, j->p.K, j->p.kproj, s);
@@ -634,9 +748,103 @@ int cvt_job_energy(cvt_job* j, double* E) {
     return 0;
 }
 
-int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
-    (void)j; (void)P; (void)n; (void)d2;
-    return fail(CVT_EINVAL, "cvt_job_point_mesh_d2: not built yet");
+int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
+    if (!j || !F_out || !nF_out) return fail(CVT_EINVAL, "null argument");
+    *F_out = nullptr;
+    *nF_out = 0;
+    if (j->nS >= (1 << 21)) return fail(CVT_EINVAL, "remesh keys pack site ids in 21 bits (nS < 2097152)");
+    const int K = j->p.K, M = K + 4, nS = j->nS;
+    const long nk = (long)nS * M;
+    if (nk > (1L << 31) - 1) return fail(CVT_EINVAL, "too many candidate triangles");
+    hipStream_t s = j->stream;
+    // temporaries (this is an output stage, not part of the timed iteration)
+    std::vector<void*> tmp;
+    auto talloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        tmp.push_back(p);
+        return p;
+    };
+    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
+    const long hw = radix_hist_words((int)nk);
+    int* knn_tmp = (int*)talloc(sizeof(int) * (size_t)nS * K);
+    auto* keys = (unsigned long long*)talloc(8 * (size_t)nk);
+    auto* keys_alt = (unsigned long long*)talloc(8 * (size_t)nk);
+    float4* site_n = (float4*)talloc(sizeof(float4) * nS);
+    int* hist = (int*)talloc(4 * (size_t)(hw + 1));
+    int* hsum = (int*)talloc(4 * (size_t)scan_partials(std::max(hw, nk) + 1));
+    int* flag = (int*)talloc(4 * (size_t)nk);
+    int* pos = (int*)talloc(4 * (size_t)(nk + 1));
+    int4* tri = (int4*)talloc(sizeof(int4) * (size_t)nk);
+    int* keep = (int*)talloc(4 * (size_t)nk);
+    int* pos2 = (int*)talloc(4 * (size_t)(nk + 1));
+    if (!knn_tmp || !keys || !keys_alt || !site_n || !hist || !hsum || !flag || !pos || !tri || !keep || !pos2) {
+        cleanup();
+        return fail(CVT_ENOMEM, "remesh temporaries");
+    }
+    launch_fill(j->stats, kStatWords, 0, s);
+    launch_grid_build(j->S[j->cur], nS, j->sgp, j->sgb, s);
+    StepArgs a = make_args(j, j->cur, 1);
+    launch_remesh_cells(a, K, knn_tmp, keys, site_n, s);
+    radix_sort_u64(keys, (int)nk, 64, keys_alt, hist, hsum, s);
+    launch_unique_mark(keys, (int)nk, flag, s);
+    launch_scan(flag, (int)nk, pos, hsum, s);
+    int nuniq = 0;
+    hipError_t e = hipMemcpyAsync(&nuniq, pos + nk, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { cleanup(); return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e)); }
+    launch_tri_finish(keys, flag, pos, (int)nk, j->S[j->cur], site_n, (float)eps, tri, keep, s);
+    launch_scan(keep, nuniq, pos2, hsum, s);
+    int nkeep = 0;
+    e = hipMemcpyAsync(&nkeep, pos2 + nuniq, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) { cleanup(); return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e)); }
+    int* dF = (int*)talloc(sizeof(int) * 3 * (size_t)std::max(nkeep, 1));
+    if (!dF) { cleanup(); return fail(CVT_ENOMEM, "remesh output"); }
+    launch_tri_compact(tri, keep, pos2, nuniq, dF, s);
+    int* hF = (int*)std::malloc(sizeof(int) * 3 * (size_t)std::max(nkeep, 1));
+    if (!hF) { cleanup(); return fail(CVT_ENOMEM, "host output"); }
+    e = hipMemcpyAsync(hF, dF, sizeof(int) * 3 * (size_t)nkeep, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    cleanup();
+    if (e != hipSuccess) { std::free(hF); return fail(CVT_EHIP, "remesh copy: %s", hipGetErrorString(e)); }
+    *F_out = hF;
+    *nF_out = nkeep;
+    return 0;
 }
+
+void cvt_free(void* p) { std::free(p); }
+
+int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
+    if (!j || (!P && n > 0) || (!d2 && n > 0) || n < 0) return fail(CVT_EINVAL, "bad argument");
+    if (!j->bvh) {
+        // the job keeps only FP32 vertices on the device: rebuild host FP64 copies of them
+        std::vector<float4> V4(j->nV);
+        std::vector<int> Fv((size_t)3 * j->nF);
+        HIPCHK(hipMemcpy(V4.data(), j->V, sizeof(float4) * j->nV, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(Fv.data(), j->F, sizeof(int) * Fv.size(), hipMemcpyDeviceToHost));
+        std::vector<double> Vd((size_t)3 * j->nV);
+        for (int v = 0; v < j->nV; ++v) { Vd[3 * v] = V4[v].x; Vd[3 * v + 1] = V4[v].y; Vd[3 * v + 2] = V4[v].z; }
+        int rc = bvh_make(Vd.data(), j->nV, Fv.data(), j->nF, j->tri, &j->bvh);
+        if (rc) return rc;
+    }
+    return bvh_query(j->bvh, P, n, d2);
+}
+
+int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out) {
+    if (!out || !V || !F || nV < 3 || nF < 1) return fail(CVT_EINVAL, "bad argument");
+    *out = nullptr;
+    if (!check_finite(V, 3L * nV)) return fail(CVT_ENONFINITE, "non-finite vertex coordinate");
+    for (long f = 0; f < 3L * nF; ++f)
+        if (F[f] < 0 || F[f] >= nV) return fail(CVT_EINVAL, "face index out of range at %ld", f);
+    return bvh_make(V, nV, F, nF, nullptr, out);
+}
+
+int cvt_bvh_point_d2(cvt_bvh* b, const double* P, int n, double* d2) {
+    if (!b || n < 0 || (n > 0 && (!P || !d2))) return fail(CVT_EINVAL, "bad argument");
+    return bvh_query(b, P, n, d2);
+}
+
+void cvt_bvh_destroy(cvt_bvh* b) { delete b; }
 
 }  // extern "C"

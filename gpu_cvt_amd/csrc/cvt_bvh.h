@@ -1,0 +1,45 @@
+// cvt_bvh.h -- device LBVH over a triangle mesh and a stable LSD radix sort.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace cvt {
+
+// Stable LSD radix sort of (u32 key, int value) pairs, 8-bit digits.
+struct RadixBufs {
+    unsigned* keys_alt = nullptr;
+    int* vals_alt = nullptr;
+    int* hist = nullptr;   // 256 * nblocks
+    int* hsum = nullptr;   // scan partials
+    int n = 0;
+};
+int radix_tile();
+long radix_hist_words(int n);
+// Sorts keys/vals in place (ping-pongs through the alt buffers, copies back if
+// the pass count is odd).  `bits` = number of significant low key bits.
+void radix_sort_pairs(unsigned* keys, int* vals, int n, int bits, RadixBufs& b, hipStream_t s);
+// Keys-only stable LSD sort of 64-bit keys; hist needs radix_hist_words(n)+1
+// ints, hsum scan_partials(radix_hist_words(n)) ints.
+void radix_sort_u64(unsigned long long* keys, int n, int bits, unsigned long long* keys_alt, int* hist, int* hsum,
+                    hipStream_t s);
+
+// LBVH: internal nodes [0, n-1), leaves [n-1, 2n-1); one triangle per leaf.
+struct BVHDev {
+    int n = 0;             // triangles
+    float4* lo = nullptr;  // 2n-1 boxes (w unused)
+    float4* hi = nullptr;
+    int2* child = nullptr; // n-1 internal nodes: (left, right) in combined numbering
+    int* parent = nullptr; // 2n-1 (root: -1)
+    int* leaf_tri = nullptr;  // n: triangle id of each leaf (Morton order)
+    const float4* tri = nullptr;  // 3*nF face table (x, y, z, vertex id bits)
+};
+
+// Work memory needed by bvh_build (bytes), so callers can allocate once.
+size_t bvh_work_bytes(int nF);
+// Builds the hierarchy for nF triangles of the face table `tri` (device),
+// using `work` (device, bvh_work_bytes) and the node arrays in `out`.
+void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const float hi[3], void* work,
+               hipStream_t s);
+// Squared distance of each point (float4 xyz) to the mesh.
+void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s);
+
+}  // namespace cvt

@@ -1,0 +1,310 @@
+// cvt_bvh.hip -- LBVH (Morton codes, Karras hierarchy, bottom-up boxes) and a
+// stackless nearest-distance traversal; replaces the reference's CPU
+// median-split BVH (evaluate.cuh:114-201) and its stack[64] query kernel
+// (evaluate.cuh:206-250).
+#include <hip/hip_runtime.h>
+
+#include "cvt_bvh.h"
+#include "cvt_kernels.h"
+
+namespace cvt {
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------------------ radix sort
+constexpr int kRadixItems = 8;
+constexpr int kRadixTile = 256 * kRadixItems;
+
+int radix_tile() { return kRadixTile; }
+long radix_hist_words(int n) { return 256L * nblk(n, kRadixTile); }
+
+template <class KeyT>
+__global__ __launch_bounds__(256) void k_radix_hist(const KeyT* __restrict__ keys, int n, int shift,
+                                                    int* __restrict__ hist) {
+    __shared__ int h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const int base = blockIdx.x * kRadixTile;
+    for (int k = 0; k < kRadixItems; ++k) {
+        const int i = base + k * 256 + threadIdx.x;
+        if (i < n) atomicAdd(&h[(unsigned)(keys[i] >> shift) & 255u], 1);
+    }
+    __syncthreads();
+    hist[threadIdx.x * gridDim.x + blockIdx.x] = h[threadIdx.x];
+}
+
+// Stable scatter: items are ranked in index order within the tile by a
+// wave-level multisplit (8 ballots per digit) and per-wave digit counts.
+template <class KeyT>
+__global__ __launch_bounds__(256) void k_radix_scatter(const KeyT* __restrict__ keys, const int* __restrict__ vals,
+                                                       int n, int shift, const int* __restrict__ offs,
+                                                       KeyT* __restrict__ keys_out, int* __restrict__ vals_out) {
+    __shared__ int run[256];
+    __shared__ int wcnt[4][256];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    run[tid] = offs[tid * gridDim.x + blockIdx.x];
+    const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const int base = blockIdx.x * kRadixTile;
+    for (int k = 0; k < kRadixItems; ++k) {
+        const int i = base + k * 256 + tid;
+        const bool valid = i < n;
+        const KeyT key = valid ? keys[i] : (KeyT)0;
+        const int val = (valid && vals) ? vals[i] : 0;
+        const unsigned d = (unsigned)(key >> shift) & 255u;
+        unsigned long long peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const int rank = __popcll(peers & lt);
+        wcnt[0][tid] = 0; wcnt[1][tid] = 0; wcnt[2][tid] = 0; wcnt[3][tid] = 0;
+        __syncthreads();
+        if (valid && rank == 0) wcnt[w][d] = __popcll(peers);
+        __syncthreads();
+        int pos = 0;
+        if (valid) {
+            pos = run[d] + rank;
+            for (int q = 0; q < w; ++q) pos += wcnt[q][d];
+        }
+        __syncthreads();
+        run[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+        if (valid) {
+            keys_out[pos] = key;
+            if (vals_out) vals_out[pos] = val;
+        }
+        __syncthreads();
+    }
+}
+
+template <class KeyT>
+__global__ void k_copy_pairs(const KeyT* __restrict__ ks, const int* __restrict__ vs, int n, KeyT* __restrict__ kd,
+                             int* __restrict__ vd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        kd[i] = ks[i];
+        if (vd) vd[i] = vs[i];
+    }
+}
+
+template <class KeyT>
+static void radix_sort_impl(KeyT* keys, int* vals, int n, int bits, KeyT* keys_alt, int* vals_alt, int* hist,
+                            int* hsum, hipStream_t s) {
+    if (n <= 1) return;
+    const int nb = nblk(n, kRadixTile);
+    KeyT* ka = keys;
+    int* va = vals;
+    KeyT* kb = keys_alt;
+    int* vb = vals ? vals_alt : nullptr;
+    int passes = 0;
+    for (int shift = 0; shift < bits; shift += 8, ++passes) {
+        hipLaunchKernelGGL(k_radix_hist<KeyT>, dim3(nb), dim3(256), 0, s, ka, n, shift, hist);
+        launch_scan(hist, 256 * nb, hist, hsum, s);
+        hipLaunchKernelGGL(k_radix_scatter<KeyT>, dim3(nb), dim3(256), 0, s, ka, va, n, shift, hist, kb, vb);
+        KeyT* tk = ka; ka = kb; kb = tk;
+        int* tv = va; va = vb; vb = tv;
+    }
+    if (passes & 1)
+        hipLaunchKernelGGL(k_copy_pairs<KeyT>, dim3(nblk(n, 256)), dim3(256), 0, s, ka, va, n, keys, vals);
+}
+
+void radix_sort_pairs(unsigned* keys, int* vals, int n, int bits, RadixBufs& b, hipStream_t s) {
+    radix_sort_impl<unsigned>(keys, vals, n, bits, b.keys_alt, b.vals_alt, b.hist, b.hsum, s);
+}
+
+void radix_sort_u64(unsigned long long* keys, int n, int bits, unsigned long long* keys_alt, int* hist, int* hsum,
+                    hipStream_t s) {
+    radix_sort_impl<unsigned long long>(keys, nullptr, n, bits, keys_alt, nullptr, hist, hsum, s);
+}
+
+// ------------------------------------------------------------------ LBVH
+__device__ __forceinline__ unsigned expand10(unsigned v) {
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+
+__global__ void k_morton(const float4* __restrict__ tri, int nF, float3 lo, float3 inv_ext, unsigned* __restrict__ keys,
+                         int* __restrict__ ids) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const float4 a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
+    const float cx = (a.x + b.x + c.x) * (1.f / 3.f), cy = (a.y + b.y + c.y) * (1.f / 3.f),
+                cz = (a.z + b.z + c.z) * (1.f / 3.f);
+    const unsigned x = (unsigned)fminf(fmaxf((cx - lo.x) * inv_ext.x * 1024.f, 0.f), 1023.f);
+    const unsigned y = (unsigned)fminf(fmaxf((cy - lo.y) * inv_ext.y * 1024.f, 0.f), 1023.f);
+    const unsigned z = (unsigned)fminf(fmaxf((cz - lo.z) * inv_ext.z * 1024.f, 0.f), 1023.f);
+    keys[f] = (expand10(x) << 2) | (expand10(y) << 1) | expand10(z);
+    ids[f] = f;
+}
+
+// common-prefix length of sorted keys i, j (index-augmented for duplicates)
+__device__ __forceinline__ int delta(const unsigned* __restrict__ k, int n, int i, int j) {
+    if (j < 0 || j >= n) return -1;
+    const unsigned a = k[i], b = k[j];
+    if (a == b) return 32 + __clz((unsigned)(i ^ j));
+    return __clz(a ^ b);
+}
+
+// Karras (2012) binary radix tree: internal node i, children in combined numbering.
+__global__ void k_karras(const unsigned* __restrict__ keys, int n, int2* __restrict__ child, int* __restrict__ parent) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+    const int dmin = delta(keys, n, i, i - d);
+    int lmax = 2;
+    while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t >= 1; t >>= 1)
+        if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+    const int j = i + l * d;
+    const int dnode = delta(keys, n, i, j);
+    int s = 0;
+    int t = l;
+    do {
+        t = (t + 1) >> 1;
+        if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    } while (t > 1);
+    const int g = i + s * d + min(d, 0);
+    const int left = (min(i, j) == g) ? (n - 1 + g) : g;
+    const int right = (max(i, j) == g + 1) ? (n - 1 + g + 1) : (g + 1);
+    child[i] = make_int2(left, right);
+    parent[left] = i;
+    parent[right] = i;
+}
+
+__global__ void k_leaf_boxes(const float4* __restrict__ tri, const int* __restrict__ leaf_tri, int n,
+                             float4* __restrict__ lo, float4* __restrict__ hi, int* __restrict__ flags) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    const int f = leaf_tri[l];
+    const float4 a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
+    lo[n - 1 + l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.f);
+    hi[n - 1 + l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.f);
+    if (l < n - 1) flags[l] = 0;
+}
+
+// Bottom-up refit: the second child to arrive at a node computes its box.
+__global__ void k_refit(int n, const int2* __restrict__ child, const int* __restrict__ parent, float4* lo, float4* hi,
+                        int* flags) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n) return;
+    int node = parent[n - 1 + l];
+    while (node >= 0) {
+        __threadfence();
+        if (atomicAdd(&flags[node], 1) == 0) return;
+        __threadfence();
+        const int2 c = child[node];
+        const float4 la = lo[c.x], lb = lo[c.y];
+        const float4 ha = hi[c.x], hb = hi[c.y];
+        lo[node] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.f);
+        hi[node] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.f);
+        node = parent[node];
+    }
+}
+
+size_t bvh_work_bytes(int nF) {
+    const long nb = radix_hist_words(nF);
+    return (size_t)nF * 4 * 4 + (size_t)(nb + 1 + scan_partials(nb) + 64) * 4 + 256;
+}
+
+void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const float hi[3], void* work,
+               hipStream_t s) {
+    out.n = nF;
+    out.tri = tri;
+    char* w = (char*)work;
+    unsigned* keys = (unsigned*)w;
+    w += (size_t)nF * 4;
+    RadixBufs rb;
+    rb.keys_alt = (unsigned*)w;
+    w += (size_t)nF * 4;
+    rb.vals_alt = (int*)w;
+    w += (size_t)nF * 4;
+    int* flags = (int*)w;
+    w += (size_t)nF * 4;
+    rb.hist = (int*)w;
+    w += (size_t)(radix_hist_words(nF) + 1) * 4;  // + the scan total
+    rb.hsum = (int*)w;
+    float3 l3 = make_float3(lo[0], lo[1], lo[2]);
+    float3 ie = make_float3(1.f / fmaxf(hi[0] - lo[0], 1e-30f), 1.f / fmaxf(hi[1] - lo[1], 1e-30f),
+                            1.f / fmaxf(hi[2] - lo[2], 1e-30f));
+    hipLaunchKernelGGL(k_morton, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, nF, l3, ie, keys, out.leaf_tri);
+    radix_sort_pairs(keys, out.leaf_tri, nF, 30, rb, s);
+    launch_fill(out.parent, 2L * nF - 1, -1, s);
+    if (nF > 1) hipLaunchKernelGGL(k_karras, dim3(nblk(nF - 1, 256)), dim3(256), 0, s, keys, nF, out.child, out.parent);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, out.leaf_tri, nF, out.lo, out.hi, flags);
+    if (nF > 1) hipLaunchKernelGGL(k_refit, dim3(nblk(nF, 256)), dim3(256), 0, s, nF, out.child, out.parent, out.lo,
+                                   out.hi, flags);
+}
+
+// evaluate.cuh:44-51
+__device__ __forceinline__ float box_d2(float3 p, float4 lo, float4 hi) {
+    const float dx = fmaxf(fmaxf(lo.x - p.x, 0.f), p.x - hi.x);
+    const float dy = fmaxf(fmaxf(lo.y - p.y, 0.f), p.y - hi.y);
+    const float dz = fmaxf(fmaxf(lo.z - p.z, 0.f), p.z - hi.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// evaluate.cuh:54-104 (point-triangle squared distance, Ericson).
+__device__ __forceinline__ float tri_d2(float3 p, float4 A, float4 B, float4 C) {
+    const float3 cp = closest_point_tri(p, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                        make_float3(C.x, C.y, C.z));
+    const float dx = cp.x - p.x, dy = cp.y - p.y, dz = cp.z - p.z;
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Stackless traversal with parent links: children are visited near-first
+// (by box distance, ties left-first); returning from a child the same order
+// is recomputed, so no stack is needed.
+__global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __restrict__ P, int np,
+                                                  float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float4 q4 = P[i];
+    const float3 q = make_float3(q4.x, q4.y, q4.z);
+    const int n = b.n;
+    float best = __builtin_huge_valf();
+    if (n == 1) {
+        const int f = b.leaf_tri[0];
+        out[i] = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
+        return;
+    }
+    int cur = 0, last = -1;
+    for (;;) {
+        if (cur >= n - 1) {  // leaf
+            const int f = b.leaf_tri[cur - (n - 1)];
+            best = fminf(best, tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]));
+            last = cur;
+            cur = b.parent[cur];
+            if (cur < 0) break;
+            continue;
+        }
+        const int2 c = b.child[cur];
+        const float dl = box_d2(q, b.lo[c.x], b.hi[c.x]);
+        const float dr = box_d2(q, b.lo[c.y], b.hi[c.y]);
+        const int nearc = dl <= dr ? c.x : c.y, farc = dl <= dr ? c.y : c.x;
+        const float dn = dl <= dr ? dl : dr, df = dl <= dr ? dr : dl;
+        int next;
+        if (last == b.parent[cur]) {
+            next = dn < best ? nearc : (df < best ? farc : b.parent[cur]);
+        } else if (last == nearc) {
+            next = df < best ? farc : b.parent[cur];
+        } else {
+            next = b.parent[cur];
+        }
+        last = cur;
+        cur = next;
+        if (cur < 0) break;
+    }
+    out[i] = best;
+}
+
+void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_point_d2, dim3(nblk(n, 256)), dim3(256), 0, s, b, P, n, out);
+}
+
+}  // namespace cvt

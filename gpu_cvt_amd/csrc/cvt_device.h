@@ -203,8 +203,9 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
 }
 
 // K nearest (K < L) in exact (key64, id) order, as slots.  Returns
-// 0 = FP32 order proven, 1 = FP64 rerank inside the list, 2 = exact rescan.
-template <int K, int L>
+// 0 = FP32 order proven, 1 = FP64 rerank inside the list, 2 = exact rescan,
+// -1 = not provable in FP32 (only when !PRECISE: the caller defers the query).
+template <int K, int L, bool PRECISE>
 __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
                                          const int* __restrict__ ids, float qx, float qy, float qz, int self_slot,
                                          int* __restrict__ slot_out) {
@@ -212,44 +213,48 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
     TopL<L> T;
     grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, T);
     const bool margin = T.d[L - 1] > T.d[K - 1] * (1.f + kRel) && T.d[K - 1] < __builtin_huge_valf();
-    if (!margin) {
-        const float tau = T.d[L - 1] * (1.f + 1e-5f);
-        grid_exact64<K>(g, start, pts, ids, qx, qy, qz, self_slot, tau, slot_out);
-        return 2;
-    }
     bool order = true;
 #pragma unroll
     for (int t = 0; t + 1 < K; ++t) order = order && (T.d[t + 1] > T.d[t] * (1.f + kRel));
 #pragma unroll
     for (int t = 0; t < K; ++t) slot_out[t] = T.s[t];
-    if (order) return 0;
-    // FP64 rerank: odd-even transposition passes on (key64, id) until sorted.
-    double kd[K];
-    int kid[K];
+    if (margin && order) return 0;
+    if constexpr (!PRECISE) {
+        return -1;
+    } else {
+        if (!margin) {
+            const float tau = T.d[L - 1] * (1.f + 1e-5f);
+            grid_exact64<K>(g, start, pts, ids, qx, qy, qz, self_slot, tau, slot_out);
+            return 2;
+        }
+        // FP64 rerank: odd-even transposition passes on (key64, id) until sorted.
+        double kd[K];
+        int kid[K];
 #pragma unroll
-    for (int t = 0; t < K; ++t) {
-        const float4 p = pts[slot_out[t]];
-        kd[t] = key64(qx, qy, qz, p.x, p.y, p.z);
-        kid[t] = ids[slot_out[t]];
-    }
-    for (int pass = 0; pass < K; ++pass) {
-        bool swapped = false;
+        for (int t = 0; t < K; ++t) {
+            const float4 p = pts[slot_out[t]];
+            kd[t] = key64(qx, qy, qz, p.x, p.y, p.z);
+            kid[t] = ids[slot_out[t]];
+        }
+        for (int pass = 0; pass < K; ++pass) {
+            bool swapped = false;
 #pragma unroll
-        for (int par = 0; par < 2; ++par) {
+            for (int par = 0; par < 2; ++par) {
 #pragma unroll
-            for (int t = par; t + 1 < K; t += 2) {
-                const bool sw = kd[t + 1] < kd[t] || (kd[t + 1] == kd[t] && kid[t + 1] < kid[t]);
-                if (sw) {
-                    double td = kd[t]; kd[t] = kd[t + 1]; kd[t + 1] = td;
-                    int ti = kid[t]; kid[t] = kid[t + 1]; kid[t + 1] = ti;
-                    int ts = slot_out[t]; slot_out[t] = slot_out[t + 1]; slot_out[t + 1] = ts;
-                    swapped = true;
+                for (int t = par; t + 1 < K; t += 2) {
+                    const bool sw = kd[t + 1] < kd[t] || (kd[t + 1] == kd[t] && kid[t + 1] < kid[t]);
+                    if (sw) {
+                        double td = kd[t]; kd[t] = kd[t + 1]; kd[t + 1] = td;
+                        int ti = kid[t]; kid[t] = kid[t + 1]; kid[t + 1] = ti;
+                        int ts = slot_out[t]; slot_out[t] = slot_out[t + 1]; slot_out[t + 1] = ts;
+                        swapped = true;
+                    }
                 }
             }
+            if (!swapped) break;
         }
-        if (!swapped) break;
+        return 1;
     }
-    return 1;
 }
 
 // ------------------------------------------------------------------- frames
@@ -308,12 +313,14 @@ struct CellOut {
     float cx, cy, area, moment;
     int nverts;
     bool ok;
+    bool defer;  // an orientation test FP32 could not decide (!PRECISE only)
 };
 
-template <int M>
+template <int M, bool PRECISE>
 __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
     CellOut o;
     o.ok = false;
+    o.defer = false;
     o.nverts = 0;
     // start: max x, then max y -- always a hull vertex
     float sx = px[0], sy = py[0];
@@ -338,16 +345,21 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
             const float t1 = ex * wy, t2 = ey * wx;
             float cr = t1 - t2;
             if (fabsf(cr) <= 1e-6f * (fabsf(t1) + fabsf(t2)) && !none && isnew) {
-                const double dex = (double)bx - (double)curx, dey = (double)by - (double)cury;
-                const double dwx = (double)px[j] - (double)curx, dwy = (double)py[j] - (double)cury;
-                const double c64 = dex * dwy - dey * dwx;
-                cr = c64 < 0.0 ? -1.f : (c64 > 0.0 ? 1.f : 0.f);
+                if constexpr (PRECISE) {
+                    const double dex = (double)bx - (double)curx, dey = (double)by - (double)cury;
+                    const double dwx = (double)px[j] - (double)curx, dwy = (double)py[j] - (double)cury;
+                    const double c64 = dex * dwy - dey * dwx;
+                    cr = c64 < 0.0 ? -1.f : (c64 > 0.0 ? 1.f : 0.f);
+                } else {
+                    o.defer = true;
+                }
             }
             const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
             const bool take = isnew && (none || cr < 0.f || (cr == 0.f && farther));
             bx = take ? px[j] : bx;
             by = take ? py[j] : by;
         }
+        if (!PRECISE && o.defer) return o;
         if (bx == curx && by == cury) return o;  // degenerate: all dual points coincide
         const float det = curx * by - cury * bx;
         const float inv = 1.0f / det;
@@ -384,6 +396,54 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
     o.cx = Cx.s * inv6A;
     o.cy = Cy.s * inv6A;
     return o;
+}
+
+// Labelled walk of the same dual hull for the remesh stage: calls
+// emit(jprev, jcur) for every cell vertex, i.e. for each pair of consecutive
+// hull points (indices into px/py; >= M-4 are the square's sides).  Exact
+// orientation (FP64 fallback) always.  Returns the vertex count (0 on failure).
+template <int M, class Emit>
+__device__ __forceinline__ int dual_hull_emit(const float (&px)[M], const float (&py)[M], Emit&& emit) {
+    float sx = px[0], sy = py[0];
+    int sj = 0;
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        const bool b = px[j] > sx || (px[j] == sx && py[j] > sy);
+        sx = b ? px[j] : sx;
+        sy = b ? py[j] : sy;
+        sj = b ? j : sj;
+    }
+    float curx = sx, cury = sy;
+    int curj = sj;
+    for (int step = 0; step <= M; ++step) {
+        float bx = curx, by = cury;
+        int bj = curj;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const float wx = px[j] - curx, wy = py[j] - cury;
+            const float ex = bx - curx, ey = by - cury;
+            const bool isnew = (wx != 0.f) || (wy != 0.f);
+            const bool none = (ex == 0.f) && (ey == 0.f);
+            const float t1 = ex * wy, t2 = ey * wx;
+            float cr = t1 - t2;
+            if (fabsf(cr) <= 1e-6f * (fabsf(t1) + fabsf(t2)) && !none && isnew) {
+                const double dex = (double)bx - (double)curx, dey = (double)by - (double)cury;
+                const double dwx = (double)px[j] - (double)curx, dwy = (double)py[j] - (double)cury;
+                const double c64 = dex * dwy - dey * dwx;
+                cr = c64 < 0.0 ? -1.f : (c64 > 0.0 ? 1.f : 0.f);
+            }
+            const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
+            const bool take = isnew && (none || cr < 0.f || (cr == 0.f && farther));
+            bx = take ? px[j] : bx;
+            by = take ? py[j] : by;
+            bj = take ? j : bj;
+        }
+        if (bx == curx && by == cury) return 0;
+        emit(curj, bj);
+        curx = bx; cury = by; curj = bj;
+        if (curx == sx && cury == sy) return step + 1;
+    }
+    return 0;
 }
 
 // ---------------------------------------------------- closest point (Ericson)

@@ -42,6 +42,7 @@ struct StepArgs {
     const unsigned char* tier;
     const int* has_prev;  // device flag
     int* stats;           // kStatWords counters of this iteration
+    int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
     int nS;
     float R;
     int freeze_mode;
@@ -70,11 +71,21 @@ struct GridBufs {
 };
 
 void launch_fill(int* p, long n, int v, hipStream_t s);
+// Exclusive scan: out[0..n-1] = prefix sums of in, out[n] = total.  bsum needs
+// scan_partials(n) ints.
+int scan_partials(long n);
+void launch_scan(const int* in, int n, int* out, int* bsum, hipStream_t s);
 void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipStream_t s);
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
 bool lloyd_supported(int K, int kproj);
 void launch_lloyd(const StepArgs& a, int K, int kproj, hipStream_t s);
+void launch_remesh_cells(const StepArgs& a, int K, int* knn_tmp, unsigned long long* keys, float4* site_n,
+                         hipStream_t s);
+void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s);
+void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
+                       const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);
+void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s);
 void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, hipStream_t s);
 
 }  // namespace cvt

@@ -8,10 +8,14 @@
 // all captured in one HIP graph per buffer parity.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cvt_device.h"
 #include "cvt_kernels.h"
 
 namespace cvt {
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 __device__ __forceinline__ int xcd_block(int b, int nb8) {
     // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -192,14 +196,28 @@ __device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, boo
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(&stats[idx], (int)__popcll(b));
 }
 
-template <int K, int KP>
-__global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int t = lb * 256 + (int)threadIdx.x;
-    const bool live = t < a.nS;
+// Stat word layout (cvt_iter_stats): 0 same, 1 low, 2 both, 3 newly frozen,
+// 4 frozen total, 5 exact rescans, 6 reranks, 7 cell failures, 8 non-finite,
+// 9 deferred to the precise pass (also that pass's work-list length).
+constexpr int kStatDeferred = 9;
+
+struct SiteFlags {
+    bool done;      // this pass produced the site's result
+    bool row_written;  // deferred after the (proven) KNN row was already written
+    bool same, low, both, newly, frozen_after, cell_fail, nonfinite;
+    int kind;       // KNN path: 0 FP32, 1 rerank, 2 exact rescan
+};
+
+// One site of one Lloyd iteration (main.cu:669-881 restated per site).
+// !PRECISE: pure FP32; returns done=false (and writes nothing) when an FP32
+// comparison cannot be proven.  PRECISE: resolves those in FP64.
+// carried: for the precise pass, -1 or the `same` flag the fast pass computed
+// before it wrote the site's KNN row (the row then already holds the new list).
+template <int K, int KP, bool PRECISE>
+__device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool live, int carried) {
+    SiteFlags r{};
     const bool eval_only = a.eval_only != 0;
     const int has_prev = eval_only ? 0 : *a.has_prev;
-
     int i = 0;
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
@@ -208,63 +226,61 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
     }
     const bool valid = live && __float_as_uint(q.w) != kInvalidKey;
     const bool fz = valid && !eval_only && a.frozen[i] != 0;
+    bool defer = false;
 
-    // ---- 1. site KNN (active) or pinned list (frozen)
-    int nb[K];
-    bool same = true;
-    int kind = 0;
-    if (valid && !fz) {
-        int slots[K];
-        kind = knn_exact<K, K + 1>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
-#pragma unroll
-        for (int k = 0; k < K; ++k) nb[k] = slots[k] >= 0 ? a.sg.ids[slots[k]] : i;
-        if (!eval_only) {
-            int* row = a.knn + (size_t)i * K;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                same = same && (row[k] == nb[k]);
-            }
-#pragma unroll
-            for (int k = 0; k < K; ++k) row[k] = nb[k];
-        }
-    } else if (valid) {
-        const int* row = a.knn + (size_t)i * K;
-#pragma unroll
-        for (int k = 0; k < K; ++k) nb[k] = row[k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < K; ++k) nb[k] = i;
-    }
-
-    // ---- 2. frame from the nearest mesh vertex's normal
+    // ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110)
     int nn = -1;
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (valid) {
         if (!fz) {
             int s1[1];
-            knn_exact<1, 2>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
+            const int k1 = knn_exact<1, 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
+            defer = defer || k1 < 0;
             nn = s1[0] >= 0 ? a.m.vg_ids[s1[0]] : -1;
         } else {
-            nn = a.knnV[(size_t)i * KP];
+            nn = a.knnV[(size_t)i * KP];  // pinned at the freeze iteration
         }
         if (nn >= 0 && nn < a.m.nV) {
             const float4 nv = a.m.N[nn];
             const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
             const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
-            if (l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f) frame_fp64_vertex(a.m, nn, u, w);
-            else frame_f32(nv.x, nv.y, nv.z, u, w);
+            const bool ill = l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f;
+            if constexpr (PRECISE) {
+                if (ill) frame_fp64_vertex(a.m, nn, u, w);
+                else frame_f32(nv.x, nv.y, nv.z, u, w);
+            } else {
+                defer = defer || ill;
+                frame_f32(nv.x, nv.y, nv.z, u, w);
+            }
         }
     }
 
-    // ---- 3. tangent-plane cell (dual hull) and centroid
-    float3 c3 = make_float3(q.x, q.y, q.z);
-    bool cell_ok = false;
-    float area = 0.f, moment = 0.f;
+    // ---- 2. site KNN (active) or pinned list (frozen), consumed straight into
+    //         the dual points p_j = (d.u, d.v) / (|d|^2 / 2) of the cell (2dClip.cuh:112-126)
+    float px[K + 4], py[K + 4];
+    int slots[K];
+    if (valid && !fz) {
+        r.kind = knn_exact<K, K + 1, PRECISE>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
+        defer = defer || r.kind < 0;
+    }
+    if (!PRECISE && defer) return r;  // nothing written: the precise pass redoes this site
+    bool same = true;
     if (valid) {
-        float px[K + 4], py[K + 4];
+        int* row = a.knn + (size_t)i * K;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const float4 pj = a.S[nb[k]];
+            float4 pj;
+            if (!fz) {
+                const int sl = slots[k] >= 0 ? slots[k] : t;
+                pj = a.sg.pts[sl];
+                const int id = a.sg.ids[sl];
+                if (!eval_only) {
+                    same = same && (carried >= 0 || row[k] == id);
+                    row[k] = id;
+                }
+            } else {
+                pj = a.S[row[k]];
+            }
             const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
             const float aa = dx * u.x + dy * u.y + dz * u.z;
             const float bb = dx * w.x + dy * w.y + dz * w.z;
@@ -273,12 +289,21 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
             px[k] = aa * ic;
             py[k] = bb * ic;
         }
+        if (carried >= 0) same = carried != 0;
+    }
+
+    // ---- 3. tangent-plane cell (dual hull) and centroid (2dClip.cuh:128-134)
+    float3 c3 = make_float3(q.x, q.y, q.z);
+    bool cell_ok = false;
+    float area = 0.f, moment = 0.f;
+    if (valid) {
         const float iR = 1.0f / a.R;
         px[K] = iR;   py[K] = 0.f;
         px[K + 1] = 0.f; py[K + 1] = iR;
         px[K + 2] = -iR; py[K + 2] = 0.f;
         px[K + 3] = 0.f; py[K + 3] = -iR;
-        const CellOut co = dual_hull_cell<K + 4>(px, py);
+        const CellOut co = dual_hull_cell<K + 4, PRECISE>(px, py);
+        if (!PRECISE && co.defer) defer = true;
         cell_ok = co.ok;
         area = co.area;
         moment = co.moment;
@@ -290,6 +315,13 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
             c3 = make_float3(nan, nan, nan);
         }
     }
+    if (!PRECISE && defer) {
+        // the row already holds the new (proven, hence identical) list: hand
+        // `same` to the precise pass with the deferred entry
+        r.same = same;
+        r.row_written = valid && !fz && !eval_only;
+        return r;
+    }
 
     // ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection
     float3 sp = c3;
@@ -300,7 +332,13 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
         if (!fz) {
             if (finite3(c3.x, c3.y, c3.z)) {
                 int sl[KP];
-                knn_exact<KP, KP + 1>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, c3.x, c3.y, c3.z, -1, sl);
+                const int kk = knn_exact<KP, KP + 1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, c3.x,
+                                                             c3.y, c3.z, -1, sl);
+                if (!PRECISE && kk < 0) {
+                    r.same = same;
+                    r.row_written = !eval_only;
+                    return r;
+                }
 #pragma unroll
                 for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? a.m.vg_ids[sl[k]] : -1;
             }
@@ -326,8 +364,8 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
                 // a face incident to an earlier list vertex was already visited
                 bool seen = false;
 #pragma unroll
-                for (int s = 0; s < k; ++s) {
-                    const float vs = __int_as_float(vl[s]);
+                for (int s2 = 0; s2 < k; ++s2) {
+                    const float vs = __int_as_float(vl[s2]);
                     seen = seen || (A.w == vs) || (B.w == vs) || (C.w == vs);
                 }
                 if (seen) continue;
@@ -341,24 +379,26 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
         if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
     }
 
-    // ---- 6. write the new site and the freeze state machine
-    bool low = false, both = false, newly = false, fz_after = fz;
+    // ---- 6. write the new site and the freeze state machine (freezetest.cuh:270-327, :451-466)
+    r.done = true;
+    r.frozen_after = valid && fz;
     if (valid && !eval_only) {
         a.Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
         if (has_prev) {
             const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
             const float ddx = sp.x - q.x, ddy = sp.y - q.y, ddz = sp.z - q.z;
-            low = (ddx * ddx + ddy * ddy + ddz * ddz) <= a.thr2[tr];
-            both = same && low;
+            r.same = same;
+            r.low = (ddx * ddx + ddy * ddy + ddz * ddz) <= a.thr2[tr];
+            r.both = r.same && r.low;
             if (a.freeze_mode != 0) {
-                int s = a.streak[i];
-                s = both ? min(s + 1, 255) : 0;
-                a.streak[i] = (unsigned char)s;
+                int st = a.streak[i];
+                st = r.both ? min(st + 1, 255) : 0;
+                a.streak[i] = (unsigned char)st;
                 const int need = max(a.need[tr] - 1, 1);
-                if (s >= need && !fz) {
+                if (st >= need && !fz) {
                     a.frozen[i] = 1;
-                    newly = true;
-                    fz_after = true;
+                    r.newly = true;
+                    r.frozen_after = true;
                 }
             }
         } else if (a.freeze_mode != 0) {
@@ -368,6 +408,8 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
         // non-finite site: keep it (it is excluded from every neighbour list)
         a.Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
     }
+    r.cell_fail = valid && !cell_ok;
+    r.nonfinite = live && !finite3(sp.x, sp.y, sp.z);
     if (live && a.dbg_nn) {
         a.dbg_nn[i] = nn;
         a.dbg_U[i] = make_float4(u.x, u.y, u.z, 0.f);
@@ -375,18 +417,194 @@ __global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
         a.dbg_cent[i] = make_float4(c3.x, c3.y, c3.z, 0.f);
         a.dbg_am[i] = make_float2(area, moment);
     }
-    if (!eval_only) {
-        const bool hp = has_prev != 0;
-        wave_count(a.stats, 0, valid && hp && same);
-        wave_count(a.stats, 1, valid && hp && low);
-        wave_count(a.stats, 2, valid && hp && both);
-        wave_count(a.stats, 3, newly);
-        wave_count(a.stats, 4, valid && fz_after);
-        wave_count(a.stats, 5, kind == 2);
-        wave_count(a.stats, 6, kind == 1);
-        wave_count(a.stats, 7, valid && !cell_ok);
-        wave_count(a.stats, 8, live && !finite3(sp.x, sp.y, sp.z));
+    return r;
+}
+
+__device__ __forceinline__ void count_site(const StepArgs& a, const SiteFlags& f, bool live) {
+    const bool hp = a.eval_only == 0 && *a.has_prev != 0;
+    const bool d = live && f.done;
+    wave_count(a.stats, 0, d && hp && f.same);
+    wave_count(a.stats, 1, d && hp && f.low);
+    wave_count(a.stats, 2, d && hp && f.both);
+    wave_count(a.stats, 3, d && f.newly);
+    wave_count(a.stats, 4, d && f.frozen_after);
+    wave_count(a.stats, 5, d && f.kind == 2);
+    wave_count(a.stats, 6, d && f.kind == 1);
+    wave_count(a.stats, 7, d && f.cell_fail);
+    wave_count(a.stats, 8, d && f.nonfinite);
+}
+
+// Fast pass over all sites (bucket-sorted order, XCD-contiguous ranges).
+template <int K, int KP>
+__global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int t = lb * 256 + (int)threadIdx.x;
+    const bool live = t < a.nS;
+    const SiteFlags f = lloyd_site<K, KP, false>(a, t, live, -1);
+    const bool deferred = live && !f.done;
+    if (deferred) {
+        const int pos = atomicAdd(&a.stats[kStatDeferred], 1);
+        a.defer_list[pos] = t | (f.row_written ? (1 << 30) : 0) | (f.same ? (1 << 29) : 0);
     }
+    count_site(a, f, live);
+}
+
+// Precise pass over the deferred sites only (FP64 where FP32 was inconclusive).
+template <int K, int KP>
+__global__ __launch_bounds__(256) void k_lloyd_precise(StepArgs a) {
+    const int n = a.stats[kStatDeferred];
+    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+        const int idx = base + (int)threadIdx.x;
+        const bool live = idx < n;
+        const int e = live ? a.defer_list[idx] : 0;
+        const int t = e & ((1 << 29) - 1);
+        const int carried = (e & (1 << 30)) ? ((e >> 29) & 1) : -1;
+        const SiteFlags f = lloyd_site<K, KP, true>(a, t, live, carried);
+        count_site(a, f, live);
+    }
+}
+
+// ------------------------------------------------------------ remesh stage
+// Dual triangles of the restricted Voronoi cells (near_project.cuh:246-352 and
+// buildmesh.cuh:170-201 restated): every cell vertex is the meeting point of
+// two consecutive hull edges, i.e. of bisectors with neighbours a and b, and
+// yields the triangle {i, a, b} (square sides give none).  Keys are the sorted
+// triple packed 21:21:21 (buildmesh.cuh:11-14); a site's unused slots hold ~0.
+__device__ __forceinline__ unsigned long long pack3(int a, int b, int c) {
+    if (a > b) { int t = a; a = b; b = t; }
+    if (b > c) { int t = b; b = c; c = t; }
+    if (a > b) { int t = a; a = b; b = t; }
+    return ((unsigned long long)a << 42) | ((unsigned long long)b << 21) | (unsigned long long)c;
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_remesh_cells(StepArgs a, int* __restrict__ knn_tmp,
+                                                      unsigned long long* __restrict__ keys,
+                                                      float4* __restrict__ site_n) {
+    constexpr int M = K + 4;
+    const int t = blockIdx.x * 256 + (int)threadIdx.x;
+    if (t >= a.nS) return;
+    const float4 q = a.sg.pts[t];
+    const int i = a.sg.ids[t];
+    unsigned long long* out = keys + (size_t)i * M;
+    for (int k = 0; k < M; ++k) out[k] = ~0ull;
+    site_n[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (__float_as_uint(q.w) == kInvalidKey) return;
+    int s1[1];
+    knn_exact<1, 2, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
+    const int nn = s1[0] >= 0 ? a.m.vg_ids[s1[0]] : -1;
+    float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
+    if (nn >= 0) {
+        const float4 nv = a.m.N[nn];
+        site_n[i] = nv;
+        const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
+        const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
+        if (l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f) frame_fp64_vertex(a.m, nn, u, w);
+        else frame_f32(nv.x, nv.y, nv.z, u, w);
+    }
+    int slots[K];
+    knn_exact<K, K + 1, true>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
+    int* row = knn_tmp + (size_t)i * K;
+    float px[M], py[M];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int sl = slots[k] >= 0 ? slots[k] : t;
+        const float4 pj = a.sg.pts[sl];
+        row[k] = a.sg.ids[sl];
+        const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
+        const float aa = dx * u.x + dy * u.y + dz * u.z;
+        const float bb = dx * w.x + dy * w.y + dz * w.z;
+        const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
+        const float ic = cc > 0.f ? 1.0f / cc : 0.f;
+        px[k] = aa * ic;
+        py[k] = bb * ic;
+    }
+    const float iR = 1.0f / a.R;
+    px[K] = iR;   py[K] = 0.f;
+    px[K + 1] = 0.f; py[K + 1] = iR;
+    px[K + 2] = -iR; py[K + 2] = 0.f;
+    px[K + 3] = 0.f; py[K + 3] = -iR;
+    int nout = 0;
+    dual_hull_emit<M>(px, py, [&](int jp, int jc) {
+        if (jp < K && jc < K && nout < M) {
+            const int b = row[jp], c = row[jc];
+            if (b != c && b != i && c != i) out[nout++] = pack3(i, b, c);
+        }
+    });
+}
+
+__global__ void k_unique_mark(const unsigned long long* __restrict__ keys, int n, int* __restrict__ flag) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const unsigned long long v = keys[k];
+    flag[k] = (v != ~0ull && (k == 0 || keys[k - 1] != v)) ? 1 : 0;
+}
+
+// Decode unique keys, drop degenerate triangles (buildmesh.cuh:225-241) and
+// orient each by the summed site normals (scripts/test_freeze.py:150-159).
+__global__ void k_tri_finish(const unsigned long long* __restrict__ keys, const int* __restrict__ flag,
+                             const int* __restrict__ pos, int n, const float4* __restrict__ S,
+                             const float4* __restrict__ site_n, float eps, int4* __restrict__ tri,
+                             int* __restrict__ keep) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || !flag[k]) return;
+    const unsigned long long v = keys[k];
+    int ia = (int)(v >> 42), ib = (int)((v >> 21) & 0x1fffff), ic = (int)(v & 0x1fffff);
+    const float4 A = S[ia], B = S[ib], C = S[ic];
+    const float ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
+    const float fx = C.x - A.x, fy = C.y - A.y, fz = C.z - A.z;
+    const float gx = C.x - B.x, gy = C.y - B.y, gz = C.z - B.z;
+    const float nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
+    const float e0 = sqrtf(ex * ex + ey * ey + ez * ez), e1 = sqrtf(gx * gx + gy * gy + gz * gz);
+    const float e2 = sqrtf(fx * fx + fy * fy + fz * fz), ar = sqrtf(nx * nx + ny * ny + nz * nz);
+    const bool ok = e0 > eps && e1 > eps && e2 > eps && ar > eps;
+    const float4 na = site_n[ia], nb = site_n[ib], nc = site_n[ic];
+    const float dot = nx * (na.x + nb.x + nc.x) + ny * (na.y + nb.y + nc.y) + nz * (na.z + nb.z + nc.z);
+    if (dot < 0.f) { const int tmp = ib; ib = ic; ic = tmp; }
+    const int p = pos[k];
+    tri[p] = make_int4(ia, ib, ic, 0);
+    keep[p] = ok ? 1 : 0;
+}
+
+__global__ void k_tri_compact(const int4* __restrict__ tri, const int* __restrict__ keep, const int* __restrict__ pos,
+                              int n, int* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || !keep[k]) return;
+    const int p = pos[k];
+    const int4 t = tri[k];
+    out[3 * p] = t.x;
+    out[3 * p + 1] = t.y;
+    out[3 * p + 2] = t.z;
+}
+
+void launch_remesh_cells(const StepArgs& a, int K, int* knn_tmp, unsigned long long* keys, float4* site_n,
+                         hipStream_t s) {
+    const dim3 g(nblk(a.nS, 256)), b(256);
+    switch (K) {
+        case 4: hipLaunchKernelGGL(k_remesh_cells<4>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 6: hipLaunchKernelGGL(k_remesh_cells<6>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 8: hipLaunchKernelGGL(k_remesh_cells<8>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 12: hipLaunchKernelGGL(k_remesh_cells<12>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 16: hipLaunchKernelGGL(k_remesh_cells<16>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 20: hipLaunchKernelGGL(k_remesh_cells<20>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 24: hipLaunchKernelGGL(k_remesh_cells<24>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 32: hipLaunchKernelGGL(k_remesh_cells<32>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        default: break;
+    }
+}
+
+void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s) {
+    hipLaunchKernelGGL(k_unique_mark, dim3(nblk(n, 256)), dim3(256), 0, s, keys, n, flag);
+}
+
+void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
+                       const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s) {
+    hipLaunchKernelGGL(k_tri_finish, dim3(nblk(n, 256)), dim3(256), 0, s, keys, flag, pos, n, S, site_n, eps, tri,
+                       keep);
+}
+
+void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s) {
+    hipLaunchKernelGGL(k_tri_compact, dim3(nblk(n, 256)), dim3(256), 0, s, tri, keep, pos, n, out);
 }
 
 __global__ void k_iter_end(int* __restrict__ stats, int* __restrict__ hist, int* __restrict__ hist_count, int cap,
@@ -400,20 +618,25 @@ __global__ void k_iter_end(int* __restrict__ stats, int* __restrict__ hist, int*
 }
 
 // ------------------------------------------------------------------ launches
-static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 void launch_fill(int* p, long n, int v, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3(nblk(n, 256)), dim3(256), 0, s, p, (int)n, v);
+}
+
+int scan_partials(long n) { return nblk(n, kScanTile) + 1; }
+
+void launch_scan(const int* in, int n, int* out, int* bsum, hipStream_t s) {
+    const int nb = nblk(n, kScanTile);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, in, n, bsum);
+    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, s, bsum, nb, out + n);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, in, n, bsum, out);
 }
 
 void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipStream_t s) {
     const int T1 = (int)g.mask + 2;  // buckets incl. the non-finite bucket
     launch_fill(b.count, T1, 0, s);
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.count, b.bucket, b.rank);
-    const int nb = nblk(T1, kScanTile);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(256), 0, s, b.count, T1, b.bsum);
-    hipLaunchKernelGGL(k_scan_bsum, dim3(1), dim3(256), 0, s, b.bsum, nb, b.start + T1);
-    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(256), 0, s, b.count, T1, b.bsum, b.start);
+    launch_scan(b.count, T1, b.start, b.bsum, s);
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.bucket, b.rank, b.start,
                        b.pts, b.ids);
 }
@@ -428,6 +651,8 @@ static void launch_lloyd_k(const StepArgs& a, int kproj, hipStream_t s) {
     const int nb8 = ((nblk(a.nS, 256) + 7) / 8) * 8;
     (void)kproj;
     hipLaunchKernelGGL((k_lloyd<K, 5>), dim3(nb8), dim3(256), 0, s, a);
+    const int nbp = std::min(nblk(a.nS, 256), 512);
+    hipLaunchKernelGGL((k_lloyd_precise<K, 5>), dim3(nbp), dim3(256), 0, s, a);
 }
 
 bool lloyd_supported(int K, int kproj) {

@@ -25,6 +25,7 @@ EXPORTS = [
     "cvt_job_info_get", "cvt_job_step", "cvt_job_step_timed", "cvt_job_get_sites", "cvt_job_get_knn",
     "cvt_job_get_knnv", "cvt_job_get_freeze", "cvt_job_get_normals", "cvt_job_set_state", "cvt_job_set_tiers",
     "cvt_job_get_debug", "cvt_job_energy", "cvt_job_point_mesh_d2", "cvt_job_profile",
+    "cvt_bvh_create", "cvt_bvh_point_d2", "cvt_bvh_destroy", "cvt_job_remesh", "cvt_free",
 ]
 
 
@@ -44,7 +45,7 @@ class IterStats(C.Structure):
         ("count_same", C.c_int), ("count_low_disp", C.c_int), ("count_both", C.c_int),
         ("count_newly_frozen", C.c_int), ("frozen_total", C.c_int), ("knn_exact_rescans", C.c_int),
         ("knn_reranks", C.c_int), ("cell_failures", C.c_int), ("nonfinite_sites", C.c_int),
-        ("reserved", C.c_int * 3),
+        ("deferred_sites", C.c_int), ("reserved", C.c_int * 2),
     ]
 
     def as_dict(self) -> dict:
@@ -122,6 +123,13 @@ def lib():
     L.cvt_job_point_mesh_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+    L.cvt_job_remesh.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
+    L.cvt_free.argtypes = [C.c_void_p]
+    L.cvt_free.restype = None
+    L.cvt_bvh_create.argtypes = [_dp, C.c_int, _ip, C.c_int, C.POINTER(C.c_void_p)]
+    L.cvt_bvh_point_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
+    L.cvt_bvh_destroy.argtypes = [C.c_void_p]
+    L.cvt_bvh_destroy.restype = None
     missing = [n for n in EXPORTS if not hasattr(L, n)]
     if missing:
         raise RuntimeError(f"{LIB_PATH} lacks exports {missing}: rebuild it")
@@ -270,8 +278,68 @@ class Job:
         _check("cvt_job_energy", lib().cvt_job_energy(self.handle, C.byref(e)))
         return e.value
 
+    def remesh(self, eps: float = 1e-6) -> np.ndarray:
+        """Dual triangles of the current cells (int32 (m, 3)), vertices = sites()."""
+        fp = C.POINTER(C.c_int)()
+        nf = C.c_int()
+        _check("cvt_job_remesh", lib().cvt_job_remesh(self.handle, float(eps), C.byref(fp), C.byref(nf)))
+        try:
+            F = np.ctypeslib.as_array(fp, shape=(nf.value * 3,)).copy().reshape(-1, 3) if nf.value else \
+                np.zeros((0, 3), np.int32)
+        finally:
+            lib().cvt_free(fp)
+        return F
+
     def point_mesh_d2(self, P) -> np.ndarray:
         P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
         out = np.zeros(len(P))
         _check("cvt_job_point_mesh_d2", lib().cvt_job_point_mesh_d2(self.handle, P.reshape(-1), len(P), out))
         return out
+
+
+class BVH:
+    """Device LBVH over a triangle mesh: squared point-to-surface distances."""
+
+    def __init__(self, V, F):
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        h = C.c_void_p()
+        _check("cvt_bvh_create", lib().cvt_bvh_create(V.reshape(-1), len(V), F.reshape(-1), len(F), C.byref(h)))
+        self._h = h
+
+    def point_d2(self, P) -> np.ndarray:
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(P))
+        _check("cvt_bvh_point_d2", lib().cvt_bvh_point_d2(self._h, P.reshape(-1), len(P), out))
+        return out
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cvt_bvh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def hausdorff(V1, F1, V2, F2, n_samples: int, seed: int, symmetric: bool = True) -> float:
+    """Hausdorff distance between two triangle meshes from area-uniform samples
+    (n_samples on each surface).  The reference measures one side only
+    (candidate -> reference, evaluate.cuh:359-383) over vertices, face
+    centroids and edge midpoints; ``symmetric=False`` gives that direction
+    (samples on mesh 1, distances to mesh 2)."""
+    from .meshgen import sample_surface
+
+    b2 = BVH(V2, F2)
+    P1 = sample_surface(np.asarray(V1, np.float64), np.asarray(F1), n_samples, seed)
+    h = float(np.sqrt(b2.point_d2(P1).max()))
+    b2.close()
+    if symmetric:
+        b1 = BVH(V1, F1)
+        P2 = sample_surface(np.asarray(V2, np.float64), np.asarray(F2), n_samples, seed + 1)
+        h = max(h, float(np.sqrt(b1.point_d2(P2).max())))
+        b1.close()
+    return h

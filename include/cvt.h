@@ -56,7 +56,8 @@ typedef struct cvt_iter_stats {
     int knn_reranks;       /* queries whose in-list order needed FP64 keys */
     int cell_failures;     /* cells whose hull walk did not close (site kept) */
     int nonfinite_sites;   /* sites with non-finite coordinates after the step */
-    int reserved[3];
+    int deferred_sites;    /* sites the FP32 pass handed to the FP64-resolving pass */
+    int reserved[2];
 } cvt_iter_stats;
 
 typedef struct cvt_job_info {
@@ -133,10 +134,31 @@ int cvt_job_get_debug(cvt_job* job, int* nn_site, double* U, double* Vf, double*
  * closest analogue scripts/testfreeze2.py:285-309).  All sites, fresh KNN. */
 int cvt_job_energy(cvt_job* job, double* E_out);
 
+/* Remeshed surface: the dual triangles {i, a, b} of every cell vertex, where
+ * a, b are the neighbours whose bisectors meet there, de-duplicated, with
+ * degenerate triangles (an edge or |cross| <= eps) removed and each triangle
+ * oriented by the summed site normals.  Vertices are the current sites.
+ * Replaces cell_poly2d_kernel + emit_faces_from_labels_kernel + the host
+ * sort/unique/filter (near_project.cuh:246-352, buildmesh.cuh:170-241,
+ * main.cu:928-974) and the Python orient_faces (scripts/test_freeze.py:150-159).
+ * *F_out (3 * *nF_out ints) is allocated by the library: release it with
+ * cvt_free.  Requires nS < 2^21 (keys pack ids in 21 bits, as the reference). */
+int cvt_job_remesh(cvt_job* job, double eps, int** F_out, int* nF_out);
+void cvt_free(void* p);
+
 /* Squared distance from each query point (n*3, host) to the job's input mesh,
  * via a device LBVH (replaces bvh_closest_d2_kernel, evaluate.cuh:206-250, and
  * the CPU BVH of evaluate.cuh:114-201).  out: n doubles. */
 int cvt_job_point_mesh_d2(cvt_job* job, const double* P, int n, double* d2_out);
+
+/* Standalone device LBVH over any triangle mesh (e.g. a remeshed surface),
+ * for two-sided Hausdorff distances.  Replaces build_bvh (evaluate.cuh:175-201)
+ * + hausdorff_bvh_gpu (evaluate.cuh:291-357); the max over the returned
+ * distances is the one-sided Hausdorff of the sample set. */
+typedef struct cvt_bvh cvt_bvh;
+int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out);
+int cvt_bvh_point_d2(cvt_bvh* bvh, const double* P, int n, double* d2_out);
+void cvt_bvh_destroy(cvt_bvh* bvh);
 
 #ifdef __cplusplus
 }

@@ -734,4 +734,85 @@ void orc_point_mesh_d2(int nP, const double* P, int nV, const double* V, int nF,
     parallel_for(nP, [&](long i) { out[i] = b.closest_d2(ld3(P, i)); });
 }
 
+// Dual-triangle extraction (remesh): near_project.cuh:201-352 (labelled
+// clip), buildmesh.cuh:170-201 (emit sorted triples), main.cu:939-973
+// (unique + is_degenerate_tri, buildmesh.cuh:225-241) and the Python
+// orient_faces (scripts/test_freeze.py:150-159).  Labels are carried through
+// the Sutherland-Hodgman clip: each polygon edge remembers the line it lies
+// on (square sides < 0), so a vertex's two lines are its two edge labels.
+// Returns the number of triangles written to tris_out (capacity nS*(K+4)*3).
+int orc_remesh(const double* S, int nS, const double* U, const double* Vv, const int* knn, int K, double R,
+               const double* Nsite, double eps, int* tris_out) {
+    std::vector<std::vector<long long>> per(nS);
+    parallel_for(nS, [&](long i) {
+        std::vector<P2> A = {{-R, -R}, {R, -R}, {R, R}, {-R, R}}, B;
+        std::vector<int> la = {-1, -2, -3, -4}, lb;
+        D3 si = ld3(S, i), u = ld3(U, i), v = ld3(Vv, i);
+        for (int t = 0; t < K; ++t) {
+            int j = knn[i * (long)K + t];
+            if (j < 0 || j >= nS || j == i) continue;
+            D3 d = sub(ld3(S, j), si);
+            double a = dot(d, u), b = dot(d, v), c = 0.5 * dot(d, d);
+            int n = (int)A.size();
+            B.clear();
+            lb.clear();
+            if (n == 0) break;
+            int sidx = n - 1;
+            double fs = a * A[sidx].x + b * A[sidx].y - c;
+            for (int e = 0; e < n; ++e) {
+                double fe = a * A[e].x + b * A[e].y - c;
+                bool ins_e = fe <= 1e-12, ins_s = fs <= 1e-12;
+                if (ins_e) {
+                    if (!ins_s) {
+                        double tt = fs / (fs - fe + 1e-30);
+                        B.push_back({A[sidx].x + tt * (A[e].x - A[sidx].x), A[sidx].y + tt * (A[e].y - A[sidx].y)});
+                        lb.push_back(la[sidx]);
+                    }
+                    B.push_back(A[e]);
+                    lb.push_back(la[e]);
+                } else if (ins_s) {
+                    double tt = fs / (fs - fe + 1e-30);
+                    B.push_back({A[sidx].x + tt * (A[e].x - A[sidx].x), A[sidx].y + tt * (A[e].y - A[sidx].y)});
+                    lb.push_back(j);
+                }
+                sidx = e;
+                fs = fe;
+            }
+            A.swap(B);
+            la.swap(lb);
+        }
+        int n = (int)A.size();
+        if (n < 3) return;
+        for (int k = 0; k < n; ++k) {
+            int b = la[(k + n - 1) % n], c = la[k];
+            if (b < 0 || c < 0 || b == c || b == i || c == i) continue;
+            long long x = i, y = b, z = c;
+            if (x > y) std::swap(x, y);
+            if (y > z) std::swap(y, z);
+            if (x > y) std::swap(x, y);
+            per[i].push_back((x << 42) | (y << 21) | z);
+        }
+    });
+    std::vector<long long> all;
+    for (auto& p : per) all.insert(all.end(), p.begin(), p.end());
+    std::sort(all.begin(), all.end());
+    all.erase(std::unique(all.begin(), all.end()), all.end());
+    int m = 0;
+    for (long long key : all) {
+        int a = (int)(key >> 42), b = (int)((key >> 21) & 0x1fffff), c = (int)(key & 0x1fffff);
+        D3 pa = ld3(S, a), pb = ld3(S, b), pc = ld3(S, c);
+        D3 nrm = cross(sub(pb, pa), sub(pc, pa));
+        double e0 = std::sqrt(dot(sub(pb, pa), sub(pb, pa))), e1 = std::sqrt(dot(sub(pc, pb), sub(pc, pb)));
+        double e2 = std::sqrt(dot(sub(pa, pc), sub(pa, pc))), ar = std::sqrt(dot(nrm, nrm));
+        if (!(e0 > eps && e1 > eps && e2 > eps && ar > eps)) continue;
+        D3 nt = add(add(ld3(Nsite, a), ld3(Nsite, b)), ld3(Nsite, c));
+        if (dot(nrm, nt) < 0) std::swap(b, c);
+        tris_out[3 * m] = a;
+        tris_out[3 * m + 1] = b;
+        tris_out[3 * m + 2] = c;
+        ++m;
+    }
+    return m;
+}
+
 }  // extern "C"

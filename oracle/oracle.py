@@ -59,6 +59,8 @@ def lib():
         L.orc_lloyd_step.argtypes = [
             C.c_int, _dp, C.c_int, _ip, _dp, _ip, _ip, C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
             _dp, _ip, _up, C.c_int, C.c_int, _ip, _ip, _up, _up, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+        L.orc_remesh.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, C.c_double, _dp, C.c_double, _ip]
+        L.orc_remesh.restype = C.c_int
         L.orc_point_mesh_d2.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp]
         _lib = L
     return _lib
@@ -242,3 +244,13 @@ def point_mesh_d2(P, V, F, use_bvh: bool = True):
     out = np.zeros(len(P))
     lib().orc_point_mesh_d2(len(P), P.reshape(-1), len(V), V.reshape(-1), len(F), F.reshape(-1), int(use_bvh), out)
     return out
+
+
+def remesh(S, U, W, knn_, R: float, Nsite, eps: float = 1e-6):
+    """Dual triangles of the tangent-plane cells (int32 (m,3)), oriented."""
+    S = _d(S)
+    n, K = knn_.shape
+    out = np.zeros(3 * n * (K + 4), np.int32)
+    m = lib().orc_remesh(S.reshape(-1), n, _d(U).reshape(-1), _d(W).reshape(-1), _i(knn_).reshape(-1), K, float(R),
+                         _d(Nsite).reshape(-1), float(eps), out)
+    return out[: 3 * m].reshape(m, 3)

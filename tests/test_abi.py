@@ -1,0 +1,91 @@
+"""C ABI checks that need no GPU: the library loads, exports every function
+include/cvt.h declares, the ctypes structs match the C layout, and argument
+validation fails with the documented status before any device work."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from gpu_cvt_amd import cvt, meshgen as mg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cvt.h")
+
+
+def header_functions():
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(cvt_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_exports_every_declared_symbol():
+    L = cvt.lib()
+    names = header_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert sorted(cvt.EXPORTS) == names
+
+
+def test_struct_layouts_match_header():
+    src = r'''
+#include <stdio.h>
+#include <stddef.h>
+#include "cvt.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cvt_params), sizeof(cvt_iter_stats), sizeof(cvt_job_info),
+         offsetof(cvt_params, debug), offsetof(cvt_job_info, device_bytes), offsetof(cvt_params, tier_disp_abs2));
+  return 0;
+}
+'''
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "probe.c")
+        exe = os.path.join(d, "probe")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        vals = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
+    assert vals == [C.sizeof(cvt.Params), C.sizeof(cvt.IterStats), C.sizeof(cvt.JobInfo),
+                    cvt.Params.debug.offset, cvt.JobInfo.device_bytes.offset, cvt.Params.tier_disp_abs2.offset]
+
+
+def test_default_params():
+    p = cvt.default_params()
+    assert (p.K, p.kproj, p.freeze_mode, p.nan_policy) == (24, 5, 0, 0)
+    assert list(p.tier_streak) == [5, 10, 15, 20, 25, 10]
+
+
+@pytest.mark.parametrize("kw, code", [
+    (dict(K=7), -1),              # unsupported K
+    (dict(K=24, kproj=3), -1),    # unsupported kproj
+    (dict(K=8, freeze_mode=5), -1),
+])
+def test_create_rejects_bad_params(kw, code):
+    V, F = mg.icosphere(2)
+    S = mg.sample_surface(V, F, 100, seed=0)
+    with pytest.raises(cvt.CVTError) as e:
+        cvt.Job(V, F, S, **kw)
+    assert e.value.code == code
+
+
+def test_create_rejects_bad_inputs():
+    V, F = mg.icosphere(2)
+    S = mg.sample_surface(V, F, 30, seed=0)
+    with pytest.raises(cvt.CVTError) as e:   # nS < K + 1 (the reference silently skips n < 1024)
+        cvt.Job(V, F, S, K=32)
+    assert e.value.code == -1
+    S2 = S.copy()
+    S2[3, 1] = np.inf
+    with pytest.raises(cvt.CVTError) as e:
+        cvt.Job(V, F, S2, K=8)
+    assert e.value.code == -5
+    F2 = F.copy()
+    F2[0, 0] = len(V)
+    with pytest.raises(cvt.CVTError) as e:
+        cvt.Job(V, F2, S, K=8)
+    assert e.value.code == -1
+    with pytest.raises(cvt.CVTError):
+        cvt.Job(V[:2], F[:0], S, K=8)
