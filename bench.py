@@ -79,6 +79,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-iters", type=int, default=5)
+    ap.add_argument("--site-cell", type=float, default=0.0, help="site grid cell scale (0 = library default)")
+    ap.add_argument("--vert-cell", type=float, default=0.0, help="vertex grid cell scale (0 = library default)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="first run one plain device copy of MIB MiB (torch), to calibrate HBM counters")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -88,10 +92,24 @@ def main(argv=None) -> int:
 
     from gpu_cvt_amd import cvt, meshgen as mg
 
+    if args.copy_calib > 0:
+        import torch
+
+        n = args.copy_calib * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written",
+              file=sys.stderr, flush=True)
+        del src, dst
+
     t_gen = time.perf_counter()
     V, F, S, K, _iters, _fz = mg.workload(args.workload)
     t_gen = time.perf_counter() - t_gen
-    params = cvt.default_params(K=K, freeze_mode=1 if args.freeze else 0, freeze_need=5, freeze_disp_rel=1e-5)
+    params = cvt.default_params(K=K, freeze_mode=1 if args.freeze else 0, freeze_need=5, freeze_disp_rel=1e-5,
+                                site_cell_scale=args.site_cell, vert_cell_scale=args.vert_cell)
     job = cvt.Job(V, F, S, params)
     info = job.info()
     print(f"[bench] workload={args.workload} nV={len(V)} nF={len(F)} nS={len(S)} K={K} gen={t_gen:.1f}s "

@@ -43,12 +43,6 @@ int fail(int code, const char* fmt, ...) {
                                           __FILE__, __LINE__);                                     \
     } while (0)
 
-unsigned next_pow2(unsigned long v) {
-    unsigned p = 1;
-    while (p < v && p < (1u << 30)) p <<= 1;
-    return p;
-}
-
 constexpr int kHistCap = 4096;
 
 }  // namespace
@@ -73,6 +67,8 @@ struct cvt_job {
     int *knn = nullptr, *knnV = nullptr;
     unsigned char *frozen = nullptr, *streak = nullptr, *tier = nullptr;
     int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
+    int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
+    float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
     // debug
     int* dbg_nn = nullptr;
     float4 *dbg_U = nullptr, *dbg_V = nullptr, *dbg_cent = nullptr;
@@ -213,20 +209,24 @@ int bvh_query(cvt_bvh* b, const double* P, int n, double* d2) {
 
 namespace {
 
-GridP make_grid(const double lo[3], const double hi[3], double h, unsigned table) {
+// Dense grid of cells of size >= h over the padded bbox; h grows until the
+// cell count fits max_cells (memory of the count/start arrays).
+GridP make_grid(const double lo[3], const double hi[3], double h, long max_cells) {
     GridP g{};
     double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
     if (!(ext > 0)) ext = 1.0;
     if (!(h > 0)) h = ext;
     int dims[3];
     for (;;) {
-        bool ok = true;
+        long tot = 1;
         for (int a = 0; a < 3; ++a) {
             double n = std::floor((hi[a] - lo[a]) / h) + 1;
-            if (n > 1023) ok = false;
-            dims[a] = (int)std::max(1.0, std::min(n, 1023.0));
+            dims[a] = (int)std::max(1.0, std::min(n, 4096.0));
+            tot *= dims[a];
         }
-        if (ok) break;
+        bool capped = false;
+        for (int a = 0; a < 3; ++a) capped = capped || (std::floor((hi[a] - lo[a]) / h) + 1 > 4096.0);
+        if (tot <= max_cells && !capped) break;
         h *= 1.1;
     }
     g.ox = (float)lo[0];
@@ -240,21 +240,19 @@ GridP make_grid(const double lo[3], const double hi[3], double h, unsigned table
     g.dx = dims[0];
     g.dy = dims[1];
     g.dz = dims[2];
-    g.mask = table - 1;
+    g.ncell = dims[0] * dims[1] * dims[2];
     return g;
 }
 
-int alloc_grid(cvt_job* j, GridBufs& b, int n, unsigned table) {
-    const long T2 = (long)table + 2;
-    const int nb = (int)((T2 + 4095) / 4096);
+int alloc_grid(cvt_job* j, GridBufs& b, int n, int ncell) {
+    const long T2 = (long)ncell + 2;
     int rc;
     if ((rc = j->alloc(&b.count, T2))) return rc;
     if ((rc = j->alloc(&b.start, T2))) return rc;
-    if ((rc = j->alloc(&b.bsum, nb + 1))) return rc;
+    if ((rc = j->alloc(&b.bsum, scan_partials(T2)))) return rc;
     if ((rc = j->alloc(&b.bucket, n))) return rc;
     if ((rc = j->alloc(&b.rank, n))) return rc;
     if ((rc = j->alloc(&b.pts, n))) return rc;
-    if ((rc = j->alloc(&b.ids, n))) return rc;
     return 0;
 }
 
@@ -271,11 +269,9 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     a.m.vg = j->vg;
     a.m.vg_start = j->vgb.start;
     a.m.vg_pts = j->vgb.pts;
-    a.m.vg_ids = j->vgb.ids;
     a.sg.g = j->sgp;
     a.sg.start = j->sgb.start;
     a.sg.pts = j->sgb.pts;
-    a.sg.ids = j->sgb.ids;
     a.S = j->S[parity];
     a.Snew = j->S[1 - parity];
     a.knn = j->knn;
@@ -286,6 +282,10 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     a.has_prev = j->has_prev;
     a.stats = j->stats;
     a.defer_list = j->defer_list;
+    a.rad2 = j->rad2;
+    a.seed_ok = eval_only ? nullptr : j->seed_ok;
+    a.dmax_prev = j->dmax + (1 - parity);
+    a.dmax_cur = eval_only ? nullptr : j->dmax + parity;
     a.nS = j->nS;
     a.R = (float)j->R;
     a.freeze_mode = j->p.freeze_mode;
@@ -317,10 +317,11 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
 void enqueue_iteration(cvt_job* j, int parity) {
     hipStream_t s = j->stream;
     launch_fill(j->stats, kStatWords, 0, s);
+    launch_fill((int*)(j->dmax + parity), 1, 0, s);
     launch_grid_build(j->S[parity], j->nS, j->sgp, j->sgb, s);
     StepArgs a = make_args(j, parity, 0);
     launch_lloyd(a, j->p.K, j->p.kproj, s);
-    launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, s);
+    launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, j->seed_ok, s);
 }
 
 int ensure_graph(cvt_job* j, int parity) {
@@ -449,24 +450,27 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         ghi[a] = j->hi[a] + pad;
     }
     const double kPi = 3.14159265358979323846;
-    const double ss = p.site_cell_scale > 0 ? p.site_cell_scale : 1.0;
-    const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.0;
-    const double rK = std::sqrt((p.K + 1) / (kPi * (nS / j->area)));
-    const double rV = std::sqrt((p.kproj + 1) / (kPi * (nV / j->area)));
-    j->sgp = make_grid(glo, ghi, ss * rK, next_pow2(std::max(1024L, 2L * nS)));
-    j->vg = make_grid(glo, ghi, vs * rV, next_pow2(std::max(1024L, 2L * nV)));
+    // cell = scale x the radius that holds K+2 (kproj+2) points at the mean
+    // surface density, so the 3x3x3 block usually covers the search
+    const double ss = p.site_cell_scale > 0 ? p.site_cell_scale : 1.4;
+    const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.4;
+    const double rK = std::sqrt((p.K + 2) / (kPi * (nS / j->area)));
+    const double rV = std::sqrt((p.kproj + 2) / (kPi * (nV / j->area)));
+    j->sgp = make_grid(glo, ghi, ss * rK, 16L * nS + (1L << 20));
+    j->vg = make_grid(glo, ghi, vs * rV, 16L * nV + (1L << 20));
 
     if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
     if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
         (rc = j->alloc(&j->F, (size_t)3 * nF)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
-        (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.mask + 1)) ||
-        (rc = alloc_grid(j, j->sgb, nS, j->sgp.mask + 1)) || (rc = j->alloc(&j->S[0], nS)) ||
+        (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.ncell)) ||
+        (rc = alloc_grid(j, j->sgb, nS, j->sgp.ncell)) || (rc = j->alloc(&j->S[0], nS)) ||
         (rc = j->alloc(&j->S[1], nS)) || (rc = j->alloc(&j->knn, (size_t)nS * p.K)) ||
         (rc = j->alloc(&j->knnV, (size_t)nS * p.kproj)) || (rc = j->alloc(&j->frozen, nS)) ||
         (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
         (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
-        (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)))
+        (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
+        (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) || (rc = j->alloc(&j->dmax, 2)))
         return bail(rc);
     if (p.debug) {
         if ((rc = j->alloc(&j->dbg_nn, nS)) || (rc = j->alloc(&j->dbg_U, nS)) || (rc = j->alloc(&j->dbg_V, nS)) ||
@@ -489,6 +493,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         hipMemsetAsync(j->tier, 0, nS, s))
         return bail(fail(CVT_EHIP, "memset failed"));
     launch_fill(j->has_prev, 1, 0, s);
+    launch_fill(j->seed_ok, 1, 0, s);
+    launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatWords, 0, s);
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
@@ -513,8 +519,8 @@ int cvt_job_info_get(cvt_job* j, cvt_job_info* o) {
     o->nV = j->nV; o->nF = j->nF; o->nS = j->nS; o->K = j->p.K; o->kproj = j->p.kproj;
     o->site_grid_dims[0] = j->sgp.dx; o->site_grid_dims[1] = j->sgp.dy; o->site_grid_dims[2] = j->sgp.dz;
     o->vert_grid_dims[0] = j->vg.dx; o->vert_grid_dims[1] = j->vg.dy; o->vert_grid_dims[2] = j->vg.dz;
-    o->site_table = (int)j->sgp.mask + 1;
-    o->vert_table = (int)j->vg.mask + 1;
+    o->site_table = j->sgp.ncell;
+    o->vert_table = j->vg.ncell;
     o->site_cell = j->sgp.h;
     o->vert_cell = j->vg.h;
     for (int a = 0; a < 3; ++a) { o->bbox_lo[a] = j->lo[a]; o->bbox_hi[a] = j->hi[a]; }
@@ -608,12 +614,13 @@ int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, do
         const int par = j->cur;
         HIPCHK(hipEventRecord(ev[0], s));
         launch_fill(j->stats, kStatWords, 0, s);
+        launch_fill((int*)(j->dmax + par), 1, 0, s);
         launch_grid_build(j->S[par], j->nS, j->sgp, j->sgb, s);
         HIPCHK(hipEventRecord(ev[1], s));
         StepArgs a = make_args(j, par, 0);
         launch_lloyd(a, j->p.K, j->p.kproj, s);
         HIPCHK(hipEventRecord(ev[2], s));
-        launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, s);
+        launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, j->seed_ok, s);
         HIPCHK(hipEventRecord(ev[3], s));
         HIPCHK(hipEventSynchronize(ev[3]));
         float f01 = 0, f12 = 0, f03 = 0;
@@ -694,6 +701,10 @@ int cvt_job_set_state(cvt_job* j, const double* S, const int* knn_prev, const in
     if (has_prev >= 0) {
         int hp = has_prev ? 1 : 0;
         HIPCHK(hipMemcpy(j->has_prev, &hp, sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (S) {  // sites moved arbitrarily: the per-site search seeds no longer bound anything
+        int zero = 0;
+        HIPCHK(hipMemcpy(j->seed_ok, &zero, sizeof(int), hipMemcpyHostToDevice));
     }
     return 0;
 }

@@ -1,6 +1,6 @@
 // cvt_device.h -- device-side building blocks of the gfx950 Lloyd step.
 //
-// Hash grid + exact K-nearest search, tangent frames, the tangent-plane cell
+// Dense cell grid + exact K-nearest search, tangent frames, the tangent-plane cell
 // as a dual convex hull, closest point on a triangle.  Written for wave64
 // CDNA: one lane per query, fixed-size register lists (no dynamic indexing,
 // no LDS, no scratch), and FP64 only where an FP32 comparison cannot be
@@ -12,20 +12,20 @@
 
 namespace cvt {
 
-// ------------------------------------------------------------------ hash grid
-// Uniform cells of size h over a padded bbox; cell (x,y,z) -> key
-// x | y<<10 | z<<20 (each < 1024) -> bucket hash(key) & mask.  Points are
-// stored bucket-sorted as float4 (x, y, z, key bits); a bucket may hold
-// points of several colliding cells, so scans compare the key.
+// ----------------------------------------------------------------- dense grid
+// Uniform cells of size h over a padded bbox, numbered x-fastest:
+// cell (x,y,z) -> (z*dy + y)*dx + x, so the 3 cells of a row along x are
+// consecutive and their points contiguous after the counting sort.  A 3x3x3
+// neighbourhood is therefore 9 contiguous runs whose 18 range bounds are all
+// loaded up front.  Points are stored cell-sorted as float4 (x, y, z, id bits);
+// cell `ncell` collects non-finite points and is never searched.
 struct GridP {
     float ox, oy, oz;
     float h, inv_h;
     float eps_abs;  // absolute rounding allowance of coordinate differences
     int dx, dy, dz;
-    unsigned mask;  // table size - 1 (power of two); bucket `mask+1` holds non-finite points
+    int ncell;
 };
-
-constexpr unsigned kInvalidKey = 0xffffffffu;
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
 
@@ -39,12 +39,8 @@ __device__ __forceinline__ void cell_of(const GridP& g, float x, float y, float 
     cz = clampi((int)floorf((z - g.oz) * g.inv_h), 0, g.dz - 1);
 }
 
-__device__ __forceinline__ unsigned cell_key(int cx, int cy, int cz) {
-    return (unsigned)cx | ((unsigned)cy << 10) | ((unsigned)cz << 20);
-}
-
-__device__ __forceinline__ unsigned cell_hash(const GridP& g, int cx, int cy, int cz) {
-    return (((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u)) & g.mask;
+__device__ __forceinline__ int cell_index(const GridP& g, int cx, int cy, int cz) {
+    return (cz * g.dy + cy) * g.dx + cx;
 }
 
 // Canonical ordering key: FP64 ((dx*dx + dy*dy) + dz*dz), dx = p - q,
@@ -69,14 +65,15 @@ __device__ __forceinline__ float d2f(float qx, float qy, float qz, float4 p) {
 // (|err| <= ~5 u d2 with u = 2^-24); comparisons use 2e-6.
 constexpr float kRel = 2e-6f;
 
-// Sorted (ascending) top-L list of (FP32 d2, slot) in registers.
+// Sorted (ascending) top-L list of (FP32 d2, slot) in registers.  init(tau)
+// fills it with sentinels (tau, -1): only candidates closer than tau enter.
 template <int L>
 struct TopL {
     float d[L];
     int s[L];
-    __device__ __forceinline__ void init() {
+    __device__ __forceinline__ void init(float tau) {
 #pragma unroll
-        for (int t = 0; t < L; ++t) { d[t] = __builtin_huge_valf(); s[t] = -1; }
+        for (int t = 0; t < L; ++t) { d[t] = tau; s[t] = -1; }
     }
     __device__ __forceinline__ float worst() const { return d[L - 1]; }
     // Branch-free sorted insertion; equal keys keep the earlier entry.
@@ -93,166 +90,231 @@ struct TopL {
     }
 };
 
-// Ring search over cells around q.  For each visited, non-culled cell calls
-// visit(bucket_begin, bucket_end, key).  cull_r2() returns the current
-// squared radius beyond which cells may be skipped; the search stops once the
-// block boundary is provably farther than that radius.
-template <class Visit, class CullR2>
-__device__ __forceinline__ void ring_search(const GridP& g, const int* __restrict__ start, float qx, float qy, float qz,
-                                            Visit&& visit, CullR2&& cull_r2) {
-    int cx, cy, cz;
-    cell_of(g, qx, qy, qz, cx, cy, cz);
-    const float h = g.h;
-    const int maxr = max(g.dx, max(g.dy, g.dz));
-    for (int r = 0; r <= maxr; ++r) {
-        for (int dz = -r; dz <= r; ++dz) {
-            const int z = cz + dz;
-            if (z < 0 || z >= g.dz) continue;
-            const float bz = dz < 0 ? fmaxf(0.f, qz - (g.oz + (z + 1) * h) - g.eps_abs)
-                             : (dz > 0 ? fmaxf(0.f, (g.oz + z * h) - qz - g.eps_abs) : 0.f);
-            for (int dy = -r; dy <= r; ++dy) {
-                const int y = cy + dy;
-                if (y < 0 || y >= g.dy) continue;
-                const float by = dy < 0 ? fmaxf(0.f, qy - (g.oy + (y + 1) * h) - g.eps_abs)
-                                 : (dy > 0 ? fmaxf(0.f, (g.oy + y * h) - qy - g.eps_abs) : 0.f);
-                const bool shell = (dz == -r) || (dz == r) || (dy == -r) || (dy == r);
-                const int step = (shell || r == 0) ? 1 : 2 * r;
-                for (int dxx = -r; dxx <= r; dxx += step) {
-                    const int x = cx + dxx;
-                    if (x < 0 || x >= g.dx) continue;
-                    const float bx = dxx < 0 ? fmaxf(0.f, qx - (g.ox + (x + 1) * h) - g.eps_abs)
-                                     : (dxx > 0 ? fmaxf(0.f, (g.ox + x * h) - qx - g.eps_abs) : 0.f);
-                    const float bd2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
-                    if (bd2 > cull_r2() * (1.f + 1e-5f)) continue;
-                    const unsigned key = cell_key(x, y, z);
-                    const unsigned b = cell_hash(g, x, y, z);
-                    visit(start[b], start[b + 1], key);
-                }
-            }
-        }
-        // distance from q to the outside of block [c-r, c+r] (inf where the block reaches the grid edge)
-        float bnd = __builtin_huge_valf();
-        if (cx - r > 0) bnd = fminf(bnd, qx - (g.ox + (cx - r) * h));
-        if (cx + r + 1 < g.dx) bnd = fminf(bnd, (g.ox + (cx + r + 1) * h) - qx);
-        if (cy - r > 0) bnd = fminf(bnd, qy - (g.oy + (cy - r) * h));
-        if (cy + r + 1 < g.dy) bnd = fminf(bnd, (g.oy + (cy + r + 1) * h) - qy);
-        if (cz - r > 0) bnd = fminf(bnd, qz - (g.oz + (cz - r) * h));
-        if (cz + r + 1 < g.dz) bnd = fminf(bnd, (g.oz + (cz + r + 1) * h) - qz);
-        if (bnd == __builtin_huge_valf()) break;
-        bnd -= g.eps_abs;
-        if (bnd > 0.f && cull_r2() < bnd * bnd * (1.f - 1e-5f)) break;
+// Lower bound of the distance along one axis from coordinate q to cell c of
+// size h starting at o, for a cell offset dc from the query's cell.
+__device__ __forceinline__ float axis_gap(float q, float o, float h, int c, int dc, float eps) {
+    return dc < 0 ? fmaxf(0.f, q - (o + (c + 1) * h) - eps) : (dc > 0 ? fmaxf(0.f, (o + c * h) - q - eps) : 0.f);
+}
+
+template <int L>
+__device__ __forceinline__ void scan_run(const float4* __restrict__ pts, int s0, int s1, float qx, float qy, float qz,
+                                         int self_slot, TopL<L>& T) {
+    int s = s0;
+    for (; s + 1 < s1; s += 2) {  // two loads in flight
+        const float4 p0 = pts[s], p1 = pts[s + 1];
+        const float d0 = d2f(qx, qy, qz, p0), d1 = d2f(qx, qy, qz, p1);
+        if (d0 < T.worst() && s != self_slot) T.insert(d0, s);
+        if (d1 < T.worst() && s + 1 != self_slot) T.insert(d1, s + 1);
+    }
+    if (s < s1) {
+        const float4 p0 = pts[s];
+        const float d0 = d2f(qx, qy, qz, p0);
+        if (d0 < T.worst() && s != self_slot) T.insert(d0, s);
     }
 }
 
-// FP32 top-L over a grid (self_slot excluded).  Exact: the returned list is
-// the FP32 top-L of the whole point set.
-template <int L>
-__device__ __forceinline__ void grid_topl(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
-                                          float qx, float qy, float qz, int self_slot, TopL<L>& T) {
-    T.init();
-    ring_search(
-        g, start, qx, qy, qz,
-        [&](int s0, int s1, unsigned key) {
-            for (int s = s0; s < s1; ++s) {
-                const float4 p = pts[s];
-                if (__float_as_uint(p.w) != key || s == self_slot) continue;
-                const float d = d2f(qx, qy, qz, p);
-                if (d < T.worst()) T.insert(d, s);
-            }
-        },
-        [&]() { return T.worst(); });
+// Distance from q to the outside of the block of radius r around its cell
+// (+inf where the block reaches the grid edge on every side).
+__device__ __forceinline__ float block_bound(const GridP& g, float qx, float qy, float qz, int cx, int cy, int cz,
+                                             int r) {
+    const float h = g.h;
+    float bnd = __builtin_huge_valf();
+    if (cx - r > 0) bnd = fminf(bnd, qx - (g.ox + (cx - r) * h));
+    if (cx + r + 1 < g.dx) bnd = fminf(bnd, (g.ox + (cx + r + 1) * h) - qx);
+    if (cy - r > 0) bnd = fminf(bnd, qy - (g.oy + (cy - r) * h));
+    if (cy + r + 1 < g.dy) bnd = fminf(bnd, (g.oy + (cy + r + 1) * h) - qy);
+    if (cz - r > 0) bnd = fminf(bnd, qz - (g.oz + (cz - r) * h));
+    if (cz + r + 1 < g.dz) bnd = fminf(bnd, (g.oz + (cz + r + 1) * h) - qz);
+    return bnd;
 }
 
-// Exact K nearest under (key64, id): FP64 insertion over every point with
-// FP32 distance <= tau.  Used only when the FP32 filter is inconclusive.
+// FP32 top-L of the whole point set (self_slot excluded), seeded with radius
+// tau: the 3x3x3 block first (9 runs, bounds loaded together); if the block
+// boundary is not provably beyond the L-th distance, the list is rebuilt over
+// blocks of radius 2, 3, ...  Returns false when fewer than L points lie
+// within tau (the caller then repeats with tau = inf).
+template <int L>
+__device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+                                          float qx, float qy, float qz, int self_slot, float tau, TopL<L>& T) {
+    int cx, cy, cz;
+    cell_of(g, qx, qy, qz, cx, cy, cz);
+    const float h = g.h;
+    T.init(tau);
+    {
+        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.dx - 1);
+        int r0[9], r1[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const int dz = k / 3 - 1, dy = k % 3 - 1;
+            const int z = cz + dz, y = cy + dy;
+            const bool in = z >= 0 && z < g.dz && y >= 0 && y < g.dy;
+            const float bz = axis_gap(qz, g.oz, h, z, dz, g.eps_abs), by = axis_gap(qy, g.oy, h, y, dy, g.eps_abs);
+            const bool keep = in && fmaf(by, by, bz * bz) <= tau * (1.f + 1e-5f);
+            const int row = (z * g.dy + y) * g.dx;
+            r0[k] = keep ? start[row + x0] : 0;
+            r1[k] = keep ? start[row + x1 + 1] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) scan_run<L>(pts, r0[k], r1[k], qx, qy, qz, self_slot, T);
+    }
+    const int maxr = max(g.dx, max(g.dy, g.dz));
+    for (int r = 1; r <= maxr; ++r) {
+        float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, r);
+        if (bnd == __builtin_huge_valf()) break;
+        bnd -= g.eps_abs;
+        if (bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f)) break;
+        // not provable yet: rebuild over the block of radius r+1
+        const int rr = r + 1;
+        T.init(tau);
+        const int x0 = max(cx - rr, 0), x1 = min(cx + rr, g.dx - 1);
+        for (int dz = -rr; dz <= rr; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= g.dz) continue;
+            const float bz = axis_gap(qz, g.oz, h, z, dz, g.eps_abs);
+            for (int dy = -rr; dy <= rr; ++dy) {
+                const int y = cy + dy;
+                if (y < 0 || y >= g.dy) continue;
+                const float by = axis_gap(qy, g.oy, h, y, dy, g.eps_abs);
+                if (fmaf(by, by, bz * bz) > T.worst() * (1.f + 1e-5f)) continue;
+                const int row = (z * g.dy + y) * g.dx;
+                scan_run<L>(pts, start[row + x0], start[row + x1 + 1], qx, qy, qz, self_slot, T);
+            }
+        }
+    }
+    return T.s[L - 1] >= 0 || tau == __builtin_huge_valf();
+}
+
+__device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float qx, float qy, float qz, int sa,
+                                         int sb) {
+    const float4 a = pts[sa], b = pts[sb];
+    const double ka = key64(qx, qy, qz, a.x, a.y, a.z), kb = key64(qx, qy, qz, b.x, b.y, b.z);
+    return ka < kb || (ka == kb && __float_as_int(a.w) < __float_as_int(b.w));
+}
+
+// Exact K nearest under (key64, id) from an FP32 top-(K+2) list.  Adjacent
+// entries closer than the FP32 error bound are "ambiguous"; isolated
+// ambiguous pairs (including the pair at the K-th boundary) are ordered by
+// FP64 keys here.  Returns false (caller defers the query to the FP64 pass)
+// when three or more entries are mutually ambiguous or the (K+2)-th entry is
+// not provably beyond the K-th.
+template <int K, int L>
+__device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
+                                             int& n_pairs) {
+    static_assert(L == K + 2, "list must hold two extra entries");
+    n_pairs = 0;
+    if (T.s[K - 1] < 0) return true;  // fewer than K points exist: nothing to order beyond them
+    const bool margin = T.s[K + 1] < 0 || T.d[K + 1] > T.d[K - 1] * (1.f + kRel);
+    if (!margin) return false;
+    bool amb[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) amb[t] = T.s[t + 1] >= 0 && !(T.d[t + 1] > T.d[t] * (1.f + kRel));
+    bool chain = false;
+#pragma unroll
+    for (int t = 0; t + 1 < K; ++t) chain = chain || (amb[t] && amb[t + 1]);
+    if (chain) return false;
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+        if (amb[t]) {
+            ++n_pairs;
+            if (key_less(pts, qx, qy, qz, T.s[t + 1], T.s[t])) {
+                const int ts = T.s[t]; T.s[t] = T.s[t + 1]; T.s[t + 1] = ts;
+                const float td = T.d[t]; T.d[t] = T.d[t + 1]; T.d[t + 1] = td;
+            }
+        }
+    }
+    return true;
+}
+
+// Exact K nearest under (key64, id) by FP64 insertion over every point with
+// FP32 distance <= tau.  Only the FP64-resolving pass uses it.
 template <int K>
 __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
-                                          const int* __restrict__ ids, float qx, float qy, float qz, int self_slot,
-                                          float tau, int* __restrict__ out_slot) {
+                                          float qx, float qy, float qz, int self_slot, float tau,
+                                          int* __restrict__ out_slot) {
     double kd[K];
     int kid[K], ks[K];
 #pragma unroll
     for (int t = 0; t < K; ++t) { kd[t] = __builtin_huge_val(); kid[t] = 0x7fffffff; ks[t] = -1; }
-    ring_search(
-        g, start, qx, qy, qz,
-        [&](int s0, int s1, unsigned key) {
-            for (int s = s0; s < s1; ++s) {
-                const float4 p = pts[s];
-                if (__float_as_uint(p.w) != key || s == self_slot) continue;
-                if (d2f(qx, qy, qz, p) > tau) continue;
-                const double d = key64(qx, qy, qz, p.x, p.y, p.z);
-                const int id = ids[s];
-                if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) continue;
+    int cx, cy, cz;
+    cell_of(g, qx, qy, qz, cx, cy, cz);
+    const float h = g.h;
+    const int maxr = max(g.dx, max(g.dy, g.dz));
+    for (int rr = 1; rr <= maxr; ++rr) {
+        // scan the shell at Chebyshev radius rr (radius 0 and 1 together on the first pass)
+        const int x0 = max(cx - rr, 0), x1 = min(cx + rr, g.dx - 1);
+        for (int dz = -rr; dz <= rr; ++dz) {
+            const int z = cz + dz;
+            if (z < 0 || z >= g.dz) continue;
+            const float bz = axis_gap(qz, g.oz, h, z, dz, g.eps_abs);
+            for (int dy = -rr; dy <= rr; ++dy) {
+                const int y = cy + dy;
+                if (y < 0 || y >= g.dy) continue;
+                const float by = axis_gap(qy, g.oy, h, y, dy, g.eps_abs);
+                if (fmaf(by, by, bz * bz) > tau * (1.f + 1e-5f)) continue;
+                const bool inner = rr > 1 && dz > -rr && dz < rr && dy > -rr && dy < rr;
+                const int row = (z * g.dy + y) * g.dx;
+                for (int part = 0; part < (inner ? 2 : 1); ++part) {
+                    int a0 = x0, a1 = x1;
+                    if (inner) {  // only the two end cells of an inner row are new
+                        a0 = part == 0 ? cx - rr : cx + rr;
+                        a1 = a0;
+                        if (a0 < 0 || a0 >= g.dx) continue;
+                    }
+                    const int s1 = start[row + a1 + 1];
+                    for (int s = start[row + a0]; s < s1; ++s) {
+                        const float4 p = pts[s];
+                        if (s == self_slot || d2f(qx, qy, qz, p) > tau) continue;
+                        const double d = key64(qx, qy, qz, p.x, p.y, p.z);
+                        const int id = __float_as_int(p.w);
+                        if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) continue;
 #pragma unroll
-                for (int t = K - 1; t >= 1; --t) {
-                    const bool ct = d < kd[t] || (d == kd[t] && id < kid[t]);
-                    const bool cp = d < kd[t - 1] || (d == kd[t - 1] && id < kid[t - 1]);
-                    kd[t] = cp ? kd[t - 1] : (ct ? d : kd[t]);
-                    kid[t] = cp ? kid[t - 1] : (ct ? id : kid[t]);
-                    ks[t] = cp ? ks[t - 1] : (ct ? s : ks[t]);
+                        for (int t = K - 1; t >= 1; --t) {
+                            const bool ct = d < kd[t] || (d == kd[t] && id < kid[t]);
+                            const bool cp = d < kd[t - 1] || (d == kd[t - 1] && id < kid[t - 1]);
+                            kd[t] = cp ? kd[t - 1] : (ct ? d : kd[t]);
+                            kid[t] = cp ? kid[t - 1] : (ct ? id : kid[t]);
+                            ks[t] = cp ? ks[t - 1] : (ct ? s : ks[t]);
+                        }
+                        const bool c0 = d < kd[0] || (d == kd[0] && id < kid[0]);
+                        kd[0] = c0 ? d : kd[0];
+                        kid[0] = c0 ? id : kid[0];
+                        ks[0] = c0 ? s : ks[0];
+                    }
                 }
-                const bool c0 = d < kd[0] || (d == kd[0] && id < kid[0]);
-                kd[0] = c0 ? d : kd[0];
-                kid[0] = c0 ? id : kid[0];
-                ks[0] = c0 ? s : ks[0];
             }
-        },
-        [&]() { return tau; });
+        }
+        float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, rr);
+        if (bnd == __builtin_huge_valf()) break;
+        bnd -= g.eps_abs;
+        if (bnd > 0.f && tau < bnd * bnd * (1.f - 1e-5f)) break;
+    }
 #pragma unroll
     for (int t = 0; t < K; ++t) out_slot[t] = ks[t];
 }
 
-// K nearest (K < L) in exact (key64, id) order, as slots.  Returns
-// 0 = FP32 order proven, 1 = FP64 rerank inside the list, 2 = exact rescan,
-// -1 = not provable in FP32 (only when !PRECISE: the caller defers the query).
-template <int K, int L, bool PRECISE>
+// K nearest (in exact (key64, id) order) of q as slots.  Returns 0 when FP32
+// (plus isolated FP64 pair checks) proved the list, 1 when the FP64 rescan
+// was needed, -1 (only !PRECISE) when the query must go to the FP64 pass.
+// tau: a provable upper bound of the (K+2)-th distance^2, or inf.
+// l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
+template <int K, bool PRECISE>
 __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
-                                         const int* __restrict__ ids, float qx, float qy, float qz, int self_slot,
-                                         int* __restrict__ slot_out) {
-    static_assert(L > K, "need at least one extra entry");
+                                         float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
+                                         float& l_out) {
+    constexpr int L = K + 2;
     TopL<L> T;
-    grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, T);
-    const bool margin = T.d[L - 1] > T.d[K - 1] * (1.f + kRel) && T.d[K - 1] < __builtin_huge_valf();
-    bool order = true;
-#pragma unroll
-    for (int t = 0; t + 1 < K; ++t) order = order && (T.d[t + 1] > T.d[t] * (1.f + kRel));
+    if (!grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T))
+        grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T);
+    l_out = T.s[L - 1] >= 0 ? T.d[L - 1] : __builtin_huge_valf();
+    int pairs;
+    const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs);
 #pragma unroll
     for (int t = 0; t < K; ++t) slot_out[t] = T.s[t];
-    if (margin && order) return 0;
+    if (ok) return 0;
     if constexpr (!PRECISE) {
         return -1;
     } else {
-        if (!margin) {
-            const float tau = T.d[L - 1] * (1.f + 1e-5f);
-            grid_exact64<K>(g, start, pts, ids, qx, qy, qz, self_slot, tau, slot_out);
-            return 2;
-        }
-        // FP64 rerank: odd-even transposition passes on (key64, id) until sorted.
-        double kd[K];
-        int kid[K];
-#pragma unroll
-        for (int t = 0; t < K; ++t) {
-            const float4 p = pts[slot_out[t]];
-            kd[t] = key64(qx, qy, qz, p.x, p.y, p.z);
-            kid[t] = ids[slot_out[t]];
-        }
-        for (int pass = 0; pass < K; ++pass) {
-            bool swapped = false;
-#pragma unroll
-            for (int par = 0; par < 2; ++par) {
-#pragma unroll
-                for (int t = par; t + 1 < K; t += 2) {
-                    const bool sw = kd[t + 1] < kd[t] || (kd[t + 1] == kd[t] && kid[t + 1] < kid[t]);
-                    if (sw) {
-                        double td = kd[t]; kd[t] = kd[t + 1]; kd[t + 1] = td;
-                        int ti = kid[t]; kid[t] = kid[t + 1]; kid[t + 1] = ti;
-                        int ts = slot_out[t]; slot_out[t] = slot_out[t + 1]; slot_out[t + 1] = ts;
-                        swapped = true;
-                    }
-                }
-            }
-            if (!swapped) break;
-        }
+        grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out);
         return 1;
     }
 }

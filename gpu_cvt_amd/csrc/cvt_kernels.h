@@ -17,17 +17,15 @@ struct MeshDev {
     const int* vf_off;    // nV+1
     const int* vf_faces;  // 3*nF
     int nV, nF;
-    GridP vg;             // static vertex hash grid
+    GridP vg;             // static vertex grid (pts: x, y, z, vertex id bits)
     const int* vg_start;
     const float4* vg_pts;
-    const int* vg_ids;
 };
 
 struct SiteGridDev {
     GridP g;
     const int* start;
-    const float4* pts;
-    const int* ids;
+    const float4* pts;    // cell-sorted sites (x, y, z, site id bits)
 };
 
 struct StepArgs {
@@ -43,6 +41,10 @@ struct StepArgs {
     const int* has_prev;  // device flag
     int* stats;           // kStatWords counters of this iteration
     int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
+    float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
+    const int* seed_ok;   // device flag: rad2 and dmax_prev are valid for this iteration
+    const float* dmax_prev;  // largest site displacement of the previous iteration
+    float* dmax_cur;         // this iteration's (atomic max, float bits)
     int nS;
     float R;
     int freeze_mode;
@@ -59,15 +61,14 @@ struct StepArgs {
     float2* dbg_am;
 };
 
-// Work buffers of one hash grid build.
+// Work buffers of one grid build.
 struct GridBufs {
-    int* count;   // T+2
-    int* start;   // T+2 (exclusive scan; start[T+1] = n)
+    int* count;   // ncell+2
+    int* start;   // ncell+2 (exclusive scan; start[ncell+1] = n)
     int* bsum;    // scan partials
     int* bucket;  // n
     int* rank;    // n
-    float4* pts;  // n   bucket-sorted (x, y, z, cell key bits)
-    int* ids;     // n   point id per sorted slot
+    float4* pts;  // n   cell-sorted (x, y, z, id bits)
 };
 
 void launch_fill(int* p, long n, int v, hipStream_t s);
@@ -86,6 +87,6 @@ void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStr
 void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
                        const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);
 void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s);
-void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, hipStream_t s);
+void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, int* seed_ok, hipStream_t s);
 
 }  // namespace cvt

@@ -36,33 +36,23 @@ __global__ void k_grid_count(const float4* __restrict__ P, int n, GridP g, int* 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = P[i];
-    unsigned b;
+    int b = g.ncell;
     if (finite3(p.x, p.y, p.z)) {
         int cx, cy, cz;
         cell_of(g, p.x, p.y, p.z, cx, cy, cz);
-        b = cell_hash(g, cx, cy, cz);
-    } else {
-        b = g.mask + 1;
+        b = cell_index(g, cx, cy, cz);
     }
-    bucket[i] = (int)b;
+    bucket[i] = b;
     rank[i] = atomicAdd(&count[b], 1);
 }
 
-__global__ void k_grid_scatter(const float4* __restrict__ P, int n, GridP g, const int* __restrict__ bucket,
-                               const int* __restrict__ rank, const int* __restrict__ start, float4* __restrict__ out,
-                               int* __restrict__ ids) {
+__global__ void k_grid_scatter(const float4* __restrict__ P, int n, const int* __restrict__ bucket,
+                               const int* __restrict__ rank, const int* __restrict__ start, float4* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float4 p = P[i];
-    unsigned key = kInvalidKey;
-    if (finite3(p.x, p.y, p.z)) {
-        int cx, cy, cz;
-        cell_of(g, p.x, p.y, p.z, cx, cy, cz);
-        key = cell_key(cx, cy, cz);
-    }
     const int pos = start[bucket[i]] + rank[i];
-    out[pos] = make_float4(p.x, p.y, p.z, __uint_as_float(key));
-    ids[pos] = i;
+    out[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
 }
 
 // Exclusive scan of n ints (3 kernels).  Block = 256 threads x 16 items.
@@ -202,6 +192,7 @@ __device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, boo
 constexpr int kStatDeferred = 9;
 
 struct SiteFlags {
+    float disp;     // |S' - S| (0 when not done or not finite)
     bool done;      // this pass produced the site's result
     bool row_written;  // deferred after the (proven) KNN row was already written
     bool same, low, both, newly, frozen_after, cell_fail, nonfinite;
@@ -222,11 +213,12 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
         q = a.sg.pts[t];
-        i = a.sg.ids[t];
+        i = __float_as_int(q.w);
     }
-    const bool valid = live && __float_as_uint(q.w) != kInvalidKey;
+    const bool valid = live && finite3(q.x, q.y, q.z);
     const bool fz = valid && !eval_only && a.frozen[i] != 0;
     bool defer = false;
+    const float inf = __builtin_huge_valf();
 
     // ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110)
     int nn = -1;
@@ -234,9 +226,10 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     if (valid) {
         if (!fz) {
             int s1[1];
-            const int k1 = knn_exact<1, 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
+            float l1;
+            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, inf, s1, l1);
             defer = defer || k1 < 0;
-            nn = s1[0] >= 0 ? a.m.vg_ids[s1[0]] : -1;
+            nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
         } else {
             nn = a.knnV[(size_t)i * KP];  // pinned at the freeze iteration
         }
@@ -260,8 +253,17 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     float px[K + 4], py[K + 4];
     int slots[K];
     if (valid && !fz) {
-        r.kind = knn_exact<K, K + 1, PRECISE>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
+        // provable seed: the previous (K+2)-th distance grown by twice the
+        // largest site displacement of the previous iteration
+        float tau = inf;
+        if (!PRECISE && a.seed_ok && *a.seed_ok) {
+            const float r0 = sqrtf(a.rad2[i]) + 2.f * a.dmax_prev[0];
+            tau = r0 * r0 * (1.f + 1e-5f);
+        }
+        float lk;
+        r.kind = knn_exact<K, PRECISE>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, tau, slots, lk);
         defer = defer || r.kind < 0;
+        if (!eval_only && r.kind >= 0) a.rad2[i] = lk;
     }
     if (!PRECISE && defer) return r;  // nothing written: the precise pass redoes this site
     bool same = true;
@@ -273,7 +275,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
             if (!fz) {
                 const int sl = slots[k] >= 0 ? slots[k] : t;
                 pj = a.sg.pts[sl];
-                const int id = a.sg.ids[sl];
+                const int id = __float_as_int(pj.w);
                 if (!eval_only) {
                     same = same && (carried >= 0 || row[k] == id);
                     row[k] = id;
@@ -332,15 +334,16 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
         if (!fz) {
             if (finite3(c3.x, c3.y, c3.z)) {
                 int sl[KP];
-                const int kk = knn_exact<KP, KP + 1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, c3.x,
-                                                             c3.y, c3.z, -1, sl);
+                float lv;
+                const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, inf, sl,
+                                                      lv);
                 if (!PRECISE && kk < 0) {
                     r.same = same;
                     r.row_written = !eval_only;
                     return r;
                 }
 #pragma unroll
-                for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? a.m.vg_ids[sl[k]] : -1;
+                for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
             }
             if (a.write_knnv) {
                 int* row = a.knnV + (size_t)i * KP;
@@ -384,6 +387,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     r.frozen_after = valid && fz;
     if (valid && !eval_only) {
         a.Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
+        r.disp = sqrtf((sp.x - q.x) * (sp.x - q.x) + (sp.y - q.y) * (sp.y - q.y) + (sp.z - q.z) * (sp.z - q.z));
         if (has_prev) {
             const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
             const float ddx = sp.x - q.x, ddy = sp.y - q.y, ddz = sp.z - q.z;
@@ -423,6 +427,11 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
 __device__ __forceinline__ void count_site(const StepArgs& a, const SiteFlags& f, bool live) {
     const bool hp = a.eval_only == 0 && *a.has_prev != 0;
     const bool d = live && f.done;
+    // largest displacement of this iteration (seeds the next iteration's searches)
+    float m = (d && isfinite(f.disp)) ? f.disp : 0.f;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.f && a.dmax_cur) atomicMax((unsigned*)a.dmax_cur, __float_as_uint(m));
     wave_count(a.stats, 0, d && hp && f.same);
     wave_count(a.stats, 1, d && hp && f.low);
     wave_count(a.stats, 2, d && hp && f.both);
@@ -485,14 +494,16 @@ __global__ __launch_bounds__(256) void k_remesh_cells(StepArgs a, int* __restric
     const int t = blockIdx.x * 256 + (int)threadIdx.x;
     if (t >= a.nS) return;
     const float4 q = a.sg.pts[t];
-    const int i = a.sg.ids[t];
+    const int i = __float_as_int(q.w);
     unsigned long long* out = keys + (size_t)i * M;
     for (int k = 0; k < M; ++k) out[k] = ~0ull;
     site_n[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (__float_as_uint(q.w) == kInvalidKey) return;
+    if (!finite3(q.x, q.y, q.z)) return;
+    const float inf = __builtin_huge_valf();
     int s1[1];
-    knn_exact<1, 2, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, a.m.vg_ids, q.x, q.y, q.z, -1, s1);
-    const int nn = s1[0] >= 0 ? a.m.vg_ids[s1[0]] : -1;
+    float l1;
+    knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, inf, s1, l1);
+    const int nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (nn >= 0) {
         const float4 nv = a.m.N[nn];
@@ -503,14 +514,15 @@ __global__ __launch_bounds__(256) void k_remesh_cells(StepArgs a, int* __restric
         else frame_f32(nv.x, nv.y, nv.z, u, w);
     }
     int slots[K];
-    knn_exact<K, K + 1, true>(a.sg.g, a.sg.start, a.sg.pts, a.sg.ids, q.x, q.y, q.z, t, slots);
+    float lk;
+    knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, inf, slots, lk);
     int* row = knn_tmp + (size_t)i * K;
     float px[M], py[M];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int sl = slots[k] >= 0 ? slots[k] : t;
         const float4 pj = a.sg.pts[sl];
-        row[k] = a.sg.ids[sl];
+        row[k] = __float_as_int(pj.w);
         const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
         const float aa = dx * u.x + dy * u.y + dz * u.z;
         const float bb = dx * w.x + dy * w.y + dz * w.z;
@@ -608,13 +620,14 @@ void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n,
 }
 
 __global__ void k_iter_end(int* __restrict__ stats, int* __restrict__ hist, int* __restrict__ hist_count, int cap,
-                           int* __restrict__ has_prev) {
+                           int* __restrict__ has_prev, int* __restrict__ seed_ok) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int c = *hist_count;
     const int slot = c < cap ? c : cap - 1;
     for (int k = 0; k < kStatWords; ++k) hist[slot * kStatWords + k] = stats[k];
     *hist_count = c + 1;
     *has_prev = 1;
+    *seed_ok = 1;
 }
 
 // ------------------------------------------------------------------ launches
@@ -633,12 +646,11 @@ void launch_scan(const int* in, int n, int* out, int* bsum, hipStream_t s) {
 }
 
 void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipStream_t s) {
-    const int T1 = (int)g.mask + 2;  // buckets incl. the non-finite bucket
+    const int T1 = g.ncell + 1;  // cells + the non-finite bucket
     launch_fill(b.count, T1, 0, s);
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.count, b.bucket, b.rank);
     launch_scan(b.count, T1, b.start, b.bsum, s);
-    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.bucket, b.rank, b.start,
-                       b.pts, b.ids);
+    hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, b.bucket, b.rank, b.start, b.pts);
 }
 
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
@@ -677,8 +689,8 @@ void launch_lloyd(const StepArgs& a, int K, int kproj, hipStream_t s) {
     }
 }
 
-void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, hipStream_t s) {
-    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, stats, hist, hist_count, cap, has_prev);
+void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, int* seed_ok, hipStream_t s) {
+    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, stats, hist, hist_count, cap, has_prev, seed_ok);
 }
 
 }  // namespace cvt
