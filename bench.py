@@ -69,6 +69,55 @@ def cpu_baseline(V, F, S, K: int, budget_s: float = 20.0):
                       f"({dt:.1f} s, {threads} threads)"}
 
 
+def bench_batch(args) -> int:
+    """Mixed-size jobs (log-uniform triangles 5k-400k, sites 1k-100k capped at
+    triangles/2, K = 24, freezing on): site-iters/s of the batched form (all
+    jobs in one graph launch per iteration) against the plain path (each job's
+    own graph, one job at a time) on the same inputs.  Both forms compute
+    bit-identical results (tests/test_gpu_batch.py), so the final Hausdorff is
+    equal by construction."""
+    from gpu_cvt_amd import cvt, meshgen as mg
+
+    t_gen = time.perf_counter()
+    work = mg.batch_workload(args.jobs, seed=7)
+    t_gen = time.perf_counter() - t_gen
+    total_sites = sum(len(w[2]) for w in work)
+    kw = dict(K=24, freeze_mode=1, freeze_need=5, freeze_disp_rel=1e-5)
+    print(f"[bench] batch: {len(work)} jobs, {total_sites} sites, "
+          f"{sum(len(w[1]) for w in work)} triangles, gen {t_gen:.1f}s", file=sys.stderr, flush=True)
+    iters = args.steps
+    # plain path: one job at a time, each job's own graph
+    plain_ms = 0.0
+    for k, (V, F, S, _tag) in enumerate(work):
+        with cvt.Job(V, F, S, **kw) as job:
+            job.step(args.warmup)
+            plain_ms += job.step_timed(iters)
+        if k % 32 == 31:
+            print(f"[bench] plain: {k + 1} jobs done", file=sys.stderr, flush=True)
+    plain_rate = total_sites * iters / (plain_ms / 1e3)
+    # batched: all jobs resident, one graph launch per iteration
+    jobs = [cvt.Job(V, F, S, **kw) for V, F, S, _tag in work]
+    with cvt.Batch(jobs) as b:
+        b.step(args.warmup)
+        batch_ms = b.step_timed(iters)
+    frozen = sum(int(j.freeze_state()[0].sum()) for j in jobs)
+    for j in jobs:
+        j.close()
+    batch_rate = total_sites * iters / (batch_ms / 1e3)
+    print(f"[bench] plain {plain_ms:.1f} ms, batched {batch_ms:.1f} ms for {iters} iterations; "
+          f"frozen at end {frozen}/{total_sites}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "batched Lloyd site-iters/s, 256 mixed jobs (vs one job at a time)",
+        "value": batch_rate, "unit": "site-iters/s", "n_gpus": 1, "steps": iters, "warmup": args.warmup,
+        "ms_per_step": batch_ms / iters, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded icospheres, tori, displaced spheres)",
+        "config": {"workload": "batch_mixed", "jobs": len(work), "sites": total_sites, "K": 24, "freezing": True},
+        "plain_value": plain_rate, "batched_over_plain": batch_rate / plain_rate,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,6 +128,9 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-iters", type=int, default=5)
+    ap.add_argument("--mode", choices=["plain", "batch"], default="plain",
+                    help="plain: the headline 1M-site job; batch: 256 mixed jobs, batched vs one at a time")
+    ap.add_argument("--jobs", type=int, default=256, help="batch mode: number of jobs")
     ap.add_argument("--site-cell", type=float, default=0.0, help="site grid cell scale (0 = library default)")
     ap.add_argument("--vert-cell", type=float, default=0.0, help="vertex grid cell scale (0 = library default)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
@@ -104,6 +156,9 @@ def main(argv=None) -> int:
         print(f"[bench] copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written",
               file=sys.stderr, flush=True)
         del src, dst
+
+    if args.mode == "batch":
+        return bench_batch(args)
 
     t_gen = time.perf_counter()
     V, F, S, K, _iters, _fz = mg.workload(args.workload)

@@ -43,8 +43,6 @@ int fail(int code, const char* fmt, ...) {
                                           __FILE__, __LINE__);                                     \
     } while (0)
 
-constexpr int kHistCap = 4096;
-
 }  // namespace
 
 struct cvt_job {
@@ -69,6 +67,11 @@ struct cvt_job {
     int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
+    int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
+    // this job as a batch of one: its descriptor and launch tables on the device
+    JobDesc* desc = nullptr;
+    BlockRef *site_blocks = nullptr, *cell_blocks = nullptr, *tile_blocks = nullptr;
+    BatchTables tables{};
     // debug
     int* dbg_nn = nullptr;
     float4 *dbg_U = nullptr, *dbg_V = nullptr, *dbg_cent = nullptr;
@@ -77,7 +80,7 @@ struct cvt_job {
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     double diag = 0, R = 0, area = 0;
     int iters_done = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t graph = nullptr;
     cvt_bvh* bvh = nullptr;  // LBVH of the input mesh, built on first use
 
     template <class T>
@@ -93,8 +96,7 @@ struct cvt_job {
     }
     ~cvt_job();
     void release() {
-        for (auto& g : graph)
-            if (g) (void)hipGraphExecDestroy(g);
+        if (graph) (void)hipGraphExecDestroy(graph);
         for (void* d : allocs) (void)hipFree(d);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -256,8 +258,9 @@ int alloc_grid(cvt_job* j, GridBufs& b, int n, int ncell) {
     return 0;
 }
 
-StepArgs make_args(cvt_job* j, int parity, int eval_only) {
-    StepArgs a{};
+JobDesc make_desc(cvt_job* j) {
+    JobDesc d{};
+    StepArgs& a = d.a;
     a.m.V = j->V;
     a.m.N = j->N;
     a.m.tri = j->tri;
@@ -272,8 +275,9 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     a.sg.g = j->sgp;
     a.sg.start = j->sgb.start;
     a.sg.pts = j->sgb.pts;
-    a.S = j->S[parity];
-    a.Snew = j->S[1 - parity];
+    a.S01[0] = j->S[0];
+    a.S01[1] = j->S[1];
+    a.parity = j->parity;
     a.knn = j->knn;
     a.knnV = j->knnV;
     a.frozen = j->frozen;
@@ -281,11 +285,12 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     a.tier = j->tier;
     a.has_prev = j->has_prev;
     a.stats = j->stats;
+    a.hist = j->hist;
+    a.hist_count = j->hist_count;
     a.defer_list = j->defer_list;
     a.rad2 = j->rad2;
-    a.seed_ok = eval_only ? nullptr : j->seed_ok;
-    a.dmax_prev = j->dmax + (1 - parity);
-    a.dmax_cur = eval_only ? nullptr : j->dmax + parity;
+    a.seed_ok = j->seed_ok;
+    a.dmax01 = j->dmax;
     a.nS = j->nS;
     a.R = (float)j->R;
     a.freeze_mode = j->p.freeze_mode;
@@ -302,7 +307,6 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
     }
     a.nan_policy = j->p.nan_policy;
     a.write_knnv = (j->p.freeze_mode != 0 || j->p.debug) ? 1 : 0;
-    a.eval_only = eval_only;
     if (j->p.debug) {
         a.dbg_nn = j->dbg_nn;
         a.dbg_U = j->dbg_U;
@@ -310,27 +314,40 @@ StepArgs make_args(cvt_job* j, int parity, int eval_only) {
         a.dbg_cent = j->dbg_cent;
         a.dbg_am = j->dbg_am;
     }
-    return a;
+    d.count = j->sgb.count;
+    d.start = j->sgb.start;
+    d.bsum = j->sgb.bsum;
+    d.bucket = j->sgb.bucket;
+    d.rank = j->sgb.rank;
+    d.ncell1 = j->sgp.ncell + 1;
+    return d;
 }
 
-// One iteration from S[parity] into S[1-parity].
-void enqueue_iteration(cvt_job* j, int parity) {
-    hipStream_t s = j->stream;
-    launch_fill(j->stats, kStatWords, 0, s);
-    launch_fill((int*)(j->dmax + parity), 1, 0, s);
-    launch_grid_build(j->S[parity], j->nS, j->sgp, j->sgb, s);
-    StepArgs a = make_args(j, parity, 0);
-    launch_lloyd(a, j->p.K, j->p.kproj, s);
-    launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, j->seed_ok, s);
+// Block tables of a set of jobs (job index k refers to descs[k]).
+struct HostTables {
+    std::vector<BlockRef> site, cell, tile;
+};
+
+void add_job_blocks(HostTables& h, int k, int nS, int ncell1) {
+    for (int b = 0; b < (nS + 255) / 256; ++b) h.site.push_back({k, b});
+    for (int b = 0; b < (ncell1 + 255) / 256; ++b) h.cell.push_back({k, b});
+    for (int b = 0; b < (ncell1 + scan_tile() - 1) / scan_tile(); ++b) h.tile.push_back({k, b});
 }
 
-int ensure_graph(cvt_job* j, int parity) {
-    if (j->graph[parity]) return 0;
+void pad_site_blocks(HostTables& h) {
+    while (h.site.size() % 8) h.site.push_back({-1, 0});  // the Lloyd launch walks 8 XCD ranges
+}
+
+// One iteration of the job's 1-job batch (graph-capturable: no host sync, no memset).
+void enqueue_iteration(cvt_job* j) { launch_batch_iteration(j->tables, j->p.K, j->p.kproj, j->stream); }
+
+int ensure_graph(cvt_job* j) {
+    if (j->graph) return 0;
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(j->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_iteration(j, parity);
+    enqueue_iteration(j);
     HIPCHK(hipStreamEndCapture(j->stream, &g));
-    hipError_t e = hipGraphInstantiate(&j->graph[parity], g, nullptr, nullptr, 0);
+    hipError_t e = hipGraphInstantiate(&j->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) return fail(CVT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
     return 0;
@@ -456,8 +473,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.4;
     const double rK = std::sqrt((p.K + 2) / (kPi * (nS / j->area)));
     const double rV = std::sqrt((p.kproj + 2) / (kPi * (nV / j->area)));
-    j->sgp = make_grid(glo, ghi, ss * rK, 16L * nS + (1L << 20));
-    j->vg = make_grid(glo, ghi, vs * rV, 16L * nV + (1L << 20));
+    j->sgp = make_grid(glo, ghi, ss * rK, 32L * nS + 4096);
+    j->vg = make_grid(glo, ghi, vs * rV, 32L * nV + 4096);
 
     if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
@@ -470,7 +487,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
         (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
-        (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) || (rc = j->alloc(&j->dmax, 2)))
+        (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {
         if ((rc = j->alloc(&j->dbg_nn, nS)) || (rc = j->alloc(&j->dbg_U, nS)) || (rc = j->alloc(&j->dbg_V, nS)) ||
@@ -493,12 +511,29 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         hipMemsetAsync(j->tier, 0, nS, s))
         return bail(fail(CVT_EHIP, "memset failed"));
     launch_fill(j->has_prev, 1, 0, s);
+    launch_fill(j->parity, 1, 0, s);
     launch_fill(j->seed_ok, 1, 0, s);
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatWords, 0, s);
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
+    {
+        // this job as a batch of one
+        JobDesc dh = make_desc(j);
+        HostTables ht;
+        add_job_blocks(ht, 0, nS, j->sgp.ncell + 1);
+        pad_site_blocks(ht);
+        if ((rc = j->alloc(&j->site_blocks, ht.site.size())) || (rc = j->alloc(&j->cell_blocks, ht.cell.size())) ||
+            (rc = j->alloc(&j->tile_blocks, ht.tile.size())))
+            return bail(rc);
+        if (up(j->desc, &dh, sizeof dh) || up(j->site_blocks, ht.site.data(), sizeof(BlockRef) * ht.site.size()) ||
+            up(j->cell_blocks, ht.cell.data(), sizeof(BlockRef) * ht.cell.size()) ||
+            up(j->tile_blocks, ht.tile.data(), sizeof(BlockRef) * ht.tile.size()))
+            return bail(fail(CVT_EHIP, "upload failed"));
+        j->tables = BatchTables{j->desc, 1, j->site_blocks, (int)ht.site.size(), j->cell_blocks,
+                                (int)ht.cell.size(), j->tile_blocks, (int)ht.tile.size()};
+    }
     hipError_t e2 = hipStreamSynchronize(s);
     if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup: %s", hipGetErrorString(e2)));
     e2 = hipGetLastError();
@@ -559,7 +594,7 @@ static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
 
 static int run_iters(cvt_job* j, int iters, double* ms) {
     int rc;
-    if ((rc = ensure_graph(j, 0)) || (rc = ensure_graph(j, 1))) return rc;
+    if ((rc = ensure_graph(j))) return rc;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (ms) {
         HIPCHK(hipEventCreate(&e0));
@@ -567,8 +602,8 @@ static int run_iters(cvt_job* j, int iters, double* ms) {
         HIPCHK(hipEventRecord(e0, j->stream));
     }
     for (int it = 0; it < iters; ++it) {
-        HIPCHK(hipGraphLaunch(j->graph[j->cur], j->stream));
-        j->cur ^= 1;
+        HIPCHK(hipGraphLaunch(j->graph, j->stream));
+        j->cur ^= 1;  // host mirror of the device parity flag
     }
     if (ms) HIPCHK(hipEventRecord(e1, j->stream));
     HIPCHK(hipStreamSynchronize(j->stream));
@@ -611,16 +646,12 @@ int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, do
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     double g = 0, l = 0, t = 0;
     for (int it = 0; it < iters; ++it) {
-        const int par = j->cur;
         HIPCHK(hipEventRecord(ev[0], s));
-        launch_fill(j->stats, kStatWords, 0, s);
-        launch_fill((int*)(j->dmax + par), 1, 0, s);
-        launch_grid_build(j->S[par], j->nS, j->sgp, j->sgb, s);
+        launch_batch_grid(j->tables, 0, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        StepArgs a = make_args(j, par, 0);
-        launch_lloyd(a, j->p.K, j->p.kproj, s);
+        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s);
         HIPCHK(hipEventRecord(ev[2], s));
-        launch_iter_end(j->stats, j->hist, j->hist_count, kHistCap, j->has_prev, j->seed_ok, s);
+        launch_batch_end(j->tables, s);
         HIPCHK(hipEventRecord(ev[3], s));
         HIPCHK(hipEventSynchronize(ev[3]));
         float f01 = 0, f12 = 0, f03 = 0;
@@ -746,10 +777,7 @@ int cvt_job_energy(cvt_job* j, double* E) {
     if (!j || !E) return fail(CVT_EINVAL, "null argument");
     if (!j->p.debug) return fail(CVT_ESTATE, "energy evaluation needs params.debug (per-site moment buffer)");
     hipStream_t s = j->stream;
-    launch_fill(j->stats, kStatWords, 0, s);
-    launch_grid_build(j->S[j->cur], j->nS, j->sgp, j->sgb, s);
-    StepArgs a = make_args(j, j->cur, 1);
-    launch_lloyd(a, j->p.K, j->p.kproj, s);
+    launch_batch_eval(j->tables, j->p.K, j->p.kproj, s);
     HIPCHK(hipStreamSynchronize(s));
     std::vector<float2> am(j->nS);
     HIPCHK(hipMemcpy(am.data(), j->dbg_am, sizeof(float2) * j->nS, hipMemcpyDeviceToHost));
@@ -793,10 +821,8 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
         cleanup();
         return fail(CVT_ENOMEM, "remesh temporaries");
     }
-    launch_fill(j->stats, kStatWords, 0, s);
-    launch_grid_build(j->S[j->cur], nS, j->sgp, j->sgb, s);
-    StepArgs a = make_args(j, j->cur, 1);
-    launch_remesh_cells(a, K, knn_tmp, keys, site_n, s);
+    launch_batch_grid(j->tables, 1, s);
+    launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
     radix_sort_u64(keys, (int)nk, 64, keys_alt, hist, hsum, s);
     launch_unique_mark(keys, (int)nk, flag, s);
     launch_scan(flag, (int)nk, pos, hsum, s);
@@ -825,6 +851,125 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
 }
 
 void cvt_free(void* p) { std::free(p); }
+
+int cvt_job_stats(cvt_job* j, int last_n, cvt_iter_stats* out) {
+    if (!j || last_n < 0 || (last_n > 0 && !out)) return fail(CVT_EINVAL, "bad argument");
+    int count = 0;
+    HIPCHK(hipMemcpy(&count, j->hist_count, sizeof(int), hipMemcpyDeviceToHost));
+    const int n = std::min(last_n, std::min(count, kHistCap));
+    for (int k = 0; k < last_n; ++k) std::memset(&out[k], 0, sizeof(cvt_iter_stats));
+    if (n == 0) return 0;
+    return read_stats(j, std::min(count, kHistCap) - n, n, out + (last_n - n));
+}
+
+// ------------------------------------------------------------------ batches
+struct cvt_batch {
+    std::vector<cvt_job*> jobs;
+    int K = 0, kproj = 0;
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocs;
+    BatchTables tables{};
+    hipGraphExec_t graph = nullptr;
+    ~cvt_batch() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
+    if (!out || !jobs || njobs < 1) return fail(CVT_EINVAL, "bad argument");
+    *out = nullptr;
+    const int K = jobs[0] ? jobs[0]->p.K : 0, kp = jobs[0] ? jobs[0]->p.kproj : 0;
+    for (int k = 0; k < njobs; ++k) {
+        if (!jobs[k]) return fail(CVT_EINVAL, "null job at %d", k);
+        if (jobs[k]->p.K != K || jobs[k]->p.kproj != kp) return fail(CVT_EINVAL, "jobs of a batch must share K and kproj");
+        for (int q = 0; q < k; ++q)
+            if (jobs[q] == jobs[k]) return fail(CVT_EINVAL, "job %d appears twice", k);
+    }
+    cvt_batch* b = new cvt_batch();
+    auto bail = [&](int rc) { delete b; return rc; };
+    b->jobs.assign(jobs, jobs + njobs);
+    b->K = K;
+    b->kproj = kp;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
+    std::vector<JobDesc> descs(njobs);
+    HostTables ht;
+    for (int k = 0; k < njobs; ++k) {
+        descs[k] = make_desc(jobs[k]);
+        add_job_blocks(ht, k, jobs[k]->nS, jobs[k]->sgp.ncell + 1);
+    }
+    pad_site_blocks(ht);
+    auto dalloc = [&](size_t bytes) -> void* {
+        void* p = nullptr;
+        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        b->allocs.push_back(p);
+        return p;
+    };
+    auto* dd = (JobDesc*)dalloc(sizeof(JobDesc) * njobs);
+    auto* sb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.site.size());
+    auto* cb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.cell.size());
+    auto* tb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.tile.size());
+    if (!dd || !sb || !cb || !tb) return bail(fail(CVT_ENOMEM, "batch tables"));
+    if (hipMemcpy(dd, descs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice) ||
+        hipMemcpy(sb, ht.site.data(), sizeof(BlockRef) * ht.site.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(cb, ht.cell.data(), sizeof(BlockRef) * ht.cell.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(tb, ht.tile.data(), sizeof(BlockRef) * ht.tile.size(), hipMemcpyHostToDevice))
+        return bail(fail(CVT_EHIP, "batch table upload failed"));
+    b->tables = BatchTables{dd, njobs, sb, (int)ht.site.size(), cb, (int)ht.cell.size(), tb, (int)ht.tile.size()};
+    hipGraph_t g = nullptr;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return bail(fail(CVT_EHIP, "capture begin failed"));
+    launch_batch_iteration(b->tables, K, kp, b->stream);
+    if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return bail(fail(CVT_EHIP, "capture end failed"));
+    hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return bail(fail(CVT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e)));
+    *out = b;
+    return 0;
+}
+
+static int batch_run(cvt_batch* b, int iters, double* ms) {
+    for (cvt_job* j : b->jobs) HIPCHK(hipStreamSynchronize(j->stream));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ms) {
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, b->stream));
+    }
+    for (int it = 0; it < iters; ++it) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    if (ms) HIPCHK(hipEventRecord(e1, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (ms) {
+        float f = 0;
+        HIPCHK(hipEventElapsedTime(&f, e0, e1));
+        *ms = f;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    for (cvt_job* j : b->jobs) {
+        j->cur ^= (iters & 1);
+        j->iters_done += iters;
+    }
+    return 0;
+}
+
+int cvt_batch_step(cvt_batch* b, int iters) {
+    if (!b || iters < 0) return fail(CVT_EINVAL, "bad argument");
+    return iters ? batch_run(b, iters, nullptr) : 0;
+}
+
+int cvt_batch_step_timed(cvt_batch* b, int iters, double* ms) {
+    if (!b || iters <= 0 || !ms) return fail(CVT_EINVAL, "bad argument");
+    return batch_run(b, iters, ms);
+}
+
+void cvt_batch_destroy(cvt_batch* b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
 
 int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
     if (!j || (!P && n > 0) || (!d2 && n > 0) || n < 0) return fail(CVT_EINVAL, "bad argument");

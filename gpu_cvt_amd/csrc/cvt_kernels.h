@@ -28,23 +28,26 @@ struct SiteGridDev {
     const float4* pts;    // cell-sorted sites (x, y, z, site id bits)
 };
 
+// Per-job pointers and parameters; lives in device memory (one per job) and is
+// read with uniform (scalar) loads by every block working on that job.
 struct StepArgs {
     MeshDev m;
     SiteGridDev sg;
-    const float4* S;      // current sites (by site id)
-    float4* Snew;         // next sites
+    float4* S01[2];       // site buffers; *parity selects the current one
+    int* parity;          // device flag, flipped at the end of every iteration
     int* knn;             // nS*K, in place: previous lists in, new lists out
     int* knnV;            // nS*kproj pinned vertex lists
     unsigned char* frozen;
     unsigned char* streak;
     const unsigned char* tier;
-    const int* has_prev;  // device flag
+    int* has_prev;        // device flag
     int* stats;           // kStatWords counters of this iteration
+    int* hist;            // kHistCap * kStatWords history of stats
+    int* hist_count;
     int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
     float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
-    const int* seed_ok;   // device flag: rad2 and dmax_prev are valid for this iteration
-    const float* dmax_prev;  // largest site displacement of the previous iteration
-    float* dmax_cur;         // this iteration's (atomic max, float bits)
+    int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
+    float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     int nS;
     float R;
     int freeze_mode;
@@ -52,7 +55,6 @@ struct StepArgs {
     int need[6];
     int nan_policy;
     int write_knnv;
-    int eval_only;        // 1: no state writes (energy evaluation)
     // optional per-site debug outputs (null when off)
     int* dbg_nn;
     float4* dbg_U;
@@ -60,6 +62,8 @@ struct StepArgs {
     float4* dbg_cent;
     float2* dbg_am;
 };
+
+constexpr int kHistCap = 4096;
 
 // Work buffers of one grid build.
 struct GridBufs {
@@ -71,6 +75,35 @@ struct GridBufs {
     float4* pts;  // n   cell-sorted (x, y, z, id bits)
 };
 
+// A job inside a batch: its step arguments plus its site-grid work buffers.
+struct JobDesc {
+    StepArgs a;
+    int* count;   // ncell+2
+    int* start;   // ncell+2
+    int* bsum;
+    int* bucket;
+    int* rank;
+    int ncell1;   // ncell + 1 (cells + the non-finite bucket)
+};
+
+// blockIdx -> (job, block within that job's range); job < 0: padding block.
+struct BlockRef {
+    int job;
+    int blk;
+};
+
+// Host-built launch tables of one batch (device pointers + block counts).
+struct BatchTables {
+    const JobDesc* descs;
+    int njobs;
+    const BlockRef* site_blocks;   // 256 sites per block, padded to a multiple of 8
+    int n_site_blocks;
+    const BlockRef* cell_blocks;   // 256 count entries per block
+    int n_cell_blocks;
+    const BlockRef* tile_blocks;   // scan tiles of kScanTile entries
+    int n_tile_blocks;
+};
+
 void launch_fill(int* p, long n, int v, hipStream_t s);
 // Exclusive scan: out[0..n-1] = prefix sums of in, out[n] = total.  bsum needs
 // scan_partials(n) ints.
@@ -80,13 +113,21 @@ void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipS
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
 bool lloyd_supported(int K, int kproj);
-void launch_lloyd(const StepArgs& a, int K, int kproj, hipStream_t s);
-void launch_remesh_cells(const StepArgs& a, int K, int* knn_tmp, unsigned long long* keys, float4* site_n,
-                         hipStream_t s);
+int scan_tile();
+// One Lloyd iteration of every job of a batch (all jobs share K and kproj).
+void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s);
+// The fused step alone over a batch, without state writes (energy / debug evaluation).
+void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s);
+// Pieces of an iteration, for per-stage timing.
+void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s);
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s);
+void launch_batch_end(const BatchTables& bt, hipStream_t s);
+void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
+                         float4* site_n, hipStream_t s);
 void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s);
 void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
                        const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);
 void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s);
-void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, int* seed_ok, hipStream_t s);
+
 
 }  // namespace cvt

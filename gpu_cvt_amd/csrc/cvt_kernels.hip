@@ -205,10 +205,12 @@ struct SiteFlags {
 // carried: for the precise pass, -1 or the `same` flag the fast pass computed
 // before it wrote the site's KNN row (the row then already holds the new list).
 template <int K, int KP, bool PRECISE>
-__device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool live, int carried) {
+__device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool eval_only, int t, bool live,
+                                                int carried) {
     SiteFlags r{};
-    const bool eval_only = a.eval_only != 0;
     const int has_prev = eval_only ? 0 : *a.has_prev;
+    const float4* __restrict__ S = a.S01[par];
+    float4* __restrict__ Snew = a.S01[par ^ 1];
     int i = 0;
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
@@ -256,8 +258,8 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
         // provable seed: the previous (K+2)-th distance grown by twice the
         // largest site displacement of the previous iteration
         float tau = inf;
-        if (!PRECISE && a.seed_ok && *a.seed_ok) {
-            const float r0 = sqrtf(a.rad2[i]) + 2.f * a.dmax_prev[0];
+        if (!PRECISE && !eval_only && *a.seed_ok) {
+            const float r0 = sqrtf(a.rad2[i]) + 2.f * a.dmax01[par ^ 1];
             tau = r0 * r0 * (1.f + 1e-5f);
         }
         float lk;
@@ -281,7 +283,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
                     row[k] = id;
                 }
             } else {
-                pj = a.S[row[k]];
+                pj = S[row[k]];
             }
             const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
             const float aa = dx * u.x + dy * u.y + dz * u.z;
@@ -386,7 +388,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     r.done = true;
     r.frozen_after = valid && fz;
     if (valid && !eval_only) {
-        a.Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
+        Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
         r.disp = sqrtf((sp.x - q.x) * (sp.x - q.x) + (sp.y - q.y) * (sp.y - q.y) + (sp.z - q.z) * (sp.z - q.z));
         if (has_prev) {
             const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
@@ -410,7 +412,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
         }
     } else if (live && !valid && !eval_only) {
         // non-finite site: keep it (it is excluded from every neighbour list)
-        a.Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
+        Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
     }
     r.cell_fail = valid && !cell_ok;
     r.nonfinite = live && !finite3(sp.x, sp.y, sp.z);
@@ -424,14 +426,15 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int t, bool l
     return r;
 }
 
-__device__ __forceinline__ void count_site(const StepArgs& a, const SiteFlags& f, bool live) {
-    const bool hp = a.eval_only == 0 && *a.has_prev != 0;
+__device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval_only, const SiteFlags& f, bool live) {
+    if (eval_only) return;
+    const bool hp = *a.has_prev != 0;
     const bool d = live && f.done;
     // largest displacement of this iteration (seeds the next iteration's searches)
     float m = (d && isfinite(f.disp)) ? f.disp : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0.f && a.dmax_cur) atomicMax((unsigned*)a.dmax_cur, __float_as_uint(m));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax((unsigned*)(a.dmax01 + par), __float_as_uint(m));
     wave_count(a.stats, 0, d && hp && f.same);
     wave_count(a.stats, 1, d && hp && f.low);
     wave_count(a.stats, 2, d && hp && f.both);
@@ -443,34 +446,162 @@ __device__ __forceinline__ void count_site(const StepArgs& a, const SiteFlags& f
     wave_count(a.stats, 8, d && f.nonfinite);
 }
 
-// Fast pass over all sites (bucket-sorted order, XCD-contiguous ranges).
+// Fast pass over all sites of all jobs of a batch (cell-sorted order within a
+// job; the block table is walked XCD-contiguously).
 template <int K, int KP>
-__global__ __launch_bounds__(256) void k_lloyd(StepArgs a) {
-    const int lb = xcd_block(blockIdx.x, gridDim.x);
-    const int t = lb * 256 + (int)threadIdx.x;
+__global__ __launch_bounds__(256) void kb_lloyd(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
+                                                int eval_only) {
+    const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int par = *a.parity;
+    const int t = br.blk * 256 + (int)threadIdx.x;
     const bool live = t < a.nS;
-    const SiteFlags f = lloyd_site<K, KP, false>(a, t, live, -1);
+    const SiteFlags f = lloyd_site<K, KP, false>(a, par, eval_only != 0, t, live, -1);
     const bool deferred = live && !f.done;
     if (deferred) {
         const int pos = atomicAdd(&a.stats[kStatDeferred], 1);
         a.defer_list[pos] = t | (f.row_written ? (1 << 30) : 0) | (f.same ? (1 << 29) : 0);
     }
-    count_site(a, f, live);
+    count_site(a, par, eval_only != 0, f, live);
 }
 
-// Precise pass over the deferred sites only (FP64 where FP32 was inconclusive).
+// Precise pass over each job's deferred sites (FP64 where FP32 was
+// inconclusive); kPreciseBlocks blocks per job stride over its list.
+constexpr int kPreciseBlocks = 4;
+
 template <int K, int KP>
-__global__ __launch_bounds__(256) void k_lloyd_precise(StepArgs a) {
+__global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restrict__ descs, int eval_only) {
+    const StepArgs& a = descs[blockIdx.x / kPreciseBlocks].a;
+    const int par = *a.parity;
     const int n = a.stats[kStatDeferred];
-    for (int base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+    for (int base = (blockIdx.x % kPreciseBlocks) * 256; base < n; base += kPreciseBlocks * 256) {
         const int idx = base + (int)threadIdx.x;
         const bool live = idx < n;
         const int e = live ? a.defer_list[idx] : 0;
         const int t = e & ((1 << 29) - 1);
         const int carried = (e & (1 << 30)) ? ((e >> 29) & 1) : -1;
-        const SiteFlags f = lloyd_site<K, KP, true>(a, t, live, carried);
-        count_site(a, f, live);
+        const SiteFlags f = lloyd_site<K, KP, true>(a, par, eval_only != 0, t, live, carried);
+        count_site(a, par, eval_only != 0, f, live);
     }
+}
+
+// ------------------------------------------------------ batched grid rebuild
+__global__ void kb_begin(const JobDesc* __restrict__ descs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const StepArgs& a = descs[j].a;
+    for (int k = 0; k < kStatWords; ++k) a.stats[k] = 0;
+    a.dmax01[*a.parity] = 0.f;
+}
+
+__global__ void kb_fill_count(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const JobDesc& d = descs[br.job];
+    const int c = br.blk * 256 + (int)threadIdx.x;
+    if (c < d.ncell1) d.count[c] = 0;
+}
+
+__global__ void kb_grid_count(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const JobDesc& d = descs[br.job];
+    const StepArgs& a = d.a;
+    const int i = br.blk * 256 + (int)threadIdx.x;
+    if (i >= a.nS) return;
+    const float4 p = a.S01[*a.parity][i];
+    const GridP& g = a.sg.g;
+    int b = g.ncell;
+    if (finite3(p.x, p.y, p.z)) {
+        int cx, cy, cz;
+        cell_of(g, p.x, p.y, p.z, cx, cy, cz);
+        b = cell_index(g, cx, cy, cz);
+    }
+    d.bucket[i] = b;
+    d.rank[i] = atomicAdd(&d.count[b], 1);
+}
+
+__global__ __launch_bounds__(256) void kb_scan_reduce(const JobDesc* __restrict__ descs,
+                                                      const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const JobDesc& d = descs[br.job];
+    const int base = br.blk * kScanTile + threadIdx.x;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k * 256;
+        s += (j < d.ncell1) ? d.count[j] : 0;
+    }
+    int total;
+    block_excl_scan(s, total);
+    if (threadIdx.x == 0) d.bsum[br.blk] = total;
+}
+
+__global__ __launch_bounds__(256) void kb_scan_bsum(const JobDesc* __restrict__ descs) {
+    const JobDesc& d = descs[blockIdx.x];
+    const int nb = (d.ncell1 + kScanTile - 1) / kScanTile;
+    int carry = 0;
+    for (int c = 0; c < nb; c += 256) {
+        const int j = c + threadIdx.x;
+        const int v = (j < nb) ? d.bsum[j] : 0;
+        int total;
+        const int ex = block_excl_scan(v, total);
+        if (j < nb) d.bsum[j] = carry + ex;
+        carry += total;
+    }
+    if (threadIdx.x == 0) d.start[d.ncell1] = carry;
+}
+
+__global__ __launch_bounds__(256) void kb_scan_down(const JobDesc* __restrict__ descs,
+                                                    const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const JobDesc& d = descs[br.job];
+    const int base = br.blk * kScanTile + threadIdx.x * kScanItems;
+    int v[kScanItems];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k;
+        v[k] = (j < d.ncell1) ? d.count[j] : 0;
+        s += v[k];
+    }
+    int total;
+    int run = block_excl_scan(s, total) + d.bsum[br.blk];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+        const int j = base + k;
+        if (j < d.ncell1) d.start[j] = run;
+        run += v[k];
+    }
+}
+
+__global__ void kb_grid_scatter(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const JobDesc& d = descs[br.job];
+    const StepArgs& a = d.a;
+    const int i = br.blk * 256 + (int)threadIdx.x;
+    if (i >= a.nS) return;
+    const float4 p = a.S01[*a.parity][i];
+    const int pos = d.start[d.bucket[i]] + d.rank[i];
+    const_cast<float4*>(a.sg.pts)[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+}
+
+// End of an iteration: stats history, flags, buffer parity.
+__global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= njobs) return;
+    const StepArgs& a = descs[j].a;
+    const int c = *a.hist_count;
+    const int slot = c < kHistCap ? c : kHistCap - 1;
+    for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
+    *a.hist_count = c + 1;
+    *a.has_prev = 1;
+    *a.seed_ok = 1;
+    *a.parity ^= 1;
 }
 
 // ------------------------------------------------------------ remesh stage
@@ -487,10 +618,11 @@ __device__ __forceinline__ unsigned long long pack3(int a, int b, int c) {
 }
 
 template <int K>
-__global__ __launch_bounds__(256) void k_remesh_cells(StepArgs a, int* __restrict__ knn_tmp,
+__global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict__ desc, int* __restrict__ knn_tmp,
                                                       unsigned long long* __restrict__ keys,
                                                       float4* __restrict__ site_n) {
     constexpr int M = K + 4;
+    const StepArgs& a = desc->a;
     const int t = blockIdx.x * 256 + (int)threadIdx.x;
     if (t >= a.nS) return;
     const float4 q = a.sg.pts[t];
@@ -589,18 +721,18 @@ __global__ void k_tri_compact(const int4* __restrict__ tri, const int* __restric
     out[3 * p + 2] = t.z;
 }
 
-void launch_remesh_cells(const StepArgs& a, int K, int* knn_tmp, unsigned long long* keys, float4* site_n,
-                         hipStream_t s) {
-    const dim3 g(nblk(a.nS, 256)), b(256);
+void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
+                         float4* site_n, hipStream_t s) {
+    const dim3 g(nblk(nS, 256)), b(256);
     switch (K) {
-        case 4: hipLaunchKernelGGL(k_remesh_cells<4>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 6: hipLaunchKernelGGL(k_remesh_cells<6>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 8: hipLaunchKernelGGL(k_remesh_cells<8>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 12: hipLaunchKernelGGL(k_remesh_cells<12>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 16: hipLaunchKernelGGL(k_remesh_cells<16>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 20: hipLaunchKernelGGL(k_remesh_cells<20>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 24: hipLaunchKernelGGL(k_remesh_cells<24>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
-        case 32: hipLaunchKernelGGL(k_remesh_cells<32>, g, b, 0, s, a, knn_tmp, keys, site_n); break;
+        case 4: hipLaunchKernelGGL(k_remesh_cells<4>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 6: hipLaunchKernelGGL(k_remesh_cells<6>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 8: hipLaunchKernelGGL(k_remesh_cells<8>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 12: hipLaunchKernelGGL(k_remesh_cells<12>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 16: hipLaunchKernelGGL(k_remesh_cells<16>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 20: hipLaunchKernelGGL(k_remesh_cells<20>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 24: hipLaunchKernelGGL(k_remesh_cells<24>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        case 32: hipLaunchKernelGGL(k_remesh_cells<32>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
         default: break;
     }
 }
@@ -617,17 +749,6 @@ void launch_tri_finish(const unsigned long long* keys, const int* flag, const in
 
 void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s) {
     hipLaunchKernelGGL(k_tri_compact, dim3(nblk(n, 256)), dim3(256), 0, s, tri, keep, pos, n, out);
-}
-
-__global__ void k_iter_end(int* __restrict__ stats, int* __restrict__ hist, int* __restrict__ hist_count, int cap,
-                           int* __restrict__ has_prev, int* __restrict__ seed_ok) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    const int c = *hist_count;
-    const int slot = c < cap ? c : cap - 1;
-    for (int k = 0; k < kStatWords; ++k) hist[slot * kStatWords + k] = stats[k];
-    *hist_count = c + 1;
-    *has_prev = 1;
-    *seed_ok = 1;
 }
 
 // ------------------------------------------------------------------ launches
@@ -658,14 +779,7 @@ void launch_normals(const float4* V, const int* F, const int* vf_off, const int*
     hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nV, 256)), dim3(256), 0, s, V, F, vf_off, vf_faces, nV, N);
 }
 
-template <int K>
-static void launch_lloyd_k(const StepArgs& a, int kproj, hipStream_t s) {
-    const int nb8 = ((nblk(a.nS, 256) + 7) / 8) * 8;
-    (void)kproj;
-    hipLaunchKernelGGL((k_lloyd<K, 5>), dim3(nb8), dim3(256), 0, s, a);
-    const int nbp = std::min(nblk(a.nS, 256), 512);
-    hipLaunchKernelGGL((k_lloyd_precise<K, 5>), dim3(nbp), dim3(256), 0, s, a);
-}
+int scan_tile() { return kScanTile; }
 
 bool lloyd_supported(int K, int kproj) {
     if (kproj != 5) return false;
@@ -675,22 +789,54 @@ bool lloyd_supported(int K, int kproj) {
     }
 }
 
-void launch_lloyd(const StepArgs& a, int K, int kproj, hipStream_t s) {
+void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
+    if (!eval_only) hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+    hipLaunchKernelGGL(kb_fill_count, dim3(bt.n_cell_blocks), dim3(256), 0, s, bt.descs, bt.cell_blocks);
+    hipLaunchKernelGGL(kb_grid_count, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
+    hipLaunchKernelGGL(kb_scan_reduce, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
+    hipLaunchKernelGGL(kb_scan_bsum, dim3(bt.njobs), dim3(256), 0, s, bt.descs);
+    hipLaunchKernelGGL(kb_scan_down, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
+    hipLaunchKernelGGL(kb_grid_scatter, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
+}
+
+template <int K>
+static void launch_lloyd_k(const BatchTables& bt, int eval_only, hipStream_t s) {
+    hipLaunchKernelGGL((kb_lloyd<K, 5>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
+                       eval_only);
+    hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), dim3(bt.njobs * kPreciseBlocks), dim3(256), 0, s, bt.descs,
+                       eval_only);
+}
+
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s) {
+    (void)kproj;
     switch (K) {
-        case 4: launch_lloyd_k<4>(a, kproj, s); break;
-        case 6: launch_lloyd_k<6>(a, kproj, s); break;
-        case 8: launch_lloyd_k<8>(a, kproj, s); break;
-        case 12: launch_lloyd_k<12>(a, kproj, s); break;
-        case 16: launch_lloyd_k<16>(a, kproj, s); break;
-        case 20: launch_lloyd_k<20>(a, kproj, s); break;
-        case 24: launch_lloyd_k<24>(a, kproj, s); break;
-        case 32: launch_lloyd_k<32>(a, kproj, s); break;
+        case 4: launch_lloyd_k<4>(bt, eval_only, s); break;
+        case 6: launch_lloyd_k<6>(bt, eval_only, s); break;
+        case 8: launch_lloyd_k<8>(bt, eval_only, s); break;
+        case 12: launch_lloyd_k<12>(bt, eval_only, s); break;
+        case 16: launch_lloyd_k<16>(bt, eval_only, s); break;
+        case 20: launch_lloyd_k<20>(bt, eval_only, s); break;
+        case 24: launch_lloyd_k<24>(bt, eval_only, s); break;
+        case 32: launch_lloyd_k<32>(bt, eval_only, s); break;
         default: break;
     }
 }
 
-void launch_iter_end(int* stats, int* hist, int* hist_count, int cap, int* has_prev, int* seed_ok, hipStream_t s) {
-    hipLaunchKernelGGL(k_iter_end, dim3(1), dim3(64), 0, s, stats, hist, hist_count, cap, has_prev, seed_ok);
+void launch_batch_end(const BatchTables& bt, hipStream_t s) {
+    hipLaunchKernelGGL(kb_end, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+}
+
+void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s) {
+    launch_batch_grid(bt, 0, s);
+    launch_batch_lloyd(bt, K, kproj, 0, s);
+    launch_batch_end(bt, s);
+}
+
+void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s) {
+    // eval_only: the stats words are reused for the deferred count only
+    hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+    launch_batch_grid(bt, 1, s);
+    launch_batch_lloyd(bt, K, kproj, 1, s);
 }
 
 }  // namespace cvt

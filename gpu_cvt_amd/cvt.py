@@ -25,7 +25,8 @@ EXPORTS = [
     "cvt_job_info_get", "cvt_job_step", "cvt_job_step_timed", "cvt_job_get_sites", "cvt_job_get_knn",
     "cvt_job_get_knnv", "cvt_job_get_freeze", "cvt_job_get_normals", "cvt_job_set_state", "cvt_job_set_tiers",
     "cvt_job_get_debug", "cvt_job_energy", "cvt_job_point_mesh_d2", "cvt_job_profile",
-    "cvt_bvh_create", "cvt_bvh_point_d2", "cvt_bvh_destroy", "cvt_job_remesh", "cvt_free",
+    "cvt_bvh_create", "cvt_bvh_point_d2", "cvt_bvh_destroy", "cvt_job_remesh", "cvt_free", "cvt_job_stats",
+    "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy",
 ]
 
 
@@ -123,6 +124,12 @@ def lib():
     L.cvt_job_point_mesh_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
+    L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.cvt_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+    L.cvt_batch_step.argtypes = [C.c_void_p, C.c_int]
+    L.cvt_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
+    L.cvt_batch_destroy.argtypes = [C.c_void_p]
+    L.cvt_batch_destroy.restype = None
     L.cvt_job_remesh.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.POINTER(C.c_int)), C.POINTER(C.c_int)]
     L.cvt_free.argtypes = [C.c_void_p]
     L.cvt_free.restype = None
@@ -278,6 +285,12 @@ class Job:
         _check("cvt_job_energy", lib().cvt_job_energy(self.handle, C.byref(e)))
         return e.value
 
+    def last_stats(self, n: int = 1):
+        """The last n per-iteration records (oldest first)."""
+        arr = (IterStats * max(n, 1))()
+        _check("cvt_job_stats", lib().cvt_job_stats(self.handle, n, C.cast(arr, C.c_void_p)))
+        return [arr[i].as_dict() for i in range(n)]
+
     def remesh(self, eps: float = 1e-6) -> np.ndarray:
         """Dual triangles of the current cells (int32 (m, 3)), vertices = sites()."""
         fp = C.POINTER(C.c_int)()
@@ -343,3 +356,39 @@ def hausdorff(V1, F1, V2, F2, n_samples: int, seed: int, symmetric: bool = True)
         h = max(h, float(np.sqrt(b1.point_d2(P2).max())))
         b1.close()
     return h
+
+
+class Batch:
+    """Jobs advanced together by one graph launch per iteration."""
+
+    def __init__(self, jobs):
+        self.jobs = list(jobs)
+        arr = (C.c_void_p * len(self.jobs))(*[j.handle for j in self.jobs])
+        h = C.c_void_p()
+        _check("cvt_batch_create", lib().cvt_batch_create(arr, len(self.jobs), C.byref(h)))
+        self._h = h
+
+    def step(self, iters: int = 1) -> None:
+        _check("cvt_batch_step", lib().cvt_batch_step(self._h, iters))
+
+    def step_timed(self, iters: int) -> float:
+        ms = C.c_double()
+        _check("cvt_batch_step_timed", lib().cvt_batch_step_timed(self._h, iters, C.byref(ms)))
+        return ms.value
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cvt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

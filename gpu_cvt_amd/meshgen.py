@@ -233,3 +233,38 @@ def workload(name: str):
         S = sample_surface(V, F, 1_000_000, seed=10)
         return V, F, S, 24, 100, False
     raise KeyError(name)
+
+
+def batch_workload(njobs: int = 256, seed: int = 7, nf_range=(5_000, 400_000), ns_range=(1_000, 100_000)):
+    """Mixed jobs: mesh type uniform over {icosphere, torus, displaced sphere},
+    triangle count log-uniform in nf_range, site count log-uniform in ns_range
+    capped at triangles/2.  Meshes are shared between jobs whose (type, size)
+    round to the same tessellation; sites are sampled per job.
+    Returns a list of (V, F, S, tag)."""
+    rng = _rng(seed)
+    cache = {}
+    jobs = []
+    for k in range(njobs):
+        typ = int(rng.integers(0, 3))
+        nf = float(np.exp(rng.uniform(np.log(nf_range[0]), np.log(nf_range[1]))))
+        ns = float(np.exp(rng.uniform(np.log(ns_range[0]), np.log(ns_range[1]))))
+        if typ == 0:
+            lvl = int(np.clip(round(np.log(nf / 20.0) / np.log(4.0)), 4, 7))
+            key = ("ico", lvl)
+            if key not in cache:
+                cache[key] = icosphere(lvl)
+        elif typ == 1:
+            nv = max(8, int(round(np.sqrt(nf / 5.0) / 4.0)) * 4)
+            key = ("torus", nv)
+            if key not in cache:
+                cache[key] = torus(1.0, 0.3, int(2.5 * nv), nv, jitter=1e-3, seed=nv)
+        else:
+            n = max(8, int(round(np.sqrt(nf / 12.0) / 8.0)) * 8)
+            key = ("dsphere", n)
+            if key not in cache:
+                cache[key] = displaced_sphere(n, seed=n, crease_seed=n + 1)
+        V, F = cache[key]
+        nsites = int(min(ns, len(F) / 2))
+        S = sample_surface(V, F, nsites, seed=seed * 100003 + k)
+        jobs.append((V, F, S, f"{key[0]}{key[1]}"))
+    return jobs

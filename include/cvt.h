@@ -151,6 +151,22 @@ void cvt_free(void* p);
  * the CPU BVH of evaluate.cuh:114-201).  out: n doubles. */
 int cvt_job_point_mesh_d2(cvt_job* job, const double* P, int n, double* d2_out);
 
+/* The last `last_n` per-iteration records of a job, oldest first (records
+ * beyond the job's history are zero-filled at the front). */
+int cvt_job_stats(cvt_job* job, int last_n, cvt_iter_stats* out);
+
+/* Batched form: many jobs (same K and kproj, any sizes and meshes) advance
+ * together, one HIP graph launch per iteration whose kernels walk a device
+ * table of jobs.  The reference runs one mesh per process (main.cu:371);
+ * this has no counterpart there.  Each job's result is bit-identical to
+ * running it alone.  The caller keeps ownership of the jobs, which must
+ * outlive the batch; jobs may still be stepped alone between batch steps. */
+typedef struct cvt_batch cvt_batch;
+int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out);
+int cvt_batch_step(cvt_batch* batch, int iters);
+int cvt_batch_step_timed(cvt_batch* batch, int iters, double* device_ms);
+void cvt_batch_destroy(cvt_batch* batch);
+
 /* Standalone device LBVH over any triangle mesh (e.g. a remeshed surface),
  * for two-sided Hausdorff distances.  Replaces build_bvh (evaluate.cuh:175-201)
  * + hausdorff_bvh_gpu (evaluate.cuh:291-357); the max over the returned

@@ -1,0 +1,71 @@
+"""Batched form: every job's result in a batch is bit-identical to running it
+alone (sites, KNN lists, freeze state), including when batch steps and solo
+steps are interleaved."""
+import numpy as np
+import pytest
+
+from gpu_cvt_amd import cvt, meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs():
+    out = []
+    V, F = mg.icosphere(3)
+    out.append((V, F, mg.sample_surface(V, F, 300, seed=1), dict()))
+    V, F = mg.icosphere(5)
+    out.append((V, F, mg.sample_surface(V, F, 2000, seed=2), dict(freeze_mode=1, freeze_disp_rel=1e-3)))
+    V, F = mg.torus(1.0, 0.3, 120, 50, jitter=1e-3, seed=3)
+    out.append((V, F, mg.sample_surface(V, F, 6000, seed=4), dict()))
+    V, F = mg.displaced_sphere(30, seed=5, crease_seed=6)
+    out.append((V, F, mg.sample_surface(V, F, 4000, seed=7), dict(freeze_mode=1, freeze_disp_rel=1e-3)))
+    V, F = mg.icosphere(2)
+    out.append((V, F, mg.sample_surface(V, F, 40, seed=8), dict()))
+    return out
+
+
+def _state(job):
+    fz, st = job.freeze_state()
+    return job.sites(), job.knn(), fz, st
+
+
+@pytest.mark.parametrize("schedule", ["batch4", "mixed"])
+def test_batch_matches_solo(schedule):
+    ins = _inputs()
+    print(f"[batch] schedule={schedule} jobs={len(ins)} sites={[len(x[2]) for x in ins]}", flush=True)
+    solo = [cvt.Job(V, F, S, K=24, **kw) for V, F, S, kw in ins]
+    for j in solo:
+        j.step(4)
+    ref = [_state(j) for j in solo]
+    for j in solo:
+        j.close()
+    jobs = [cvt.Job(V, F, S, K=24, **kw) for V, F, S, kw in ins]
+    with cvt.Batch(jobs) as b:
+        if schedule == "batch4":
+            b.step(4)
+        else:
+            b.step(1)
+            jobs[1].step(1)          # one job advances alone in between
+            jobs[3].step(1)
+            b.step(1)                # jobs 1 and 3 are now one iteration ahead ...
+            for k in (0, 2, 4):
+                jobs[k].step(1)      # ... the others catch up alone
+            for k in (0, 2, 4):
+                jobs[k].step(1)
+            for k in (1, 3):
+                jobs[k].step(1)
+    for k, j in enumerate(jobs):
+        got = _state(j)
+        for a, r, name in zip(got, ref[k], ("sites", "knn", "frozen", "streak")):
+            assert np.array_equal(a, r), f"job {k}: {name} differs between batch and solo"
+        j.close()
+
+
+def test_batch_rejects_mixed_k():
+    V, F = mg.icosphere(3)
+    a = cvt.Job(V, F, mg.sample_surface(V, F, 200, 1), K=24)
+    b = cvt.Job(V, F, mg.sample_surface(V, F, 200, 2), K=12)
+    with pytest.raises(cvt.CVTError):
+        cvt.Batch([a, b])
+    a.close()
+    b.close()
