@@ -16,7 +16,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/cvt.h"
+#include "cvt.h"
 #include "cvt_bvh.h"
 #include "cvt_kernels.h"
 
@@ -293,6 +293,12 @@ JobDesc make_desc(cvt_job* j) {
     a.dmax01 = j->dmax;
     a.nS = j->nS;
     a.R = (float)j->R;
+    {
+        // radius holding ~1.6 (K+2) sites at the mean surface density
+        const double kPi = 3.14159265358979323846;
+        const double r = std::sqrt(1.6 * (j->p.K + 2) / (kPi * (j->nS / j->area)));
+        a.tau0 = (float)(r * r);
+    }
     a.freeze_mode = j->p.freeze_mode;
     const double base2 = j->p.freeze_disp_abs2 > 0 ? j->p.freeze_disp_abs2
                                                    : (j->p.freeze_disp_rel * j->diag) * (j->p.freeze_disp_rel * j->diag);

@@ -184,6 +184,61 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
     return T.s[L - 1] >= 0 || tau == __builtin_huge_valf();
 }
 
+// Collect-then-insert variant of grid_topl for the FP32 pass: every point of
+// the 3x3x3 block closer than tau is appended to this thread's LDS column
+// (lds[k * 256], k < CAP; conflict-free since the stride is a multiple of the
+// bank count), then the at most CAP candidates are inserted.  Divergent lanes
+// thus pay a 4-op append per candidate instead of a full sorted insertion.
+// Returns false -- the caller then runs grid_topl -- when fewer than L or more
+// than CAP points lie within tau, or the block is not provably large enough.
+template <int L, int CAP>
+__device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __restrict__ start,
+                                                  const float4* __restrict__ pts, float qx, float qy, float qz,
+                                                  int self_slot, float tau, int* lds, TopL<L>& T) {
+    int cx, cy, cz;
+    cell_of(g, qx, qy, qz, cx, cy, cz);
+    const float h = g.h;
+    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.dx - 1);
+    int r0[9], r1[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int dz = k / 3 - 1, dy = k % 3 - 1;
+        const int z = cz + dz, y = cy + dy;
+        const bool in = z >= 0 && z < g.dz && y >= 0 && y < g.dy;
+        const float bz = axis_gap(qz, g.oz, h, z, dz, g.eps_abs), by = axis_gap(qy, g.oy, h, y, dy, g.eps_abs);
+        const bool keep = in && fmaf(by, by, bz * bz) <= tau * (1.f + 1e-5f);
+        const int row = (z * g.dy + y) * g.dx;
+        r0[k] = keep ? start[row + x0] : 0;
+        r1[k] = keep ? start[row + x1 + 1] : 0;
+    }
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        int s = r0[k];
+        for (; s + 1 < r1[k]; s += 2) {
+            const float4 p0 = pts[s], p1 = pts[s + 1];
+            const bool a0 = d2f(qx, qy, qz, p0) < tau && s != self_slot;
+            const bool a1 = d2f(qx, qy, qz, p1) < tau && s + 1 != self_slot;
+            if (a0) { if (cnt < CAP) lds[cnt * 256] = s; ++cnt; }
+            if (a1) { if (cnt < CAP) lds[cnt * 256] = s + 1; ++cnt; }
+        }
+        if (s < r1[k]) {
+            const float4 p0 = pts[s];
+            if (d2f(qx, qy, qz, p0) < tau && s != self_slot) { if (cnt < CAP) lds[cnt * 256] = s; ++cnt; }
+        }
+    }
+    if (cnt < L || cnt > CAP) return false;
+    T.init(__builtin_huge_valf());
+    for (int k = 0; k < cnt; ++k) {
+        const int sl = lds[k * 256];
+        T.insert(d2f(qx, qy, qz, pts[sl]), sl);
+    }
+    float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, 1);
+    if (bnd == __builtin_huge_valf()) return true;
+    bnd -= g.eps_abs;
+    return bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f);
+}
+
 __device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float qx, float qy, float qz, int sa,
                                          int sb) {
     const float4 a = pts[sa], b = pts[sb];
@@ -297,13 +352,19 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
 // was needed, -1 (only !PRECISE) when the query must go to the FP64 pass.
 // tau: a provable upper bound of the (K+2)-th distance^2, or inf.
 // l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
-template <int K, bool PRECISE>
+// lds (optional, with collect_tau finite): try grid_collect_topl first.
+template <int K, bool PRECISE, int CAP = 0>
 __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
                                          float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
-                                         float& l_out) {
+                                         float& l_out, int* lds = nullptr, float collect_tau = 0.f) {
     constexpr int L = K + 2;
     TopL<L> T;
-    if (!grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T))
+    bool have = false;
+    if constexpr (CAP > 0) {
+        if (lds && collect_tau > 0.f)
+            have = grid_collect_topl<L, CAP>(g, start, pts, qx, qy, qz, self_slot, collect_tau, lds, T);
+    }
+    if (!have && !grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T))
         grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T);
     l_out = T.s[L - 1] >= 0 ? T.d[L - 1] : __builtin_huge_valf();
     int pairs;

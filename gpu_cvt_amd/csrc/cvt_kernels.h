@@ -50,6 +50,7 @@ struct StepArgs {
     float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     int nS;
     float R;
+    float tau0;           // first-iteration candidate radius^2 guess (mean-density estimate)
     int freeze_mode;
     float thr2[6];
     int need[6];

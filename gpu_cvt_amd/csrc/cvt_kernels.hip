@@ -204,9 +204,11 @@ struct SiteFlags {
 // comparison cannot be proven.  PRECISE: resolves those in FP64.
 // carried: for the precise pass, -1 or the `same` flag the fast pass computed
 // before it wrote the site's KNN row (the row then already holds the new list).
+constexpr int kCollectCap = 48;  // LDS candidates per thread in the FP32 pass
+
 template <int K, int KP, bool PRECISE>
 __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool eval_only, int t, bool live,
-                                                int carried) {
+                                                int carried, int* lds = nullptr) {
     SiteFlags r{};
     const int has_prev = eval_only ? 0 : *a.has_prev;
     const float4* __restrict__ S = a.S01[par];
@@ -257,13 +259,21 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
     if (valid && !fz) {
         // provable seed: the previous (K+2)-th distance grown by twice the
         // largest site displacement of the previous iteration
-        float tau = inf;
+        float tau = inf, ctau = a.tau0;
         if (!PRECISE && !eval_only && *a.seed_ok) {
-            const float r0 = sqrtf(a.rad2[i]) + 2.f * a.dmax01[par ^ 1];
+            const float rp = sqrtf(a.rad2[i]);
+            const float r0 = rp + 2.f * a.dmax01[par ^ 1];
             tau = r0 * r0 * (1.f + 1e-5f);
+            // collection radius: a guess (no proof needed), from this site's own move
+            const float4 sp = a.S01[par ^ 1][i];
+            const float di = sqrtf((q.x - sp.x) * (q.x - sp.x) + (q.y - sp.y) * (q.y - sp.y) +
+                                   (q.z - sp.z) * (q.z - sp.z));
+            const float rc = 1.05f * rp + 2.5f * di;
+            ctau = fminf(rc * rc, tau);
         }
         float lk;
-        r.kind = knn_exact<K, PRECISE>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, tau, slots, lk);
+        r.kind = knn_exact<K, PRECISE, PRECISE ? 0 : kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t,
+                                                                   tau, slots, lk, lds, ctau);
         defer = defer || r.kind < 0;
         if (!eval_only && r.kind >= 0) a.rad2[i] = lk;
     }
@@ -451,13 +461,14 @@ __device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval
 template <int K, int KP>
 __global__ __launch_bounds__(256) void kb_lloyd(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
                                                 int eval_only) {
+    __shared__ int lds[kCollectCap * 256];
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
     const int par = *a.parity;
     const int t = br.blk * 256 + (int)threadIdx.x;
     const bool live = t < a.nS;
-    const SiteFlags f = lloyd_site<K, KP, false>(a, par, eval_only != 0, t, live, -1);
+    const SiteFlags f = lloyd_site<K, KP, false>(a, par, eval_only != 0, t, live, -1, lds + threadIdx.x);
     const bool deferred = live && !f.done;
     if (deferred) {
         const int pos = atomicAdd(&a.stats[kStatDeferred], 1);
