@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcvt_hip.so")
-SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip"]
+SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip", "cvt_server.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVT_OFFLOAD_ARCH", "gfx950")
 
@@ -22,7 +22,7 @@ FLAGS = [
     # keep hipcc from emitting packed FP32 VALU ops
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-    "-Wall", "-Wno-unused-result",
+    "-Wall", "-Wno-unused-result", "-pthread",
 ]
 
 
@@ -45,7 +45,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -23,10 +23,12 @@
 using namespace cvt;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
 
-int fail(int code, const char* fmt, ...) {
+// Records the calling thread's error message (cvt_last_error) and returns code.
+namespace cvt {
+__attribute__((visibility("hidden"))) int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -35,6 +37,9 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+}  // namespace cvt
+
+namespace {
 
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
@@ -594,6 +599,7 @@ static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
         o.cell_failures = w[7];
         o.nonfinite_sites = w[8];
         o.deferred_sites = w[9];
+        std::memcpy(&o.max_disp, &w[10], 4);
     }
     return 0;
 }
@@ -793,40 +799,39 @@ int cvt_job_energy(cvt_job* j, double* E) {
     return 0;
 }
 
-int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
-    if (!j || !F_out || !nF_out) return fail(CVT_EINVAL, "null argument");
-    *F_out = nullptr;
-    *nF_out = 0;
+// Device remesh into temporaries owned by `tmp` (freed by the caller): dF
+// receives 3 * nkeep vertex ids.
+struct DevTemps {
+    std::vector<void*> p;
+    void* get(size_t bytes) {
+        void* q = nullptr;
+        if (hipMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        p.push_back(q);
+        return q;
+    }
+    ~DevTemps() { for (void* q : p) (void)hipFree(q); }
+};
+
+static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int* nkeep_out) {
     if (j->nS >= (1 << 21)) return fail(CVT_EINVAL, "remesh keys pack site ids in 21 bits (nS < 2097152)");
     const int K = j->p.K, M = K + 4, nS = j->nS;
     const long nk = (long)nS * M;
     if (nk > (1L << 31) - 1) return fail(CVT_EINVAL, "too many candidate triangles");
     hipStream_t s = j->stream;
-    // temporaries (this is an output stage, not part of the timed iteration)
-    std::vector<void*> tmp;
-    auto talloc = [&](size_t bytes) -> void* {
-        void* p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
-        tmp.push_back(p);
-        return p;
-    };
-    auto cleanup = [&]() { for (void* p : tmp) (void)hipFree(p); };
     const long hw = radix_hist_words((int)nk);
-    int* knn_tmp = (int*)talloc(sizeof(int) * (size_t)nS * K);
-    auto* keys = (unsigned long long*)talloc(8 * (size_t)nk);
-    auto* keys_alt = (unsigned long long*)talloc(8 * (size_t)nk);
-    float4* site_n = (float4*)talloc(sizeof(float4) * nS);
-    int* hist = (int*)talloc(4 * (size_t)(hw + 1));
-    int* hsum = (int*)talloc(4 * (size_t)scan_partials(std::max(hw, nk) + 1));
-    int* flag = (int*)talloc(4 * (size_t)nk);
-    int* pos = (int*)talloc(4 * (size_t)(nk + 1));
-    int4* tri = (int4*)talloc(sizeof(int4) * (size_t)nk);
-    int* keep = (int*)talloc(4 * (size_t)nk);
-    int* pos2 = (int*)talloc(4 * (size_t)(nk + 1));
-    if (!knn_tmp || !keys || !keys_alt || !site_n || !hist || !hsum || !flag || !pos || !tri || !keep || !pos2) {
-        cleanup();
+    int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
+    auto* keys = (unsigned long long*)T.get(8 * (size_t)nk);
+    auto* keys_alt = (unsigned long long*)T.get(8 * (size_t)nk);
+    float4* site_n = (float4*)T.get(sizeof(float4) * nS);
+    int* hist = (int*)T.get(4 * (size_t)(hw + 1));
+    int* hsum = (int*)T.get(4 * (size_t)scan_partials(std::max(hw, nk) + 1));
+    int* flag = (int*)T.get(4 * (size_t)nk);
+    int* pos = (int*)T.get(4 * (size_t)(nk + 1));
+    int4* tri = (int4*)T.get(sizeof(int4) * (size_t)nk);
+    int* keep = (int*)T.get(4 * (size_t)nk);
+    int* pos2 = (int*)T.get(4 * (size_t)(nk + 1));
+    if (!knn_tmp || !keys || !keys_alt || !site_n || !hist || !hsum || !flag || !pos || !tri || !keep || !pos2)
         return fail(CVT_ENOMEM, "remesh temporaries");
-    }
     launch_batch_grid(j->tables, 1, s);
     launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
     radix_sort_u64(keys, (int)nk, 64, keys_alt, hist, hsum, s);
@@ -835,24 +840,67 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
     int nuniq = 0;
     hipError_t e = hipMemcpyAsync(&nuniq, pos + nk, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { cleanup(); return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e)); }
+    if (e != hipSuccess) return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e));
     launch_tri_finish(keys, flag, pos, (int)nk, j->S[j->cur], site_n, (float)eps, tri, keep, s);
     launch_scan(keep, nuniq, pos2, hsum, s);
     int nkeep = 0;
     e = hipMemcpyAsync(&nkeep, pos2 + nuniq, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) { cleanup(); return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e)); }
-    int* dF = (int*)talloc(sizeof(int) * 3 * (size_t)std::max(nkeep, 1));
-    if (!dF) { cleanup(); return fail(CVT_ENOMEM, "remesh output"); }
+    if (e != hipSuccess) return fail(CVT_EHIP, "remesh: %s", hipGetErrorString(e));
+    int* dF = (int*)T.get(sizeof(int) * 3 * (size_t)std::max(nkeep, 1));
+    if (!dF) return fail(CVT_ENOMEM, "remesh output");
     launch_tri_compact(tri, keep, pos2, nuniq, dF, s);
+    *dF_out = dF;
+    *nkeep_out = nkeep;
+    return 0;
+}
+
+int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
+    if (!j || !F_out || !nF_out) return fail(CVT_EINVAL, "null argument");
+    *F_out = nullptr;
+    *nF_out = 0;
+    DevTemps T;
+    int* dF = nullptr;
+    int nkeep = 0;
+    int rc = remesh_device(j, eps, T, &dF, &nkeep);
+    if (rc) return rc;
     int* hF = (int*)std::malloc(sizeof(int) * 3 * (size_t)std::max(nkeep, 1));
-    if (!hF) { cleanup(); return fail(CVT_ENOMEM, "host output"); }
-    e = hipMemcpyAsync(hF, dF, sizeof(int) * 3 * (size_t)nkeep, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    cleanup();
+    if (!hF) return fail(CVT_ENOMEM, "host output");
+    hipError_t e = hipMemcpyAsync(hF, dF, sizeof(int) * 3 * (size_t)nkeep, hipMemcpyDeviceToHost, j->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     if (e != hipSuccess) { std::free(hF); return fail(CVT_EHIP, "remesh copy: %s", hipGetErrorString(e)); }
     *F_out = hF;
     *nF_out = nkeep;
+    return 0;
+}
+
+static int ensure_bvh(cvt_job* j);
+
+int cvt_job_hausdorff(cvt_job* j, double eps, double* h_out, int* nF_out) {
+    if (!j || !h_out) return fail(CVT_EINVAL, "null argument");
+    int rc;
+    if ((rc = ensure_bvh(j))) return rc;
+    DevTemps T;
+    int* dF = nullptr;
+    int nkeep = 0;
+    if ((rc = remesh_device(j, eps, T, &dF, &nkeep))) return rc;
+    if (nF_out) *nF_out = nkeep;
+    if (nkeep == 0) { *h_out = 0; return 0; }
+    float4* P = (float4*)T.get(sizeof(float4) * 4 * (size_t)nkeep);
+    float* d2 = (float*)T.get(sizeof(float) * 4 * (size_t)nkeep);
+    unsigned* mx = (unsigned*)T.get(sizeof(unsigned));
+    if (!P || !d2 || !mx) return fail(CVT_ENOMEM, "hausdorff temporaries");
+    hipStream_t s = j->stream;
+    launch_tri_probes(dF, nkeep, j->S[j->cur], P, s);
+    bvh_point_d2(j->bvh->d, P, 4 * nkeep, d2, s);
+    launch_fill((int*)mx, 1, 0, s);
+    launch_max_float(d2, 4 * nkeep, mx, s);
+    unsigned hb = 0;
+    HIPCHK(hipMemcpyAsync(&hb, mx, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    float m;
+    std::memcpy(&m, &hb, 4);
+    *h_out = std::sqrt((double)m);
     return 0;
 }
 
@@ -977,8 +1025,7 @@ void cvt_batch_destroy(cvt_batch* b) {
     delete b;
 }
 
-int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
-    if (!j || (!P && n > 0) || (!d2 && n > 0) || n < 0) return fail(CVT_EINVAL, "bad argument");
+static int ensure_bvh(cvt_job* j) {
     if (!j->bvh) {
         // the job keeps only FP32 vertices on the device: rebuild host FP64 copies of them
         std::vector<float4> V4(j->nV);
@@ -990,6 +1037,13 @@ int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
         int rc = bvh_make(Vd.data(), j->nV, Fv.data(), j->nF, j->tri, &j->bvh);
         if (rc) return rc;
     }
+    return 0;
+}
+
+int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
+    if (!j || (!P && n > 0) || (!d2 && n > 0) || n < 0) return fail(CVT_EINVAL, "bad argument");
+    int rc = ensure_bvh(j);
+    if (rc) return rc;
     return bvh_query(j->bvh, P, n, d2);
 }
 

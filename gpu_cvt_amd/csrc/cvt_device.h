@@ -163,9 +163,10 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
         if (bnd == __builtin_huge_valf()) break;
         bnd -= g.eps_abs;
         if (bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f)) break;
-        // not provable yet: rebuild over the block of radius r+1
+        // not provable yet: add the shell at Chebyshev radius r+1 (the list
+        // only shrinks, so rows pruned against it earlier stay pruned; the
+        // whole search is O(cells) even for a query far outside the grid)
         const int rr = r + 1;
-        T.init(tau);
         const int x0 = max(cx - rr, 0), x1 = min(cx + rr, g.dx - 1);
         for (int dz = -rr; dz <= rr; ++dz) {
             const int z = cz + dz;
@@ -177,7 +178,12 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
                 const float by = axis_gap(qy, g.oy, h, y, dy, g.eps_abs);
                 if (fmaf(by, by, bz * bz) > T.worst() * (1.f + 1e-5f)) continue;
                 const int row = (z * g.dy + y) * g.dx;
-                scan_run<L>(pts, start[row + x0], start[row + x1 + 1], qx, qy, qz, self_slot, T);
+                if (dz == -rr || dz == rr || dy == -rr || dy == rr) {
+                    scan_run<L>(pts, start[row + x0], start[row + x1 + 1], qx, qy, qz, self_slot, T);
+                } else {  // inner row: only its two end cells are new
+                    if (cx - rr >= 0) scan_run<L>(pts, start[row + cx - rr], start[row + cx - rr + 1], qx, qy, qz, self_slot, T);
+                    if (cx + rr < g.dx) scan_run<L>(pts, start[row + cx + rr], start[row + cx + rr + 1], qx, qy, qz, self_slot, T);
+                }
             }
         }
     }

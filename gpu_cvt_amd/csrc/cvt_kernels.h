@@ -128,6 +128,8 @@ void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsig
 void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s);
 void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
                        const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);
+void launch_tri_probes(const int* F, int nF, const float4* S, float4* P, hipStream_t s);
+void launch_max_float(const float* v, int n, unsigned* out, hipStream_t s);
 void launch_tri_compact(const int4* tri, const int* keep, const int* pos, int n, int* out, hipStream_t s);
 
 

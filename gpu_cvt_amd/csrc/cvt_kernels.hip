@@ -190,6 +190,7 @@ __device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, boo
 // 4 frozen total, 5 exact rescans, 6 reranks, 7 cell failures, 8 non-finite,
 // 9 deferred to the precise pass (also that pass's work-list length).
 constexpr int kStatDeferred = 9;
+constexpr int kStatMaxDisp = 10;  // float bits
 
 struct SiteFlags {
     float disp;     // |S' - S| (0 when not done or not finite)
@@ -608,6 +609,7 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     const StepArgs& a = descs[j].a;
     const int c = *a.hist_count;
     const int slot = c < kHistCap ? c : kHistCap - 1;
+    a.stats[kStatMaxDisp] = __float_as_int(a.dmax01[*a.parity]);
     for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
     *a.hist_count = c + 1;
     *a.has_prev = 1;
@@ -686,6 +688,37 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
             if (b != c && b != i && c != i) out[nout++] = pack3(i, b, c);
         }
     });
+}
+
+// Hausdorff probes of a triangle set (evaluate.cuh:252-283 without the
+// vertices): per face its centroid and three edge midpoints.
+__global__ void k_tri_probes(const int* __restrict__ F, int nF, const float4* __restrict__ S,
+                             float4* __restrict__ P) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const float4 a = S[F[3 * f]], b = S[F[3 * f + 1]], c = S[F[3 * f + 2]];
+    P[4 * f] = make_float4((a.x + b.x + c.x) * (1.f / 3.f), (a.y + b.y + c.y) * (1.f / 3.f),
+                           (a.z + b.z + c.z) * (1.f / 3.f), 0.f);
+    P[4 * f + 1] = make_float4(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.f);
+    P[4 * f + 2] = make_float4(0.5f * (b.x + c.x), 0.5f * (b.y + c.y), 0.5f * (b.z + c.z), 0.f);
+    P[4 * f + 3] = make_float4(0.5f * (c.x + a.x), 0.5f * (c.y + a.y), 0.5f * (c.z + a.z), 0.f);
+}
+
+// max of n non-negative floats into *out (float bits, atomic max; *out zeroed by the caller)
+__global__ void k_max_float(const float* __restrict__ v, int n, unsigned* __restrict__ out) {
+    float m = 0.f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) m = fmaxf(m, v[i]);
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+void launch_tri_probes(const int* F, int nF, const float4* S, float4* P, hipStream_t s) {
+    if (nF > 0) hipLaunchKernelGGL(k_tri_probes, dim3(nblk(nF, 256)), dim3(256), 0, s, F, nF, S, P);
+}
+
+void launch_max_float(const float* v, int n, unsigned* out, hipStream_t s) {
+    if (n > 0) hipLaunchKernelGGL(k_max_float, dim3(std::min(nblk(n, 256), 1024)), dim3(256), 0, s, v, n, out);
 }
 
 __global__ void k_unique_mark(const unsigned long long* __restrict__ keys, int n, int* __restrict__ flag) {

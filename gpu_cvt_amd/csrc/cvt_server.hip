@@ -1,0 +1,369 @@
+// cvt_server.hip -- continuous-batching request loop over the public C ABI.
+//
+// The reference processes one mesh per process invocation (main.cu:371) and
+// has no server.  Here requests (mesh + sites + parameters) arrive at any
+// time; an ingest thread turns them into device-resident jobs while their
+// footprint fits the HBM budget, and a scheduler thread advances every active
+// job together, E iterations per epoch, as one batch graph (cvt_batch).  At
+// each epoch boundary the scheduler checks convergence -- the one-sided
+// Hausdorff distance of the remeshed surface (cvt_job_hausdorff), the check
+// the reference runs every DUMP_STRIDE = 10 iterations (main.cu:395,1034) --
+// retires finished jobs (freeing their memory) and admits jobs that became
+// ready, rebuilding the batch only when its membership changed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "cvt.h"
+
+namespace cvt {
+int fail(int code, const char* fmt, ...);
+}
+using cvt::fail;
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+    unsigned long long ticket = 0;
+    std::vector<double> V, S;
+    std::vector<int> F;
+    cvt_params p{};
+    double tol_rel = 1e-3;
+    int max_iters = 100;
+    // lifecycle
+    cvt_result res{};
+    cvt_job* job = nullptr;
+    unsigned long long bytes = 0;
+    double diag = 0;
+    int next_check = 0;
+    std::vector<double> S_out;
+};
+
+double secs(Clock::time_point t0) { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+}  // namespace
+
+struct cvt_server {
+    cvt_server_config cfg{};
+    Clock::time_point t0;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Request*> pending;  // submitted, not yet on the device
+    std::vector<Request*> ready;   // device job created, waiting for the next epoch
+    std::vector<Request*> active;  // in the batch
+    std::map<unsigned long long, Request*> all;
+    unsigned long long next_ticket = 1;
+    unsigned long long used = 0, peak = 0;
+    bool stop = false;
+    std::thread ingest_th, sched_th;
+    // counters
+    double site_iters = 0, busy_s = 0;
+    long epochs = 0, rebuilds = 0, checks = 0;
+    int failures = 0;
+
+    void ingest_loop();
+    void sched_loop();
+    void finish(Request* r, int status);
+};
+
+// Conservative estimate of a job's device footprint before creating it (the
+// exact figure replaces it once the job exists).
+static unsigned long long estimate_bytes(int nV, int nF, int nS, int K) {
+    const double v = 16.0 * 2 * nV + 4.0 * (nV + 1) + 32.0 * nV;   // V, N, CSR offsets, vertex grid
+    const double f = 48.0 * nF + 12.0 * nF + 12.0 * nF;            // face table, F, CSR faces
+    const double s = nS * (16.0 * 2 + 4.0 * K + 20 + 3 + 4 + 4 + 16 + 4 * 2 + 4 + 4 * 32);
+    return (unsigned long long)((v + f + s) * 1.25) + (1ull << 20);
+}
+
+void cvt_server::finish(Request* r, int status) {
+    // called by the scheduler without the lock held
+    if (r->job) {
+        r->S_out.resize(3 * (size_t)r->res.nS);
+        if (cvt_job_get_sites(r->job, r->S_out.data()) != CVT_OK) status = 4;
+        cvt_job_destroy(r->job);
+        r->job = nullptr;
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    used -= r->bytes;
+    r->res.status = status;
+    r->res.t_done = secs(t0);
+    cv.notify_all();
+}
+
+void cvt_server::ingest_loop() {
+    for (;;) {
+        Request* r = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] {
+                if (stop) return true;
+                if (pending.empty()) return false;
+                Request* f = pending.front();
+                const unsigned long long est = estimate_bytes((int)f->V.size() / 3, (int)f->F.size() / 3,
+                                                              (int)f->S.size() / 3, f->p.K);
+                return used + est <= cfg.hbm_budget_bytes || (active.empty() && ready.empty() && used == 0);
+            });
+            if (stop) return;
+            r = pending.front();
+            pending.pop_front();
+            r->bytes = estimate_bytes((int)r->V.size() / 3, (int)r->F.size() / 3, (int)r->S.size() / 3, r->p.K);
+            used += r->bytes;
+        }
+        cvt_job* job = nullptr;
+        const int rc = cvt_job_create(r->V.data(), (int)r->V.size() / 3, r->F.data(), (int)r->F.size() / 3,
+                                      r->S.data(), (int)r->S.size() / 3, &r->p, &job);
+        std::lock_guard<std::mutex> lk(mu);
+        if (rc != CVT_OK) {
+            used -= r->bytes;
+            r->res.status = 4;
+            r->res.error = rc;
+            r->res.t_done = secs(t0);
+            ++failures;
+        } else {
+            cvt_job_info info;
+            cvt_job_info_get(job, &info);
+            // the job's own buffers plus the LBVH its first convergence check
+            // builds over the input mesh (about 96 bytes per face)
+            const unsigned long long b = info.device_bytes + 96ull * (unsigned long long)info.nF;
+            used = used - r->bytes + b;
+            peak = std::max(peak, used);
+            r->bytes = b;
+            r->diag = info.bbox_diag;
+            r->res.bbox_diag = info.bbox_diag;
+            r->job = job;
+            r->res.t_admit = secs(t0);
+            r->res.status = 1;
+            r->next_check = cfg.epoch_iters;
+            // the request's host copies are no longer needed
+            std::vector<double>().swap(r->V);
+            std::vector<int>().swap(r->F);
+            std::vector<double>().swap(r->S);
+            ready.push_back(r);
+        }
+        cv.notify_all();
+    }
+}
+
+void cvt_server::sched_loop() {
+    // One batch per (K, kproj) group; a group's batch is rebuilt only when its
+    // membership changes (a job joined or retired).
+    struct Group {
+        std::vector<Request*> members;
+        cvt_batch* batch = nullptr;
+        bool dirty = true;
+    };
+    std::map<std::pair<int, int>, Group> groups;
+    for (;;) {
+        std::vector<Request*> joined;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv.wait(lk, [&] { return stop || !ready.empty() || !active.empty(); });
+            if (stop) break;
+            joined.swap(ready);
+            active.insert(active.end(), joined.begin(), joined.end());
+        }
+        for (Request* r : joined) {
+            Group& g = groups[{r->p.K, r->p.kproj}];
+            g.members.push_back(r);
+            g.dirty = true;
+        }
+        const auto e0 = Clock::now();
+        double si = 0;
+        long nrebuild = 0, ncheck = 0;
+        std::vector<Request*> done;
+        std::vector<int> status;
+        for (auto& kv : groups) {
+            Group& g = kv.second;
+            if (g.members.empty()) continue;
+            std::vector<cvt_job*> jobs;
+            for (Request* r : g.members) jobs.push_back(r->job);
+            if (g.dirty) {
+                if (g.batch) cvt_batch_destroy(g.batch);
+                g.batch = nullptr;
+                if (cvt_batch_create(jobs.data(), (int)jobs.size(), &g.batch) != CVT_OK) g.batch = nullptr;
+                g.dirty = false;
+                ++nrebuild;
+            }
+            if (g.batch) {
+                cvt_batch_step(g.batch, cfg.epoch_iters);
+            } else {
+                for (cvt_job* j : jobs) cvt_job_step(j, cfg.epoch_iters, nullptr);
+            }
+            // convergence at the epoch boundary
+            std::vector<Request*> keep;
+            for (Request* r : g.members) {
+                {
+                    std::lock_guard<std::mutex> lk(mu);  // res is read by cvt_server_poll
+                    r->res.iterations += cfg.epoch_iters;
+                }
+                si += (double)r->res.nS * cfg.epoch_iters;
+                cvt_iter_stats st;
+                std::memset(&st, 0, sizeof st);
+                cvt_job_stats(r->job, 1, &st);
+                const bool gate = r->res.iterations >= cfg.min_check_iters &&
+                                  ((double)st.max_disp <= cfg.check_disp_rel * r->diag ||
+                                   st.frozen_total >= cfg.check_frozen_frac * r->res.nS);
+                const bool last = r->res.iterations >= r->max_iters;
+                int st_done = 0;
+                if (gate || last) {
+                    double h = -1;
+                    if (cvt_job_hausdorff(r->job, 1e-6, &h, nullptr) == CVT_OK) {
+                        std::lock_guard<std::mutex> lk(mu);
+                        r->res.hausdorff = h;
+                        ++ncheck;
+                    }
+                    if (h >= 0 && h <= r->tol_rel * r->diag) st_done = 2;
+                    else if (last) st_done = 3;
+                }
+                if (st_done) {
+                    done.push_back(r);
+                    status.push_back(st_done);
+                    g.dirty = true;
+                } else {
+                    keep.push_back(r);
+                }
+            }
+            if (g.dirty && g.batch) {
+                // the retiring jobs are destroyed below: drop the batch that names them first
+                cvt_batch_destroy(g.batch);
+                g.batch = nullptr;
+            }
+            g.members.swap(keep);
+        }
+        const double dt = secs(e0);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            site_iters += si;
+            busy_s += dt;
+            ++epochs;
+            rebuilds += nrebuild;
+            checks += ncheck;
+            for (Request* r : done) active.erase(std::find(active.begin(), active.end(), r));
+        }
+        for (size_t k = 0; k < done.size(); ++k) finish(done[k], status[k]);
+    }
+    for (auto& kv : groups)
+        if (kv.second.batch) cvt_batch_destroy(kv.second.batch);
+}
+
+extern "C" {
+
+void cvt_server_config_default(cvt_server_config* c) {
+    std::memset(c, 0, sizeof *c);
+    c->hbm_budget_bytes = 200ull << 30;
+    c->epoch_iters = 10;
+    c->min_check_iters = 10;
+    c->check_disp_rel = 1e-3;
+    c->check_frozen_frac = 0.5;
+}
+
+int cvt_server_create(const cvt_server_config* cfg, cvt_server** out) {
+    if (!out) return fail(CVT_EINVAL, "null argument");
+    cvt_server* s = new cvt_server();
+    if (cfg) s->cfg = *cfg;
+    else cvt_server_config_default(&s->cfg);
+    if (s->cfg.epoch_iters < 1) s->cfg.epoch_iters = 1;
+    s->t0 = Clock::now();
+    s->ingest_th = std::thread([s] { s->ingest_loop(); });
+    s->sched_th = std::thread([s] { s->sched_loop(); });
+    *out = s;
+    return CVT_OK;
+}
+
+int cvt_server_submit(cvt_server* s, const double* V, int nV, const int* F, int nF, const double* S0, int nS,
+                      const cvt_params* p, double tol_hausdorff_rel, int max_iters, unsigned long long* ticket) {
+    if (!s || !V || !F || !S0 || !ticket || nV < 3 || nF < 1 || nS < 1 || max_iters < 1 || !(tol_hausdorff_rel >= 0))
+        return fail(CVT_EINVAL, "bad argument");
+    Request* r = new Request();
+    r->V.assign(V, V + 3L * nV);
+    r->F.assign(F, F + 3L * nF);
+    r->S.assign(S0, S0 + 3L * nS);
+    if (p) r->p = *p;
+    else cvt_params_default(&r->p);
+    r->tol_rel = tol_hausdorff_rel;
+    r->max_iters = max_iters;
+    r->res.nS = nS;
+    r->res.hausdorff = -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    r->ticket = s->next_ticket++;
+    r->res.t_arrive = secs(s->t0);
+    s->all[r->ticket] = r;
+    s->pending.push_back(r);
+    *ticket = r->ticket;
+    s->cv.notify_all();
+    return CVT_OK;
+}
+
+int cvt_server_poll(cvt_server* s, unsigned long long ticket, cvt_result* out) {
+    if (!s || !out) return fail(CVT_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return fail(CVT_EINVAL, "unknown ticket %llu", ticket);
+    *out = it->second->res;
+    return CVT_OK;
+}
+
+int cvt_server_get_sites(cvt_server* s, unsigned long long ticket, double* S_out) {
+    if (!s || !S_out) return fail(CVT_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->all.find(ticket);
+    if (it == s->all.end()) return fail(CVT_EINVAL, "unknown ticket %llu", ticket);
+    Request* r = it->second;
+    if (r->res.status < 2 || r->S_out.empty()) return fail(CVT_ESTATE, "request %llu has not finished", ticket);
+    std::memcpy(S_out, r->S_out.data(), sizeof(double) * r->S_out.size());
+    return CVT_OK;
+}
+
+int cvt_server_wait_idle(cvt_server* s, double timeout_s) {
+    if (!s) return fail(CVT_EINVAL, "null argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
+        return s->pending.empty() && s->ready.empty() && s->active.empty() && s->used == 0;
+    });
+    return ok ? CVT_OK : fail(CVT_ESTATE, "timed out waiting for the server to drain");
+}
+
+int cvt_server_info_get(cvt_server* s, cvt_server_info* out) {
+    if (!s || !out) return fail(CVT_EINVAL, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    std::memset(out, 0, sizeof *out);
+    out->site_iters = s->site_iters;
+    out->busy_seconds = s->busy_s;
+    out->peak_bytes = s->peak;
+    out->used_bytes = s->used;
+    out->epochs = s->epochs;
+    out->rebuilds = s->rebuilds;
+    out->hausdorff_checks = s->checks;
+    out->pending = (int)s->pending.size();
+    out->active = (int)(s->active.size() + s->ready.size());
+    out->failures = s->failures;
+    return CVT_OK;
+}
+
+void cvt_server_destroy(cvt_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        s->cv.notify_all();
+    }
+    s->ingest_th.join();
+    s->sched_th.join();
+    for (auto& kv : s->all) {
+        if (kv.second->job) cvt_job_destroy(kv.second->job);
+        delete kv.second;
+    }
+    delete s;
+}
+
+}  // extern "C"

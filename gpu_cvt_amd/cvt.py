@@ -26,7 +26,9 @@ EXPORTS = [
     "cvt_job_get_knnv", "cvt_job_get_freeze", "cvt_job_get_normals", "cvt_job_set_state", "cvt_job_set_tiers",
     "cvt_job_get_debug", "cvt_job_energy", "cvt_job_point_mesh_d2", "cvt_job_profile",
     "cvt_bvh_create", "cvt_bvh_point_d2", "cvt_bvh_destroy", "cvt_job_remesh", "cvt_free", "cvt_job_stats",
-    "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy",
+    "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy", "cvt_job_hausdorff",
+    "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
+    "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
 ]
 
 
@@ -46,7 +48,7 @@ class IterStats(C.Structure):
         ("count_same", C.c_int), ("count_low_disp", C.c_int), ("count_both", C.c_int),
         ("count_newly_frozen", C.c_int), ("frozen_total", C.c_int), ("knn_exact_rescans", C.c_int),
         ("knn_reranks", C.c_int), ("cell_failures", C.c_int), ("nonfinite_sites", C.c_int),
-        ("deferred_sites", C.c_int), ("reserved", C.c_int * 2),
+        ("deferred_sites", C.c_int), ("max_disp", C.c_float), ("reserved", C.c_int * 1),
     ]
 
     def as_dict(self) -> dict:
@@ -63,6 +65,36 @@ class JobInfo(C.Structure):
         ("bbox_diag", C.c_double), ("clip_R", C.c_double), ("area", C.c_double),
         ("device_bytes", C.c_ulonglong), ("iterations_done", C.c_int), ("reserved", C.c_int * 7),
     ]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_ulonglong), ("epoch_iters", C.c_int), ("min_check_iters", C.c_int),
+        ("check_disp_rel", C.c_double), ("check_frozen_frac", C.c_double), ("reserved", C.c_int * 8),
+    ]
+
+
+class Result(C.Structure):
+    _fields_ = [
+        ("status", C.c_int), ("error", C.c_int), ("nS", C.c_int), ("iterations", C.c_int),
+        ("hausdorff", C.c_double), ("bbox_diag", C.c_double),
+        ("t_arrive", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double), ("reserved", C.c_int * 4),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [
+        ("site_iters", C.c_double), ("busy_seconds", C.c_double),
+        ("peak_bytes", C.c_ulonglong), ("used_bytes", C.c_ulonglong),
+        ("epochs", C.c_longlong), ("rebuilds", C.c_longlong), ("hausdorff_checks", C.c_longlong),
+        ("pending", C.c_int), ("active", C.c_int), ("failures", C.c_int), ("reserved", C.c_int * 5),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
 class CVTError(RuntimeError):
@@ -125,6 +157,7 @@ def lib():
     L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double)]
     L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.cvt_job_hausdorff.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cvt_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.cvt_batch_step.argtypes = [C.c_void_p, C.c_int]
     L.cvt_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
@@ -136,6 +169,17 @@ def lib():
     L.cvt_bvh_create.argtypes = [_dp, C.c_int, _ip, C.c_int, C.POINTER(C.c_void_p)]
     L.cvt_bvh_point_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_bvh_destroy.argtypes = [C.c_void_p]
+    L.cvt_server_config_default.argtypes = [C.POINTER(ServerConfig)]
+    L.cvt_server_config_default.restype = None
+    L.cvt_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
+    L.cvt_server_submit.argtypes = [C.c_void_p, _dp, C.c_int, _ip, C.c_int, _dp, C.c_int, C.POINTER(Params),
+                                    C.c_double, C.c_int, C.POINTER(C.c_ulonglong)]
+    L.cvt_server_poll.argtypes = [C.c_void_p, C.c_ulonglong, C.POINTER(Result)]
+    L.cvt_server_get_sites.argtypes = [C.c_void_p, C.c_ulonglong, _dp]
+    L.cvt_server_wait_idle.argtypes = [C.c_void_p, C.c_double]
+    L.cvt_server_info_get.argtypes = [C.c_void_p, C.POINTER(ServerInfo)]
+    L.cvt_server_destroy.argtypes = [C.c_void_p]
+    L.cvt_server_destroy.restype = None
     L.cvt_bvh_destroy.restype = None
     missing = [n for n in EXPORTS if not hasattr(L, n)]
     if missing:
@@ -303,6 +347,13 @@ class Job:
             lib().cvt_free(fp)
         return F
 
+    def hausdorff(self, eps: float = 1e-6):
+        """(one-sided remesh->input Hausdorff, remeshed triangle count), on the device."""
+        h = C.c_double()
+        nf = C.c_int()
+        _check("cvt_job_hausdorff", lib().cvt_job_hausdorff(self.handle, float(eps), C.byref(h), C.byref(nf)))
+        return h.value, nf.value
+
     def point_mesh_d2(self, P) -> np.ndarray:
         P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
         out = np.zeros(len(P))
@@ -379,6 +430,80 @@ class Batch:
     def close(self) -> None:
         if getattr(self, "_h", None):
             lib().cvt_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+STATUS_NAMES = {0: "queued", 1: "running", 2: "converged", 3: "max_iters", 4: "failed"}
+
+
+class Server:
+    """Continuous-batching request loop (cvt_server_*): submit meshes + sites
+    at any time; admitted jobs advance together in epochs; each retires when
+    its remeshed surface is within ``tol_rel * bbox_diag`` of the input
+    (one-sided Hausdorff) or after ``max_iters`` iterations."""
+
+    def __init__(self, hbm_budget_bytes: int | None = None, **kw):
+        L = lib()
+        cfg = ServerConfig()
+        L.cvt_server_config_default(C.byref(cfg))
+        if hbm_budget_bytes is not None:
+            cfg.hbm_budget_bytes = int(hbm_budget_bytes)
+        for k, v in kw.items():
+            setattr(cfg, k, v)
+        self.config = cfg
+        h = C.c_void_p()
+        _check("cvt_server_create", L.cvt_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._nS = {}
+
+    def submit(self, V, F, S0, tol_rel: float = 1e-3, max_iters: int = 100, params: Params | None = None,
+               **kw) -> int:
+        p = params if params is not None else default_params(**kw)
+        V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        S0 = np.ascontiguousarray(S0, dtype=np.float64).reshape(-1, 3)
+        t = C.c_ulonglong()
+        _check("cvt_server_submit", lib().cvt_server_submit(self._h, V.reshape(-1), len(V), F.reshape(-1), len(F),
+                                                            S0.reshape(-1), len(S0), C.byref(p), float(tol_rel),
+                                                            int(max_iters), C.byref(t)))
+        self._nS[t.value] = len(S0)
+        return t.value
+
+    def poll(self, ticket: int) -> dict:
+        r = Result()
+        _check("cvt_server_poll", lib().cvt_server_poll(self._h, ticket, C.byref(r)))
+        d = r.as_dict()
+        d["state"] = STATUS_NAMES.get(d["status"], "?")
+        return d
+
+    def sites(self, ticket: int) -> np.ndarray:
+        out = np.zeros((self._nS[ticket], 3))
+        _check("cvt_server_get_sites", lib().cvt_server_get_sites(self._h, ticket, out.reshape(-1)))
+        return out
+
+    def wait_idle(self, timeout_s: float = 600.0) -> None:
+        _check("cvt_server_wait_idle", lib().cvt_server_wait_idle(self._h, float(timeout_s)))
+
+    def info(self) -> dict:
+        o = ServerInfo()
+        _check("cvt_server_info_get", lib().cvt_server_info_get(self._h, C.byref(o)))
+        return o.as_dict()
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().cvt_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

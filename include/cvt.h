@@ -57,7 +57,8 @@ typedef struct cvt_iter_stats {
     int cell_failures;     /* cells whose hull walk did not close (site kept) */
     int nonfinite_sites;   /* sites with non-finite coordinates after the step */
     int deferred_sites;    /* sites the FP32 pass handed to the FP64-resolving pass */
-    int reserved[2];
+    float max_disp;        /* largest |S' - S| of the iteration */
+    int reserved[1];
 } cvt_iter_stats;
 
 typedef struct cvt_job_info {
@@ -146,6 +147,14 @@ int cvt_job_energy(cvt_job* job, double* E_out);
 int cvt_job_remesh(cvt_job* job, double eps, int** F_out, int* nF_out);
 void cvt_free(void* p);
 
+/* One-sided Hausdorff distance from the remeshed surface of the current sites
+ * to the input mesh, over the deterministic probes of evaluate.cuh:252-283
+ * (face centroids and edge midpoints; the vertices are the sites, which lie on
+ * the mesh), computed on the device (remesh, probes, LBVH, max).  Replaces
+ * hausdorff_cand_to_ref_gpu (evaluate.cuh:359-383).  nF_out (nullable): the
+ * remeshed triangle count. */
+int cvt_job_hausdorff(cvt_job* job, double eps, double* h_out, int* nF_out);
+
 /* Squared distance from each query point (n*3, host) to the job's input mesh,
  * via a device LBVH (replaces bvh_closest_d2_kernel, evaluate.cuh:206-250, and
  * the CPU BVH of evaluate.cuh:114-201).  out: n doubles. */
@@ -175,6 +184,62 @@ typedef struct cvt_bvh cvt_bvh;
 int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out);
 int cvt_bvh_point_d2(cvt_bvh* bvh, const double* P, int n, double* d2_out);
 void cvt_bvh_destroy(cvt_bvh* bvh);
+
+/* Serving form: a continuous-batching request loop.  The reference runs one
+ * mesh per process invocation (main.cu:371-402) and checks convergence every
+ * DUMP_STRIDE = 10 iterations with a Hausdorff evaluation (main.cu:395,
+ * 1034-1060); it has no server, so these entry points have no direct
+ * counterpart there.  Requests are admitted while the sum of their device
+ * footprints stays within hbm_budget_bytes; all admitted jobs advance together
+ * in epochs of epoch_iters iterations (one batch graph); at an epoch boundary a
+ * job whose max displacement or frozen fraction passes the gate is checked by
+ * cvt_job_hausdorff and retires when h <= tol_hausdorff_rel * bbox_diag, or
+ * when it reaches max_iters.  A job's result does not depend on which other
+ * jobs shared its epochs. */
+typedef struct cvt_server_config {
+    unsigned long long hbm_budget_bytes; /* admission budget (default 200 GiB) */
+    int epoch_iters;          /* iterations between convergence checks (default 10) */
+    int min_check_iters;      /* no check before this many iterations (default 10) */
+    double check_disp_rel;    /* gate: max_disp <= this * bbox_diag (default 1e-3) */
+    double check_frozen_frac; /* gate: frozen_total >= this * nS (default 0.5) */
+    int reserved[8];
+} cvt_server_config;
+
+/* status: 0 queued, 1 running, 2 converged, 3 reached max_iters, 4 failed.
+ * Times in seconds since the server was created. */
+typedef struct cvt_result {
+    int status;
+    int error;          /* cvt_job_create's status when status == 4 */
+    int nS;
+    int iterations;
+    double hausdorff;   /* last checked value (-1 if never checked) */
+    double bbox_diag;
+    double t_arrive, t_admit, t_done;
+    int reserved[4];
+} cvt_result;
+
+typedef struct cvt_server_info {
+    double site_iters;        /* sum over epochs of (active sites x iterations) */
+    double busy_seconds;      /* wall time of epochs incl. convergence checks */
+    unsigned long long peak_bytes, used_bytes;
+    long long epochs, rebuilds, hausdorff_checks;
+    int pending, active, failures;
+    int reserved[5];
+} cvt_server_info;
+
+typedef struct cvt_server cvt_server;
+void cvt_server_config_default(cvt_server_config* c);
+int cvt_server_create(const cvt_server_config* cfg, cvt_server** out);
+/* Copies the inputs; returns a ticket.  p may be null (defaults). */
+int cvt_server_submit(cvt_server* s, const double* V, int nV, const int* F, int nF, const double* S0, int nS,
+                      const cvt_params* p, double tol_hausdorff_rel, int max_iters, unsigned long long* ticket);
+int cvt_server_poll(cvt_server* s, unsigned long long ticket, cvt_result* out);
+/* Final sites of a finished request (status 2 or 3), nS*3 doubles. */
+int cvt_server_get_sites(cvt_server* s, unsigned long long ticket, double* S_out);
+/* Blocks until no request is queued or running, or timeout_s passes (CVT_ESTATE). */
+int cvt_server_wait_idle(cvt_server* s, double timeout_s);
+int cvt_server_info_get(cvt_server* s, cvt_server_info* out);
+void cvt_server_destroy(cvt_server* s);
 
 #ifdef __cplusplus
 }

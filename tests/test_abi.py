@@ -39,6 +39,8 @@ def test_struct_layouts_match_header():
 int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(cvt_params), sizeof(cvt_iter_stats), sizeof(cvt_job_info),
          offsetof(cvt_params, debug), offsetof(cvt_job_info, device_bytes), offsetof(cvt_params, tier_disp_abs2));
+  printf("%zu %zu %zu %zu %zu\n", sizeof(cvt_server_config), sizeof(cvt_result), sizeof(cvt_server_info),
+         offsetof(cvt_result, t_done), offsetof(cvt_server_info, pending));
   return 0;
 }
 '''
@@ -49,7 +51,9 @@ int main(void) {
         subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
         vals = [int(x) for x in subprocess.run([exe], capture_output=True, text=True, check=True).stdout.split()]
     assert vals == [C.sizeof(cvt.Params), C.sizeof(cvt.IterStats), C.sizeof(cvt.JobInfo),
-                    cvt.Params.debug.offset, cvt.JobInfo.device_bytes.offset, cvt.Params.tier_disp_abs2.offset]
+                    cvt.Params.debug.offset, cvt.JobInfo.device_bytes.offset, cvt.Params.tier_disp_abs2.offset,
+                    C.sizeof(cvt.ServerConfig), C.sizeof(cvt.Result), C.sizeof(cvt.ServerInfo),
+                    cvt.Result.t_done.offset, cvt.ServerInfo.pending.offset]
 
 
 def test_default_params():
@@ -89,3 +93,21 @@ def test_create_rejects_bad_inputs():
     assert e.value.code == -1
     with pytest.raises(cvt.CVTError):
         cvt.Job(V[:2], F[:0], S, K=8)
+
+
+def test_server_rejects_bad_requests():
+    """Argument checks of the request loop run before any device work."""
+    V, F = mg.icosphere(2)
+    S = mg.sample_surface(V, F, 50, seed=0)
+    L = cvt.lib()
+    t = C.c_ulonglong()
+    p = cvt.default_params()
+    # null server handle
+    rc = L.cvt_server_submit(None, V.reshape(-1), len(V), F.astype(np.int32).reshape(-1), len(F), S.reshape(-1),
+                             len(S), C.byref(p), 1e-3, 10, C.byref(t))
+    assert rc == -1
+    cfg = cvt.ServerConfig()
+    L.cvt_server_config_default(C.byref(cfg))
+    assert cfg.hbm_budget_bytes == 200 << 30 and cfg.epoch_iters == 10
+    r = cvt.Result()
+    assert L.cvt_server_poll(None, 1, C.byref(r)) == -1
