@@ -58,7 +58,7 @@ struct cvt_job {
     unsigned long long bytes = 0;
 
     // mesh
-    float4 *V = nullptr, *N = nullptr, *tri = nullptr;
+    float4 *V = nullptr, *N = nullptr, *FU = nullptr, *FW = nullptr, *tri = nullptr;
     int *F = nullptr, *vf_off = nullptr, *vf_faces = nullptr;
     GridP vg{};
     GridBufs vgb{};
@@ -268,6 +268,8 @@ JobDesc make_desc(cvt_job* j) {
     StepArgs& a = d.a;
     a.m.V = j->V;
     a.m.N = j->N;
+    a.m.FU = j->FU;
+    a.m.FW = j->FW;
     a.m.tri = j->tri;
     a.m.F = j->F;
     a.m.vf_off = j->vf_off;
@@ -489,7 +491,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
 
     if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
-    if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
+    if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->FU, nV)) ||
+        (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
         (rc = j->alloc(&j->F, (size_t)3 * nF)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
         (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.ncell)) ||
         (rc = alloc_grid(j, j->sgb, nS, j->sgp.ncell)) || (rc = j->alloc(&j->S[0], nS)) ||
@@ -528,6 +531,11 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatWords, 0, s);
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
+    {
+        MeshDev m{};
+        m.V = j->V; m.N = j->N; m.F = j->F; m.vf_off = j->vf_off; m.vf_faces = j->vf_faces; m.nV = nV; m.nF = nF;
+        launch_frames(m, j->FU, j->FW, s);
+    }
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
     {
         // this job as a batch of one

@@ -12,6 +12,8 @@ constexpr int kStatWords = 12;  // mirrors cvt_iter_stats (int fields)
 struct MeshDev {
     const float4* V;      // nV  (x, y, z, 0)
     const float4* N;      // nV  unit vertex normals (FP32)
+    const float4* FU;     // nV  tangent frame u of each vertex normal (site_normals.cuh:68-110)
+    const float4* FW;     // nV  tangent frame v
     const float4* tri;    // 3*nF  (x, y, z, vertex id as float bits) per corner
     const int* F;         // 3*nF  vertex ids
     const int* vf_off;    // nV+1
@@ -111,6 +113,9 @@ void launch_fill(int* p, long n, int v, hipStream_t s);
 int scan_partials(long n);
 void launch_scan(const int* in, int n, int* out, int* bsum, hipStream_t s);
 void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipStream_t s);
+// Per-vertex tangent frames: FP32 from N, or from the FP64 normal where the
+// axis switch |n_z| > 0.9 is within FP32 error (or N is degenerate).
+void launch_frames(const MeshDev& m, float4* FU, float4* FW, hipStream_t s);
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
 bool lloyd_supported(int K, int kproj);

@@ -181,6 +181,19 @@ __device__ __noinline__ void frame_fp64_vertex(const MeshDev& m, int vtx, float3
     frame_f64(sx, sy, sz, u, v);
 }
 
+__global__ void k_vertex_frames(MeshDev m, float4* __restrict__ FU, float4* __restrict__ FW) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= m.nV) return;
+    const float4 nv = m.N[v];
+    const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
+    const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
+    float3 u, w;
+    if (l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f) frame_fp64_vertex(m, v, u, w);
+    else frame_f32(nv.x, nv.y, nv.z, u, w);
+    FU[v] = make_float4(u.x, u.y, u.z, 0.f);
+    FW[v] = make_float4(w.x, w.y, w.z, 0.f);
+}
+
 __device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, bool pred) {
     const unsigned long long b = __ballot(pred);
     if ((threadIdx.x & 63) == 0 && b) atomicAdd(&stats[idx], (int)__popcll(b));
@@ -239,17 +252,9 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
             nn = a.knnV[(size_t)i * KP];  // pinned at the freeze iteration
         }
         if (nn >= 0 && nn < a.m.nV) {
-            const float4 nv = a.m.N[nn];
-            const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
-            const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
-            const bool ill = l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f;
-            if constexpr (PRECISE) {
-                if (ill) frame_fp64_vertex(a.m, nn, u, w);
-                else frame_f32(nv.x, nv.y, nv.z, u, w);
-            } else {
-                defer = defer || ill;
-                frame_f32(nv.x, nv.y, nv.z, u, w);
-            }
+            const float4 fu = a.m.FU[nn], fw = a.m.FW[nn];
+            u = make_float3(fu.x, fu.y, fu.z);
+            w = make_float3(fw.x, fw.y, fw.z);
         }
     }
 
@@ -257,11 +262,13 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
     //         the dual points p_j = (d.u, d.v) / (|d|^2 / 2) of the cell (2dClip.cuh:112-126)
     float px[K + 4], py[K + 4];
     int slots[K];
-    if (valid && !fz) {
+    // the precise pass reuses a row the fast pass already proved and wrote
+    const bool use_row = fz || (PRECISE && carried >= 0);
+    if (valid && !use_row) {
         // provable seed: the previous (K+2)-th distance grown by twice the
         // largest site displacement of the previous iteration
         float tau = inf, ctau = a.tau0;
-        if (!PRECISE && !eval_only && *a.seed_ok) {
+        if (!eval_only && *a.seed_ok) {
             const float rp = sqrtf(a.rad2[i]);
             const float r0 = rp + 2.f * a.dmax01[par ^ 1];
             tau = r0 * r0 * (1.f + 1e-5f);
@@ -285,7 +292,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             float4 pj;
-            if (!fz) {
+            if (!use_row) {
                 const int sl = slots[k] >= 0 ? slots[k] : t;
                 pj = a.sg.pts[sl];
                 const int id = __float_as_int(pj.w);
@@ -317,6 +324,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
         px[K + 1] = 0.f; py[K + 1] = iR;
         px[K + 2] = -iR; py[K + 2] = 0.f;
         px[K + 3] = 0.f; py[K + 3] = -iR;
+        // near-collinear orientation tests fall back to FP64 inline in both passes
         const CellOut co = dual_hull_cell<K + 4, PRECISE>(px, py);
         if (!PRECISE && co.defer) defer = true;
         cell_ok = co.ok;
@@ -651,12 +659,10 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
     const int nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (nn >= 0) {
-        const float4 nv = a.m.N[nn];
-        site_n[i] = nv;
-        const float l2 = nv.x * nv.x + nv.y * nv.y + nv.z * nv.z;
-        const float nzr = fabsf(nv.z) * rsqrtf(l2 + 1e-30f);
-        if (l2 < 1e-12f || fabsf(nzr - 0.9f) < 1e-5f) frame_fp64_vertex(a.m, nn, u, w);
-        else frame_f32(nv.x, nv.y, nv.z, u, w);
+        site_n[i] = a.m.N[nn];
+        const float4 fu = a.m.FU[nn], fw = a.m.FW[nn];
+        u = make_float3(fu.x, fu.y, fu.z);
+        w = make_float3(fw.x, fw.y, fw.z);
     }
     int slots[K];
     float lk;
@@ -821,6 +827,10 @@ void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipS
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s) {
     hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nV, 256)), dim3(256), 0, s, V, F, vf_off, vf_faces, nV, N);
+}
+
+void launch_frames(const MeshDev& m, float4* FU, float4* FW, hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_frames, dim3(nblk(m.nV, 256)), dim3(256), 0, s, m, FU, FW);
 }
 
 int scan_tile() { return kScanTile; }
