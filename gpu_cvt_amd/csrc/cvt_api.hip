@@ -72,6 +72,7 @@ struct cvt_job {
     int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
+    float *radF = nullptr, *radV = nullptr;  // per-site vertex-search seed radii
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
@@ -297,6 +298,8 @@ JobDesc make_desc(cvt_job* j) {
     a.defer_list = j->defer_list;
     a.rad2 = j->rad2;
     a.seed_ok = j->seed_ok;
+    a.radF = j->radF;
+    a.radV = j->radV;
     a.dmax01 = j->dmax;
     a.nS = j->nS;
     a.R = (float)j->R;
@@ -501,7 +504,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
         (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
-        (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
+        (rc = j->alloc(&j->radF, nS)) || (rc = j->alloc(&j->radV, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {
@@ -527,6 +531,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill(j->has_prev, 1, 0, s);
     launch_fill(j->parity, 1, 0, s);
     launch_fill(j->seed_ok, 1, 0, s);
+    launch_fill((int*)j->radF, nS, 0x7f800000, s);  // +inf: no seed yet
+    launch_fill((int*)j->radV, nS, 0x7f800000, s);
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatWords, 0, s);

@@ -113,6 +113,18 @@ __device__ __forceinline__ void scan_run(const float4* __restrict__ pts, int s0,
     }
 }
 
+// x-cells of the 3x3x3 block that can hold a point within sqrt(tau): the
+// neighbouring cell on a side is dropped when its x-gap alone exceeds it.
+__device__ __forceinline__ void block_xrange(const GridP& g, float qx, int cx, float tau, int& x0, int& x1) {
+    x0 = max(cx - 1, 0);
+    x1 = min(cx + 1, g.dx - 1);
+    const float lim = tau * (1.f + 1e-5f);
+    const float gl = axis_gap(qx, g.ox, g.h, cx - 1, -1, g.eps_abs);
+    const float gr = axis_gap(qx, g.ox, g.h, cx + 1, 1, g.eps_abs);
+    if (gl * gl > lim) x0 = cx;
+    if (gr * gr > lim) x1 = cx;
+}
+
 // Distance from q to the outside of the block of radius r around its cell
 // (+inf where the block reaches the grid edge on every side).
 __device__ __forceinline__ float block_bound(const GridP& g, float qx, float qy, float qz, int cx, int cy, int cz,
@@ -141,7 +153,8 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
     const float h = g.h;
     T.init(tau);
     {
-        const int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.dx - 1);
+        int x0, x1;
+        block_xrange(g, qx, cx, tau, x0, x1);
         int r0[9], r1[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
@@ -204,7 +217,8 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __r
     int cx, cy, cz;
     cell_of(g, qx, qy, qz, cx, cy, cz);
     const float h = g.h;
-    const int x0 = max(cx - 1, 0), x1 = min(cx + 1, g.dx - 1);
+    int x0, x1;
+    block_xrange(g, qx, cx, tau, x0, x1);
     int r0[9], r1[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -445,16 +459,23 @@ struct CellOut {
     bool defer;  // an orientation test FP32 could not decide (!PRECISE only)
 };
 
-template <int M, bool PRECISE>
-__device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
+// Gift-wrapping walk of the dual hull over the points j < NB and the last 4
+// (the clip square), accumulating the cell (polar dual) area, centroid and
+// second moment with compensated sums.  rin2 receives the squared distance
+// from the origin to the nearest hull edge line (the hull's inradius about
+// the origin, which the clip square keeps inside).
+template <int M, int NB, bool PRECISE>
+__device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (&py)[M], float& rin2) {
     CellOut o;
     o.ok = false;
     o.defer = false;
     o.nverts = 0;
+    rin2 = __builtin_huge_valf();
     // start: max x, then max y -- always a hull vertex
-    float sx = px[0], sy = py[0];
+    float sx = px[M - 4], sy = py[M - 4];
 #pragma unroll
-    for (int j = 1; j < M; ++j) {
+    for (int jj = 0; jj < NB + 3; ++jj) {
+        const int j = jj < NB ? jj : jj - NB + M - 3;
         const bool b = px[j] > sx || (px[j] == sx && py[j] > sy);
         sx = b ? px[j] : sx;
         sy = b ? py[j] : sy;
@@ -463,10 +484,11 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
     Kahan A2, Cx, Cy, I12;
     A2.init(); Cx.init(); Cy.init(); I12.init();
     float fvx = 0.f, fvy = 0.f, pvx = 0.f, pvy = 0.f;
-    for (int step = 0; step <= M; ++step) {
+    for (int step = 0; step <= NB + 4; ++step) {
         float bx = curx, by = cury;
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
+        for (int jj = 0; jj < NB + 4; ++jj) {
+            const int j = jj < NB ? jj : jj - NB + M - 4;
             const float wx = px[j] - curx, wy = py[j] - cury;
             const float ex = bx - curx, ey = by - cury;
             const bool isnew = (wx != 0.f) || (wy != 0.f);
@@ -491,6 +513,8 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
         if (!PRECISE && o.defer) return o;
         if (bx == curx && by == cury) return o;  // degenerate: all dual points coincide
         const float det = curx * by - cury * bx;
+        const float ex = bx - curx, ey = by - cury;
+        rin2 = fminf(rin2, det * det / (ex * ex + ey * ey));
         const float inv = 1.0f / det;
         const float vx = (by - cury) * inv, vy = (curx - bx) * inv;
         if (step == 0) {
@@ -525,6 +549,31 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
     o.cx = Cx.s * inv6A;
     o.cy = Cy.s * inv6A;
     return o;
+}
+
+// Tangent-plane cell of a site as the polar dual of the convex hull of its
+// dual points p_j = d_j / (|d_j|^2 / 2) (the half-plane x.d_j <= |d_j|^2/2 is
+// x.p_j <= 1) and of the clip square's (+-1/R, 0), (0, +-1/R).  Cell vertices
+// are the duals of consecutive hull points.  The walk first runs over the 12
+// nearest neighbours and the square; when every farther neighbour's dual
+// point lies strictly inside that hull's inscribed circle about the origin,
+// the full hull is the same polygon and the walk is done, otherwise it is
+// repeated over all points.
+template <int M, bool PRECISE>
+__device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
+    constexpr int kNear = 12;
+    float rin2;
+    if constexpr (M - 4 > kNear + 4) {
+        const CellOut o = hull_walk<M, kNear, PRECISE>(px, py, rin2);
+        if (o.ok) {
+            const float lim = rin2 * (1.f - 1e-4f);
+            bool inside = true;
+#pragma unroll
+            for (int j = kNear; j < M - 4; ++j) inside = inside && (px[j] * px[j] + py[j] * py[j] < lim);
+            if (inside) return o;
+        }
+    }
+    return hull_walk<M, M - 4, PRECISE>(px, py, rin2);
 }
 
 // Labelled walk of the same dual hull for the remesh stage: calls

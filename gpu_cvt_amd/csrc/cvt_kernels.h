@@ -49,6 +49,8 @@ struct StepArgs {
     int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
     float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
     int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
+    float* radF;          // nS: 3rd-nearest-vertex distance of the site (frame search seed), +inf if none
+    float* radV;          // nS: 7th-nearest-vertex distance of the centroid + |projection - centroid|, +inf if none
     float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     int nS;
     float R;

@@ -243,10 +243,24 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (valid) {
         if (!fz) {
+            // seed: the 3rd-nearest-vertex distance of the previous position
+            // plus the site's displacement bounds the new one (triangle inequality)
+            float tau1 = inf;
+            if (!eval_only && *a.seed_ok) {
+                const float rf = a.radF[i];
+                if (rf < inf) {
+                    const float4 sp0 = a.S01[par ^ 1][i];
+                    const float dq = sqrtf((q.x - sp0.x) * (q.x - sp0.x) + (q.y - sp0.y) * (q.y - sp0.y) +
+                                           (q.z - sp0.z) * (q.z - sp0.z));
+                    const float rr = (rf + dq) * (1.f + 1e-4f);
+                    tau1 = rr * rr;
+                }
+            }
             int s1[1];
             float l1;
-            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, inf, s1, l1);
+            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1);
             defer = defer || k1 < 0;
+            if (!eval_only && k1 >= 0) a.radF[i] = l1 < inf ? sqrtf(l1) : inf;
             nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
         } else {
             nn = a.knnV[(size_t)i * KP];  // pinned at the freeze iteration
@@ -348,15 +362,27 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
 
     // ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection
     float3 sp = c3;
+    float lv = inf;  // 7th-nearest-vertex distance^2 of the centroid (when searched)
     int vl[KP];
 #pragma unroll
     for (int k = 0; k < KP; ++k) vl[k] = -1;
     if (valid && !eval_only) {
         if (!fz) {
             if (finite3(c3.x, c3.y, c3.z)) {
+                // seed: the previous centroid's 7th-nearest-vertex distance plus
+                // |projection - centroid| (radV) plus |c3 - site| bounds this one
+                float tauv = inf;
+                if (!eval_only && *a.seed_ok) {
+                    const float rv = a.radV[i];
+                    if (rv < inf) {
+                        const float dc = sqrtf((c3.x - q.x) * (c3.x - q.x) + (c3.y - q.y) * (c3.y - q.y) +
+                                               (c3.z - q.z) * (c3.z - q.z));
+                        const float rr = (rv + dc) * (1.f + 1e-4f);
+                        tauv = rr * rr;
+                    }
+                }
                 int sl[KP];
-                float lv;
-                const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, inf, sl,
+                const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, tauv, sl,
                                                       lv);
                 if (!PRECISE && kk < 0) {
                     r.same = same;
@@ -401,6 +427,11 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
             }
         }
         if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
+        if (!fz) {
+            const float dp = sqrtf((sp.x - c3.x) * (sp.x - c3.x) + (sp.y - c3.y) * (sp.y - c3.y) +
+                                   (sp.z - c3.z) * (sp.z - c3.z));
+            a.radV[i] = (lv < inf && finite3(c3.x, c3.y, c3.z) && dp < inf) ? sqrtf(lv) + dp : inf;
+        }
     }
 
     // ---- 6. write the new site and the freeze state machine (freezetest.cuh:270-327, :451-466)
