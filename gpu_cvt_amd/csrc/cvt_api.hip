@@ -18,6 +18,7 @@
 
 #include "cvt.h"
 #include "cvt_bvh.h"
+#include "cvt_internal.h"
 #include "cvt_kernels.h"
 
 using namespace cvt;
@@ -26,9 +27,8 @@ namespace {
 thread_local std::string g_err;
 }  // namespace
 
-// Records the calling thread's error message (cvt_last_error) and returns code.
 namespace cvt {
-__attribute__((visibility("hidden"))) int fail(int code, const char* fmt, ...) {
+int fail(int code, const char* fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -136,7 +136,8 @@ int bvh_alloc(cvt_bvh* b, void** p, size_t bytes) {
 }
 
 // Build an LBVH over (V, F) host arrays (FP32 on the device).
-int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_tri, cvt_bvh** out) {
+int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_tri, cvt_bvh** out,
+             const float* box_lo = nullptr, const float* box_hi = nullptr) {
     cvt_bvh* b = new cvt_bvh();
     b->nF = nF;
     auto bail = [&](int rc) { delete b; return rc; };
@@ -145,6 +146,10 @@ int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
     std::vector<float4> tri;
     if (!dev_tri) tri.resize((size_t)3 * nF);
+    if (dev_tri && box_lo) {  // device face table and a known box: no host pass
+        for (int a = 0; a < 3; ++a) { lo[a] = box_lo[a]; hi[a] = box_hi[a]; }
+        nF = -nF;  // skip the loop below
+    }
     for (int f = 0; f < nF; ++f)
         for (int k = 0; k < 3; ++k) {
             const int v = Fh[3 * f + k];
@@ -158,6 +163,7 @@ int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_
             }
         }
     (void)nV;
+    if (nF < 0) nF = -nF;
     int rc;
     void* p;
     if (dev_tri) {
@@ -813,18 +819,9 @@ int cvt_job_energy(cvt_job* j, double* E) {
     return 0;
 }
 
-// Device remesh into temporaries owned by `tmp` (freed by the caller): dF
-// receives 3 * nkeep vertex ids.
-struct DevTemps {
-    std::vector<void*> p;
-    void* get(size_t bytes) {
-        void* q = nullptr;
-        if (hipMalloc(&q, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
-        p.push_back(q);
-        return q;
-    }
-    ~DevTemps() { for (void* q : p) (void)hipFree(q); }
-};
+// Device remesh into temporaries from `T` (the caller synchronises before
+// reusing them): dF receives 3 * nkeep vertex ids.
+using DevTemps = TempPool;
 
 static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int* nkeep_out) {
     if (j->nS >= (1 << 21)) return fail(CVT_EINVAL, "remesh keys pack site ids in 21 bits (nS < 2097152)");
@@ -891,10 +888,18 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
 static int ensure_bvh(cvt_job* j);
 
 int cvt_job_hausdorff(cvt_job* j, double eps, double* h_out, int* nF_out) {
+    TempPool T;
+    return job_hausdorff(j, eps, T, h_out, nF_out);
+}
+
+}  // extern "C"
+
+namespace cvt {
+int job_hausdorff(cvt_job* j, double eps, TempPool& T, double* h_out, int* nF_out) {
     if (!j || !h_out) return fail(CVT_EINVAL, "null argument");
     int rc;
     if ((rc = ensure_bvh(j))) return rc;
-    DevTemps T;
+    T.reset();
     int* dF = nullptr;
     int nkeep = 0;
     if ((rc = remesh_device(j, eps, T, &dF, &nkeep))) return rc;
@@ -917,6 +922,9 @@ int cvt_job_hausdorff(cvt_job* j, double eps, double* h_out, int* nF_out) {
     *h_out = std::sqrt((double)m);
     return 0;
 }
+}  // namespace cvt
+
+extern "C" {
 
 void cvt_free(void* p) { std::free(p); }
 
@@ -1041,14 +1049,14 @@ void cvt_batch_destroy(cvt_batch* b) {
 
 static int ensure_bvh(cvt_job* j) {
     if (!j->bvh) {
-        // the job keeps only FP32 vertices on the device: rebuild host FP64 copies of them
-        std::vector<float4> V4(j->nV);
-        std::vector<int> Fv((size_t)3 * j->nF);
-        HIPCHK(hipMemcpy(V4.data(), j->V, sizeof(float4) * j->nV, hipMemcpyDeviceToHost));
-        HIPCHK(hipMemcpy(Fv.data(), j->F, sizeof(int) * Fv.size(), hipMemcpyDeviceToHost));
-        std::vector<double> Vd((size_t)3 * j->nV);
-        for (int v = 0; v < j->nV; ++v) { Vd[3 * v] = V4[v].x; Vd[3 * v + 1] = V4[v].y; Vd[3 * v + 2] = V4[v].z; }
-        int rc = bvh_make(Vd.data(), j->nV, Fv.data(), j->nF, j->tri, &j->bvh);
+        // the job's face table is already on the device; its vertex bbox
+        // (rounded outward to FP32) bounds the Morton quantisation
+        float lo[3], hi[3];
+        for (int a = 0; a < 3; ++a) {
+            lo[a] = std::nextafter((float)j->lo[a], -INFINITY);
+            hi[a] = std::nextafter((float)j->hi[a], INFINITY);
+        }
+        int rc = bvh_make(nullptr, j->nV, nullptr, j->nF, j->tri, &j->bvh, lo, hi);
         if (rc) return rc;
     }
     return 0;

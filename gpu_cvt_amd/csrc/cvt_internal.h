@@ -1,0 +1,57 @@
+// cvt_internal.h -- library-internal helpers shared by the C ABI translation
+// units (not part of the public header).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "cvt.h"
+
+namespace cvt {
+
+// Records the calling thread's error message (cvt_last_error) and returns code.
+__attribute__((visibility("hidden"))) int fail(int code, const char* fmt, ...);
+
+// Device temporaries handed out in call order.  reset() starts a new round of
+// get() calls that reuse the previous round's buffers (grown when too small);
+// the owner must have synchronised all work that used them.  A pool that is
+// not reused simply frees everything when destroyed.
+struct TempPool {
+    struct Slot {
+        void* p = nullptr;
+        size_t cap = 0;
+    };
+    std::vector<Slot> slots;
+    size_t next = 0;
+    void reset() { next = 0; }
+    void* get(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (next == slots.size()) slots.emplace_back();
+        Slot& s = slots[next++];
+        if (s.cap < bytes) {
+            if (s.p) (void)hipFree(s.p);
+            s.p = nullptr;
+            s.cap = 0;
+            const size_t want = std::max(bytes, s.cap + s.cap / 2);
+            if (hipMalloc(&s.p, want) != hipSuccess) return nullptr;
+            s.cap = want;
+        }
+        return s.p;
+    }
+    size_t bytes() const {
+        size_t b = 0;
+        for (const Slot& s : slots) b += s.cap;
+        return b;
+    }
+    ~TempPool() {
+        for (Slot& s : slots)
+            if (s.p) (void)hipFree(s.p);
+    }
+};
+
+// cvt_job_hausdorff with caller-owned temporaries (the serving loop keeps one
+// pool for all of its convergence checks).
+__attribute__((visibility("hidden"))) int job_hausdorff(cvt_job* j, double eps, TempPool& pool, double* h_out, int* nF_out);
+
+}  // namespace cvt

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -23,10 +24,8 @@
 #include <vector>
 
 #include "cvt.h"
+#include "cvt_internal.h"
 
-namespace cvt {
-int fail(int code, const char* fmt, ...);
-}
 using cvt::fail;
 
 namespace {
@@ -46,6 +45,7 @@ struct Request {
     unsigned long long bytes = 0;
     double diag = 0;
     int next_check = 0;
+    double checked_h = -1;  // this epoch's Hausdorff check (-1: none)
     std::vector<double> S_out;
 };
 
@@ -65,7 +65,9 @@ struct cvt_server {
     unsigned long long next_ticket = 1;
     unsigned long long used = 0, peak = 0;
     bool stop = false;
-    std::thread ingest_th, sched_th;
+    std::vector<std::thread> ingest_th;
+    std::thread sched_th;
+    std::vector<cvt::TempPool> check_pools;  // device temporaries, one set per check worker
     // counters
     double site_iters = 0, busy_s = 0;
     long epochs = 0, rebuilds = 0, checks = 0;
@@ -199,9 +201,13 @@ void cvt_server::sched_loop() {
             } else {
                 for (cvt_job* j : jobs) cvt_job_step(j, cfg.epoch_iters, nullptr);
             }
-            // convergence at the epoch boundary
-            std::vector<Request*> keep;
-            for (Request* r : g.members) {
+            // convergence at the epoch boundary: gate each job, then run the
+            // Hausdorff checks of the gated jobs in parallel host threads (each
+            // on its job's own stream, with its own device temporaries)
+            std::vector<Request*> check;
+            std::vector<char> is_last(g.members.size());
+            for (size_t k = 0; k < g.members.size(); ++k) {
+                Request* r = g.members[k];
                 {
                     std::lock_guard<std::mutex> lk(mu);  // res is read by cvt_server_poll
                     r->res.iterations += cfg.epoch_iters;
@@ -213,18 +219,37 @@ void cvt_server::sched_loop() {
                 const bool gate = r->res.iterations >= cfg.min_check_iters &&
                                   ((double)st.max_disp <= cfg.check_disp_rel * r->diag ||
                                    st.frozen_total >= cfg.check_frozen_frac * r->res.nS);
-                const bool last = r->res.iterations >= r->max_iters;
-                int st_done = 0;
-                if (gate || last) {
+                is_last[k] = r->res.iterations >= r->max_iters;
+                r->checked_h = -1;
+                if (gate || is_last[k]) check.push_back(r);
+            }
+            const int nth = (int)std::min<size_t>(check_pools.size(), check.size());
+            std::atomic<size_t> next{0};
+            std::atomic<long> nok{0};
+            auto worker = [&](int w) {
+                for (size_t c = next++; c < check.size(); c = next++) {
                     double h = -1;
-                    if (cvt_job_hausdorff(r->job, 1e-6, &h, nullptr) == CVT_OK) {
-                        std::lock_guard<std::mutex> lk(mu);
-                        r->res.hausdorff = h;
-                        ++ncheck;
+                    if (cvt::job_hausdorff(check[c]->job, 1e-6, check_pools[w], &h, nullptr) == CVT_OK) {
+                        check[c]->checked_h = h;
+                        ++nok;
                     }
-                    if (h >= 0 && h <= r->tol_rel * r->diag) st_done = 2;
-                    else if (last) st_done = 3;
                 }
+            };
+            std::vector<std::thread> ths;
+            for (int w = 1; w < nth; ++w) ths.emplace_back(worker, w);
+            if (!check.empty()) worker(0);
+            for (std::thread& t : ths) t.join();
+            ncheck += nok;
+            std::vector<Request*> keep;
+            for (size_t k = 0; k < g.members.size(); ++k) {
+                Request* r = g.members[k];
+                int st_done = 0;
+                if (r->checked_h >= 0) {
+                    std::lock_guard<std::mutex> lk(mu);
+                    r->res.hausdorff = r->checked_h;
+                }
+                if (r->checked_h >= 0 && r->checked_h <= r->tol_rel * r->diag) st_done = 2;
+                else if (is_last[k]) st_done = 3;
                 if (st_done) {
                     done.push_back(r);
                     status.push_back(st_done);
@@ -265,6 +290,8 @@ void cvt_server_config_default(cvt_server_config* c) {
     c->min_check_iters = 10;
     c->check_disp_rel = 1e-3;
     c->check_frozen_frac = 0.5;
+    c->ingest_threads = 4;
+    c->check_threads = 8;
 }
 
 int cvt_server_create(const cvt_server_config* cfg, cvt_server** out) {
@@ -274,7 +301,9 @@ int cvt_server_create(const cvt_server_config* cfg, cvt_server** out) {
     else cvt_server_config_default(&s->cfg);
     if (s->cfg.epoch_iters < 1) s->cfg.epoch_iters = 1;
     s->t0 = Clock::now();
-    s->ingest_th = std::thread([s] { s->ingest_loop(); });
+    s->check_pools.resize(std::max(1, std::min(s->cfg.check_threads, 64)));
+    const int nth = std::max(1, std::min(s->cfg.ingest_threads, 64));
+    for (int k = 0; k < nth; ++k) s->ingest_th.emplace_back([s] { s->ingest_loop(); });
     s->sched_th = std::thread([s] { s->sched_loop(); });
     *out = s;
     return CVT_OK;
@@ -357,7 +386,7 @@ void cvt_server_destroy(cvt_server* s) {
         s->stop = true;
         s->cv.notify_all();
     }
-    s->ingest_th.join();
+    for (std::thread& t : s->ingest_th) t.join();
     s->sched_th.join();
     for (auto& kv : s->all) {
         if (kv.second->job) cvt_job_destroy(kv.second->job);

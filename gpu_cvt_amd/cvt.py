@@ -70,7 +70,8 @@ class JobInfo(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_ulonglong), ("epoch_iters", C.c_int), ("min_check_iters", C.c_int),
-        ("check_disp_rel", C.c_double), ("check_frozen_frac", C.c_double), ("reserved", C.c_int * 8),
+        ("check_disp_rel", C.c_double), ("check_frozen_frac", C.c_double), ("ingest_threads", C.c_int),
+        ("check_threads", C.c_int), ("reserved", C.c_int * 6),
     ]
 
 

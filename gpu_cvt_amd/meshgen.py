@@ -14,6 +14,8 @@ from __future__ import annotations
 
 import math
 
+import zlib
+
 import numpy as np
 
 
@@ -268,3 +270,55 @@ def batch_workload(njobs: int = 256, seed: int = 7, nf_range=(5_000, 400_000), n
         S = sample_surface(V, F, nsites, seed=seed * 100003 + k)
         jobs.append((V, F, S, f"{key[0]}{key[1]}"))
     return jobs
+
+
+def serve_trace(n_arrivals: int = 2000, rate: float = 200.0, seed: int = 8, large_frac: float = 0.05,
+                nf_range=(5_000, 400_000), ns_range=(1_000, 100_000)):
+    """Request trace for the serving loop: Poisson arrivals at `rate` jobs/s;
+    each request is drawn from the batched-form size distribution (mesh type
+    uniform over icosphere / torus / displaced sphere, triangles log-uniform in
+    nf_range, sites log-uniform in ns_range capped at triangles/2), or with
+    probability large_frac is a large job (the 1,997,568-triangle displaced
+    sphere with 1M sites).  Meshes are shared by requests whose size rounds to
+    the same tessellation and site sets by requests whose site count rounds to
+    the same of 16 log-spaced buckets per decade, so generation stays bounded.
+    Returns a list of (t_arrive_s, V, F, S, tag), ordered by arrival."""
+    rng = _rng(seed)
+    t = 0.0
+    mcache, scache = {}, {}
+    large = None
+    out = []
+    for k in range(n_arrivals):
+        t += float(rng.exponential(1.0 / rate))
+        if rng.uniform() < large_frac:
+            if large is None:
+                V, F, S, _K, _it, _fz = workload("dsphere_2m_1m")
+                large = (V, F, S)
+            out.append((t, *large, "large"))
+            continue
+        typ = int(rng.integers(0, 3))
+        nf = float(np.exp(rng.uniform(np.log(nf_range[0]), np.log(nf_range[1]))))
+        ns = float(np.exp(rng.uniform(np.log(ns_range[0]), np.log(ns_range[1]))))
+        if typ == 0:
+            key = ("ico", int(np.clip(round(np.log(nf / 20.0) / np.log(4.0)), 4, 7)))
+        elif typ == 1:
+            key = ("torus", max(8, int(round(np.sqrt(nf / 5.0) / 4.0)) * 4))
+        else:
+            key = ("dsphere", max(8, int(round(np.sqrt(nf / 12.0) / 8.0)) * 8))
+        if key not in mcache:
+            if key[0] == "ico":
+                mcache[key] = icosphere(key[1])
+            elif key[0] == "torus":
+                mcache[key] = torus(1.0, 0.3, int(2.5 * key[1]), key[1], jitter=1e-3, seed=key[1])
+            else:
+                mcache[key] = displaced_sphere(key[1], seed=key[1], crease_seed=key[1] + 1)
+        V, F = mcache[key]
+        nsites = int(min(ns, len(F) / 2))
+        b = int(round(np.log10(max(nsites, 1)) * 16))
+        skey = (key, b)
+        if skey not in scache:
+            n_b = int(min(round(10 ** (b / 16.0)), len(F) // 2))
+            seed_s = (seed * 1_000_003 + zlib.crc32(repr(skey).encode())) & 0x7FFFFFFF
+            scache[skey] = sample_surface(V, F, max(n_b, 64), seed=seed_s)
+        out.append((t, V, F, scache[skey], f"{key[0]}{key[1]}"))
+    return out

@@ -202,7 +202,9 @@ typedef struct cvt_server_config {
     int min_check_iters;      /* no check before this many iterations (default 10) */
     double check_disp_rel;    /* gate: max_disp <= this * bbox_diag (default 1e-3) */
     double check_frozen_frac; /* gate: frozen_total >= this * nS (default 0.5) */
-    int reserved[8];
+    int ingest_threads;       /* threads creating device jobs from requests (default 4) */
+    int check_threads;        /* threads running one epoch's convergence checks (default 8) */
+    int reserved[6];
 } cvt_server_config;
 
 /* status: 0 queued, 1 running, 2 converged, 3 reached max_iters, 4 failed.
