@@ -37,6 +37,23 @@ def algorithmic_bytes(nS: int, nV: int, nF: int, K: int) -> int:
     return nS * (24 + 4 * K) + 28 * nV + 24 * nF + 4
 
 
+def traffic_from_profile(workload: str):
+    """HBM bytes per launch of the fused Lloyd kernel, as recorded by the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (each
+    counter scaled by its copy-kernel calibration factor); (None, None) when no
+    summary exists for the workload.  The counters cannot be read from inside
+    this process, so the figure comes from the profile run of the same command."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"kb_lloyd_{workload}_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None, None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1].strip()), os.path.relpath(path, os.path.dirname(path) + "/..")
+    return None, None
+
+
 def cpu_baseline(V, F, S, K: int, budget_s: float = 20.0):
     """Time the FP64 oracle's full Lloyd step on host threads (bounded sample)."""
     from oracle import oracle as orc
@@ -288,8 +305,9 @@ def main(argv=None) -> int:
     prof = job.profile(args.profile_iters)
     alg = algorithmic_bytes(len(S), len(V), len(F), K)
     achieved = alg / (prof["lloyd_ms"] * 1e-3) / 1e9
+    traffic, traffic_src = traffic_from_profile(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic}
     whole_iter_gbs = alg / (ms_per_step * 1e-3) / 1e9
     print(f"[bench] timed {args.steps} steps: {ms:.3f} ms device ({wall * 1e3:.1f} ms wall); "
           f"profile {prof}; kernel {achieved:.1f} GB/s; whole iteration {whole_iter_gbs:.1f} GB/s",
@@ -321,6 +339,8 @@ def main(argv=None) -> int:
         "roofline": roofline,
         "cpu_baseline": cpu,
         "whole_iteration_hbm_frac": round(whole_iter_gbs / HBM_PEAK_GBS, 5),
+        "algorithmic_bytes_per_launch": alg,
+        "traffic_source": traffic_src,
         "stage_ms": prof,
     }
     print(json.dumps(out), flush=True)

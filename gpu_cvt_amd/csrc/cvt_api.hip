@@ -670,29 +670,33 @@ int cvt_job_step_timed(cvt_job* j, int iters, double* ms) {
     return run_iters(j, iters, ms);
 }
 
-int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, double* iter_ms) {
+int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, double* precise_ms, double* iter_ms) {
     if (!j) return fail(CVT_EINVAL, "null job");
     if (iters <= 0) return fail(CVT_EINVAL, "iters must be > 0");
     hipStream_t s = j->stream;
-    hipEvent_t ev[4];
+    hipEvent_t ev[5];
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
-    double g = 0, l = 0, t = 0;
+    double g = 0, l = 0, pr = 0, t = 0;
     for (int it = 0; it < iters; ++it) {
         HIPCHK(hipEventRecord(ev[0], s));
         launch_batch_grid(j->tables, 0, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s);
+        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 1);
         HIPCHK(hipEventRecord(ev[2], s));
-        launch_batch_end(j->tables, s);
+        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 2);
         HIPCHK(hipEventRecord(ev[3], s));
-        HIPCHK(hipEventSynchronize(ev[3]));
-        float f01 = 0, f12 = 0, f03 = 0;
+        launch_batch_end(j->tables, s);
+        HIPCHK(hipEventRecord(ev[4], s));
+        HIPCHK(hipEventSynchronize(ev[4]));
+        float f01 = 0, f12 = 0, f23 = 0, f04 = 0;
         HIPCHK(hipEventElapsedTime(&f01, ev[0], ev[1]));
         HIPCHK(hipEventElapsedTime(&f12, ev[1], ev[2]));
-        HIPCHK(hipEventElapsedTime(&f03, ev[0], ev[3]));
+        HIPCHK(hipEventElapsedTime(&f23, ev[2], ev[3]));
+        HIPCHK(hipEventElapsedTime(&f04, ev[0], ev[4]));
         g += f01;
         l += f12;
-        t += f03;
+        pr += f23;
+        t += f04;
         j->cur ^= 1;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
@@ -700,6 +704,7 @@ int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, do
     j->iters_done += iters;
     if (grid_ms) *grid_ms = g / iters;
     if (lloyd_ms) *lloyd_ms = l / iters;
+    if (precise_ms) *precise_ms = pr / iters;
     if (iter_ms) *iter_ms = t / iters;
     return 0;
 }

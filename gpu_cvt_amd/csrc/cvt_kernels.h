@@ -128,7 +128,8 @@ void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t
 void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s);
 // Pieces of an iteration, for per-stage timing.
 void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s);
-void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s);
+// which: 1 = the FP32 pass, 2 = the FP64-resolving pass over its deferred sites, 3 = both
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
                          float4* site_n, hipStream_t s);

@@ -885,24 +885,26 @@ void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
 }
 
 template <int K>
-static void launch_lloyd_k(const BatchTables& bt, int eval_only, hipStream_t s) {
-    hipLaunchKernelGGL((kb_lloyd<K, 5>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
-                       eval_only);
-    hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), dim3(bt.njobs * kPreciseBlocks), dim3(256), 0, s, bt.descs,
-                       eval_only);
+static void launch_lloyd_k(const BatchTables& bt, int eval_only, int which, hipStream_t s) {
+    if (which & 1)
+        hipLaunchKernelGGL((kb_lloyd<K, 5>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
+                           eval_only);
+    if (which & 2)
+        hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), dim3(bt.njobs * kPreciseBlocks), dim3(256), 0, s, bt.descs,
+                           eval_only);
 }
 
-void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s) {
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which) {
     (void)kproj;
     switch (K) {
-        case 4: launch_lloyd_k<4>(bt, eval_only, s); break;
-        case 6: launch_lloyd_k<6>(bt, eval_only, s); break;
-        case 8: launch_lloyd_k<8>(bt, eval_only, s); break;
-        case 12: launch_lloyd_k<12>(bt, eval_only, s); break;
-        case 16: launch_lloyd_k<16>(bt, eval_only, s); break;
-        case 20: launch_lloyd_k<20>(bt, eval_only, s); break;
-        case 24: launch_lloyd_k<24>(bt, eval_only, s); break;
-        case 32: launch_lloyd_k<32>(bt, eval_only, s); break;
+        case 4: launch_lloyd_k<4>(bt, eval_only, which, s); break;
+        case 6: launch_lloyd_k<6>(bt, eval_only, which, s); break;
+        case 8: launch_lloyd_k<8>(bt, eval_only, which, s); break;
+        case 12: launch_lloyd_k<12>(bt, eval_only, which, s); break;
+        case 16: launch_lloyd_k<16>(bt, eval_only, which, s); break;
+        case 20: launch_lloyd_k<20>(bt, eval_only, which, s); break;
+        case 24: launch_lloyd_k<24>(bt, eval_only, which, s); break;
+        case 32: launch_lloyd_k<32>(bt, eval_only, which, s); break;
         default: break;
     }
 }

@@ -156,7 +156,7 @@ def lib():
     L.cvt_job_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.cvt_job_point_mesh_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
-                                  C.POINTER(C.c_double)]
+                                  C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.cvt_job_hausdorff.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cvt_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
@@ -275,10 +275,12 @@ class Job:
         return ms.value
 
     def profile(self, iters: int) -> dict:
-        """Mean device ms per iteration: site-grid rebuild, Lloyd kernel, whole iteration."""
-        g, l, t = C.c_double(), C.c_double(), C.c_double()
-        _check("cvt_job_profile", lib().cvt_job_profile(self.handle, iters, C.byref(g), C.byref(l), C.byref(t)))
-        return {"grid_ms": g.value, "lloyd_ms": l.value, "iter_ms": t.value}
+        """Mean device ms per iteration: site-grid rebuild, FP32 Lloyd kernel,
+        FP64-resolving pass, whole iteration."""
+        g, l, pr, t = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        _check("cvt_job_profile", lib().cvt_job_profile(self.handle, iters, C.byref(g), C.byref(l), C.byref(pr),
+                                                        C.byref(t)))
+        return {"grid_ms": g.value, "lloyd_ms": l.value, "precise_ms": pr.value, "iter_ms": t.value}
 
     # ----------------------------------------------------------------- state
     def sites(self) -> np.ndarray:

@@ -103,9 +103,12 @@ int cvt_job_step_timed(cvt_job* job, int iters, double* device_ms);
 /* Per-stage device timing: runs `iters` iterations launched eagerly on the
  * job's stream with HIP events around each stage; returns the mean device
  * milliseconds per iteration of the site-grid rebuild (grid_ms), of the fused
- * Lloyd kernel (lloyd_ms) and of the whole iteration (iter_ms).  Replaces the
- * per-stage cudaEvent pairs of main.cu:692-803. */
-int cvt_job_profile(cvt_job* job, int iters, double* grid_ms, double* lloyd_ms, double* iter_ms);
+ * FP32 Lloyd kernel (lloyd_ms), of the FP64-resolving pass over the sites the
+ * FP32 pass deferred (precise_ms) and of the whole iteration (iter_ms).  Any
+ * output may be null.  Replaces the per-stage cudaEvent pairs of
+ * main.cu:692-803. */
+int cvt_job_profile(cvt_job* job, int iters, double* grid_ms, double* lloyd_ms, double* precise_ms,
+                    double* iter_ms);
 
 /* State access (host arrays).  knn rows are in (d^2, index) order; knnv are
  * the pinned/last vertex lists (nS*kproj); frozen/streak one byte per site. */
