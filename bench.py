@@ -249,6 +249,9 @@ def main(argv=None) -> int:
                     help="plain: the headline 1M-site job; batch: 256 mixed jobs, batched vs one at a time; "
                          "serve: Poisson request trace through the serving loop vs FIFO plain path")
     ap.add_argument("--arrivals", type=int, default=2000, help="serve mode: requests in the trace")
+    ap.add_argument("--hausdorff-samples", type=int, default=0,
+                    help="plain mode: after the timed steps, symmetric Hausdorff remesh<->input with this many "
+                         "area-uniform samples per surface (0 = skip)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve mode: arrivals per second")
     ap.add_argument("--jobs", type=int, default=256, help="batch mode: number of jobs")
     ap.add_argument("--site-cell", type=float, default=0.0, help="site grid cell scale (0 = library default)")
@@ -302,6 +305,18 @@ def main(argv=None) -> int:
     ms_per_step = ms / args.steps
     value = len(S) * args.steps / (ms / 1e3)
 
+    quality = None
+    if args.hausdorff_samples > 0:
+        # symmetric Hausdorff between the remeshed surface and the input mesh,
+        # area-uniform samples on both, distances through device LBVHs
+        T = job.remesh()
+        Sf = job.sites()
+        h = cvt.hausdorff(Sf, T, V, F, args.hausdorff_samples, seed=3, symmetric=True)
+        h1 = cvt.hausdorff(Sf, T, V, F, args.hausdorff_samples, seed=3, symmetric=False)
+        quality = {"iterations": args.warmup + 1 + args.steps, "remeshed_triangles": int(len(T)),
+                   "hausdorff_symmetric": h, "hausdorff_remesh_to_input": h1,
+                   "bbox_diag": info["bbox_diag"], "samples_per_surface": args.hausdorff_samples}
+        print(f"[bench] quality {quality}", file=sys.stderr, flush=True)
     prof = job.profile(args.profile_iters)
     alg = algorithmic_bytes(len(S), len(V), len(F), K)
     achieved = alg / (prof["lloyd_ms"] * 1e-3) / 1e9
@@ -342,6 +357,7 @@ def main(argv=None) -> int:
         "algorithmic_bytes_per_launch": alg,
         "traffic_source": traffic_src,
         "stage_ms": prof,
+        "quality": quality,
     }
     print(json.dumps(out), flush=True)
     return 0
