@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcvt_hip.so")
+CLI = os.path.join(HERE, "gpu_cvt")
 SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip", "cvt_server.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVT_OFFLOAD_ARCH", "gfx950")
@@ -50,6 +51,15 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # the `gpu_cvt` command line (host C++ over the C ABI)
+    cli_tmp = CLI + ".tmp"
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "cvt_cli.cpp"),
+           "-I", os.path.join(ROOT, "include"), "-L", HERE, "-lcvt_hip", "-Wl,-rpath,$ORIGIN",
+           "-Wl,-rpath-link,/opt/rocm/lib", "-o", cli_tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"cli build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(cli_tmp, CLI)
     return LIB
 
 

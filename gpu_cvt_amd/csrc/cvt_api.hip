@@ -785,6 +785,26 @@ int cvt_job_set_tiers(cvt_job* j, const unsigned char* tier) {
     return 0;
 }
 
+int cvt_job_compute_tiers(cvt_job* j, int scheme, unsigned char* tier_out, double* nv_out, double* L_out) {
+    if (!j) return fail(CVT_EINVAL, "null job");
+    if (scheme != 1 && scheme != 2) return fail(CVT_EINVAL, "tier scheme must be 1 or 2");
+    const int n = j->nS;
+    TempPool T;
+    int* nn = (int*)T.get(sizeof(int) * n);
+    double* dnv = (double*)T.get(sizeof(double) * n);
+    double* dL = (double*)T.get(sizeof(double) * n);
+    if (!nn || !dnv || !dL) return fail(CVT_ENOMEM, "tier temporaries");
+    hipStream_t s = j->stream;
+    launch_batch_grid(j->tables, 1, s);
+    launch_site_tiers(j->desc, n, j->p.K, scheme, nn, j->tier, dnv, dL, s);
+    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipGetLastError());
+    if (tier_out) HIPCHK(hipMemcpy(tier_out, j->tier, n, hipMemcpyDeviceToHost));
+    if (nv_out) HIPCHK(hipMemcpy(nv_out, dnv, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (L_out) HIPCHK(hipMemcpy(L_out, dL, sizeof(double) * n, hipMemcpyDeviceToHost));
+    return 0;
+}
+
 int cvt_job_get_debug(cvt_job* j, int* nn, double* U, double* Vf, double* cent, double* area, double* moment) {
     if (!j) return fail(CVT_EINVAL, "null job");
     if (!j->p.debug) return fail(CVT_ESTATE, "job was created without params.debug");

@@ -133,6 +133,11 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
                          float4* site_n, hipStream_t s);
+// Per-site freeze tiers (scheme 1: normal variation, 5 tiers; scheme 2: plus
+// the covariance split, 6 tiers) into tier[nS]; nv_out / L_out (nullable)
+// receive the per-site scores.  The site grid must be current.
+void launch_site_tiers(const JobDesc* desc, int nS, int K, int scheme, int* nn_tmp, unsigned char* tier,
+                       double* nv_out, double* L_out, hipStream_t s);
 void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s);
 void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
                        const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);

@@ -656,6 +656,127 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     *a.parity ^= 1;
 }
 
+// ------------------------------------------------------------ curvature tiers
+// Freeze tiers from normal variation (main.cu:41-149, 640-667 restated per
+// site): n_i is the normal of site i's nearest mesh vertex (the reference's
+// sites are the vertices themselves, main.cu:493, so this is the same normal
+// there); over the site's K nearest sites j,
+//   nv = 1 - mean_j clamp(n_i . n_j, -1, 1)                     (main.cu:45-62)
+//   L  = (l3 - l2) / l3 of the covariance of the n_j, eigenvalues from two
+//        20-step power iterations with deflation                (main.cu:64-127)
+// tier = #{thresholds 0.15, 0.35, 0.55, 0.80 that nv reaches}; scheme 2 moves
+// tier 4 to tier 5 when L > 0.80 (main.cu:129-148).  FP64, no contraction, in
+// the oracle's operation order.
+__constant__ double kTierThr[4] = {0.15, 0.35, 0.55, 0.80};
+
+__global__ void k_site_nn(const JobDesc* __restrict__ desc, int* __restrict__ nn_out) {
+    const StepArgs& a = desc->a;
+    const int i = blockIdx.x * 256 + (int)threadIdx.x;
+    if (i >= a.nS) return;
+    const float4 q = a.S01[*a.parity][i];
+    int nn = -1;
+    if (finite3(q.x, q.y, q.z)) {
+        int s1[1];
+        float l1;
+        knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, __builtin_huge_valf(), s1, l1);
+        nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
+    }
+    nn_out[i] = nn;
+}
+
+__device__ __noinline__ void sym3_eig_power(const double (&C)[9], double (&lam)[3]) {
+#pragma clang fp contract(off)
+    double v[3] = {1.0, 0.0, 0.0};
+    for (int it = 0; it < 20; ++it) {
+        const double w0 = C[0] * v[0] + C[1] * v[1] + C[2] * v[2];
+        const double w1 = C[3] * v[0] + C[4] * v[1] + C[5] * v[2];
+        const double w2 = C[6] * v[0] + C[7] * v[1] + C[8] * v[2];
+        const double n = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        if (n < 1e-30) break;
+        v[0] = w0 / n; v[1] = w1 / n; v[2] = w2 / n;
+    }
+    lam[0] = v[0] * (C[0] * v[0] + C[1] * v[1] + C[2] * v[2]) + v[1] * (C[3] * v[0] + C[4] * v[1] + C[5] * v[2]) +
+             v[2] * (C[6] * v[0] + C[7] * v[1] + C[8] * v[2]);
+    double D[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) D[3 * r + c] = C[3 * r + c] - lam[0] * v[r] * v[c];
+    double u[3] = {0.0, 1.0, 0.0};
+    for (int it = 0; it < 20; ++it) {
+        const double w0 = D[0] * u[0] + D[1] * u[1] + D[2] * u[2];
+        const double w1 = D[3] * u[0] + D[4] * u[1] + D[5] * u[2];
+        const double w2 = D[6] * u[0] + D[7] * u[1] + D[8] * u[2];
+        const double n = sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        if (n < 1e-30) break;
+        u[0] = w0 / n; u[1] = w1 / n; u[2] = w2 / n;
+    }
+    lam[1] = u[0] * (D[0] * u[0] + D[1] * u[1] + D[2] * u[2]) + u[1] * (D[3] * u[0] + D[4] * u[1] + D[5] * u[2]) +
+             u[2] * (D[6] * u[0] + D[7] * u[1] + D[8] * u[2]);
+    lam[2] = C[0] + C[4] + C[8] - lam[0] - lam[1];
+}
+
+template <int K>
+__global__ __launch_bounds__(256) void k_site_tiers(const JobDesc* __restrict__ desc, const int* __restrict__ nn,
+                                                    int scheme, unsigned char* __restrict__ tier,
+                                                    double* __restrict__ nv_out, double* __restrict__ L_out) {
+#pragma clang fp contract(off)
+    const StepArgs& a = desc->a;
+    const int t = blockIdx.x * 256 + (int)threadIdx.x;
+    if (t >= a.nS) return;
+    const float4 q = a.sg.pts[t];
+    const int i = __float_as_int(q.w);
+    double nv = 0.0, L = 0.0;
+    const int vi = nn[i];
+    if (finite3(q.x, q.y, q.z) && vi >= 0) {
+        int slots[K];
+        float lk;
+        knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, __builtin_huge_valf(), slots, lk);
+        const float4 ni = a.m.N[vi];
+        double sum_cos = 0.0, mean0 = 0.0, mean1 = 0.0, mean2 = 0.0;
+        int count = 0;
+        for (int k = 0; k < K; ++k) {
+            if (slots[k] < 0) continue;
+            const int vj = nn[__float_as_int(a.sg.pts[slots[k]].w)];
+            if (vj < 0) continue;
+            const float4 nj = a.m.N[vj];
+            double c = (double)ni.x * (double)nj.x + (double)ni.y * (double)nj.y + (double)ni.z * (double)nj.z;
+            c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+            sum_cos += c;
+            mean0 += (double)nj.x;
+            mean1 += (double)nj.y;
+            mean2 += (double)nj.z;
+            ++count;
+        }
+        nv = count > 0 ? 1.0 - sum_cos / (double)count : 0.0;
+        if (scheme == 2 && count >= 2) {
+            mean0 /= count; mean1 /= count; mean2 /= count;
+            double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < K; ++k) {
+                if (slots[k] < 0) continue;
+                const int vj = nn[__float_as_int(a.sg.pts[slots[k]].w)];
+                if (vj < 0) continue;
+                const float4 nj = a.m.N[vj];
+                const double n0 = (double)nj.x - mean0, n1 = (double)nj.y - mean1, n2 = (double)nj.z - mean2;
+                C[0] += n0 * n0; C[1] += n0 * n1; C[2] += n0 * n2;
+                C[3] += n1 * n0; C[4] += n1 * n1; C[5] += n1 * n2;
+                C[6] += n2 * n0; C[7] += n2 * n1; C[8] += n2 * n2;
+            }
+            double lam[3];
+            sym3_eig_power(C, lam);
+            double x;
+            if (lam[0] > lam[1]) { x = lam[0]; lam[0] = lam[1]; lam[1] = x; }
+            if (lam[1] > lam[2]) { x = lam[1]; lam[1] = lam[2]; lam[2] = x; }
+            if (lam[0] > lam[1]) { x = lam[0]; lam[0] = lam[1]; lam[1] = x; }
+            if (lam[2] > 1e-30) L = (lam[2] - lam[1]) / lam[2];
+        }
+    }
+    int tr = 0;
+    while (tr < 4 && nv >= kTierThr[tr]) ++tr;
+    if (scheme == 2 && tr == 4 && L > 0.80) tr = 5;
+    tier[i] = (unsigned char)tr;
+    if (nv_out) nv_out[i] = nv;
+    if (L_out) L_out[i] = L;
+}
+
 // ------------------------------------------------------------ remesh stage
 // Dual triangles of the restricted Voronoi cells (near_project.cuh:246-352 and
 // buildmesh.cuh:170-201 restated): every cell vertex is the meeting point of
@@ -814,6 +935,23 @@ void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsig
         case 20: hipLaunchKernelGGL(k_remesh_cells<20>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
         case 24: hipLaunchKernelGGL(k_remesh_cells<24>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
         case 32: hipLaunchKernelGGL(k_remesh_cells<32>, g, b, 0, s, desc, knn_tmp, keys, site_n); break;
+        default: break;
+    }
+}
+
+void launch_site_tiers(const JobDesc* desc, int nS, int K, int scheme, int* nn_tmp, unsigned char* tier,
+                       double* nv_out, double* L_out, hipStream_t s) {
+    const dim3 g(nblk(nS, 256)), b(256);
+    hipLaunchKernelGGL(k_site_nn, g, b, 0, s, desc, nn_tmp);
+    switch (K) {
+        case 4: hipLaunchKernelGGL(k_site_tiers<4>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 6: hipLaunchKernelGGL(k_site_tiers<6>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 8: hipLaunchKernelGGL(k_site_tiers<8>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 12: hipLaunchKernelGGL(k_site_tiers<12>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 16: hipLaunchKernelGGL(k_site_tiers<16>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 20: hipLaunchKernelGGL(k_site_tiers<20>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 24: hipLaunchKernelGGL(k_site_tiers<24>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
+        case 32: hipLaunchKernelGGL(k_site_tiers<32>, g, b, 0, s, desc, nn_tmp, scheme, tier, nv_out, L_out); break;
         default: break;
     }
 }

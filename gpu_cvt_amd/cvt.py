@@ -29,6 +29,7 @@ EXPORTS = [
     "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy", "cvt_job_hausdorff",
     "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
     "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
+    "cvt_job_compute_tiers",
 ]
 
 
@@ -152,6 +153,7 @@ def lib():
     L.cvt_job_get_normals.argtypes = [C.c_void_p, _dp]
     L.cvt_job_set_state.argtypes = [C.c_void_p, _dpn, _ipn, _ipn, _upn, _upn, C.c_int]
     L.cvt_job_set_tiers.argtypes = [C.c_void_p, _up]
+    L.cvt_job_compute_tiers.argtypes = [C.c_void_p, C.c_int, _upn, _dpn, _dpn]
     L.cvt_job_get_debug.argtypes = [C.c_void_p, _ipn, _dpn, _dpn, _dpn, _dpn, _dpn]
     L.cvt_job_energy.argtypes = [C.c_void_p, C.POINTER(C.c_double)]
     L.cvt_job_point_mesh_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
@@ -317,6 +319,16 @@ class Job:
 
     def set_tiers(self, tier) -> None:
         _check("cvt_job_set_tiers", lib().cvt_job_set_tiers(self.handle, np.ascontiguousarray(tier, np.uint8)))
+
+    def compute_tiers(self, scheme: int):
+        """Device freeze tiers (scheme 1: 5-tier normal variation; 2: 6-tier with
+        the covariance split); stored in the job for freeze_mode 2.  Returns
+        (tier uint8, nv float64, L float64) per site."""
+        tier = np.zeros(self.nS, np.uint8)
+        nv = np.zeros(self.nS)
+        L = np.zeros(self.nS)
+        _check("cvt_job_compute_tiers", lib().cvt_job_compute_tiers(self.handle, int(scheme), tier, nv, L))
+        return tier, nv, L
 
     def debug(self) -> dict:
         n = self.nS

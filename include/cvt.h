@@ -127,6 +127,17 @@ int cvt_job_set_state(cvt_job* job, const double* S, const int* knn_prev, const 
 /* Per-site tier ids 0..5 for freeze_mode 2 (main.cu:640-667 computes them on the host). */
 int cvt_job_set_tiers(cvt_job* job, const unsigned char* tier);
 
+/* Per-site freeze tiers computed on the device from the current sites
+ * (replaces the host tiering of main.cu:41-149,640-667): n_i is the normal of
+ * site i's nearest mesh vertex (the reference's sites are the vertices);
+ * nv = 1 - mean over the K nearest sites of clamp(n_i.n_j); scheme 1 (mode 1):
+ * tier = number of thresholds {0.15, 0.35, 0.55, 0.80} nv reaches; scheme 2
+ * (mode 2): tier 4 becomes 5 when the normal covariance split
+ * L = (l3 - l2)/l3 exceeds 0.80 (20-step power iterations, FP64).  The tiers
+ * are stored in the job (as cvt_job_set_tiers) and copied to tier_out; nv_out
+ * and L_out (nullable, nS doubles) receive the scores. */
+int cvt_job_compute_tiers(cvt_job* job, int scheme, unsigned char* tier_out, double* nv_out, double* L_out);
+
 /* Intermediates of the last step (requires params.debug): nearest vertex used
  * for the frame, frame U/V (nS*3), pre-projection centroid (nS*3), cell area
  * and polar second moment about the site.  Any pointer may be null. */

@@ -584,6 +584,96 @@ void orc_knn_grid(const double* P, int nP, const double* Q, int nQ, int k, int e
 // Tangent-plane cell + centroid for n sites given frames and neighbour lists.
 // Outputs: cent (n*3, NaN for degenerate cells), area, moment (polar second
 // moment of the cell about its site), npoly (vertex count).
+// Freeze tiers (main.cu:41-149 restated; the reference indexes normals by
+// site because its sites are the vertices, main.cu:647-649 -- here nn[i] names
+// the vertex whose normal site i uses).  nv = 1 - mean clamp(n_i.n_j) over the
+// K listed neighbours (main.cu:45-62); scheme 2 adds L = (l3 - l2) / l3 of the
+// neighbours' normal covariance, eigenvalues from two 20-step power iterations
+// with deflation (main.cu:64-127), and tier 4 -> 5 when L > 0.80
+// (main.cu:129-148).
+static void sym3_eig_power(const double C[9], double lam[3]) {
+    double v[3] = {1.0, 0.0, 0.0};
+    for (int it = 0; it < 20; ++it) {
+        const double w0 = C[0] * v[0] + C[1] * v[1] + C[2] * v[2];
+        const double w1 = C[3] * v[0] + C[4] * v[1] + C[5] * v[2];
+        const double w2 = C[6] * v[0] + C[7] * v[1] + C[8] * v[2];
+        const double n = std::sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        if (n < 1e-30) break;
+        v[0] = w0 / n; v[1] = w1 / n; v[2] = w2 / n;
+    }
+    lam[0] = v[0] * (C[0] * v[0] + C[1] * v[1] + C[2] * v[2]) + v[1] * (C[3] * v[0] + C[4] * v[1] + C[5] * v[2]) +
+             v[2] * (C[6] * v[0] + C[7] * v[1] + C[8] * v[2]);
+    double D[9];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) D[3 * r + c] = C[3 * r + c] - lam[0] * v[r] * v[c];
+    double u[3] = {0.0, 1.0, 0.0};
+    for (int it = 0; it < 20; ++it) {
+        const double w0 = D[0] * u[0] + D[1] * u[1] + D[2] * u[2];
+        const double w1 = D[3] * u[0] + D[4] * u[1] + D[5] * u[2];
+        const double w2 = D[6] * u[0] + D[7] * u[1] + D[8] * u[2];
+        const double n = std::sqrt(w0 * w0 + w1 * w1 + w2 * w2);
+        if (n < 1e-30) break;
+        u[0] = w0 / n; u[1] = w1 / n; u[2] = w2 / n;
+    }
+    lam[1] = u[0] * (D[0] * u[0] + D[1] * u[1] + D[2] * u[2]) + u[1] * (D[3] * u[0] + D[4] * u[1] + D[5] * u[2]) +
+             u[2] * (D[6] * u[0] + D[7] * u[1] + D[8] * u[2]);
+    lam[2] = C[0] + C[4] + C[8] - lam[0] - lam[1];
+}
+
+void orc_tiers(int nS, int K, const int* knn, const int* nn, const double* N, int nV, int scheme, unsigned char* tier,
+               double* nv_out, double* L_out) {
+    static const double thr[4] = {0.15, 0.35, 0.55, 0.80};
+    parallel_for(nS, [&](long i) {
+        double nv = 0.0, L = 0.0;
+        const int vi = nn[i];
+        if (vi >= 0 && vi < nV) {
+            const double* ni = N + 3L * vi;
+            double sum_cos = 0.0, m0 = 0.0, m1 = 0.0, m2 = 0.0;
+            int count = 0;
+            for (int k = 0; k < K; ++k) {
+                const int j = knn[i * (long)K + k];
+                if (j < 0 || j >= nS) continue;
+                const int vj = nn[j];
+                if (vj < 0 || vj >= nV) continue;
+                const double* nj = N + 3L * vj;
+                double c = ni[0] * nj[0] + ni[1] * nj[1] + ni[2] * nj[2];
+                c = c > 1.0 ? 1.0 : (c < -1.0 ? -1.0 : c);
+                sum_cos += c;
+                m0 += nj[0]; m1 += nj[1]; m2 += nj[2];
+                ++count;
+            }
+            nv = count > 0 ? 1.0 - sum_cos / (double)count : 0.0;
+            if (scheme == 2 && count >= 2) {
+                m0 /= count; m1 /= count; m2 /= count;
+                double C[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                for (int k = 0; k < K; ++k) {
+                    const int j = knn[i * (long)K + k];
+                    if (j < 0 || j >= nS) continue;
+                    const int vj = nn[j];
+                    if (vj < 0 || vj >= nV) continue;
+                    const double* nj = N + 3L * vj;
+                    const double a0 = nj[0] - m0, a1 = nj[1] - m1, a2 = nj[2] - m2;
+                    C[0] += a0 * a0; C[1] += a0 * a1; C[2] += a0 * a2;
+                    C[3] += a1 * a0; C[4] += a1 * a1; C[5] += a1 * a2;
+                    C[6] += a2 * a0; C[7] += a2 * a1; C[8] += a2 * a2;
+                }
+                double lam[3];
+                sym3_eig_power(C, lam);
+                if (lam[0] > lam[1]) std::swap(lam[0], lam[1]);
+                if (lam[1] > lam[2]) std::swap(lam[1], lam[2]);
+                if (lam[0] > lam[1]) std::swap(lam[0], lam[1]);
+                if (lam[2] > 1e-30) L = (lam[2] - lam[1]) / lam[2];
+            }
+        }
+        int t = 0;
+        while (t < 4 && nv >= thr[t]) ++t;
+        if (scheme == 2 && t == 4 && L > 0.80) t = 5;
+        tier[i] = (unsigned char)t;
+        if (nv_out) nv_out[i] = nv;
+        if (L_out) L_out[i] = L;
+    });
+}
+
 void orc_cells(const double* S, int nS, const double* U, const double* Vv, const int* knn, int K, double R,
                double* cent, double* area, double* moment, int* npoly) {
     parallel_for(nS, [&](long i) {

@@ -62,6 +62,7 @@ def lib():
         L.orc_remesh.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, C.c_double, _dp, C.c_double, _ip]
         L.orc_remesh.restype = C.c_int
         L.orc_point_mesh_d2.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp]
+        L.orc_tiers.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _up, _dp, _dp]
         _lib = L
     return _lib
 
@@ -254,3 +255,16 @@ def remesh(S, U, W, knn_, R: float, Nsite, eps: float = 1e-6):
     m = lib().orc_remesh(S.reshape(-1), n, _d(U).reshape(-1), _d(W).reshape(-1), _i(knn_).reshape(-1), K, float(R),
                          _d(Nsite).reshape(-1), float(eps), out)
     return out[: 3 * m].reshape(m, 3)
+
+
+def tiers(knn_, nn, N, scheme: int):
+    """Freeze tiers from neighbour lists knn_ (n, K), each site's vertex nn (n,)
+    and vertex normals N (nV, 3): returns (tier uint8, nv, L)."""
+    knn_ = _i(knn_)
+    n, K = knn_.shape
+    N = _d(N)
+    tier = np.zeros(n, np.uint8)
+    nv = np.zeros(n)
+    L = np.zeros(n)
+    lib().orc_tiers(n, K, knn_.reshape(-1), _i(nn), N.reshape(-1), len(N), int(scheme), tier, nv, L)
+    return tier, nv, L

@@ -111,3 +111,14 @@ def test_server_rejects_bad_requests():
     assert cfg.hbm_budget_bytes == 200 << 30 and cfg.epoch_iters == 10
     r = cvt.Result()
     assert L.cvt_server_poll(None, 1, C.byref(r)) == -1
+
+
+def test_cli_usage_and_missing_file(tmp_path):
+    """The gpu_cvt command line (main.cu:371-402 arguments) fails cleanly
+    without a mesh, before touching a GPU."""
+    exe = os.path.join(ROOT, "gpu_cvt_amd", "gpu_cvt")
+    assert os.path.exists(exe), "gpu_cvt not built"
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 1 and "usage" in r.stderr
+    r = subprocess.run([exe, str(tmp_path / "none.obj")], capture_output=True, text=True, cwd=tmp_path)
+    assert r.returncode == 1 and "cannot read" in r.stderr

@@ -190,3 +190,60 @@ def test_energy_decreases_and_matches_oracle(orc):
     assert e_gpu < e0
     assert abs(e_gpu - e_orc) <= 1e-3 * e_orc
     assert np.isfinite(Sg).all()
+
+
+def _cube(n):
+    """Closed cube surface [-1, 1]^3, n x n quads per face, welded edges."""
+    g = np.linspace(-1.0, 1.0, n + 1)
+    pts, idx, faces = [], {}, []
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in idx:
+            idx[key] = len(pts)
+            pts.append(p)
+        return idx[key]
+
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            u, v = [a for a in range(3) if a != axis]
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[axis], p[u], p[v] = sgn, g[i + di], g[j + dj]
+                        q.append(vid(p))
+                    if (sgn > 0) == (axis != 1):
+                        faces += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+                    else:
+                        faces += [[q[0], q[2], q[1]], [q[0], q[3], q[2]]]
+    return np.array(pts), np.array(faces, np.int32)
+
+
+@pytest.mark.parametrize("scheme,shape", [(1, "dsphere"), (2, "dsphere"), (1, "cube"), (2, "cube")])
+def test_device_tiers_match_oracle(orc, scheme, shape):
+    """Freeze tiers computed on the device equal the oracle's from the same
+    FP32 sites and the device's vertex normals (exact tier ids; nv and L to
+    FP64 rounding).  The cube's sharp edges and corners reach tiers above 1."""
+    if shape == "dsphere":
+        V, F = mg.displaced_sphere(30, seed=4, crease_seed=5)
+    else:
+        V, F = _cube(24)
+    S = mg.sample_surface(V, F, 6000, seed=6)
+    K = 24
+    print(f"[tiers] scheme={scheme} shape={shape} nS={len(S)} K={K}", flush=True)
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=K) as job:
+        job.step(2)
+        S32 = orc.f32r(job.sites())
+        tier_g, nv_g, L_g = job.compute_tiers(scheme)
+        Ng = job.normals()
+    knn_o = orc.knn(S32, S32, K, exclude_self=True)
+    nn_o = orc.knn(mesh.V, S32, 1, exclude_self=False)[:, 0]
+    tier_o, nv_o, L_o = orc.tiers(knn_o, nn_o, Ng, scheme)
+    print(f"[tiers] histogram {np.bincount(tier_g, minlength=6).tolist()}", flush=True)
+    assert np.array_equal(tier_g, tier_o)
+    assert np.abs(nv_g - nv_o).max() < 1e-12
+    assert np.abs(L_g - L_o).max() < 1e-9
+    assert len(np.unique(tier_g)) >= (3 if shape == "cube" else 2)

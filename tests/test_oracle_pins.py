@@ -390,3 +390,66 @@ def test_remesh_sphere_is_closed_manifold(orc):
     assert abs(len(T) - (2 * len(S) - 4)) <= 0.01 * len(T)
     a, b, c = S[T[:, 0]], S[T[:, 1]], S[T[:, 2]]
     assert np.all((np.cross(b - a, c - a) * (a + b + c)).sum(1) > 0)  # outward
+
+
+# ---------------------------------------------------------------------- tiers
+def _tier_case(normal0, neighbour_normals):
+    """Site 0 with the given normal and neighbours 1..K with theirs (sites are
+    their own vertices); returns the oracle's (tier, nv, L) of site 0 for
+    schemes 1 and 2."""
+    K = len(neighbour_normals)
+    N = np.vstack([normal0, neighbour_normals]).astype(np.float64)
+    n = K + 1
+    knn = np.zeros((n, K), np.int32)
+    knn[0] = np.arange(1, K + 1)
+    for i in range(1, n):
+        knn[i] = [j for j in range(n) if j != i][:K]
+    nn = np.arange(n, dtype=np.int32)
+    return [orc_tiers_site0(knn, nn, N, s) for s in (1, 2)]
+
+
+def orc_tiers_site0(knn, nn, N, scheme):
+    from oracle import oracle as o
+    t, nv, L = o.tiers(knn, nn, N, scheme)
+    return int(t[0]), float(nv[0]), float(L[0])
+
+
+def test_tiers_plane_is_tier0(orc):
+    K = 16
+    (t1, nv1, L1), (t2, nv2, L2) = _tier_case([0, 0, 1], np.tile([0.0, 0.0, 1.0], (K, 1)))
+    assert (t1, t2) == (0, 0) and nv1 == 0.0 and L2 == 0.0
+
+
+@pytest.mark.parametrize("k2, theta_deg", [(4, 30.0), (8, 60.0), (8, 90.0), (12, 100.0), (8, 150.0), (16, 120.0)])
+def test_tiers_crease_closed_form(orc, k2, theta_deg):
+    """k1 = K - k2 neighbours share the site's normal a, k2 have normal b at
+    angle theta: nv = (k2/K)(1 - cos theta); the normals form two clusters, so
+    their covariance has rank one and L = 1 (up to rounding)."""
+    K = 16
+    th = np.radians(theta_deg)
+    a = np.array([0.0, 0.0, 1.0])
+    b = np.array([np.sin(th), 0.0, np.cos(th)])
+    nb = np.vstack([np.tile(a, (K - k2, 1)), np.tile(b, (k2, 1))])
+    (t1, nv1, _), (t2, nv2, L2) = _tier_case(a, nb)
+    nv_exp = (k2 / K) * (1.0 - np.cos(th))
+    assert abs(nv1 - nv_exp) < 1e-12 and abs(nv2 - nv_exp) < 1e-12
+    tier_exp = int(np.sum(nv_exp >= np.array([0.15, 0.35, 0.55, 0.80])))
+    assert t1 == tier_exp
+    if k2 < K:
+        assert abs(L2 - 1.0) < 1e-9
+        assert t2 == (5 if tier_exp == 4 else tier_exp)
+    else:  # one cluster only: zero covariance, L = 0
+        assert L2 == 0.0 and t2 == tier_exp
+
+
+def test_tiers_cone_has_no_split(orc):
+    """Neighbour normals evenly spaced on a cone of half-angle 85 deg about the
+    site's normal: nv = 1 - cos 85 deg (tier 4) and the covariance has two equal
+    largest eigenvalues, so L ~ 0 and scheme 2 keeps tier 4 (a crease gives 5)."""
+    K = 16
+    al = np.radians(85.0)
+    phi = 2 * np.pi * np.arange(K) / K
+    nb = np.stack([np.sin(al) * np.cos(phi), np.sin(al) * np.sin(phi), np.full(K, np.cos(al))], axis=1)
+    (t1, nv1, _), (t2, nv2, L2) = _tier_case([0.0, 0.0, 1.0], nb)
+    assert abs(nv1 - (1 - np.cos(al))) < 1e-12
+    assert t1 == 4 and t2 == 4 and L2 < 1e-6
