@@ -67,12 +67,18 @@ struct cvt_job {
     GridBufs sgb{};
     float4* S[2] = {nullptr, nullptr};
     int cur = 0;
+    // Sites are numbered internally in the order of the first cell sort, so
+    // that per-site arrays are read and written nearly in processing order;
+    // perm maps internal -> caller index, inv the reverse, cid is perm on the
+    // device (KNN ties break by the caller's index).
+    std::vector<int> perm, inv;
+    int* cid = nullptr;
     int *knn = nullptr, *knnV = nullptr;
     unsigned char *frozen = nullptr, *streak = nullptr, *tier = nullptr;
     int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
-    float *radF = nullptr, *radV = nullptr;  // per-site vertex-search seed radii
+    float4 *fcache = nullptr, *vcache = nullptr, *gaps = nullptr;  // per-site vertex-search caches
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
@@ -289,6 +295,7 @@ JobDesc make_desc(cvt_job* j) {
     a.sg.g = j->sgp;
     a.sg.start = j->sgb.start;
     a.sg.pts = j->sgb.pts;
+    a.sg.cid = j->cid;
     a.S01[0] = j->S[0];
     a.S01[1] = j->S[1];
     a.parity = j->parity;
@@ -304,8 +311,9 @@ JobDesc make_desc(cvt_job* j) {
     a.defer_list = j->defer_list;
     a.rad2 = j->rad2;
     a.seed_ok = j->seed_ok;
-    a.radF = j->radF;
-    a.radV = j->radV;
+    a.fcache = j->fcache;
+    a.vcache = j->vcache;
+    a.gaps = j->gaps;
     a.dmax01 = j->dmax;
     a.nS = j->nS;
     a.R = (float)j->R;
@@ -328,7 +336,6 @@ JobDesc make_desc(cvt_job* j) {
         }
     }
     a.nan_policy = j->p.nan_policy;
-    a.write_knnv = (j->p.freeze_mode != 0 || j->p.debug) ? 1 : 0;
     if (j->p.debug) {
         a.dbg_nn = j->dbg_nn;
         a.dbg_U = j->dbg_U;
@@ -511,7 +518,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
-        (rc = j->alloc(&j->radF, nS)) || (rc = j->alloc(&j->radV, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {
@@ -519,15 +526,40 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
             (rc = j->alloc(&j->dbg_cent, nS)) || (rc = j->alloc(&j->dbg_am, nS)))
             return bail(rc);
     }
+    // internal numbering: sites stably sorted by their cell in the site grid
+    {
+        const GridP& g = j->sgp;
+        std::vector<long long> key(nS);
+        for (int i = 0; i < nS; ++i) {
+            const float x = (float)S0[3 * i], y = (float)S0[3 * i + 1], z = (float)S0[3 * i + 2];
+            auto cl = [](float v, float o, float ih, int d) {
+                const int c = (int)std::floor((v - o) * ih);
+                return c < 0 ? 0 : (c >= d ? d - 1 : c);
+            };
+            const long c = ((long)cl(z, g.oz, g.inv_h, g.dz) * g.dy + cl(y, g.oy, g.inv_h, g.dy)) * g.dx +
+                           cl(x, g.ox, g.inv_h, g.dx);
+            key[i] = (c << 31) | i;
+        }
+        std::sort(key.begin(), key.end());
+        j->perm.resize(nS);
+        j->inv.resize(nS);
+        for (int k = 0; k < nS; ++k) {
+            j->perm[k] = (int)(key[k] & 0x7fffffff);
+            j->inv[j->perm[k]] = k;
+        }
+    }
     std::vector<float4> S4(nS);
-    for (int i = 0; i < nS; ++i) S4[i] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
+    for (int k = 0; k < nS; ++k) {
+        const int i = j->perm[k];
+        S4[k] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
+    }
     std::vector<int> Fv(Fh, Fh + 3L * nF);
     hipStream_t s = j->stream;
     auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
     if (up(j->V, V4.data(), sizeof(float4) * nV) || up(j->tri, tri.data(), sizeof(float4) * tri.size()) ||
         up(j->F, Fv.data(), sizeof(int) * Fv.size()) || up(j->vf_off, vf_off.data(), sizeof(int) * vf_off.size()) ||
         up(j->vf_faces, vf_faces.data(), sizeof(int) * vf_faces.size()) ||
-        up(j->S[0], S4.data(), sizeof(float4) * nS))
+        up(j->S[0], S4.data(), sizeof(float4) * nS) || up(j->cid, j->perm.data(), sizeof(int) * nS))
         return bail(fail(CVT_EHIP, "upload failed"));
     launch_fill(j->knn, (long)nS * p.K, 0, s);
     launch_fill(j->knnV, (long)nS * p.kproj, -1, s);
@@ -537,8 +569,9 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill(j->has_prev, 1, 0, s);
     launch_fill(j->parity, 1, 0, s);
     launch_fill(j->seed_ok, 1, 0, s);
-    launch_fill((int*)j->radF, nS, 0x7f800000, s);  // +inf: no seed yet
-    launch_fill((int*)j->radV, nS, 0x7f800000, s);
+    launch_fill((int*)j->fcache, 4L * nS, 0x7f800000, s);  // +inf: no cached search yet
+    launch_fill((int*)j->vcache, 4L * nS, 0x7f800000, s);
+    launch_fill((int*)j->gaps, 4L * nS, (int)0xbf800000, s);  // gaps -1: never reusable
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatWords, 0, s);
@@ -713,26 +746,47 @@ int cvt_job_get_sites(cvt_job* j, double* S) {
     if (!j || !S) return fail(CVT_EINVAL, "null argument");
     std::vector<float4> h(j->nS);
     HIPCHK(hipMemcpy(h.data(), j->S[j->cur], sizeof(float4) * j->nS, hipMemcpyDeviceToHost));
-    for (int i = 0; i < j->nS; ++i) { S[3 * i] = h[i].x; S[3 * i + 1] = h[i].y; S[3 * i + 2] = h[i].z; }
+    for (int k = 0; k < j->nS; ++k) {
+        const int i = j->perm[k];
+        S[3 * i] = h[k].x; S[3 * i + 1] = h[k].y; S[3 * i + 2] = h[k].z;
+    }
     return 0;
 }
 
 int cvt_job_get_knn(cvt_job* j, int* out) {
     if (!j || !out) return fail(CVT_EINVAL, "null argument");
-    HIPCHK(hipMemcpy(out, j->knn, sizeof(int) * (size_t)j->nS * j->p.K, hipMemcpyDeviceToHost));
+    const int K = j->p.K;
+    std::vector<int> h((size_t)j->nS * K);
+    HIPCHK(hipMemcpy(h.data(), j->knn, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < j->nS; ++k) {
+        int* o = out + (size_t)j->perm[k] * K;
+        const int* r = h.data() + (size_t)k * K;
+        for (int t = 0; t < K; ++t) o[t] = (r[t] >= 0 && r[t] < j->nS) ? j->perm[r[t]] : r[t];
+    }
     return 0;
 }
 
 int cvt_job_get_knnv(cvt_job* j, int* out) {
     if (!j || !out) return fail(CVT_EINVAL, "null argument");
-    HIPCHK(hipMemcpy(out, j->knnV, sizeof(int) * (size_t)j->nS * j->p.kproj, hipMemcpyDeviceToHost));
+    const int kp = j->p.kproj;
+    std::vector<int> h((size_t)j->nS * kp);
+    HIPCHK(hipMemcpy(h.data(), j->knnV, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < j->nS; ++k)
+        std::memcpy(out + (size_t)j->perm[k] * kp, h.data() + (size_t)k * kp, sizeof(int) * kp);
     return 0;
 }
 
 int cvt_job_get_freeze(cvt_job* j, unsigned char* fz, unsigned char* st) {
     if (!j) return fail(CVT_EINVAL, "null job");
-    if (fz) HIPCHK(hipMemcpy(fz, j->frozen, j->nS, hipMemcpyDeviceToHost));
-    if (st) HIPCHK(hipMemcpy(st, j->streak, j->nS, hipMemcpyDeviceToHost));
+    std::vector<unsigned char> h(j->nS);
+    if (fz) {
+        HIPCHK(hipMemcpy(h.data(), j->frozen, j->nS, hipMemcpyDeviceToHost));
+        for (int k = 0; k < j->nS; ++k) fz[j->perm[k]] = h[k];
+    }
+    if (st) {
+        HIPCHK(hipMemcpy(h.data(), j->streak, j->nS, hipMemcpyDeviceToHost));
+        for (int k = 0; k < j->nS; ++k) st[j->perm[k]] = h[k];
+    }
     return 0;
 }
 
@@ -751,26 +805,44 @@ int cvt_job_set_state(cvt_job* j, const double* S, const int* knn_prev, const in
     if (S) {
         if (!check_finite(S, 3L * nS)) return fail(CVT_ENONFINITE, "non-finite site coordinate");
         std::vector<float4> h(nS);
-        for (int i = 0; i < nS; ++i) h[i] = make_float4((float)S[3 * i], (float)S[3 * i + 1], (float)S[3 * i + 2], 0.f);
+        for (int k = 0; k < nS; ++k) {
+            const int i = j->perm[k];
+            h[k] = make_float4((float)S[3 * i], (float)S[3 * i + 1], (float)S[3 * i + 2], 0.f);
+        }
         HIPCHK(hipMemcpy(j->S[j->cur], h.data(), sizeof(float4) * nS, hipMemcpyHostToDevice));
     }
     if (knn_prev) {
-        for (long k = 0; k < (long)nS * j->p.K; ++k)
+        const int K = j->p.K;
+        for (long k = 0; k < (long)nS * K; ++k)
             if (knn_prev[k] < 0 || knn_prev[k] >= nS) return fail(CVT_EINVAL, "knn_prev entry out of range");
-        HIPCHK(hipMemcpy(j->knn, knn_prev, sizeof(int) * (size_t)nS * j->p.K, hipMemcpyHostToDevice));
+        std::vector<int> h((size_t)nS * K);
+        for (int k = 0; k < nS; ++k)
+            for (int t = 0; t < K; ++t) h[(size_t)k * K + t] = j->inv[knn_prev[(size_t)j->perm[k] * K + t]];
+        HIPCHK(hipMemcpy(j->knn, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
     }
     if (knnv_prev) {
-        for (long k = 0; k < (long)nS * j->p.kproj; ++k)
+        const int kp = j->p.kproj;
+        for (long k = 0; k < (long)nS * kp; ++k)
             if (knnv_prev[k] < -1 || knnv_prev[k] >= j->nV) return fail(CVT_EINVAL, "knnv_prev entry out of range");
-        HIPCHK(hipMemcpy(j->knnV, knnv_prev, sizeof(int) * (size_t)nS * j->p.kproj, hipMemcpyHostToDevice));
+        std::vector<int> h((size_t)nS * kp);
+        for (int k = 0; k < nS; ++k)
+            std::memcpy(h.data() + (size_t)k * kp, knnv_prev + (size_t)j->perm[k] * kp, sizeof(int) * kp);
+        HIPCHK(hipMemcpy(j->knnV, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
     }
-    if (frozen) HIPCHK(hipMemcpy(j->frozen, frozen, nS, hipMemcpyHostToDevice));
-    if (streak) HIPCHK(hipMemcpy(j->streak, streak, nS, hipMemcpyHostToDevice));
+    std::vector<unsigned char> hb(nS);
+    if (frozen) {
+        for (int k = 0; k < nS; ++k) hb[k] = frozen[j->perm[k]];
+        HIPCHK(hipMemcpy(j->frozen, hb.data(), nS, hipMemcpyHostToDevice));
+    }
+    if (streak) {
+        for (int k = 0; k < nS; ++k) hb[k] = streak[j->perm[k]];
+        HIPCHK(hipMemcpy(j->streak, hb.data(), nS, hipMemcpyHostToDevice));
+    }
     if (has_prev >= 0) {
         int hp = has_prev ? 1 : 0;
         HIPCHK(hipMemcpy(j->has_prev, &hp, sizeof(int), hipMemcpyHostToDevice));
     }
-    if (S) {  // sites moved arbitrarily: the per-site search seeds no longer bound anything
+    if (S || knnv_prev) {  // the per-site search seeds and caches no longer describe the state
         int zero = 0;
         HIPCHK(hipMemcpy(j->seed_ok, &zero, sizeof(int), hipMemcpyHostToDevice));
     }
@@ -781,7 +853,9 @@ int cvt_job_set_tiers(cvt_job* j, const unsigned char* tier) {
     if (!j || !tier) return fail(CVT_EINVAL, "null argument");
     for (int i = 0; i < j->nS; ++i)
         if (tier[i] > 5) return fail(CVT_EINVAL, "tier id > 5 at site %d", i);
-    HIPCHK(hipMemcpy(j->tier, tier, j->nS, hipMemcpyHostToDevice));
+    std::vector<unsigned char> h(j->nS);
+    for (int k = 0; k < j->nS; ++k) h[k] = tier[j->perm[k]];
+    HIPCHK(hipMemcpy(j->tier, h.data(), j->nS, hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -799,9 +873,20 @@ int cvt_job_compute_tiers(cvt_job* j, int scheme, unsigned char* tier_out, doubl
     launch_site_tiers(j->desc, n, j->p.K, scheme, nn, j->tier, dnv, dL, s);
     HIPCHK(hipStreamSynchronize(s));
     HIPCHK(hipGetLastError());
-    if (tier_out) HIPCHK(hipMemcpy(tier_out, j->tier, n, hipMemcpyDeviceToHost));
-    if (nv_out) HIPCHK(hipMemcpy(nv_out, dnv, sizeof(double) * n, hipMemcpyDeviceToHost));
-    if (L_out) HIPCHK(hipMemcpy(L_out, dL, sizeof(double) * n, hipMemcpyDeviceToHost));
+    if (tier_out) {
+        std::vector<unsigned char> h(n);
+        HIPCHK(hipMemcpy(h.data(), j->tier, n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) tier_out[j->perm[k]] = h[k];
+    }
+    std::vector<double> hd(n);
+    if (nv_out) {
+        HIPCHK(hipMemcpy(hd.data(), dnv, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) nv_out[j->perm[k]] = hd[k];
+    }
+    if (L_out) {
+        HIPCHK(hipMemcpy(hd.data(), dL, sizeof(double) * n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) L_out[j->perm[k]] = hd[k];
+    }
     return 0;
 }
 
@@ -809,12 +894,20 @@ int cvt_job_get_debug(cvt_job* j, int* nn, double* U, double* Vf, double* cent, 
     if (!j) return fail(CVT_EINVAL, "null job");
     if (!j->p.debug) return fail(CVT_ESTATE, "job was created without params.debug");
     const int n = j->nS;
-    if (nn) HIPCHK(hipMemcpy(nn, j->dbg_nn, sizeof(int) * n, hipMemcpyDeviceToHost));
+    const std::vector<int>& P = j->perm;
+    if (nn) {
+        std::vector<int> hi(n);
+        HIPCHK(hipMemcpy(hi.data(), j->dbg_nn, sizeof(int) * n, hipMemcpyDeviceToHost));
+        for (int k = 0; k < n; ++k) nn[P[k]] = hi[k];
+    }
     std::vector<float4> h(n);
     auto get3 = [&](float4* d, double* o) -> int {
         if (!o) return 0;
         HIPCHK(hipMemcpy(h.data(), d, sizeof(float4) * n, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) { o[3 * i] = h[i].x; o[3 * i + 1] = h[i].y; o[3 * i + 2] = h[i].z; }
+        for (int k = 0; k < n; ++k) {
+            const int i = P[k];
+            o[3 * i] = h[k].x; o[3 * i + 1] = h[k].y; o[3 * i + 2] = h[k].z;
+        }
         return 0;
     };
     int rc;
@@ -822,9 +915,9 @@ int cvt_job_get_debug(cvt_job* j, int* nn, double* U, double* Vf, double* cent, 
     if (area || moment) {
         std::vector<float2> am(n);
         HIPCHK(hipMemcpy(am.data(), j->dbg_am, sizeof(float2) * n, hipMemcpyDeviceToHost));
-        for (int i = 0; i < n; ++i) {
-            if (area) area[i] = am[i].x;
-            if (moment) moment[i] = am[i].y;
+        for (int k = 0; k < n; ++k) {
+            if (area) area[P[k]] = am[k].x;
+            if (moment) moment[P[k]] = am[k].y;
         }
     }
     return 0;
@@ -838,8 +931,8 @@ int cvt_job_energy(cvt_job* j, double* E) {
     HIPCHK(hipStreamSynchronize(s));
     std::vector<float2> am(j->nS);
     HIPCHK(hipMemcpy(am.data(), j->dbg_am, sizeof(float2) * j->nS, hipMemcpyDeviceToHost));
-    double sum = 0;
-    for (int i = 0; i < j->nS; ++i) sum += am[i].y;
+    double sum = 0;  // in the caller's site order
+    for (int i = 0; i < j->nS; ++i) sum += am[j->inv[i]].y;
     *E = sum;
     return 0;
 }
@@ -905,6 +998,7 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
     hipError_t e = hipMemcpyAsync(hF, dF, sizeof(int) * 3 * (size_t)nkeep, hipMemcpyDeviceToHost, j->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(j->stream);
     if (e != hipSuccess) { std::free(hF); return fail(CVT_EHIP, "remesh copy: %s", hipGetErrorString(e)); }
+    for (long k = 0; k < 3L * nkeep; ++k) hF[k] = j->perm[hF[k]];  // internal -> caller site index
     *F_out = hF;
     *nF_out = nkeep;
     return 0;

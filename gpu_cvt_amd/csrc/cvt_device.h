@@ -259,11 +259,18 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __r
     return bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f);
 }
 
+// Tie-break index of a stored point: its id bits, or cid[id] when the store
+// numbers points internally (the caller's index decides ties).
+__device__ __forceinline__ int tie_id(const float4& p, const int* __restrict__ cid) {
+    const int id = __float_as_int(p.w);
+    return cid ? cid[id] : id;
+}
+
 __device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float qx, float qy, float qz, int sa,
-                                         int sb) {
+                                         int sb, const int* __restrict__ cid) {
     const float4 a = pts[sa], b = pts[sb];
     const double ka = key64(qx, qy, qz, a.x, a.y, a.z), kb = key64(qx, qy, qz, b.x, b.y, b.z);
-    return ka < kb || (ka == kb && __float_as_int(a.w) < __float_as_int(b.w));
+    return ka < kb || (ka == kb && tie_id(a, cid) < tie_id(b, cid));
 }
 
 // Exact K nearest under (key64, id) from an FP32 top-(K+2) list.  Adjacent
@@ -274,7 +281,7 @@ __device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float q
 // not provably beyond the K-th.
 template <int K, int L>
 __device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
-                                             int& n_pairs) {
+                                             int& n_pairs, const int* __restrict__ cid) {
     static_assert(L == K + 2, "list must hold two extra entries");
     n_pairs = 0;
     if (T.s[K - 1] < 0) return true;  // fewer than K points exist: nothing to order beyond them
@@ -291,7 +298,7 @@ __device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, flo
     for (int t = 0; t < K; ++t) {
         if (amb[t]) {
             ++n_pairs;
-            if (key_less(pts, qx, qy, qz, T.s[t + 1], T.s[t])) {
+            if (key_less(pts, qx, qy, qz, T.s[t + 1], T.s[t], cid)) {
                 const int ts = T.s[t]; T.s[t] = T.s[t + 1]; T.s[t + 1] = ts;
                 const float td = T.d[t]; T.d[t] = T.d[t + 1]; T.d[t + 1] = td;
             }
@@ -305,7 +312,7 @@ __device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, flo
 template <int K>
 __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
                                           float qx, float qy, float qz, int self_slot, float tau,
-                                          int* __restrict__ out_slot) {
+                                          int* __restrict__ out_slot, const int* __restrict__ cid) {
     double kd[K];
     int kid[K], ks[K];
 #pragma unroll
@@ -340,7 +347,7 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
                         const float4 p = pts[s];
                         if (s == self_slot || d2f(qx, qy, qz, p) > tau) continue;
                         const double d = key64(qx, qy, qz, p.x, p.y, p.z);
-                        const int id = __float_as_int(p.w);
+                        const int id = tie_id(p, cid);
                         if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) continue;
 #pragma unroll
                         for (int t = K - 1; t >= 1; --t) {
@@ -376,7 +383,8 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
 template <int K, bool PRECISE, int CAP = 0>
 __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
                                          float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
-                                         float& l_out, int* lds = nullptr, float collect_tau = 0.f) {
+                                         float& l_out, int* lds = nullptr, float collect_tau = 0.f,
+                                         float* d_out = nullptr, const int* __restrict__ cid = nullptr) {
     constexpr int L = K + 2;
     TopL<L> T;
     bool have = false;
@@ -388,14 +396,22 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
         grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T);
     l_out = T.s[L - 1] >= 0 ? T.d[L - 1] : __builtin_huge_valf();
     int pairs;
-    const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs);
+    const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs, cid);
 #pragma unroll
     for (int t = 0; t < K; ++t) slot_out[t] = T.s[t];
+    if (d_out) {
+#pragma unroll
+        for (int t = 0; t < L; ++t) d_out[t] = T.s[t] >= 0 ? T.d[t] : __builtin_huge_valf();
+    }
     if (ok) return 0;
     if constexpr (!PRECISE) {
         return -1;
     } else {
-        grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out);
+        grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
+        if (d_out) {  // FP32 distances no longer follow the rescanned list: report no margin
+#pragma unroll
+            for (int t = 0; t < L; ++t) d_out[t] = 0.f;
+        }
         return 1;
     }
 }

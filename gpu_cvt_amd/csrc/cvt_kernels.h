@@ -27,7 +27,8 @@ struct MeshDev {
 struct SiteGridDev {
     GridP g;
     const int* start;
-    const float4* pts;    // cell-sorted sites (x, y, z, site id bits)
+    const float4* pts;    // cell-sorted sites (x, y, z, internal site index bits)
+    const int* cid;       // internal site index -> caller's index (KNN ties break by the caller's index)
 };
 
 // Per-job pointers and parameters; lives in device memory (one per job) and is
@@ -49,8 +50,9 @@ struct StepArgs {
     int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
     float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
     int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
-    float* radF;          // nS: 3rd-nearest-vertex distance of the site (frame search seed), +inf if none
-    float* radV;          // nS: 7th-nearest-vertex distance of the centroid + |projection - centroid|, +inf if none
+    float4* fcache;       // nS: (site position at its last vertex search, 3rd-nearest distance there; +inf: none)
+    float4* vcache;       // nS: (centroid at its last kproj search, 7th-nearest distance there; +inf: none)
+    float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, 5th/6th gap at vcache, 0)
     float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     int nS;
     float R;
@@ -59,7 +61,6 @@ struct StepArgs {
     float thr2[6];
     int need[6];
     int nan_policy;
-    int write_knnv;
     // optional per-site debug outputs (null when off)
     int* dbg_nn;
     float4* dbg_U;

@@ -239,29 +239,53 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
     const float inf = __builtin_huge_valf();
 
     // ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110)
+    // Search cache (per site): fc = (position of the last vertex search, its
+    // 3rd-nearest distance), gp = (gap between the 1st and 2nd nearest distances
+    // there, that nearest vertex, the centroid's 5th/6th gap, -), vc = (centroid
+    // of the last kproj search, its 7th-nearest distance).  A point that moved by
+    // d keeps its nearest vertex while 2d < gap (every other vertex gains at most
+    // d on the nearest), so the search is skipped; otherwise the cached radius
+    // plus d seeds it (triangle inequality).
+    const bool use_cache = !eval_only && *a.seed_ok;
+    float4 fc = make_float4(0.f, 0.f, 0.f, inf), gp = make_float4(-1.f, 0.f, -1.f, 0.f);
+    bool fc_new = false, vc_new = false;
+    float4 vc_out = make_float4(0.f, 0.f, 0.f, inf);
     int nn = -1;
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (valid) {
         if (!fz) {
-            // seed: the 3rd-nearest-vertex distance of the previous position
-            // plus the site's displacement bounds the new one (triangle inequality)
             float tau1 = inf;
-            if (!eval_only && *a.seed_ok) {
-                const float rf = a.radF[i];
-                if (rf < inf) {
-                    const float4 sp0 = a.S01[par ^ 1][i];
-                    const float dq = sqrtf((q.x - sp0.x) * (q.x - sp0.x) + (q.y - sp0.y) * (q.y - sp0.y) +
-                                           (q.z - sp0.z) * (q.z - sp0.z));
-                    const float rr = (rf + dq) * (1.f + 1e-4f);
-                    tau1 = rr * rr;
+            bool have = false;
+            if (use_cache) {
+                fc = a.fcache[i];
+                gp = a.gaps[i];
+                if (fc.w < inf) {
+                    const float dq = sqrtf((q.x - fc.x) * (q.x - fc.x) + (q.y - fc.y) * (q.y - fc.y) +
+                                           (q.z - fc.z) * (q.z - fc.z));
+                    if (gp.x - 2.f * dq > 2e-5f * fc.w + 4.f * a.m.vg.eps_abs) {
+                        nn = __float_as_int(gp.y);
+                        have = true;
+                    } else {
+                        const float rr = (fc.w + dq) * (1.f + 1e-4f);
+                        tau1 = rr * rr;
+                    }
                 }
             }
-            int s1[1];
-            float l1;
-            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1);
-            defer = defer || k1 < 0;
-            if (!eval_only && k1 >= 0) a.radF[i] = l1 < inf ? sqrtf(l1) : inf;
-            nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
+            if (!have) {
+                int s1[1];
+                float l1, d1[3];
+                const int k1 =
+                    knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1, nullptr,
+                                          0.f, d1);
+                defer = defer || k1 < 0;
+                nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
+                if (k1 >= 0 && !eval_only) {
+                    fc = make_float4(q.x, q.y, q.z, d1[2] < inf ? sqrtf(d1[2]) : inf);
+                    gp.x = nn >= 0 ? sqrtf(d1[1]) - sqrtf(d1[0]) : -1.f;
+                    gp.y = __int_as_float(nn);
+                    fc_new = true;
+                }
+            }
         } else {
             nn = a.knnV[(size_t)i * KP];  // pinned at the freeze iteration
         }
@@ -295,7 +319,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
         }
         float lk;
         r.kind = knn_exact<K, PRECISE, PRECISE ? 0 : kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t,
-                                                                   tau, slots, lk, lds, ctau);
+                                                                   tau, slots, lk, lds, ctau, nullptr, a.sg.cid);
         defer = defer || r.kind < 0;
         if (!eval_only && r.kind >= 0) a.rad2[i] = lk;
     }
@@ -362,41 +386,71 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
 
     // ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection
     float3 sp = c3;
-    float lv = inf;  // 7th-nearest-vertex distance^2 of the centroid (when searched)
     int vl[KP];
 #pragma unroll
     for (int k = 0; k < KP; ++k) vl[k] = -1;
     if (valid && !eval_only) {
         if (!fz) {
             if (finite3(c3.x, c3.y, c3.z)) {
-                // seed: the previous centroid's 7th-nearest-vertex distance plus
-                // |projection - centroid| (radV) plus |c3 - site| bounds this one
+                // the 5 nearest vertices keep their set while 2|c3 - c_s| is below
+                // the 5th/6th gap at the cached centroid c_s; only their order is
+                // recomputed (exact FP64 keys, index ties)
                 float tauv = inf;
-                if (!eval_only && *a.seed_ok) {
-                    const float rv = a.radV[i];
-                    if (rv < inf) {
-                        const float dc = sqrtf((c3.x - q.x) * (c3.x - q.x) + (c3.y - q.y) * (c3.y - q.y) +
-                                               (c3.z - q.z) * (c3.z - q.z));
-                        const float rr = (rv + dc) * (1.f + 1e-4f);
-                        tauv = rr * rr;
+                bool have = false;
+                if (use_cache) {
+                    const float4 vc = a.vcache[i];
+                    if (vc.w < inf) {
+                        const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
+                                               (c3.z - vc.z) * (c3.z - vc.z));
+                        if (gp.z - 2.f * dc > 2e-5f * vc.w + 4.f * a.m.vg.eps_abs) {
+                            const int* row = a.knnV + (size_t)i * KP;
+                            double key[KP];
+#pragma unroll
+                            for (int k = 0; k < KP; ++k) {
+                                vl[k] = row[k];
+                                const float4 pv = a.m.V[vl[k]];
+                                key[k] = key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z);
+                            }
+#pragma unroll
+                            for (int p0 = 1; p0 < KP; ++p0) {  // insertion sort of KP entries by (key, id)
+#pragma unroll
+                                for (int k = p0; k >= 1; --k) {
+                                    const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && vl[k] < vl[k - 1]);
+                                    const double tk = key[k];
+                                    const int tv = vl[k];
+                                    key[k] = sw ? key[k - 1] : key[k];
+                                    vl[k] = sw ? vl[k - 1] : vl[k];
+                                    key[k - 1] = sw ? tk : key[k - 1];
+                                    vl[k - 1] = sw ? tv : vl[k - 1];
+                                }
+                            }
+                            have = true;
+                        } else {
+                            const float rr = (vc.w + dc) * (1.f + 1e-4f);
+                            tauv = rr * rr;
+                        }
                     }
                 }
-                int sl[KP];
-                const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, tauv, sl,
-                                                      lv);
-                if (!PRECISE && kk < 0) {
-                    r.same = same;
-                    r.row_written = !eval_only;
-                    return r;
+                if (!have) {
+                    int sl[KP];
+                    float lv, dv[KP + 2];
+                    const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1,
+                                                          tauv, sl, lv, nullptr, 0.f, dv);
+                    if (!PRECISE && kk < 0) {
+                        r.same = same;
+                        r.row_written = !eval_only;
+                        return r;
+                    }
+#pragma unroll
+                    for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
+                    vc_out = make_float4(c3.x, c3.y, c3.z, dv[KP + 1] < inf ? sqrtf(dv[KP + 1]) : inf);
+                    gp.z = vl[KP - 1] >= 0 ? (dv[KP] < inf ? sqrtf(dv[KP]) : inf) - sqrtf(dv[KP - 1]) : -1.f;
+                    vc_new = true;
                 }
-#pragma unroll
-                for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
             }
-            if (a.write_knnv) {
-                int* row = a.knnV + (size_t)i * KP;
+            int* row = a.knnV + (size_t)i * KP;
 #pragma unroll
-                for (int k = 0; k < KP; ++k) row[k] = vl[k];
-            }
+            for (int k = 0; k < KP; ++k) row[k] = vl[k];
         } else {
             const int* row = a.knnV + (size_t)i * KP;
 #pragma unroll
@@ -427,11 +481,10 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
             }
         }
         if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
-        if (!fz) {
-            const float dp = sqrtf((sp.x - c3.x) * (sp.x - c3.x) + (sp.y - c3.y) * (sp.y - c3.y) +
-                                   (sp.z - c3.z) * (sp.z - c3.z));
-            a.radV[i] = (lv < inf && finite3(c3.x, c3.y, c3.z) && dp < inf) ? sqrtf(lv) + dp : inf;
-        }
+        // refreshed search caches
+        if (fc_new) a.fcache[i] = fc;
+        if (vc_new) a.vcache[i] = vc_out;
+        if (fc_new || vc_new) a.gaps[i] = gp;
     }
 
     // ---- 6. write the new site and the freeze state machine (freezetest.cuh:270-327, :451-466)
@@ -729,7 +782,8 @@ __global__ __launch_bounds__(256) void k_site_tiers(const JobDesc* __restrict__ 
     if (finite3(q.x, q.y, q.z) && vi >= 0) {
         int slots[K];
         float lk;
-        knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, __builtin_huge_valf(), slots, lk);
+        knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, __builtin_huge_valf(), slots, lk, nullptr,
+                           0.f, nullptr, a.sg.cid);
         const float4 ni = a.m.N[vi];
         double sum_cos = 0.0, mean0 = 0.0, mean1 = 0.0, mean2 = 0.0;
         int count = 0;
@@ -818,7 +872,8 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
     }
     int slots[K];
     float lk;
-    knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, inf, slots, lk);
+    knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, inf, slots, lk, nullptr, 0.f, nullptr,
+                       a.sg.cid);
     int* row = knn_tmp + (size_t)i * K;
     float px[M], py[M];
 #pragma unroll
