@@ -462,13 +462,41 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->diag = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
     j->R = p.clip_R_mode == 1 ? p.clip_R : (double)(float)emax;  // main.cu:226 (FP32 max bbox edge)
     if (!(j->R > 0)) return bail(fail(CVT_EINVAL, "clip half-size R must be > 0"));
+    // faces are stored in the Morton order of their centroids (face ids never
+    // leave the library), so the faces a query visits sit close in memory
+    std::vector<int> Fi((size_t)3 * nF);
+    {
+        std::vector<unsigned long long> key(nF);
+        auto spread = [](unsigned x) {
+            unsigned long long v = x & 0x3ff;
+            v = (v | (v << 16)) & 0x30000ffull;
+            v = (v | (v << 8)) & 0x300f00full;
+            v = (v | (v << 4)) & 0x30c30c3ull;
+            v = (v | (v << 2)) & 0x9249249ull;
+            return v;
+        };
+        for (int f = 0; f < nF; ++f) {
+            unsigned q[3];
+            for (int a = 0; a < 3; ++a) {
+                const double c = ((double)(&V4[Fh[3 * f]].x)[a] + (&V4[Fh[3 * f + 1]].x)[a] + (&V4[Fh[3 * f + 2]].x)[a]) / 3.0;
+                const double t = e[a] > 0 ? (c - j->lo[a]) / e[a] : 0.0;
+                q[a] = (unsigned)std::min(1023.0, std::max(0.0, t * 1024.0));
+            }
+            key[f] = ((spread(q[0]) << 2 | spread(q[1]) << 1 | spread(q[2])) << 32) | (unsigned)f;
+        }
+        std::sort(key.begin(), key.end());
+        for (int k = 0; k < nF; ++k) {
+            const int f = (int)(key[k] & 0xffffffffu);
+            for (int c = 0; c < 3; ++c) Fi[3L * k + c] = Fh[3L * f + c];
+        }
+    }
     std::vector<float4> tri((size_t)3 * nF);
     double area = 0;
     for (int f = 0; f < nF; ++f) {
-        const float4 A = V4[Fh[3 * f]], B = V4[Fh[3 * f + 1]], C = V4[Fh[3 * f + 2]];
+        const float4 A = V4[Fi[3 * f]], B = V4[Fi[3 * f + 1]], C = V4[Fi[3 * f + 2]];
         for (int k = 0; k < 3; ++k) {
-            const float4 P = V4[Fh[3 * f + k]];
-            int id = Fh[3 * f + k];
+            const float4 P = V4[Fi[3 * f + k]];
+            int id = Fi[3 * f + k];
             float w;
             std::memcpy(&w, &id, 4);
             tri[(size_t)3 * f + k] = make_float4(P.x, P.y, P.z, w);
@@ -481,12 +509,12 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->area = area > 0 ? area : emax * emax;
     // vertex -> incident-face CSR in face order (main.cu:259-285)
     std::vector<int> vf_off(nV + 1, 0), vf_faces((size_t)3 * nF);
-    for (long f = 0; f < 3L * nF; ++f) vf_off[Fh[f] + 1]++;
+    for (long f = 0; f < 3L * nF; ++f) vf_off[Fi[f] + 1]++;
     for (int v = 0; v < nV; ++v) vf_off[v + 1] += vf_off[v];
     {
         std::vector<int> cur(vf_off.begin(), vf_off.end() - 1);
         for (int f = 0; f < nF; ++f)
-            for (int k = 0; k < 3; ++k) vf_faces[cur[Fh[3 * f + k]]++] = f;
+            for (int k = 0; k < 3; ++k) vf_faces[cur[Fi[3 * f + k]]++] = f;
     }
     // padded grid bbox
     double glo[3], ghi[3];
@@ -553,7 +581,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         const int i = j->perm[k];
         S4[k] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
     }
-    std::vector<int> Fv(Fh, Fh + 3L * nF);
+    std::vector<int>& Fv = Fi;
     hipStream_t s = j->stream;
     auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
     if (up(j->V, V4.data(), sizeof(float4) * nV) || up(j->tri, tri.data(), sizeof(float4) * tri.size()) ||
