@@ -1035,15 +1035,39 @@ int cvt_job_remesh(cvt_job* j, double eps, int** F_out, int* nF_out) {
 static int ensure_bvh(cvt_job* j);
 
 int cvt_job_hausdorff(cvt_job* j, double eps, double* h_out, int* nF_out) {
+    if (!h_out) return fail(CVT_EINVAL, "null argument");
     TempPool T;
-    return job_hausdorff(j, eps, T, h_out, nF_out);
+    return job_hausdorff(j, eps, T, -1.0, nullptr, true, h_out, nF_out);
+}
+
+int cvt_job_hausdorff_within(cvt_job* j, double eps, double tol, int* within, double* h_out, int* nF_out) {
+    if (!within || !(tol >= 0)) return fail(CVT_EINVAL, "bad argument");
+    TempPool T;
+    double h = -1;
+    const int rc = job_hausdorff(j, eps, T, tol, within, false, h_out ? &h : nullptr, nF_out);
+    if (!rc && h_out) *h_out = h;
+    return rc;
 }
 
 }  // extern "C"
 
 namespace cvt {
-int job_hausdorff(cvt_job* j, double eps, TempPool& T, double* h_out, int* nF_out) {
-    if (!j || !h_out) return fail(CVT_EINVAL, "null argument");
+// Largest float f with sqrt((double)f) <= T: "d2 <= f" in FP32 is then exactly
+// "sqrt(d2) <= T", the test the exact path applies to the maximum.
+static float sq_threshold(double T) {
+    float f = (float)(T * T);
+    while (f > 0.f && std::sqrt((double)f) > T) f = std::nextafter(f, 0.f);
+    for (;;) {
+        const float g = std::nextafter(f, INFINITY);
+        if (!(std::sqrt((double)g) <= T)) break;
+        f = g;
+    }
+    return f;
+}
+
+int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, bool exact_always, double* h_out,
+                  int* nF_out) {
+    if (!j) return fail(CVT_EINVAL, "null argument");
     int rc;
     if ((rc = ensure_bvh(j))) return rc;
     T.reset();
@@ -1051,22 +1075,45 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double* h_out, int* nF_ou
     int nkeep = 0;
     if ((rc = remesh_device(j, eps, T, &dF, &nkeep))) return rc;
     if (nF_out) *nF_out = nkeep;
-    if (nkeep == 0) { *h_out = 0; return 0; }
+    if (nkeep == 0) {
+        if (within) *within = 1;
+        if (h_out) *h_out = 0;
+        return 0;
+    }
     float4* P = (float4*)T.get(sizeof(float4) * 4 * (size_t)nkeep);
     float* d2 = (float*)T.get(sizeof(float) * 4 * (size_t)nkeep);
     unsigned* mx = (unsigned*)T.get(sizeof(unsigned));
     if (!P || !d2 || !mx) return fail(CVT_ENOMEM, "hausdorff temporaries");
     hipStream_t s = j->stream;
     launch_tri_probes(dF, nkeep, j->S[j->cur], P, s);
-    bvh_point_d2(j->bvh->d, P, 4 * nkeep, d2, s);
-    launch_fill((int*)mx, 1, 0, s);
-    launch_max_float(d2, 4 * nkeep, mx, s);
-    unsigned hb = 0;
-    HIPCHK(hipMemcpyAsync(&hb, mx, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    float m;
-    std::memcpy(&m, &hb, 4);
-    *h_out = std::sqrt((double)m);
+    auto max_of = [&](float& m) -> int {
+        launch_fill((int*)mx, 1, 0, s);
+        launch_max_float(d2, 4 * nkeep, mx, s);
+        unsigned hb = 0;
+        HIPCHK(hipMemcpyAsync(&hb, mx, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        std::memcpy(&m, &hb, 4);
+        return 0;
+    };
+    bool in = true;
+    if (tol >= 0 && within) {
+        const float thr2 = sq_threshold(tol);
+        bvh_point_within(j->bvh->d, P, 4 * nkeep, thr2, d2, s);
+        float m;
+        if ((rc = max_of(m))) return rc;
+        in = m <= thr2;
+        *within = in ? 1 : 0;
+    }
+    if (h_out) {
+        if (in || exact_always) {
+            bvh_point_d2(j->bvh->d, P, 4 * nkeep, d2, s);
+            float m;
+            if ((rc = max_of(m))) return rc;
+            *h_out = std::sqrt((double)m);
+        } else {
+            *h_out = -1;
+        }
+    }
     return 0;
 }
 }  // namespace cvt

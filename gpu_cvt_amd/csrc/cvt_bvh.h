@@ -41,5 +41,7 @@ void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const 
                hipStream_t s);
 // Squared distance of each point (float4 xyz) to the mesh.
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s);
+// out[i] <= thr2 iff some triangle lies within thr2 of P[i] (+inf otherwise)
+void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s);
 
 }  // namespace cvt

@@ -302,6 +302,61 @@ __global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __rest
     out[i] = best;
 }
 
+// Bounded query: out[i] = the squared distance of the first triangle found
+// within thr2 of P[i] (<= thr2), or +inf when none is.  Only boxes within thr2
+// are entered and the walk stops at the first hit, so a probe near the surface
+// costs a few node visits instead of a proof of its nearest triangle.
+__global__ __launch_bounds__(256) void k_point_within(BVHDev b, const float4* __restrict__ P, int np, float thr2,
+                                                      float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float4 q4 = P[i];
+    const float3 q = make_float3(q4.x, q4.y, q4.z);
+    const int n = b.n;
+    const float inf = __builtin_huge_valf();
+    if (n == 1) {
+        const int f = b.leaf_tri[0];
+        const float d = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
+        out[i] = d <= thr2 ? d : inf;
+        return;
+    }
+    float hit = inf;
+    int cur = 0, last = -1;
+    for (;;) {
+        if (cur >= n - 1) {  // leaf
+            const int f = b.leaf_tri[cur - (n - 1)];
+            const float d = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
+            if (d <= thr2) { hit = d; break; }
+            last = cur;
+            cur = b.parent[cur];
+            if (cur < 0) break;
+            continue;
+        }
+        const int2 c = b.child[cur];
+        const float dl = box_d2(q, b.lo[c.x], b.hi[c.x]);
+        const float dr = box_d2(q, b.lo[c.y], b.hi[c.y]);
+        const int nearc = dl <= dr ? c.x : c.y, farc = dl <= dr ? c.y : c.x;
+        const float dn = dl <= dr ? dl : dr, df = dl <= dr ? dr : dl;
+        int next;
+        if (last == b.parent[cur]) {
+            next = dn <= thr2 ? nearc : (df <= thr2 ? farc : b.parent[cur]);
+        } else if (last == nearc) {
+            next = df <= thr2 ? farc : b.parent[cur];
+        } else {
+            next = b.parent[cur];
+        }
+        last = cur;
+        cur = next;
+        if (cur < 0) break;
+    }
+    out[i] = hit;
+}
+
+void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_point_within, dim3(nblk(n, 256)), dim3(256), 0, s, b, P, n, thr2, out);
+}
+
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_point_d2, dim3(nblk(n, 256)), dim3(256), 0, s, b, P, n, out);

@@ -50,8 +50,12 @@ struct TempPool {
     }
 };
 
-// cvt_job_hausdorff with caller-owned temporaries (the serving loop keeps one
-// pool for all of its convergence checks).
-__attribute__((visibility("hidden"))) int job_hausdorff(cvt_job* j, double eps, TempPool& pool, double* h_out, int* nF_out);
+// One-sided Hausdorff of the remeshed surface with caller-owned temporaries
+// (the serving loop keeps one pool per check worker).  tol >= 0 with `within`:
+// first a bounded query decides whether every probe lies within tol
+// (*within); the exact distance (h_out, nullable) is then computed only when
+// it does or exact_always is set (else *h_out = -1).  tol < 0: exact only.
+__attribute__((visibility("hidden"))) int job_hausdorff(cvt_job* j, double eps, TempPool& pool, double tol,
+                                                        int* within, bool exact_always, double* h_out, int* nF_out);
 
 }  // namespace cvt

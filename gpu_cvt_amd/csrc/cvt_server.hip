@@ -45,7 +45,8 @@ struct Request {
     unsigned long long bytes = 0;
     double diag = 0;
     int next_check = 0;
-    double checked_h = -1;  // this epoch's Hausdorff check (-1: none)
+    double checked_h = -1;  // this epoch's Hausdorff distance (-1: not computed)
+    int checked_in = 0;     // this epoch's check found every probe within tolerance
     std::vector<double> S_out;
 };
 
@@ -69,7 +70,7 @@ struct cvt_server {
     std::thread sched_th;
     std::vector<cvt::TempPool> check_pools;  // device temporaries, one set per check worker
     // counters
-    double site_iters = 0, busy_s = 0;
+    double site_iters = 0, busy_s = 0, rebuild_s = 0, step_s = 0, check_s = 0;
     long epochs = 0, rebuilds = 0, checks = 0;
     int failures = 0;
 
@@ -182,6 +183,7 @@ void cvt_server::sched_loop() {
         const auto e0 = Clock::now();
         double si = 0;
         long nrebuild = 0, ncheck = 0;
+        double t_rebuild = 0, t_step = 0, t_check = 0;
         std::vector<Request*> done;
         std::vector<int> status;
         for (auto& kv : groups) {
@@ -189,6 +191,7 @@ void cvt_server::sched_loop() {
             if (g.members.empty()) continue;
             std::vector<cvt_job*> jobs;
             for (Request* r : g.members) jobs.push_back(r->job);
+            const auto r0 = Clock::now();
             if (g.dirty) {
                 if (g.batch) cvt_batch_destroy(g.batch);
                 g.batch = nullptr;
@@ -196,11 +199,15 @@ void cvt_server::sched_loop() {
                 g.dirty = false;
                 ++nrebuild;
             }
+            const auto r1 = Clock::now();
             if (g.batch) {
                 cvt_batch_step(g.batch, cfg.epoch_iters);
             } else {
                 for (cvt_job* j : jobs) cvt_job_step(j, cfg.epoch_iters, nullptr);
             }
+            const auto r2 = Clock::now();
+            t_rebuild += std::chrono::duration<double>(r1 - r0).count();
+            t_step += std::chrono::duration<double>(r2 - r1).count();
             // convergence at the epoch boundary: gate each job, then run the
             // Hausdorff checks of the gated jobs in parallel host threads (each
             // on its job's own stream, with its own device temporaries)
@@ -221,6 +228,7 @@ void cvt_server::sched_loop() {
                                    st.frozen_total >= cfg.check_frozen_frac * r->res.nS);
                 is_last[k] = r->res.iterations >= r->max_iters;
                 r->checked_h = -1;
+                r->checked_in = 0;
                 if (gate || is_last[k]) check.push_back(r);
             }
             const int nth = (int)std::min<size_t>(check_pools.size(), check.size());
@@ -228,17 +236,26 @@ void cvt_server::sched_loop() {
             std::atomic<long> nok{0};
             auto worker = [&](int w) {
                 for (size_t c = next++; c < check.size(); c = next++) {
+                    Request* r = check[c];
+                    // bounded test first; the exact distance only for a job that
+                    // converged or reached its last iteration
+                    const bool last = r->res.iterations >= r->max_iters;
                     double h = -1;
-                    if (cvt::job_hausdorff(check[c]->job, 1e-6, check_pools[w], &h, nullptr) == CVT_OK) {
-                        check[c]->checked_h = h;
+                    int in = 0;
+                    if (cvt::job_hausdorff(r->job, 1e-6, check_pools[w], r->tol_rel * r->diag, &in, last, &h,
+                                           nullptr) == CVT_OK) {
+                        r->checked_h = h;
+                        r->checked_in = in;
                         ++nok;
                     }
                 }
             };
+            const auto c0 = Clock::now();
             std::vector<std::thread> ths;
             for (int w = 1; w < nth; ++w) ths.emplace_back(worker, w);
             if (!check.empty()) worker(0);
             for (std::thread& t : ths) t.join();
+            t_check += secs(c0);
             ncheck += nok;
             std::vector<Request*> keep;
             for (size_t k = 0; k < g.members.size(); ++k) {
@@ -248,7 +265,7 @@ void cvt_server::sched_loop() {
                     std::lock_guard<std::mutex> lk(mu);
                     r->res.hausdorff = r->checked_h;
                 }
-                if (r->checked_h >= 0 && r->checked_h <= r->tol_rel * r->diag) st_done = 2;
+                if (r->checked_h >= 0 && r->checked_in) st_done = 2;
                 else if (is_last[k]) st_done = 3;
                 if (st_done) {
                     done.push_back(r);
@@ -273,6 +290,9 @@ void cvt_server::sched_loop() {
             ++epochs;
             rebuilds += nrebuild;
             checks += ncheck;
+            rebuild_s += t_rebuild;
+            step_s += t_step;
+            check_s += t_check;
             for (Request* r : done) active.erase(std::find(active.begin(), active.end(), r));
         }
         for (size_t k = 0; k < done.size(); ++k) finish(done[k], status[k]);
@@ -376,6 +396,9 @@ int cvt_server_info_get(cvt_server* s, cvt_server_info* out) {
     out->pending = (int)s->pending.size();
     out->active = (int)(s->active.size() + s->ready.size());
     out->failures = s->failures;
+    out->step_seconds = s->step_s;
+    out->check_seconds = s->check_s;
+    out->rebuild_seconds = s->rebuild_s;
     return CVT_OK;
 }
 

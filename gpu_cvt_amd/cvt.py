@@ -29,7 +29,7 @@ EXPORTS = [
     "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy", "cvt_job_hausdorff",
     "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
     "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
-    "cvt_job_compute_tiers",
+    "cvt_job_compute_tiers", "cvt_job_hausdorff_within",
 ]
 
 
@@ -93,6 +93,7 @@ class ServerInfo(C.Structure):
         ("peak_bytes", C.c_ulonglong), ("used_bytes", C.c_ulonglong),
         ("epochs", C.c_longlong), ("rebuilds", C.c_longlong), ("hausdorff_checks", C.c_longlong),
         ("pending", C.c_int), ("active", C.c_int), ("failures", C.c_int), ("reserved", C.c_int * 5),
+        ("step_seconds", C.c_double), ("check_seconds", C.c_double), ("rebuild_seconds", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -161,6 +162,8 @@ def lib():
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.cvt_job_hausdorff.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
+    L.cvt_job_hausdorff_within.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cvt_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
     L.cvt_batch_step.argtypes = [C.c_void_p, C.c_int]
     L.cvt_batch_step_timed.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double)]
@@ -368,6 +371,15 @@ class Job:
         nf = C.c_int()
         _check("cvt_job_hausdorff", lib().cvt_job_hausdorff(self.handle, float(eps), C.byref(h), C.byref(nf)))
         return h.value, nf.value
+
+    def hausdorff_within(self, tol: float, eps: float = 1e-6):
+        """(converged, exact distance if converged else -1): whether the
+        one-sided Hausdorff distance is <= tol, by a bounded device query."""
+        w = C.c_int()
+        h = C.c_double()
+        _check("cvt_job_hausdorff_within", lib().cvt_job_hausdorff_within(self.handle, float(eps), float(tol),
+                                                                          C.byref(w), C.byref(h), None))
+        return bool(w.value), h.value
 
     def point_mesh_d2(self, P) -> np.ndarray:
         P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)

@@ -169,6 +169,13 @@ void cvt_free(void* p);
  * remeshed triangle count. */
 int cvt_job_hausdorff(cvt_job* job, double eps, double* h_out, int* nF_out);
 
+/* Convergence test for the same one-sided distance: *within = 1 iff every
+ * probe lies within tol of the input mesh (i.e. cvt_job_hausdorff <= tol), by
+ * a bounded LBVH query that stops at the first triangle within tol.  h_out
+ * (nullable) receives the exact distance when *within is 1, else -1.  Replaces
+ * the dH <= tolerance decision of the reference's dump loop (main.cu:1034). */
+int cvt_job_hausdorff_within(cvt_job* job, double eps, double tol, int* within, double* h_out, int* nF_out);
+
 /* Squared distance from each query point (n*3, host) to the job's input mesh,
  * via a device LBVH (replaces bvh_closest_d2_kernel, evaluate.cuh:206-250, and
  * the CPU BVH of evaluate.cuh:114-201).  out: n doubles. */
@@ -241,6 +248,9 @@ typedef struct cvt_server_info {
     long long epochs, rebuilds, hausdorff_checks;
     int pending, active, failures;
     int reserved[5];
+    double step_seconds;      /* of busy_seconds: batch steps */
+    double check_seconds;     /* convergence checks */
+    double rebuild_seconds;   /* batch graph rebuilds after membership changes */
 } cvt_server_info;
 
 typedef struct cvt_server cvt_server;

@@ -66,3 +66,17 @@ def test_remesh_and_hausdorff_match_oracle(orc, case):
     print(f"[remesh] {name}: hausdorff device {h_dev:.6e} oracle {h_ref:.6e}", flush=True)
     assert abs(h_dev - h_ref) <= 1e-3 * h_ref + 1e-7 * mesh.bbox_diag
     job.close()
+
+
+def test_hausdorff_within_agrees_with_exact():
+    """The bounded convergence test says "within" exactly when the exact
+    one-sided distance is <= tol, and then reports that same distance."""
+    V, F = mg.displaced_sphere(30, seed=24, crease_seed=25)
+    with cvt.Job(V, F, mg.sample_surface(V, F, 5000, seed=27), K=24) as job:
+        job.step(6)
+        h, _ = job.hausdorff()
+        for tol in (0.5 * h, np.nextafter(h, 0), h, 1.5 * h):
+            w, hw = job.hausdorff_within(tol)
+            print(f"[within] h={h:.9e} tol={tol:.9e} within={w} h_within={hw:.9e}", flush=True)
+            assert w == (h <= tol)
+            assert (hw == h) if w else (hw == -1.0)
