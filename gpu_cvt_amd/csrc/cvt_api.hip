@@ -62,6 +62,7 @@ struct cvt_job {
     int *F = nullptr, *vf_off = nullptr, *vf_faces = nullptr;
     GridP vg{};
     GridBufs vgb{};
+    PointBVH vb{};  // LBVH over the vertex grid's points
     // sites
     GridP sgp{};
     GridBufs sgb{};
@@ -290,6 +291,7 @@ JobDesc make_desc(cvt_job* j) {
     a.m.nV = j->nV;
     a.m.nF = j->nF;
     a.m.vg = j->vg;
+    a.m.vb = j->vb;
     a.m.vg_start = j->vgb.start;
     a.m.vg_pts = j->vgb.pts;
     a.sg.g = j->sgp;
@@ -610,6 +612,25 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         launch_frames(m, j->FU, j->FW, s);
     }
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
+    TempPool build_tmp;  // lives until the set-up synchronises below
+    {
+        // point LBVH over the vertex grid's cell-sorted points (leaf = slot)
+        BVHDev bd;
+        float4 *lo4 = nullptr, *hi4 = nullptr;
+        int2* ch = nullptr;
+        int *par = nullptr, *leaf = nullptr;
+        if ((rc = j->alloc(&lo4, 2L * nV - 1)) || (rc = j->alloc(&hi4, 2L * nV - 1)) ||
+            (rc = j->alloc(&ch, std::max(1, nV - 1))) || (rc = j->alloc(&par, 2L * nV - 1)) ||
+            (rc = j->alloc(&leaf, nV)))
+            return bail(rc);
+        void* work = build_tmp.get(bvh_work_bytes(nV));
+        if (!work) return bail(fail(CVT_ENOMEM, "vertex LBVH work"));
+        bd.lo = lo4; bd.hi = hi4; bd.child = ch; bd.parent = par; bd.leaf_tri = leaf;
+        float blo[3], bhi[3];
+        for (int a = 0; a < 3; ++a) { blo[a] = (float)j->lo[a]; bhi[a] = (float)j->hi[a]; }
+        bvh_build(bd, j->vgb.pts, nV, blo, bhi, work, s, 1);
+        j->vb = PointBVH{nV, lo4, hi4, ch, par, leaf};
+    }
     {
         // this job as a batch of one
         JobDesc dh = make_desc(j);

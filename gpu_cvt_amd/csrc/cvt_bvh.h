@@ -37,8 +37,9 @@ struct BVHDev {
 size_t bvh_work_bytes(int nF);
 // Builds the hierarchy for nF triangles of the face table `tri` (device),
 // using `work` (device, bvh_work_bytes) and the node arrays in `out`.
+// ncorner = 1 builds it over points (tri then holds one float4 per point).
 void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const float hi[3], void* work,
-               hipStream_t s);
+               hipStream_t s, int ncorner = 3);
 // Squared distance of each point (float4 xyz) to the mesh.
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s);
 // out[i] <= thr2 iff some triangle lies within thr2 of P[i] (+inf otherwise)

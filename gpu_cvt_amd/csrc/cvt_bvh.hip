@@ -127,13 +127,21 @@ __device__ __forceinline__ unsigned expand10(unsigned v) {
     return v;
 }
 
-__global__ void k_morton(const float4* __restrict__ tri, int nF, float3 lo, float3 inv_ext, unsigned* __restrict__ keys,
-                         int* __restrict__ ids) {
+// Primitives are `nc` consecutive float4 corners: 3 for a triangle, 1 for a point.
+__global__ void k_morton(const float4* __restrict__ tri, int nF, int nc, float3 lo, float3 inv_ext,
+                         unsigned* __restrict__ keys, int* __restrict__ ids) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= nF) return;
-    const float4 a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
-    const float cx = (a.x + b.x + c.x) * (1.f / 3.f), cy = (a.y + b.y + c.y) * (1.f / 3.f),
-                cz = (a.z + b.z + c.z) * (1.f / 3.f);
+    float cx, cy, cz;
+    if (nc == 3) {
+        const float4 a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
+        cx = (a.x + b.x + c.x) * (1.f / 3.f);
+        cy = (a.y + b.y + c.y) * (1.f / 3.f);
+        cz = (a.z + b.z + c.z) * (1.f / 3.f);
+    } else {
+        const float4 a = tri[f];
+        cx = a.x; cy = a.y; cz = a.z;
+    }
     const unsigned x = (unsigned)fminf(fmaxf((cx - lo.x) * inv_ext.x * 1024.f, 0.f), 1023.f);
     const unsigned y = (unsigned)fminf(fmaxf((cy - lo.y) * inv_ext.y * 1024.f, 0.f), 1023.f);
     const unsigned z = (unsigned)fminf(fmaxf((cz - lo.z) * inv_ext.z * 1024.f, 0.f), 1023.f);
@@ -176,12 +184,12 @@ __global__ void k_karras(const unsigned* __restrict__ keys, int n, int2* __restr
     parent[right] = i;
 }
 
-__global__ void k_leaf_boxes(const float4* __restrict__ tri, const int* __restrict__ leaf_tri, int n,
+__global__ void k_leaf_boxes(const float4* __restrict__ tri, int nc, const int* __restrict__ leaf_tri, int n,
                              float4* __restrict__ lo, float4* __restrict__ hi, int* __restrict__ flags) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
     const int f = leaf_tri[l];
-    const float4 a = tri[3 * f], b = tri[3 * f + 1], c = tri[3 * f + 2];
+    const float4 a = tri[nc * f], b = tri[nc * f + (nc == 3 ? 1 : 0)], c = tri[nc * f + (nc == 3 ? 2 : 0)];
     lo[n - 1 + l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.f);
     hi[n - 1 + l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.f);
     if (l < n - 1) flags[l] = 0;
@@ -212,7 +220,7 @@ size_t bvh_work_bytes(int nF) {
 }
 
 void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const float hi[3], void* work,
-               hipStream_t s) {
+               hipStream_t s, int ncorner) {
     out.n = nF;
     out.tri = tri;
     char* w = (char*)work;
@@ -231,11 +239,12 @@ void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const 
     float3 l3 = make_float3(lo[0], lo[1], lo[2]);
     float3 ie = make_float3(1.f / fmaxf(hi[0] - lo[0], 1e-30f), 1.f / fmaxf(hi[1] - lo[1], 1e-30f),
                             1.f / fmaxf(hi[2] - lo[2], 1e-30f));
-    hipLaunchKernelGGL(k_morton, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, nF, l3, ie, keys, out.leaf_tri);
+    hipLaunchKernelGGL(k_morton, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, nF, ncorner, l3, ie, keys, out.leaf_tri);
     radix_sort_pairs(keys, out.leaf_tri, nF, 30, rb, s);
     launch_fill(out.parent, 2L * nF - 1, -1, s);
     if (nF > 1) hipLaunchKernelGGL(k_karras, dim3(nblk(nF - 1, 256)), dim3(256), 0, s, keys, nF, out.child, out.parent);
-    hipLaunchKernelGGL(k_leaf_boxes, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, out.leaf_tri, nF, out.lo, out.hi, flags);
+    hipLaunchKernelGGL(k_leaf_boxes, dim3(nblk(nF, 256)), dim3(256), 0, s, tri, ncorner, out.leaf_tri, nF, out.lo, out.hi,
+                       flags);
     if (nF > 1) hipLaunchKernelGGL(k_refit, dim3(nblk(nF, 256)), dim3(256), 0, s, nF, out.child, out.parent, out.lo,
                                    out.hi, flags);
 }

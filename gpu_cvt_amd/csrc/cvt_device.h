@@ -140,6 +140,74 @@ __device__ __forceinline__ float block_bound(const GridP& g, float qx, float qy,
     return bnd;
 }
 
+// LBVH over the points of a grid's cell-sorted store (leaf payload = the
+// point's slot there): internal nodes [0, n-1), leaves [n-1, 2n-1).  The
+// exact fallback for queries far from every point, where the ring search of
+// the grid would have to sweep O((distance / cell)^3) cells.
+struct PointBVH {
+    int n;
+    const float4* lo;
+    const float4* hi;
+    const int2* child;
+    const int* parent;
+    const int* leaf;
+};
+
+__device__ __forceinline__ float box_gap2(float qx, float qy, float qz, float4 lo, float4 hi) {
+    const float dx = fmaxf(fmaxf(lo.x - qx, 0.f), qx - hi.x);
+    const float dy = fmaxf(fmaxf(lo.y - qy, 0.f), qy - hi.y);
+    const float dz = fmaxf(fmaxf(lo.z - qz, 0.f), qz - hi.z);
+    return dx * dx + dy * dy + dz * dz;
+}
+
+// Top-L of all points (self_slot excluded) by the same FP32 distance as the
+// grid search, by a near-first stackless walk that enters a box only while it
+// may hold a point closer than the current L-th (with a 1e-6 slack for the
+// box distance's different rounding).
+template <int L>
+__device__ __noinline__ void bvh_topl(const PointBVH& b, const float4* __restrict__ pts, float qx, float qy, float qz,
+                                      int self_slot, TopL<L>& T) {
+    T.init(__builtin_huge_valf());
+    const int n = b.n;
+    if (n <= 0) return;
+    if (n == 1) {
+        const int sl = b.leaf[0];
+        if (sl != self_slot) T.insert(d2f(qx, qy, qz, pts[sl]), sl);
+        return;
+    }
+    int cur = 0, last = -1;
+    for (;;) {
+        if (cur >= n - 1) {
+            const int sl = b.leaf[cur - (n - 1)];
+            if (sl != self_slot) {
+                const float d = d2f(qx, qy, qz, pts[sl]);
+                if (d < T.worst()) T.insert(d, sl);
+            }
+            last = cur;
+            cur = b.parent[cur];
+            if (cur < 0) break;
+            continue;
+        }
+        const int2 c = b.child[cur];
+        const float dl = box_gap2(qx, qy, qz, b.lo[c.x], b.hi[c.x]);
+        const float dr = box_gap2(qx, qy, qz, b.lo[c.y], b.hi[c.y]);
+        const int nearc = dl <= dr ? c.x : c.y, farc = dl <= dr ? c.y : c.x;
+        const float dn = dl <= dr ? dl : dr, df = dl <= dr ? dr : dl;
+        const float lim = T.worst() * (1.f + 1e-6f);
+        int next;
+        if (last == b.parent[cur]) {
+            next = dn <= lim ? nearc : (df <= lim ? farc : b.parent[cur]);
+        } else if (last == nearc) {
+            next = df <= lim ? farc : b.parent[cur];
+        } else {
+            next = b.parent[cur];
+        }
+        last = cur;
+        cur = next;
+        if (cur < 0) break;
+    }
+}
+
 // FP32 top-L of the whole point set (self_slot excluded), seeded with radius
 // tau: the 3x3x3 block first (9 runs, bounds loaded together); if the block
 // boundary is not provably beyond the L-th distance, the list is rebuilt over
@@ -147,7 +215,8 @@ __device__ __forceinline__ float block_bound(const GridP& g, float qx, float qy,
 // within tau (the caller then repeats with tau = inf).
 template <int L>
 __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
-                                          float qx, float qy, float qz, int self_slot, float tau, TopL<L>& T) {
+                                          float qx, float qy, float qz, int self_slot, float tau, TopL<L>& T,
+                                          int max_r = 0, bool* limited = nullptr) {
     int cx, cy, cz;
     cell_of(g, qx, qy, qz, cx, cy, cz);
     const float h = g.h;
@@ -176,6 +245,10 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
         if (bnd == __builtin_huge_valf()) break;
         bnd -= g.eps_abs;
         if (bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f)) break;
+        if (max_r > 0 && r >= max_r) {  // the caller finishes with its exact fallback
+            *limited = true;
+            return false;
+        }
         // not provable yet: add the shell at Chebyshev radius r+1 (the list
         // only shrinks, so rows pruned against it earlier stay pruned; the
         // whole search is O(cells) even for a query far outside the grid)
@@ -307,6 +380,70 @@ __device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, flo
     return true;
 }
 
+// Sorted insertion into an exact (FP64 key, tie id) top-K.
+template <int K>
+__device__ __forceinline__ void exact_insert(double (&kd)[K], int (&kid)[K], int (&ks)[K], double d, int id, int s) {
+    if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) return;
+#pragma unroll
+    for (int t = K - 1; t >= 1; --t) {
+        const bool ct = d < kd[t] || (d == kd[t] && id < kid[t]);
+        const bool cp = d < kd[t - 1] || (d == kd[t - 1] && id < kid[t - 1]);
+        kd[t] = cp ? kd[t - 1] : (ct ? d : kd[t]);
+        kid[t] = cp ? kid[t - 1] : (ct ? id : kid[t]);
+        ks[t] = cp ? ks[t - 1] : (ct ? s : ks[t]);
+    }
+    const bool c0 = d < kd[0] || (d == kd[0] && id < kid[0]);
+    kd[0] = c0 ? d : kd[0];
+    kid[0] = c0 ? id : kid[0];
+    ks[0] = c0 ? s : ks[0];
+}
+
+// The exact rescan over a point LBVH: every point within tau (boxes within
+// tau, with slack) enters the FP64-keyed top-K.
+template <int K>
+__device__ __noinline__ void bvh_exact64(const PointBVH& b, const float4* __restrict__ pts, float qx, float qy,
+                                         float qz, int self_slot, float tau, int* __restrict__ out_slot,
+                                         const int* __restrict__ cid) {
+    double kd[K];
+    int kid[K], ks[K];
+#pragma unroll
+    for (int t = 0; t < K; ++t) { kd[t] = __builtin_huge_val(); kid[t] = 0x7fffffff; ks[t] = -1; }
+    const int n = b.n;
+    const float lim = tau * (1.f + 1e-5f);
+    if (n == 1) {
+        const int sl = b.leaf[0];
+        const float4 p = pts[sl];
+        if (sl != self_slot && d2f(qx, qy, qz, p) <= tau)
+            exact_insert<K>(kd, kid, ks, key64(qx, qy, qz, p.x, p.y, p.z), tie_id(p, cid), sl);
+    } else if (n > 1) {
+        int cur = 0, last = -1;
+        for (;;) {
+            if (cur >= n - 1) {
+                const int sl = b.leaf[cur - (n - 1)];
+                const float4 p = pts[sl];
+                if (sl != self_slot && d2f(qx, qy, qz, p) <= tau)
+                    exact_insert<K>(kd, kid, ks, key64(qx, qy, qz, p.x, p.y, p.z), tie_id(p, cid), sl);
+                last = cur;
+                cur = b.parent[cur];
+                if (cur < 0) break;
+                continue;
+            }
+            const int2 c = b.child[cur];
+            const bool inl = box_gap2(qx, qy, qz, b.lo[c.x], b.hi[c.x]) <= lim;
+            const bool inr = box_gap2(qx, qy, qz, b.lo[c.y], b.hi[c.y]) <= lim;
+            int next;
+            if (last == b.parent[cur]) next = inl ? c.x : (inr ? c.y : b.parent[cur]);
+            else if (last == c.x) next = inr ? c.y : b.parent[cur];
+            else next = b.parent[cur];
+            last = cur;
+            cur = next;
+            if (cur < 0) break;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t) out_slot[t] = ks[t];
+}
+
 // Exact K nearest under (key64, id) by FP64 insertion over every point with
 // FP32 distance <= tau.  Only the FP64-resolving pass uses it.
 template <int K>
@@ -346,21 +483,7 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
                     for (int s = start[row + a0]; s < s1; ++s) {
                         const float4 p = pts[s];
                         if (s == self_slot || d2f(qx, qy, qz, p) > tau) continue;
-                        const double d = key64(qx, qy, qz, p.x, p.y, p.z);
-                        const int id = tie_id(p, cid);
-                        if (!(d < kd[K - 1] || (d == kd[K - 1] && id < kid[K - 1]))) continue;
-#pragma unroll
-                        for (int t = K - 1; t >= 1; --t) {
-                            const bool ct = d < kd[t] || (d == kd[t] && id < kid[t]);
-                            const bool cp = d < kd[t - 1] || (d == kd[t - 1] && id < kid[t - 1]);
-                            kd[t] = cp ? kd[t - 1] : (ct ? d : kd[t]);
-                            kid[t] = cp ? kid[t - 1] : (ct ? id : kid[t]);
-                            ks[t] = cp ? ks[t - 1] : (ct ? s : ks[t]);
-                        }
-                        const bool c0 = d < kd[0] || (d == kd[0] && id < kid[0]);
-                        kd[0] = c0 ? d : kd[0];
-                        kid[0] = c0 ? id : kid[0];
-                        ks[0] = c0 ? s : ks[0];
+                        exact_insert<K>(kd, kid, ks, key64(qx, qy, qz, p.x, p.y, p.z), tie_id(p, cid), s);
                     }
                 }
             }
@@ -384,7 +507,8 @@ template <int K, bool PRECISE, int CAP = 0>
 __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
                                          float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
                                          float& l_out, int* lds = nullptr, float collect_tau = 0.f,
-                                         float* d_out = nullptr, const int* __restrict__ cid = nullptr) {
+                                         float* d_out = nullptr, const int* __restrict__ cid = nullptr,
+                                         const PointBVH* pbvh = nullptr) {
     constexpr int L = K + 2;
     TopL<L> T;
     bool have = false;
@@ -392,8 +516,15 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
         if (lds && collect_tau > 0.f)
             have = grid_collect_topl<L, CAP>(g, start, pts, qx, qy, qz, self_slot, collect_tau, lds, T);
     }
-    if (!have && !grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T))
-        grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T);
+    bool limited = false;
+    if (!have) {
+        // with a point LBVH, the ring search stops after two shells and the
+        // LBVH finishes the query
+        const int max_r = pbvh ? 2 : 0;
+        if (!grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T, max_r, &limited) && !limited)
+            grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T, max_r, &limited);
+        if (limited) bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, T);
+    }
     l_out = T.s[L - 1] >= 0 ? T.d[L - 1] : __builtin_huge_valf();
     int pairs;
     const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs, cid);
@@ -407,7 +538,8 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
     if constexpr (!PRECISE) {
         return -1;
     } else {
-        grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
+        if (limited) bvh_exact64<K>(*pbvh, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
+        else grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
         if (d_out) {  // FP32 distances no longer follow the rescanned list: report no margin
 #pragma unroll
             for (int t = 0; t < L; ++t) d_out[t] = 0.f;

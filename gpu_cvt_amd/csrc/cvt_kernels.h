@@ -20,6 +20,7 @@ struct MeshDev {
     const int* vf_faces;  // 3*nF
     int nV, nF;
     GridP vg;             // static vertex grid (pts: x, y, z, vertex id bits)
+    PointBVH vb;          // LBVH over the vertex grid's points (far-query fallback)
     const int* vg_start;
     const float4* vg_pts;
 };

@@ -276,7 +276,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
                 float l1, d1[3];
                 const int k1 =
                     knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1, nullptr,
-                                          0.f, d1);
+                                          0.f, d1, nullptr, &a.m.vb);
                 defer = defer || k1 < 0;
                 nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
                 if (k1 >= 0 && !eval_only) {
@@ -435,7 +435,7 @@ __device__ __forceinline__ SiteFlags lloyd_site(const StepArgs& a, int par, bool
                     int sl[KP];
                     float lv, dv[KP + 2];
                     const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1,
-                                                          tauv, sl, lv, nullptr, 0.f, dv);
+                                                          tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
                     if (!PRECISE && kk < 0) {
                         r.same = same;
                         r.row_written = !eval_only;
@@ -731,7 +731,8 @@ __global__ void k_site_nn(const JobDesc* __restrict__ desc, int* __restrict__ nn
     if (finite3(q.x, q.y, q.z)) {
         int s1[1];
         float l1;
-        knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, __builtin_huge_valf(), s1, l1);
+        knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, __builtin_huge_valf(), s1, l1, nullptr,
+                           0.f, nullptr, nullptr, &a.m.vb);
         nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
     }
     nn_out[i] = nn;
@@ -861,7 +862,8 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
     const float inf = __builtin_huge_valf();
     int s1[1];
     float l1;
-    knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, inf, s1, l1);
+    knn_exact<1, true>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, inf, s1, l1, nullptr, 0.f, nullptr, nullptr,
+                       &a.m.vb);
     const int nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
     float3 u = make_float3(1.f, 0.f, 0.f), w = make_float3(0.f, 1.f, 0.f);
     if (nn >= 0) {

@@ -34,6 +34,11 @@ def _case(name):
         V, F = mg.torus(1.0, 0.3, 150, 60, jitter=1e-3, seed=1)
         S = mg.sample_surface(V, F, 8000, seed=2)
         return V, F, S, 24
+    if name == "cap_far":  # sites on a small cap: boundary cells open to the clip square, far centroids
+        V, F = mg.icosphere(4)
+        S = mg.sample_surface(V, F, 20000, seed=12)
+        S = S[S[:, 2] > 0.85][:400]
+        return V, F, S, 8
     if name == "dsphere_k32":
         V, F = mg.displaced_sphere(40, seed=4, crease_seed=5)
         S = mg.sample_surface(V, F, 6000, seed=6)
@@ -75,7 +80,7 @@ def _stage_check(orc, mesh, job, S32, K, prev_state=None):
     return dict(knn=knn_g, cent=dbg["cent"], Snew=Snew_g, err_c=err_c, err_p=err_p)
 
 
-@pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32"])
+@pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32", "cap_far"])
 def test_stages_teacher_forced(orc, name):
     V, F, S, K = _case(name)
     print(f"[parity] case={name} nV={len(V)} nF={len(F)} nS={len(S)} K={K}", flush=True)
