@@ -5,6 +5,7 @@ Usage: python -m gpu_cvt_amd.build [--jobs N]
 from __future__ import annotations
 
 import os
+import re
 import subprocess
 import sys
 from concurrent.futures import ThreadPoolExecutor
@@ -15,6 +16,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcvt_hip.so")
 CLI = os.path.join(HERE, "gpu_cvt")
 SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip", "cvt_server.hip"]
+# cvt_lloyd.hip is compiled once per neighbour count K (-DCVT_K=k)
+PER_K_SOURCE = "cvt_lloyd.hip"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVT_OFFLOAD_ARCH", "gfx950")
 
@@ -27,13 +30,25 @@ FLAGS = [
 ]
 
 
+def k_values() -> list[int]:
+    """The K list of CVT_FOR_EACH_K in cvt_kernels.h (the run-time dispatch)."""
+    with open(os.path.join(CSRC, "cvt_kernels.h")) as f:
+        for line in f:
+            if line.startswith("#define CVT_FOR_EACH_K(X)"):
+                return [int(t) for t in re.findall(r"X\((\d+)\)", line)]
+    raise RuntimeError("CVT_FOR_EACH_K not found in cvt_kernels.h")
+
+
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_obj")
     os.makedirs(objdir, exist_ok=True)
+    units = [(src, src.replace(".hip", ".o"), []) for src in SOURCES]
+    units += [(PER_K_SOURCE, f"cvt_lloyd_k{k}.o", [f"-DCVT_K={k}"]) for k in k_values()]
 
-    def compile_one(src: str) -> str:
-        obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    def compile_one(unit) -> str:
+        src, oname, defs = unit
+        obj = os.path.join(objdir, oname)
+        cmd = [HIPCC, *FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -43,8 +58,10 @@ def build(verbose: bool = False) -> str:
             print(r.stderr, file=sys.stderr)
         return obj
 
-    with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(compile_one, SOURCES))
+    jobs = int(os.environ.get("MAX_JOBS", "0")) or min(16, os.cpu_count() or 1)
+    # the largest units first, so the pool drains evenly
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        objs = list(ex.map(compile_one, sorted(units, key=lambda u: u[0] != PER_K_SOURCE)))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -64,6 +64,16 @@ __device__ __forceinline__ float d2f(float qx, float qy, float qz, float4 p) {
 // Relative error allowance of d2f against the exact squared distance
 // (|err| <= ~5 u d2 with u = 2^-24); comparisons use 2e-6.
 constexpr float kRel = 2e-6f;
+// Allowance for lists sorted by truncated keys (the low 6 bits of the FP32
+// distance bits replaced by a candidate number, see grid_collect_topl): a
+// truncated value t of a distance d satisfies t <= d < t (1 + 2^-17), so
+// (1 + 2^-17)(1 + kRel) - 1 < 1e-5 covers both.
+constexpr float kRelKey = 1e-5f;
+
+// Median of three unsigned values (compiles to one v_med3_u32).
+__device__ __forceinline__ unsigned umed3(unsigned a, unsigned b, unsigned c) {
+    return max(min(a, b), min(max(a, b), c));
+}
 
 // Sorted (ascending) top-L list of (FP32 d2, slot) in registers.  init(tau)
 // fills it with sentinels (tau, -1): only candidates closer than tau enter.
@@ -276,17 +286,25 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
     return T.s[L - 1] >= 0 || tau == __builtin_huge_valf();
 }
 
-// Collect-then-insert variant of grid_topl for the FP32 pass: every point of
+// Collect-then-select variant of grid_topl for the FP32 pass: every point of
 // the 3x3x3 block closer than tau is appended to this thread's LDS column
 // (lds[k * 256], k < CAP; conflict-free since the stride is a multiple of the
-// bank count), then the at most CAP candidates are inserted.  Divergent lanes
-// thus pay a 4-op append per candidate instead of a full sorted insertion.
-// Returns false -- the caller then runs grid_topl -- when fewer than L or more
-// than CAP points lie within tau, or the block is not provably large enough.
+// bank count), so divergent lanes pay a 4-op append per candidate.  The L
+// nearest of the collected points are then selected into a sorted list of
+// 32-bit keys: the FP32 distance bits with the low 6 bits replaced by the
+// candidate number k (non-negative floats order like their bits, and k makes
+// every key unique), so one sorted insertion is one median-of-three per list
+// entry.  The list comes back with each key's truncated distance (a lower
+// bound within a factor 1 + 2^-17 of the FP32 distance; the caller orders
+// with kRelKey) and the slot of candidate k; worst_up receives an upper bound
+// of the FP32 L-th distance.  Returns false -- the caller then runs grid_topl
+// -- when fewer than L or more than CAP points lie within tau, or the block
+// is not provably large enough.
 template <int L, int CAP>
 __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __restrict__ start,
                                                   const float4* __restrict__ pts, float qx, float qy, float qz,
-                                                  int self_slot, float tau, int* lds, TopL<L>& T) {
+                                                  int self_slot, float tau, int* lds, TopL<L>& T, float& worst_up) {
+    static_assert(CAP <= 64, "candidate numbers must fit in the 6 low key bits");
     int cx, cy, cz;
     cell_of(g, qx, qy, qz, cx, cy, cz);
     const float h = g.h;
@@ -321,15 +339,25 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __r
         }
     }
     if (cnt < L || cnt > CAP) return false;
-    T.init(__builtin_huge_valf());
+    unsigned key[L];
+#pragma unroll
+    for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
     for (int k = 0; k < cnt; ++k) {
-        const int sl = lds[k * 256];
-        T.insert(d2f(qx, qy, qz, pts[sl]), sl);
+        const unsigned x = (__float_as_uint(d2f(qx, qy, qz, pts[lds[k * 256]])) & ~63u) | (unsigned)k;
+#pragma unroll
+        for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
+        key[0] = min(key[0], x);
     }
+#pragma unroll
+    for (int t = 0; t < L; ++t) {
+        T.d[t] = __uint_as_float(key[t] & ~63u);
+        T.s[t] = lds[(key[t] & 63u) * 256];
+    }
+    worst_up = __uint_as_float(key[L - 1] | 63u);
     float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, 1);
     if (bnd == __builtin_huge_valf()) return true;
     bnd -= g.eps_abs;
-    return bnd > 0.f && T.worst() < bnd * bnd * (1.f - 1e-5f);
+    return bnd > 0.f && worst_up < bnd * bnd * (1.f - 1e-5f);
 }
 
 // Tie-break index of a stored point: its id bits, or cid[id] when the store
@@ -347,22 +375,23 @@ __device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float q
 }
 
 // Exact K nearest under (key64, id) from an FP32 top-(K+2) list.  Adjacent
-// entries closer than the FP32 error bound are "ambiguous"; isolated
+// entries closer than the relative error bound rel (kRel for FP32 distances,
+// kRelKey for truncated keys) are "ambiguous"; isolated
 // ambiguous pairs (including the pair at the K-th boundary) are ordered by
 // FP64 keys here.  Returns false (caller defers the query to the FP64 pass)
 // when three or more entries are mutually ambiguous or the (K+2)-th entry is
 // not provably beyond the K-th.
 template <int K, int L>
 __device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
-                                             int& n_pairs, const int* __restrict__ cid) {
+                                             int& n_pairs, const int* __restrict__ cid, float rel = kRel) {
     static_assert(L == K + 2, "list must hold two extra entries");
     n_pairs = 0;
     if (T.s[K - 1] < 0) return true;  // fewer than K points exist: nothing to order beyond them
-    const bool margin = T.s[K + 1] < 0 || T.d[K + 1] > T.d[K - 1] * (1.f + kRel);
+    const bool margin = T.s[K + 1] < 0 || T.d[K + 1] > T.d[K - 1] * (1.f + rel);
     if (!margin) return false;
     bool amb[K];
 #pragma unroll
-    for (int t = 0; t < K; ++t) amb[t] = T.s[t + 1] >= 0 && !(T.d[t + 1] > T.d[t] * (1.f + kRel));
+    for (int t = 0; t < K; ++t) amb[t] = T.s[t + 1] >= 0 && !(T.d[t + 1] > T.d[t] * (1.f + rel));
     bool chain = false;
 #pragma unroll
     for (int t = 0; t + 1 < K; ++t) chain = chain || (amb[t] && amb[t + 1]);
@@ -512,10 +541,12 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
     constexpr int L = K + 2;
     TopL<L> T;
     bool have = false;
+    float worst_up = 0.f;  // collect path: upper bound of the FP32 L-th distance
     if constexpr (CAP > 0) {
         if (lds && collect_tau > 0.f)
-            have = grid_collect_topl<L, CAP>(g, start, pts, qx, qy, qz, self_slot, collect_tau, lds, T);
+            have = grid_collect_topl<L, CAP>(g, start, pts, qx, qy, qz, self_slot, collect_tau, lds, T, worst_up);
     }
+    const bool keyed = have;  // T.d holds truncated keys
     bool limited = false;
     if (!have) {
         // with a point LBVH, the ring search stops after two shells and the
@@ -525,9 +556,9 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
             grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T, max_r, &limited);
         if (limited) bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, T);
     }
-    l_out = T.s[L - 1] >= 0 ? T.d[L - 1] : __builtin_huge_valf();
+    l_out = T.s[L - 1] >= 0 ? (keyed ? worst_up : T.d[L - 1]) : __builtin_huge_valf();
     int pairs;
-    const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs, cid);
+    const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs, cid, keyed ? kRelKey : kRel);
 #pragma unroll
     for (int t = 0; t < K; ++t) slot_out[t] = T.s[t];
     if (d_out) {

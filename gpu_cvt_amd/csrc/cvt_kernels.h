@@ -7,6 +7,26 @@
 namespace cvt {
 
 constexpr int kStatWords = 12;  // mirrors cvt_iter_stats (int fields)
+// Stat word layout (cvt_iter_stats): 0 same, 1 low, 2 both, 3 newly frozen,
+// 4 frozen total, 5 exact rescans, 6 reranks, 7 cell failures, 8 non-finite,
+// 9 deferred to the precise pass (also that pass's work-list length), 10 the
+// largest site displacement (float bits).
+constexpr int kStatDeferred = 9;
+constexpr int kStatMaxDisp = 10;
+// Blocks per job of the precise pass (they stride over the job's deferred list).
+constexpr int kPreciseBlocks = 4;
+
+// Neighbour counts K with compiled Lloyd kernels; each has its own object
+// (cvt_lloyd.hip built with -DCVT_K=k).
+#define CVT_FOR_EACH_K(X) X(4) X(6) X(8) X(12) X(16) X(20) X(24) X(32)
+
+__device__ __forceinline__ int xcd_block(int b, int nb8) {
+    // Blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+    // contiguous range of (spatially sorted) work so its L2 sees neighbours.
+    // nb8 (the launched grid) is a multiple of 8, so this is a bijection.
+    const int per = nb8 >> 3;
+    return (b & 7) * per + (b >> 3);
+}
 
 // Static per-job mesh data on the device.
 struct MeshDev {
@@ -140,6 +160,15 @@ void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsig
 // receive the per-site scores.  The site grid must be current.
 void launch_site_tiers(const JobDesc* desc, int nS, int K, int scheme, int* nn_tmp, unsigned char* tier,
                        double* nv_out, double* L_out, hipStream_t s);
+// Per-K launchers, defined in cvt_lloyd.hip (one explicit instantiation per object).
+template <int K>
+void launch_lloyd_k(const BatchTables& bt, int eval_only, int which, hipStream_t s);
+template <int K>
+void launch_remesh_cells_k(const JobDesc* desc, int nS, int* knn_tmp, unsigned long long* keys, float4* site_n,
+                           hipStream_t s);
+template <int K>
+void launch_site_tiers_k(const JobDesc* desc, int nS, int scheme, const int* nn, unsigned char* tier,
+                         double* nv_out, double* L_out, hipStream_t s);
 void launch_unique_mark(const unsigned long long* keys, int n, int* flag, hipStream_t s);
 void launch_tri_finish(const unsigned long long* keys, const int* flag, const int* pos, int n, const float4* S,
                        const float4* site_n, float eps, int4* tri, int* keep, hipStream_t s);
