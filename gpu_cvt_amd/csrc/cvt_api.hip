@@ -80,6 +80,8 @@ struct cvt_job {
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
     float4 *fcache = nullptr, *vcache = nullptr, *gaps = nullptr;  // per-site vertex-search caches
+    int* fnn = nullptr;       // FP32-pass hand-offs: frame vertex per site,
+    float4* cent = nullptr;   // centroid and cell flags per site
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
@@ -280,43 +282,45 @@ int alloc_grid(cvt_job* j, GridBufs& b, int n, int ncell) {
 JobDesc make_desc(cvt_job* j) {
     JobDesc d{};
     StepArgs& a = d.a;
-    a.m.V = j->V;
-    a.m.N = j->N;
-    a.m.FU = j->FU;
-    a.m.FW = j->FW;
-    a.m.tri = j->tri;
-    a.m.F = j->F;
-    a.m.vf_off = j->vf_off;
-    a.m.vf_faces = j->vf_faces;
+    a.m.V = to_g(j->V);
+    a.m.N = to_g(j->N);
+    a.m.FU = to_g(j->FU);
+    a.m.FW = to_g(j->FW);
+    a.m.tri = to_g(j->tri);
+    a.m.F = to_g(j->F);
+    a.m.vf_off = to_g(j->vf_off);
+    a.m.vf_faces = to_g(j->vf_faces);
     a.m.nV = j->nV;
     a.m.nF = j->nF;
     a.m.vg = j->vg;
     a.m.vb = j->vb;
-    a.m.vg_start = j->vgb.start;
-    a.m.vg_pts = j->vgb.pts;
+    a.m.vg_start = to_g(j->vgb.start);
+    a.m.vg_pts = to_g(j->vgb.pts);
     a.sg.g = j->sgp;
-    a.sg.start = j->sgb.start;
-    a.sg.pts = j->sgb.pts;
-    a.sg.cid = j->cid;
-    a.S01[0] = j->S[0];
-    a.S01[1] = j->S[1];
-    a.parity = j->parity;
-    a.knn = j->knn;
-    a.knnV = j->knnV;
-    a.frozen = j->frozen;
-    a.streak = j->streak;
-    a.tier = j->tier;
-    a.has_prev = j->has_prev;
-    a.stats = j->stats;
-    a.hist = j->hist;
-    a.hist_count = j->hist_count;
-    a.defer_list = j->defer_list;
-    a.rad2 = j->rad2;
-    a.seed_ok = j->seed_ok;
-    a.fcache = j->fcache;
-    a.vcache = j->vcache;
-    a.gaps = j->gaps;
-    a.dmax01 = j->dmax;
+    a.sg.start = to_g(j->sgb.start);
+    a.sg.pts = to_g(j->sgb.pts);
+    a.sg.cid = to_g(j->cid);
+    a.S01[0] = to_g(j->S[0]);
+    a.S01[1] = to_g(j->S[1]);
+    a.parity = to_g(j->parity);
+    a.knn = to_g(j->knn);
+    a.knnV = to_g(j->knnV);
+    a.frozen = to_g(j->frozen);
+    a.streak = to_g(j->streak);
+    a.tier = to_g(j->tier);
+    a.has_prev = to_g(j->has_prev);
+    a.stats = to_g(j->stats);
+    a.hist = to_g(j->hist);
+    a.hist_count = to_g(j->hist_count);
+    a.defer_list = to_g(j->defer_list);
+    a.rad2 = to_g(j->rad2);
+    a.seed_ok = to_g(j->seed_ok);
+    a.fcache = to_g(j->fcache);
+    a.vcache = to_g(j->vcache);
+    a.gaps = to_g(j->gaps);
+    a.fnn = to_g(j->fnn);
+    a.cent = to_g(j->cent);
+    a.dmax01 = to_g(j->dmax);
     a.nS = j->nS;
     a.R = (float)j->R;
     {
@@ -339,17 +343,17 @@ JobDesc make_desc(cvt_job* j) {
     }
     a.nan_policy = j->p.nan_policy;
     if (j->p.debug) {
-        a.dbg_nn = j->dbg_nn;
-        a.dbg_U = j->dbg_U;
-        a.dbg_V = j->dbg_V;
-        a.dbg_cent = j->dbg_cent;
-        a.dbg_am = j->dbg_am;
+        a.dbg_nn = to_g(j->dbg_nn);
+        a.dbg_U = to_g(j->dbg_U);
+        a.dbg_V = to_g(j->dbg_V);
+        a.dbg_cent = to_g(j->dbg_cent);
+        a.dbg_am = to_g(j->dbg_am);
     }
-    d.count = j->sgb.count;
-    d.start = j->sgb.start;
-    d.bsum = j->sgb.bsum;
-    d.bucket = j->sgb.bucket;
-    d.rank = j->sgb.rank;
+    d.count = to_g(j->sgb.count);
+    d.start = to_g(j->sgb.start);
+    d.bsum = to_g(j->sgb.bsum);
+    d.bucket = to_g(j->sgb.bucket);
+    d.rank = to_g(j->sgb.rank);
     d.ncell1 = j->sgp.ncell + 1;
     return d;
 }
@@ -549,6 +553,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
         (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->cent, nS)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {
@@ -608,7 +613,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
     {
         MeshDev m{};
-        m.V = j->V; m.N = j->N; m.F = j->F; m.vf_off = j->vf_off; m.vf_faces = j->vf_faces; m.nV = nV; m.nF = nF;
+        m.V = to_g(j->V); m.N = to_g(j->N); m.F = to_g(j->F); m.vf_off = to_g(j->vf_off);
+        m.vf_faces = to_g(j->vf_faces); m.nV = nV; m.nF = nF;
         launch_frames(m, j->FU, j->FW, s);
     }
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
@@ -629,7 +635,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         float blo[3], bhi[3];
         for (int a = 0; a < 3; ++a) { blo[a] = (float)j->lo[a]; bhi[a] = (float)j->hi[a]; }
         bvh_build(bd, j->vgb.pts, nV, blo, bhi, work, s, 1);
-        j->vb = PointBVH{nV, lo4, hi4, ch, par, leaf};
+        j->vb = PointBVH{nV, to_g(lo4), to_g(hi4), to_g(ch), to_g(par), to_g(leaf)};
     }
     {
         // this job as a batch of one

@@ -10,6 +10,21 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Global-memory pointers.  Device code reaches every buffer through pointers
+// held in device-side job tables; declared in the global address space (in
+// the device compilation), their accesses compile to global memory
+// instructions instead of flat ones, which would also hold the LDS/scalar
+// wait counter and need full 64-bit address arithmetic per access.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define CVT_G __attribute__((address_space(1)))
+#else
+#define CVT_G
+#endif
+
+// Host-side assignment into CVT_G fields (an identity in the host compilation).
+template <class T>
+__host__ __device__ inline CVT_G T* to_g(T* p) { return (CVT_G T*)p; }
+
 namespace cvt {
 
 // ----------------------------------------------------------------- dense grid
@@ -107,7 +122,7 @@ __device__ __forceinline__ float axis_gap(float q, float o, float h, int c, int 
 }
 
 template <int L>
-__device__ __forceinline__ void scan_run(const float4* __restrict__ pts, int s0, int s1, float qx, float qy, float qz,
+__device__ __forceinline__ void scan_run(const CVT_G float4* __restrict__ pts, int s0, int s1, float qx, float qy, float qz,
                                          int self_slot, TopL<L>& T) {
     int s = s0;
     for (; s + 1 < s1; s += 2) {  // two loads in flight
@@ -156,11 +171,11 @@ __device__ __forceinline__ float block_bound(const GridP& g, float qx, float qy,
 // the grid would have to sweep O((distance / cell)^3) cells.
 struct PointBVH {
     int n;
-    const float4* lo;
-    const float4* hi;
-    const int2* child;
-    const int* parent;
-    const int* leaf;
+    const CVT_G float4* lo;
+    const CVT_G float4* hi;
+    const CVT_G int2* child;
+    const CVT_G int* parent;
+    const CVT_G int* leaf;
 };
 
 __device__ __forceinline__ float box_gap2(float qx, float qy, float qz, float4 lo, float4 hi) {
@@ -175,7 +190,7 @@ __device__ __forceinline__ float box_gap2(float qx, float qy, float qz, float4 l
 // may hold a point closer than the current L-th (with a 1e-6 slack for the
 // box distance's different rounding).
 template <int L>
-__device__ __noinline__ void bvh_topl(const PointBVH& b, const float4* __restrict__ pts, float qx, float qy, float qz,
+__device__ __noinline__ void bvh_topl(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy, float qz,
                                       int self_slot, TopL<L>& T) {
     T.init(__builtin_huge_valf());
     const int n = b.n;
@@ -224,7 +239,7 @@ __device__ __noinline__ void bvh_topl(const PointBVH& b, const float4* __restric
 // blocks of radius 2, 3, ...  Returns false when fewer than L points lie
 // within tau (the caller then repeats with tau = inf).
 template <int L>
-__device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+__device__ __forceinline__ bool grid_topl(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
                                           float qx, float qy, float qz, int self_slot, float tau, TopL<L>& T,
                                           int max_r = 0, bool* limited = nullptr) {
     int cx, cy, cz;
@@ -301,8 +316,8 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const int* __restrict_
 // -- when fewer than L or more than CAP points lie within tau, or the block
 // is not provably large enough.
 template <int L, int CAP>
-__device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __restrict__ start,
-                                                  const float4* __restrict__ pts, float qx, float qy, float qz,
+__device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G int* __restrict__ start,
+                                                  const CVT_G float4* __restrict__ pts, float qx, float qy, float qz,
                                                   int self_slot, float tau, int* lds, TopL<L>& T, float& worst_up) {
     static_assert(CAP <= 64, "candidate numbers must fit in the 6 low key bits");
     int cx, cy, cz;
@@ -362,13 +377,13 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const int* __r
 
 // Tie-break index of a stored point: its id bits, or cid[id] when the store
 // numbers points internally (the caller's index decides ties).
-__device__ __forceinline__ int tie_id(const float4& p, const int* __restrict__ cid) {
+__device__ __forceinline__ int tie_id(const float4& p, const CVT_G int* __restrict__ cid) {
     const int id = __float_as_int(p.w);
     return cid ? cid[id] : id;
 }
 
-__device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float qx, float qy, float qz, int sa,
-                                         int sb, const int* __restrict__ cid) {
+__device__ __forceinline__ bool key_less(const CVT_G float4* __restrict__ pts, float qx, float qy, float qz, int sa,
+                                         int sb, const CVT_G int* __restrict__ cid) {
     const float4 a = pts[sa], b = pts[sb];
     const double ka = key64(qx, qy, qz, a.x, a.y, a.z), kb = key64(qx, qy, qz, b.x, b.y, b.z);
     return ka < kb || (ka == kb && tie_id(a, cid) < tie_id(b, cid));
@@ -382,8 +397,8 @@ __device__ __forceinline__ bool key_less(const float4* __restrict__ pts, float q
 // when three or more entries are mutually ambiguous or the (K+2)-th entry is
 // not provably beyond the K-th.
 template <int K, int L>
-__device__ __forceinline__ bool resolve_topk(const float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
-                                             int& n_pairs, const int* __restrict__ cid, float rel = kRel) {
+__device__ __forceinline__ bool resolve_topk(const CVT_G float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
+                                             int& n_pairs, const CVT_G int* __restrict__ cid, float rel = kRel) {
     static_assert(L == K + 2, "list must hold two extra entries");
     n_pairs = 0;
     if (T.s[K - 1] < 0) return true;  // fewer than K points exist: nothing to order beyond them
@@ -430,9 +445,9 @@ __device__ __forceinline__ void exact_insert(double (&kd)[K], int (&kid)[K], int
 // The exact rescan over a point LBVH: every point within tau (boxes within
 // tau, with slack) enters the FP64-keyed top-K.
 template <int K>
-__device__ __noinline__ void bvh_exact64(const PointBVH& b, const float4* __restrict__ pts, float qx, float qy,
+__device__ __noinline__ void bvh_exact64(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy,
                                          float qz, int self_slot, float tau, int* __restrict__ out_slot,
-                                         const int* __restrict__ cid) {
+                                         const CVT_G int* __restrict__ cid) {
     double kd[K];
     int kid[K], ks[K];
 #pragma unroll
@@ -476,9 +491,9 @@ __device__ __noinline__ void bvh_exact64(const PointBVH& b, const float4* __rest
 // Exact K nearest under (key64, id) by FP64 insertion over every point with
 // FP32 distance <= tau.  Only the FP64-resolving pass uses it.
 template <int K>
-__device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+__device__ __noinline__ void grid_exact64(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
                                           float qx, float qy, float qz, int self_slot, float tau,
-                                          int* __restrict__ out_slot, const int* __restrict__ cid) {
+                                          int* __restrict__ out_slot, const CVT_G int* __restrict__ cid) {
     double kd[K];
     int kid[K], ks[K];
 #pragma unroll
@@ -533,10 +548,10 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const int* __restrict_
 // l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
 // lds (optional, with collect_tau finite): try grid_collect_topl first.
 template <int K, bool PRECISE, int CAP = 0>
-__device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__ start, const float4* __restrict__ pts,
+__device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
                                          float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
                                          float& l_out, int* lds = nullptr, float collect_tau = 0.f,
-                                         float* d_out = nullptr, const int* __restrict__ cid = nullptr,
+                                         float* d_out = nullptr, const CVT_G int* __restrict__ cid = nullptr,
                                          const PointBVH* pbvh = nullptr) {
     constexpr int L = K + 2;
     TopL<L> T;
@@ -554,7 +569,13 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
         const int max_r = pbvh ? 2 : 0;
         if (!grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T, max_r, &limited) && !limited)
             grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T, max_r, &limited);
-        if (limited) bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, T);
+        if (limited) {
+            // the out-of-line walk gets a copy: a list whose address reaches a
+            // call would live in scratch memory on every path, not only this one
+            TopL<L> Tb = T;
+            bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, Tb);
+            T = Tb;
+        }
     }
     l_out = T.s[L - 1] >= 0 ? (keyed ? worst_up : T.d[L - 1]) : __builtin_huge_valf();
     int pairs;
@@ -569,8 +590,11 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const int* __restrict__
     if constexpr (!PRECISE) {
         return -1;
     } else {
-        if (limited) bvh_exact64<K>(*pbvh, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
-        else grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), slot_out, cid);
+        int sb[K];  // out-of-line rescans write a copy (see above)
+        if (limited) bvh_exact64<K>(*pbvh, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), sb, cid);
+        else grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), sb, cid);
+#pragma unroll
+        for (int t = 0; t < K; ++t) slot_out[t] = sb[t];
         if (d_out) {  // FP32 distances no longer follow the rescanned list: report no margin
 #pragma unroll
             for (int t = 0; t < L; ++t) d_out[t] = 0.f;

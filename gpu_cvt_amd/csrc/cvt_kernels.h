@@ -30,26 +30,26 @@ __device__ __forceinline__ int xcd_block(int b, int nb8) {
 
 // Static per-job mesh data on the device.
 struct MeshDev {
-    const float4* V;      // nV  (x, y, z, 0)
-    const float4* N;      // nV  unit vertex normals (FP32)
-    const float4* FU;     // nV  tangent frame u of each vertex normal (site_normals.cuh:68-110)
-    const float4* FW;     // nV  tangent frame v
-    const float4* tri;    // 3*nF  (x, y, z, vertex id as float bits) per corner
-    const int* F;         // 3*nF  vertex ids
-    const int* vf_off;    // nV+1
-    const int* vf_faces;  // 3*nF
+    const CVT_G float4* V;      // nV  (x, y, z, 0)
+    const CVT_G float4* N;      // nV  unit vertex normals (FP32)
+    const CVT_G float4* FU;     // nV  tangent frame u of each vertex normal (site_normals.cuh:68-110)
+    const CVT_G float4* FW;     // nV  tangent frame v
+    const CVT_G float4* tri;    // 3*nF  (x, y, z, vertex id as float bits) per corner
+    const CVT_G int* F;         // 3*nF  vertex ids
+    const CVT_G int* vf_off;    // nV+1
+    const CVT_G int* vf_faces;  // 3*nF
     int nV, nF;
     GridP vg;             // static vertex grid (pts: x, y, z, vertex id bits)
     PointBVH vb;          // LBVH over the vertex grid's points (far-query fallback)
-    const int* vg_start;
-    const float4* vg_pts;
+    const CVT_G int* vg_start;
+    const CVT_G float4* vg_pts;
 };
 
 struct SiteGridDev {
     GridP g;
-    const int* start;
-    const float4* pts;    // cell-sorted sites (x, y, z, internal site index bits)
-    const int* cid;       // internal site index -> caller's index (KNN ties break by the caller's index)
+    const CVT_G int* start;
+    const CVT_G float4* pts;    // cell-sorted sites (x, y, z, internal site index bits)
+    const CVT_G int* cid;       // internal site index -> caller's index (KNN ties break by the caller's index)
 };
 
 // Per-job pointers and parameters; lives in device memory (one per job) and is
@@ -57,24 +57,26 @@ struct SiteGridDev {
 struct StepArgs {
     MeshDev m;
     SiteGridDev sg;
-    float4* S01[2];       // site buffers; *parity selects the current one
-    int* parity;          // device flag, flipped at the end of every iteration
-    int* knn;             // nS*K, in place: previous lists in, new lists out
-    int* knnV;            // nS*kproj pinned vertex lists
-    unsigned char* frozen;
-    unsigned char* streak;
-    const unsigned char* tier;
-    int* has_prev;        // device flag
-    int* stats;           // kStatWords counters of this iteration
-    int* hist;            // kHistCap * kStatWords history of stats
-    int* hist_count;
-    int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
-    float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
-    int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
-    float4* fcache;       // nS: (site position at its last vertex search, 3rd-nearest distance there; +inf: none)
-    float4* vcache;       // nS: (centroid at its last kproj search, 7th-nearest distance there; +inf: none)
-    float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, 5th/6th gap at vcache, 0)
-    float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
+    CVT_G float4* S01[2];       // site buffers; *parity selects the current one
+    CVT_G int* parity;          // device flag, flipped at the end of every iteration
+    CVT_G int* knn;             // nS*K, in place: previous lists in, new lists out
+    CVT_G int* knnV;            // nS*kproj pinned vertex lists
+    CVT_G unsigned char* frozen;
+    CVT_G unsigned char* streak;
+    const CVT_G unsigned char* tier;
+    CVT_G int* has_prev;        // device flag
+    CVT_G int* stats;           // kStatWords counters of this iteration
+    CVT_G int* hist;            // kHistCap * kStatWords history of stats
+    CVT_G int* hist_count;
+    CVT_G int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
+    CVT_G float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
+    CVT_G int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
+    CVT_G float4* fcache;       // nS: (site position at its last vertex search, 3rd-nearest distance there; +inf: none)
+    CVT_G float4* vcache;       // nS: (centroid at its last kproj search, 7th-nearest distance there; +inf: none)
+    CVT_G float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, 5th/6th gap at vcache, 0)
+    CVT_G float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
+    CVT_G int* fnn;             // nS: frame vertex of this iteration (hand-off of the FP32 pass; -2: deferred)
+    CVT_G float4* cent;         // nS: centroid and cell-phase flags (hand-off of the FP32 pass)
     int nS;
     float R;
     float tau0;           // first-iteration candidate radius^2 guess (mean-density estimate)
@@ -83,11 +85,11 @@ struct StepArgs {
     int need[6];
     int nan_policy;
     // optional per-site debug outputs (null when off)
-    int* dbg_nn;
-    float4* dbg_U;
-    float4* dbg_V;
-    float4* dbg_cent;
-    float2* dbg_am;
+    CVT_G int* dbg_nn;
+    CVT_G float4* dbg_U;
+    CVT_G float4* dbg_V;
+    CVT_G float4* dbg_cent;
+    CVT_G float2* dbg_am;
 };
 
 constexpr int kHistCap = 4096;
@@ -105,11 +107,11 @@ struct GridBufs {
 // A job inside a batch: its step arguments plus its site-grid work buffers.
 struct JobDesc {
     StepArgs a;
-    int* count;   // ncell+2
-    int* start;   // ncell+2
-    int* bsum;
-    int* bucket;
-    int* rank;
+    CVT_G int* count;   // ncell+2
+    CVT_G int* start;   // ncell+2
+    CVT_G int* bsum;
+    CVT_G int* bucket;
+    CVT_G int* rank;
     int ncell1;   // ncell + 1 (cells + the non-finite bucket)
 };
 

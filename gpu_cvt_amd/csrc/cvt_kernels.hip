@@ -2,7 +2,8 @@
 //
 // One iteration (the body of the reference's main.cu:669-881) is:
 //   site grid rebuild   : kb_begin, kb_fill_count, kb_grid_count, kb_scan_* x3, kb_grid_scatter
-//   fused per-site step : kb_lloyd, kb_lloyd_precise (cvt_lloyd.hip, one object per K)
+//   per-site step       : kb_frame, kb_cell (cvt_lloyd.hip, one object per K),
+//                         kb_project, then kb_lloyd_precise (cvt_lloyd.hip)
 //   iteration epilogue  : kb_end (stats history, parity flip)
 // all captured in one HIP graph.  This file holds everything that does not
 // depend on K, plus the K dispatch of the per-K launchers.
@@ -12,6 +13,7 @@
 
 #include "cvt_device.h"
 #include "cvt_kernels.h"
+#include "cvt_step.h"
 
 namespace cvt {
 
@@ -187,6 +189,51 @@ __global__ void k_vertex_frames(MeshDev m, float4* __restrict__ FU, float4* __re
     FW[v] = make_float4(w.x, w.y, w.z, 0.f);
 }
 
+// ------------------------------------------------------ FP32 Lloyd pass, K-independent phases
+// FP32 pass, phase 1 over all sites of all jobs of a batch (cell-sorted order
+// within a job; the block table is walked XCD-contiguously).
+template <int KP>
+__global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
+                                                int eval_only) {
+    const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int t = br.blk * 256 + (int)threadIdx.x;
+    if (t >= a.nS) return;
+    SiteCtx c = site_ctx(a, eval_only != 0, t, true);
+    const bool ok = phase_frame<KP, false>(a, eval_only != 0, c);
+    a.fnn[c.i] = ok ? c.nn : kFrameDeferred;
+    if (!ok) defer_site(a, t, false, false);
+}
+
+// FP32 pass, phase 3 (not run for an evaluation-only step).
+template <int KP>
+__global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int par = *a.parity;
+    const int t = br.blk * 256 + (int)threadIdx.x;
+    const bool live = t < a.nS;
+    SiteFlags f{};
+    if (live) {
+        const SiteCtx c = site_ctx(a, false, t, true);
+        const float4 ce = a.cent[c.i];
+        const int flags = c.valid ? __float_as_int(ce.w) : kCentDone;
+        if (flags & kCentDone) {
+            CellRes cr;
+            cr.c3 = make_float3(ce.x, ce.y, ce.z);
+            cr.same = (flags & kCentSame) != 0;
+            cr.cell_ok = (flags & kCentFail) == 0;
+            cr.kind = (flags >> kCentKindShift) & 3;
+            cr.area = cr.moment = 0.f;
+            phase_project<KP, false>(a, par, false, c, cr, f);
+            if (!f.done) defer_site(a, t, f.row_written, f.same);
+        }
+    }
+    count_site(a, par, false, f, live);
+}
+
 // ------------------------------------------------------ batched grid rebuild
 __global__ void kb_begin(const JobDesc* __restrict__ descs, int njobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -288,7 +335,7 @@ __global__ void kb_grid_scatter(const JobDesc* __restrict__ descs, const BlockRe
     if (i >= a.nS) return;
     const float4 p = a.S01[*a.parity][i];
     const int pos = d.start[d.bucket[i]] + d.rank[i];
-    const_cast<float4*>(a.sg.pts)[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    const_cast<CVT_G float4*>(a.sg.pts)[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
 }
 
 // End of an iteration: stats history, flags, buffer parity.
@@ -490,8 +537,7 @@ void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
     hipLaunchKernelGGL(kb_grid_scatter, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
 }
 
-void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which) {
-    (void)kproj;
+static void launch_lloyd_part(const BatchTables& bt, int K, int eval_only, int which, hipStream_t s) {
     switch (K) {
 #define CVT_CASE(k) \
     case k: launch_lloyd_k<k>(bt, eval_only, which, s); break;
@@ -499,6 +545,17 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
 #undef CVT_CASE
         default: break;
     }
+}
+
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which) {
+    (void)kproj;
+    if (which & 1) {
+        const dim3 g(bt.n_site_blocks), b(256);
+        hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
+        launch_lloyd_part(bt, K, eval_only, 1, s);
+        if (!eval_only) hipLaunchKernelGGL((kb_project<5>), g, b, 0, s, bt.descs, bt.site_blocks);
+    }
+    if (which & 2) launch_lloyd_part(bt, K, eval_only, 2, s);
 }
 
 void launch_batch_end(const BatchTables& bt, hipStream_t s) {

@@ -1,0 +1,426 @@
+// cvt_step.h -- device code of one Lloyd step per site, in phases shared by
+// the FP32 kernels (kb_frame and kb_project in cvt_kernels.hip, kb_cell per K
+// in cvt_lloyd.hip) and the FP64-resolving pass (cvt_lloyd.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "cvt_device.h"
+#include "cvt_kernels.h"
+
+namespace cvt {
+
+__device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, bool pred) {
+    const unsigned long long b = __ballot(pred);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&stats[idx], (int)__popcll(b));
+}
+
+struct SiteFlags {
+    float disp;     // |S' - S| (0 when not done or not finite)
+    bool done;      // this pass produced the site's result
+    bool row_written;  // deferred after the (proven) KNN row was already written
+    bool same, low, both, newly, frozen_after, cell_fail, nonfinite;
+    int kind;       // KNN path: 0 FP32, 1 rerank, 2 exact rescan
+};
+
+constexpr int kCollectCap = 48;  // LDS candidates per thread in the FP32 pass
+
+// One site of one Lloyd iteration (main.cu:669-881 restated per site), in
+// three phases: the frame (nearest mesh vertex), the cell (site KNN, dual
+// hull, centroid), and the projection (centroid -> mesh kproj-NN, nearest
+// point on the incident faces, freeze state).  The FP32 pass runs each phase
+// as its own kernel, so the light vertex-search phases are not held to the
+// occupancy of the register-heavy cell phase; the hand-offs go through
+// memory (fnn: the frame vertex; cent: the centroid and the cell phase's
+// flags).  The FP64-resolving pass runs all three in one thread.
+// !PRECISE: pure FP32; a phase reports a deferral (and the site is handed to
+// the precise pass) when an FP32 comparison cannot be proven.
+struct SiteCtx {
+    int i;          // internal site index
+    float4 q;       // position
+    bool live, valid, fz, use_cache;
+    int nn;         // frame vertex (-1: none)
+    float3 u, w;    // tangent frame
+};
+
+struct CellRes {
+    float3 c3;      // centroid (NaN when the cell failed)
+    float area, moment;
+    bool same, cell_ok;
+    int kind;
+};
+
+// Hand-off words of the FP32 pass.
+constexpr int kFrameDeferred = -2;   // fnn: the frame phase deferred the site
+constexpr int kCentDone = 1;         // cent.w bits: cell phase done,
+constexpr int kCentSame = 2;         //   KNN list unchanged,
+constexpr int kCentFail = 4;         //   cell failed,
+constexpr int kCentKindShift = 3;    //   KNN path (2 bits)
+
+__device__ __forceinline__ SiteCtx site_ctx(const StepArgs& a, bool eval_only, int t, bool live) {
+    SiteCtx c;
+    c.live = live;
+    c.i = 0;
+    c.q = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+        c.q = a.sg.pts[t];
+        c.i = __float_as_int(c.q.w);
+    }
+    c.valid = live && finite3(c.q.x, c.q.y, c.q.z);
+    c.fz = c.valid && !eval_only && a.frozen[c.i] != 0;
+    c.use_cache = !eval_only && *a.seed_ok;
+    c.nn = -1;
+    c.u = make_float3(1.f, 0.f, 0.f);
+    c.w = make_float3(0.f, 1.f, 0.f);
+    return c;
+}
+
+__device__ __forceinline__ void set_frame(const StepArgs& a, SiteCtx& c) {
+    if (c.nn >= 0 && c.nn < a.m.nV) {
+        const float4 fu = a.m.FU[c.nn], fw = a.m.FW[c.nn];
+        c.u = make_float3(fu.x, fu.y, fu.z);
+        c.w = make_float3(fw.x, fw.y, fw.z);
+    }
+}
+
+// ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110).
+// Search cache (per site): fcache = (position of the last vertex search, its
+// 3rd-nearest distance), gaps.x = gap between the 1st and 2nd nearest
+// distances there, gaps.y = that nearest vertex.  A point that moved by d
+// keeps its nearest vertex while 2d < gap (every other vertex gains at most d
+// on the nearest), so the search is skipped; otherwise the cached radius plus
+// d seeds it (triangle inequality).  A proven search refreshes the cache at
+// once (it is a valid proof for this position whatever the later phases do).
+// Returns false when the FP32 search could not be proven.
+template <int KP, bool PRECISE>
+__device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, SiteCtx& c) {
+    if (!c.valid) return true;
+    const float inf = __builtin_huge_valf();
+    bool ok = true;
+    if (!c.fz) {
+        const float4 q = c.q;
+        float tau1 = inf;
+        bool have = false;
+        float4 fc = make_float4(0.f, 0.f, 0.f, inf), gp = make_float4(-1.f, 0.f, -1.f, 0.f);
+        if (c.use_cache) {
+            fc = a.fcache[c.i];
+            gp = a.gaps[c.i];
+            if (fc.w < inf) {
+                const float dq = sqrtf((q.x - fc.x) * (q.x - fc.x) + (q.y - fc.y) * (q.y - fc.y) +
+                                       (q.z - fc.z) * (q.z - fc.z));
+                if (gp.x - 2.f * dq > 2e-5f * fc.w + 4.f * a.m.vg.eps_abs) {
+                    c.nn = __float_as_int(gp.y);
+                    have = true;
+                } else {
+                    const float rr = (fc.w + dq) * (1.f + 1e-4f);
+                    tau1 = rr * rr;
+                }
+            }
+        }
+        if (!have) {
+            int s1[1];
+            float l1, d1[3];
+            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1,
+                                                 nullptr, 0.f, d1, nullptr, &a.m.vb);
+            ok = k1 >= 0;
+            c.nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
+            if (ok && !eval_only) {
+                if (!c.use_cache) gp = a.gaps[c.i];  // keep the centroid-search fields
+                gp.x = c.nn >= 0 ? sqrtf(d1[1]) - sqrtf(d1[0]) : -1.f;
+                gp.y = __int_as_float(c.nn);
+                a.fcache[c.i] = make_float4(q.x, q.y, q.z, d1[2] < inf ? sqrtf(d1[2]) : inf);
+                a.gaps[c.i] = gp;
+            }
+        }
+    } else {
+        c.nn = a.knnV[(size_t)c.i * KP];  // pinned at the freeze iteration
+    }
+    set_frame(a, c);
+    return ok;
+}
+
+// ---- 2. site KNN (active) or pinned list (frozen), consumed straight into the
+//         dual points p_j = (d.u, d.v) / (|d|^2 / 2) of the cell (2dClip.cuh:112-126);
+// ---- 3. tangent-plane cell (dual hull) and centroid (2dClip.cuh:128-134).
+// carried: for the precise pass, -1 or the `same` flag the fast pass computed
+// before it wrote the site's KNN row (the row then already holds the new list).
+// Returns false (before writing the row) when the FP32 KNN could not be proven.
+template <int K, bool PRECISE>
+__device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval_only, int t, const SiteCtx& c,
+                                           int carried, int* lds, CellRes& r) {
+    const float4 q = c.q;
+    const float inf = __builtin_huge_valf();
+    const float4* __restrict__ S = a.S01[par];
+    r.c3 = make_float3(q.x, q.y, q.z);
+    r.area = 0.f;
+    r.moment = 0.f;
+    r.same = true;
+    r.cell_ok = false;
+    r.kind = 0;
+    if (!c.valid) return true;
+    float px[K + 4], py[K + 4];
+    int slots[K];
+    // the precise pass reuses a row the fast pass already proved and wrote
+    const bool use_row = c.fz || (PRECISE && carried >= 0);
+    if (!use_row) {
+        // provable seed: the previous (K+2)-th distance grown by twice the
+        // largest site displacement of the previous iteration
+        float tau = inf, ctau = a.tau0;
+        if (!eval_only && *a.seed_ok) {
+            const float rp = sqrtf(a.rad2[c.i]);
+            const float r0 = rp + 2.f * a.dmax01[par ^ 1];
+            tau = r0 * r0 * (1.f + 1e-5f);
+            // collection radius: a guess (no proof needed), from this site's own move
+            const float4 sp = a.S01[par ^ 1][c.i];
+            const float di = sqrtf((q.x - sp.x) * (q.x - sp.x) + (q.y - sp.y) * (q.y - sp.y) +
+                                   (q.z - sp.z) * (q.z - sp.z));
+            const float rc = 1.05f * rp + 2.5f * di;
+            ctau = fminf(rc * rc, tau);
+        }
+        float lk;
+        r.kind = knn_exact<K, PRECISE, PRECISE ? 0 : kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t,
+                                                                   tau, slots, lk, lds, ctau, nullptr, a.sg.cid);
+        if (r.kind < 0) return false;
+        if (!eval_only) a.rad2[c.i] = lk;
+    }
+    const float3 u = c.u, w = c.w;
+    int* row = a.knn + (size_t)c.i * K;
+    bool same = true;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        float4 pj;
+        if (!use_row) {
+            const int sl = slots[k] >= 0 ? slots[k] : t;
+            pj = a.sg.pts[sl];
+            const int id = __float_as_int(pj.w);
+            if (!eval_only) {
+                same = same && (carried >= 0 || row[k] == id);
+                row[k] = id;
+            }
+        } else {
+            pj = S[row[k]];
+        }
+        const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
+        const float aa = dx * u.x + dy * u.y + dz * u.z;
+        const float bb = dx * w.x + dy * w.y + dz * w.z;
+        const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
+        const float ic = cc > 0.f ? 1.0f / cc : 0.f;
+        px[k] = aa * ic;
+        py[k] = bb * ic;
+    }
+    r.same = carried >= 0 ? carried != 0 : same;
+    const float iR = 1.0f / a.R;
+    px[K] = iR;   py[K] = 0.f;
+    px[K + 1] = 0.f; py[K + 1] = iR;
+    px[K + 2] = -iR; py[K + 2] = 0.f;
+    px[K + 3] = 0.f; py[K + 3] = -iR;
+    // near-collinear orientation tests are decided in FP64 inline (both passes)
+    const CellOut co = dual_hull_cell<K + 4, true>(px, py);
+    r.cell_ok = co.ok;
+    r.area = co.area;
+    r.moment = co.moment;
+    if (co.ok) {
+        r.c3 = make_float3(q.x + co.cx * u.x + co.cy * w.x, q.y + co.cx * u.y + co.cy * w.y,
+                           q.z + co.cx * u.z + co.cy * w.z);
+    } else {
+        const float nan = __builtin_nanf("");
+        r.c3 = make_float3(nan, nan, nan);
+    }
+    return true;
+}
+
+__device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c, const CellRes& cr) {
+    if (c.live && a.dbg_nn) {
+        a.dbg_nn[c.i] = c.nn;
+        a.dbg_U[c.i] = make_float4(c.u.x, c.u.y, c.u.z, 0.f);
+        a.dbg_V[c.i] = make_float4(c.w.x, c.w.y, c.w.z, 0.f);
+        a.dbg_cent[c.i] = make_float4(cr.c3.x, cr.c3.y, cr.c3.z, 0.f);
+        a.dbg_am[c.i] = make_float2(cr.area, cr.moment);
+    }
+}
+
+// ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection;
+// ---- 6. the new site and the freeze state machine (freezetest.cuh:270-327, :451-466).
+// Returns r.done = false (r.row_written, r.same set for the precise pass) when
+// the FP32 kproj search could not be proven.
+template <int KP, bool PRECISE>
+__device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
+                                              const CellRes& cr, SiteFlags& r) {
+    const float inf = __builtin_huge_valf();
+    const int has_prev = eval_only ? 0 : *a.has_prev;
+    float4* __restrict__ Snew = a.S01[par ^ 1];
+    const float4 q = c.q;
+    const float3 c3 = cr.c3;
+    const int i = c.i;
+    r.kind = cr.kind;
+    float3 sp = c3;
+    if (c.valid && !eval_only) {
+        int vl[KP];
+#pragma unroll
+        for (int k = 0; k < KP; ++k) vl[k] = -1;
+        if (!c.fz) {
+            if (finite3(c3.x, c3.y, c3.z)) {
+                // the 5 nearest vertices keep their set while 2|c3 - c_s| is below
+                // the 5th/6th gap (gaps.z) at the cached centroid c_s (vcache);
+                // only their order is recomputed (exact FP64 keys, index ties)
+                float tauv = inf;
+                bool have = false;
+                if (c.use_cache) {
+                    const float4 vc = a.vcache[i];
+                    if (vc.w < inf) {
+                        const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
+                                               (c3.z - vc.z) * (c3.z - vc.z));
+                        if (a.gaps[i].z - 2.f * dc > 2e-5f * vc.w + 4.f * a.m.vg.eps_abs) {
+                            const int* row = a.knnV + (size_t)i * KP;
+                            double key[KP];
+#pragma unroll
+                            for (int k = 0; k < KP; ++k) {
+                                vl[k] = row[k];
+                                const float4 pv = a.m.V[vl[k]];
+                                key[k] = key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z);
+                            }
+#pragma unroll
+                            for (int p0 = 1; p0 < KP; ++p0) {  // insertion sort of KP entries by (key, id)
+#pragma unroll
+                                for (int k = p0; k >= 1; --k) {
+                                    const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && vl[k] < vl[k - 1]);
+                                    const double tk = key[k];
+                                    const int tv = vl[k];
+                                    key[k] = sw ? key[k - 1] : key[k];
+                                    vl[k] = sw ? vl[k - 1] : vl[k];
+                                    key[k - 1] = sw ? tk : key[k - 1];
+                                    vl[k - 1] = sw ? tv : vl[k - 1];
+                                }
+                            }
+                            have = true;
+                        } else {
+                            const float rr = (vc.w + dc) * (1.f + 1e-4f);
+                            tauv = rr * rr;
+                        }
+                    }
+                }
+                if (!have) {
+                    int sl[KP];
+                    float lv, dv[KP + 2];
+                    const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1,
+                                                          tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
+                    if (!PRECISE && kk < 0) {
+                        r.same = cr.same;
+                        r.row_written = true;
+                        return;
+                    }
+#pragma unroll
+                    for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
+                    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 1] < inf ? sqrtf(dv[KP + 1]) : inf);
+                    float4 gp = a.gaps[i];
+                    gp.z = vl[KP - 1] >= 0 ? (dv[KP] < inf ? sqrtf(dv[KP]) : inf) - sqrtf(dv[KP - 1]) : -1.f;
+                    a.gaps[i] = gp;
+                }
+            }
+            int* row = a.knnV + (size_t)i * KP;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) row[k] = vl[k];
+        } else {
+            const int* row = a.knnV + (size_t)i * KP;
+#pragma unroll
+            for (int k = 0; k < KP; ++k) vl[k] = row[k];
+        }
+        float best = 1e30f;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            const int v = vl[k];
+            if ((unsigned)v >= (unsigned)a.m.nV) continue;
+            const int e = a.m.vf_off[v + 1];
+            for (int ii = a.m.vf_off[v]; ii < e; ++ii) {
+                const int f = a.m.vf_faces[ii];
+                const float4 A = a.m.tri[3 * f], B = a.m.tri[3 * f + 1], C = a.m.tri[3 * f + 2];
+                // a face incident to an earlier list vertex was already visited
+                bool seen = false;
+#pragma unroll
+                for (int s2 = 0; s2 < k; ++s2) {
+                    const float vs = __int_as_float(vl[s2]);
+                    seen = seen || (A.w == vs) || (B.w == vs) || (C.w == vs);
+                }
+                if (seen) continue;
+                const float3 p = closest_point_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                                   make_float3(C.x, C.y, C.z));
+                const float ddx = p.x - c3.x, ddy = p.y - c3.y, ddz = p.z - c3.z;
+                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+                if (d2 < best) { best = d2; sp = p; }
+            }
+        }
+        if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
+    }
+
+    r.done = true;
+    r.frozen_after = c.valid && c.fz;
+    if (c.valid && !eval_only) {
+        Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
+        r.disp = sqrtf((sp.x - q.x) * (sp.x - q.x) + (sp.y - q.y) * (sp.y - q.y) + (sp.z - q.z) * (sp.z - q.z));
+        if (has_prev) {
+            const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
+            const float ddx = sp.x - q.x, ddy = sp.y - q.y, ddz = sp.z - q.z;
+            r.same = cr.same;
+            r.low = (ddx * ddx + ddy * ddy + ddz * ddz) <= a.thr2[tr];
+            r.both = r.same && r.low;
+            if (a.freeze_mode != 0) {
+                int st = a.streak[i];
+                st = r.both ? min(st + 1, 255) : 0;
+                a.streak[i] = (unsigned char)st;
+                const int need = max(a.need[tr] - 1, 1);
+                if (st >= need && !c.fz) {
+                    a.frozen[i] = 1;
+                    r.newly = true;
+                    r.frozen_after = true;
+                }
+            }
+        } else if (a.freeze_mode != 0) {
+            a.streak[i] = 0;
+        }
+    } else if (c.live && !c.valid && !eval_only) {
+        // non-finite site: keep it (it is excluded from every neighbour list)
+        Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
+    }
+    r.cell_fail = c.valid && !cr.cell_ok;
+    r.nonfinite = c.live && !finite3(sp.x, sp.y, sp.z);
+}
+
+// The whole step of one site in one thread (the precise pass).
+template <int K, int KP>
+__device__ __forceinline__ SiteFlags lloyd_site_precise(const StepArgs& a, int par, bool eval_only, int t, bool live,
+                                                        int carried) {
+    SiteFlags r{};
+    SiteCtx c = site_ctx(a, eval_only, t, live);
+    phase_frame<KP, true>(a, eval_only, c);
+    CellRes cr;
+    phase_cell<K, true>(a, par, eval_only, t, c, carried, nullptr, cr);
+    phase_project<KP, true>(a, par, eval_only, c, cr, r);
+    write_debug(a, c, cr);
+    return r;
+}
+
+__device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval_only, const SiteFlags& f, bool live) {
+    if (eval_only) return;
+    const bool hp = *a.has_prev != 0;
+    const bool d = live && f.done;
+    // largest displacement of this iteration (seeds the next iteration's searches)
+    float m = (d && isfinite(f.disp)) ? f.disp : 0.f;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax((unsigned*)(a.dmax01 + par), __float_as_uint(m));
+    wave_count(a.stats, 0, d && hp && f.same);
+    wave_count(a.stats, 1, d && hp && f.low);
+    wave_count(a.stats, 2, d && hp && f.both);
+    wave_count(a.stats, 3, d && f.newly);
+    wave_count(a.stats, 4, d && f.frozen_after);
+    wave_count(a.stats, 5, d && f.kind == 2);
+    wave_count(a.stats, 6, d && f.kind == 1);
+    wave_count(a.stats, 7, d && f.cell_fail);
+    wave_count(a.stats, 8, d && f.nonfinite);
+}
+
+__device__ __forceinline__ void defer_site(const StepArgs& a, int t, bool row_written, bool same) {
+    const int pos = atomicAdd(&a.stats[kStatDeferred], 1);
+    a.defer_list[pos] = t | (row_written ? (1 << 30) : 0) | (same ? (1 << 29) : 0);
+}
+
+}  // namespace cvt
