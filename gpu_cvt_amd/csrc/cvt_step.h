@@ -324,28 +324,49 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
 #pragma unroll
             for (int k = 0; k < KP; ++k) vl[k] = row[k];
         }
+        // faces incident to the list vertices, in list order; a face incident
+        // to an earlier list vertex was already visited.  The fan bounds are
+        // loaded together and faces go kFaceBatch at a time, so that more
+        // loads are in flight per round trip.
         float best = 1e30f;
+        int o0[KP], o1[KP];
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
-            const int v = vl[k];
-            if ((unsigned)v >= (unsigned)a.m.nV) continue;
-            const int e = a.m.vf_off[v + 1];
-            for (int ii = a.m.vf_off[v]; ii < e; ++ii) {
-                const int f = a.m.vf_faces[ii];
-                const float4 A = a.m.tri[3 * f], B = a.m.tri[3 * f + 1], C = a.m.tri[3 * f + 2];
-                // a face incident to an earlier list vertex was already visited
-                bool seen = false;
+            const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
+            o0[k] = ok ? a.m.vf_off[vl[k]] : 0;
+            o1[k] = ok ? a.m.vf_off[vl[k] + 1] : 0;
+        }
+        auto visit = [&](const float4& A, const float4& B, const float4& C, int k) {
+            bool seen = false;
 #pragma unroll
-                for (int s2 = 0; s2 < k; ++s2) {
-                    const float vs = __int_as_float(vl[s2]);
-                    seen = seen || (A.w == vs) || (B.w == vs) || (C.w == vs);
+            for (int s2 = 0; s2 < KP - 1; ++s2) {
+                const float vs = __int_as_float(vl[s2]);
+                seen = seen || (s2 < k && ((A.w == vs) || (B.w == vs) || (C.w == vs)));
+            }
+            if (seen) return;
+            const float3 p = closest_point_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                               make_float3(C.x, C.y, C.z));
+            const float ddx = p.x - c3.x, ddy = p.y - c3.y, ddz = p.z - c3.z;
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
+            if (d2 < best) { best = d2; sp = p; }
+        };
+        constexpr int kFaceBatch = 4;
+#pragma unroll
+        for (int k = 0; k < KP; ++k) {
+            for (int ii = o0[k]; ii < o1[k]; ii += kFaceBatch) {
+                int f[kFaceBatch];
+                float4 A[kFaceBatch], B[kFaceBatch], C[kFaceBatch];
+#pragma unroll
+                for (int b = 0; b < kFaceBatch; ++b) f[b] = a.m.vf_faces[min(ii + b, o1[k] - 1)];
+#pragma unroll
+                for (int b = 0; b < kFaceBatch; ++b) {
+                    A[b] = a.m.tri[3 * f[b]];
+                    B[b] = a.m.tri[3 * f[b] + 1];
+                    C[b] = a.m.tri[3 * f[b] + 2];
                 }
-                if (seen) continue;
-                const float3 p = closest_point_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
-                                                   make_float3(C.x, C.y, C.z));
-                const float ddx = p.x - c3.x, ddy = p.y - c3.y, ddz = p.z - c3.z;
-                const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-                if (d2 < best) { best = d2; sp = p; }
+#pragma unroll
+                for (int b = 0; b < kFaceBatch; ++b)
+                    if (ii + b < o1[k]) visit(A[b], B[b], C[b], k);
             }
         }
         if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
