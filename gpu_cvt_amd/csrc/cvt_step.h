@@ -350,7 +350,7 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
             const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
             if (d2 < best) { best = d2; sp = p; }
         };
-        constexpr int kFaceBatch = 4;
+        constexpr int kFaceBatch = 6;
 #pragma unroll
         for (int k = 0; k < KP; ++k) {
             for (int ii = o0[k]; ii < o1[k]; ii += kFaceBatch) {
