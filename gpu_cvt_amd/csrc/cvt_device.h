@@ -662,23 +662,30 @@ struct CellOut {
     bool defer;  // an orientation test FP32 could not decide (!PRECISE only)
 };
 
-// Gift-wrapping walk of the dual hull over the points j < NB and the last 4
-// (the clip square), accumulating the cell (polar dual) area, centroid and
-// second moment with compensated sums.  rin2 receives the squared distance
-// from the origin to the nearest hull edge line (the hull's inradius about
-// the origin, which the clip square keeps inside).
-template <int M, int NB, bool PRECISE>
-__device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (&py)[M], float& rin2) {
+// Gift-wrapping walk of the dual hull over the points j < NB and, with SQ,
+// the last 4 (the clip square), accumulating the cell (polar dual) area,
+// centroid and second moment with compensated sums.  rin2 receives the
+// squared distance from the origin to the nearest hull edge line, inside
+// whether the origin is strictly inside the hull (every step turns
+// counter-clockwise about it; the clip square guarantees it).
+// Each step scans the points with branch-free FP32 orientation tests; a step
+// in which a test is within FP32 error or exactly collinear is redone with
+// exact decisions (FP64 orientation on the same FP32 inputs, the farther
+// point on a tie), which is what every step would compute.
+template <int M, int NB, bool SQ>
+__device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (&py)[M], float& rin2, bool& inside) {
+    constexpr int NP = NB + (SQ ? 4 : 0);
     CellOut o;
     o.ok = false;
     o.defer = false;
     o.nverts = 0;
     rin2 = __builtin_huge_valf();
+    inside = true;
     // start: max x, then max y -- always a hull vertex
-    float sx = px[M - 4], sy = py[M - 4];
+    float sx = px[0], sy = py[0];
 #pragma unroll
-    for (int jj = 0; jj < NB + 3; ++jj) {
-        const int j = jj < NB ? jj : jj - NB + M - 3;
+    for (int jj = 1; jj < NP; ++jj) {
+        const int j = jj < NB ? jj : jj - NB + M - 4;
         const bool b = px[j] > sx || (px[j] == sx && py[j] > sy);
         sx = b ? px[j] : sx;
         sy = b ? py[j] : sy;
@@ -687,35 +694,51 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
     Kahan A2, Cx, Cy, I12;
     A2.init(); Cx.init(); Cy.init(); I12.init();
     float fvx = 0.f, fvy = 0.f, pvx = 0.f, pvy = 0.f;
-    for (int step = 0; step <= NB + 4; ++step) {
+    for (int step = 0; step <= NP; ++step) {
         float bx = curx, by = cury;
+        bool amb = false;
 #pragma unroll
-        for (int jj = 0; jj < NB + 4; ++jj) {
+        for (int jj = 0; jj < NP; ++jj) {
             const int j = jj < NB ? jj : jj - NB + M - 4;
             const float wx = px[j] - curx, wy = py[j] - cury;
             const float ex = bx - curx, ey = by - cury;
-            const bool isnew = (wx != 0.f) || (wy != 0.f);
-            const bool none = (ex == 0.f) && (ey == 0.f);
             const float t1 = ex * wy, t2 = ey * wx;
-            float cr = t1 - t2;
-            if (fabsf(cr) <= 1e-6f * (fabsf(t1) + fabsf(t2)) && !none && isnew) {
-                if constexpr (PRECISE) {
+            const float cr = t1 - t2;
+            // bitwise, not short-circuit, logic: no branches in the scan
+            const bool none = (ex == 0.f) & (ey == 0.f);
+            const bool isnew = (wx != 0.f) | (wy != 0.f);
+            amb = amb | (!none & isnew & ((fabsf(cr) < 1e-6f * (fabsf(t1) + fabsf(t2))) | (cr == 0.f)));
+            const bool take = none | (cr < 0.f);
+            bx = take ? px[j] : bx;
+            by = take ? py[j] : by;
+        }
+        if (amb) {
+            bx = curx;
+            by = cury;
+#pragma unroll
+            for (int jj = 0; jj < NP; ++jj) {
+                const int j = jj < NB ? jj : jj - NB + M - 4;
+                const float wx = px[j] - curx, wy = py[j] - cury;
+                const float ex = bx - curx, ey = by - cury;
+                const bool isnew = (wx != 0.f) || (wy != 0.f);
+                const bool none = (ex == 0.f) && (ey == 0.f);
+                const float t1 = ex * wy, t2 = ey * wx;
+                float cr = t1 - t2;
+                if (fabsf(cr) <= 1e-6f * (fabsf(t1) + fabsf(t2)) && !none && isnew) {
                     const double dex = (double)bx - (double)curx, dey = (double)by - (double)cury;
                     const double dwx = (double)px[j] - (double)curx, dwy = (double)py[j] - (double)cury;
                     const double c64 = dex * dwy - dey * dwx;
                     cr = c64 < 0.0 ? -1.f : (c64 > 0.0 ? 1.f : 0.f);
-                } else {
-                    o.defer = true;
                 }
+                const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
+                const bool take = isnew && (none || cr < 0.f || (cr == 0.f && farther));
+                bx = take ? px[j] : bx;
+                by = take ? py[j] : by;
             }
-            const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
-            const bool take = isnew && (none || cr < 0.f || (cr == 0.f && farther));
-            bx = take ? px[j] : bx;
-            by = take ? py[j] : by;
         }
-        if (!PRECISE && o.defer) return o;
         if (bx == curx && by == cury) return o;  // degenerate: all dual points coincide
         const float det = curx * by - cury * bx;
+        inside = inside && det > 0.f;
         const float ex = bx - curx, ey = by - cury;
         rin2 = fminf(rin2, det * det / (ex * ex + ey * ey));
         const float inv = 1.0f / det;
@@ -758,25 +781,27 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
 // dual points p_j = d_j / (|d_j|^2 / 2) (the half-plane x.d_j <= |d_j|^2/2 is
 // x.p_j <= 1) and of the clip square's (+-1/R, 0), (0, +-1/R).  Cell vertices
 // are the duals of consecutive hull points.  The walk first runs over the 12
-// nearest neighbours and the square; when every farther neighbour's dual
-// point lies strictly inside that hull's inscribed circle about the origin,
-// the full hull is the same polygon and the walk is done, otherwise it is
-// repeated over all points.
-template <int M, bool PRECISE>
+// nearest neighbours alone; when that hull strictly contains the origin and
+// every other dual point -- the farther neighbours' and the square's, all at
+// distance 1/R -- lies strictly inside its inscribed circle about the origin,
+// the full hull is the same polygon, started at the same point, and the walk
+// is done; otherwise it is repeated over all points.
+template <int M>
 __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
     constexpr int kNear = 12;
     float rin2;
+    bool inside;
     if constexpr (M - 4 > kNear + 4) {
-        const CellOut o = hull_walk<M, kNear, PRECISE>(px, py, rin2);
-        if (o.ok) {
+        const CellOut o = hull_walk<M, kNear, false>(px, py, rin2, inside);
+        if (o.ok && inside) {
             const float lim = rin2 * (1.f - 1e-4f);
-            bool inside = true;
+            bool in = px[M - 4] * px[M - 4] < lim;
 #pragma unroll
-            for (int j = kNear; j < M - 4; ++j) inside = inside && (px[j] * px[j] + py[j] * py[j] < lim);
-            if (inside) return o;
+            for (int j = kNear; j < M - 4; ++j) in = in && (px[j] * px[j] + py[j] * py[j] < lim);
+            if (in) return o;
         }
     }
-    return hull_walk<M, M - 4, PRECISE>(px, py, rin2);
+    return hull_walk<M, M - 4, true>(px, py, rin2, inside);
 }
 
 // Labelled walk of the same dual hull for the remesh stage: calls

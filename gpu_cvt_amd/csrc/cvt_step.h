@@ -214,7 +214,7 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
     px[K + 2] = -iR; py[K + 2] = 0.f;
     px[K + 3] = 0.f; py[K + 3] = -iR;
     // near-collinear orientation tests are decided in FP64 inline (both passes)
-    const CellOut co = dual_hull_cell<K + 4, true>(px, py);
+    const CellOut co = dual_hull_cell<K + 4>(px, py);
     r.cell_ok = co.ok;
     r.area = co.area;
     r.moment = co.moment;
