@@ -357,11 +357,24 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
     unsigned key[L];
 #pragma unroll
     for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
-    for (int k = 0; k < cnt; ++k) {
-        const unsigned x = (__float_as_uint(d2f(qx, qy, qz, pts[lds[k * 256]])) & ~63u) | (unsigned)k;
+    // four candidates per round trip; past cnt the key is ~0u, whose
+    // insertion leaves the list unchanged
+    constexpr int kB = 4;
+    for (int k = 0; k < cnt; k += kB) {
+        int sl[kB];
+        float4 p[kB];
 #pragma unroll
-        for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
-        key[0] = min(key[0], x);
+        for (int b = 0; b < kB; ++b) sl[b] = lds[min(k + b, cnt - 1) * 256];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) p[b] = pts[sl[b]];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) {
+            const unsigned x = k + b < cnt ? ((__float_as_uint(d2f(qx, qy, qz, p[b])) & ~63u) | (unsigned)(k + b))
+                                           : 0xffffffffu;
+#pragma unroll
+            for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
+            key[0] = min(key[0], x);
+        }
     }
 #pragma unroll
     for (int t = 0; t < L; ++t) {
