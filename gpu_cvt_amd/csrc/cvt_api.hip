@@ -365,7 +365,7 @@ struct HostTables {
 
 void add_job_blocks(HostTables& h, int k, int nS, int ncell1) {
     for (int b = 0; b < (nS + 255) / 256; ++b) h.site.push_back({k, b});
-    for (int b = 0; b < (ncell1 + 255) / 256; ++b) h.cell.push_back({k, b});
+    for (int b = 0; b < (ncell1 + 1023) / 1024; ++b) h.cell.push_back({k, b});  // 1024 counters per block
     for (int b = 0; b < (ncell1 + scan_tile() - 1) / scan_tile(); ++b) h.tile.push_back({k, b});
 }
 

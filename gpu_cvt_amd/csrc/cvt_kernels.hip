@@ -247,8 +247,13 @@ __global__ void kb_fill_count(const JobDesc* __restrict__ descs, const BlockRef*
     const BlockRef br = bt[blockIdx.x];
     if (br.job < 0) return;
     const JobDesc& d = descs[br.job];
-    const int c = br.blk * 256 + (int)threadIdx.x;
-    if (c < d.ncell1) d.count[c] = 0;
+    // four counters per thread (one int4 where the run is whole)
+    const int c = 4 * (br.blk * 256 + (int)threadIdx.x);
+    if (c + 4 <= d.ncell1) {
+        *(CVT_G int4*)(d.count + c) = make_int4(0, 0, 0, 0);
+    } else {
+        for (int k = c; k < d.ncell1; ++k) d.count[k] = 0;
+    }
 }
 
 __global__ void kb_grid_count(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
@@ -257,17 +262,39 @@ __global__ void kb_grid_count(const JobDesc* __restrict__ descs, const BlockRef*
     const JobDesc& d = descs[br.job];
     const StepArgs& a = d.a;
     const int i = br.blk * 256 + (int)threadIdx.x;
-    if (i >= a.nS) return;
-    const float4 p = a.S01[*a.parity][i];
-    const GridP& g = a.sg.g;
-    int b = g.ncell;
-    if (finite3(p.x, p.y, p.z)) {
-        int cx, cy, cz;
-        cell_of(g, p.x, p.y, p.z, cx, cy, cz);
-        b = cell_index(g, cx, cy, cz);
+    const bool live = i < a.nS;
+    int b = -1;
+    if (live) {
+        const float4 p = a.S01[*a.parity][i];
+        const GridP& g = a.sg.g;
+        b = g.ncell;
+        if (finite3(p.x, p.y, p.z)) {
+            int cx, cy, cz;
+            cell_of(g, p.x, p.y, p.z, cx, cy, cz);
+            b = cell_index(g, cx, cy, cz);
+        }
+        d.bucket[i] = b;
     }
-    d.bucket[i] = b;
-    d.rank[i] = atomicAdd(&d.count[b], 1);
+    // Sites arrive nearly in cell order, so the lanes of a wave share few
+    // cells: the lanes are grouped by cell first, then every group's lowest
+    // lane adds the group size with one atomic (all groups at once), and each
+    // lane's rank is the group base plus its position in the group.
+    const int lane = (int)threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long todo = __ballot(live);
+    int leader = lane;
+    unsigned long long grp = 0;
+    while (todo) {
+        const int l = __ffsll((long long)todo) - 1;
+        const int bl = __shfl(b, l, 64);
+        const unsigned long long g = __ballot(b == bl) & todo;
+        if (live && b == bl) { leader = l; grp = g; }
+        todo &= ~g;
+    }
+    int base = 0;
+    if (live && lane == leader) base = atomicAdd(&d.count[b], (int)__popcll(grp));
+    base = __shfl(base, leader, 64);
+    if (live) d.rank[i] = base + (int)__popcll(grp & below);
 }
 
 __global__ __launch_bounds__(256) void kb_scan_reduce(const JobDesc* __restrict__ descs,
@@ -310,19 +337,42 @@ __global__ __launch_bounds__(256) void kb_scan_down(const JobDesc* __restrict__ 
     const int base = br.blk * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
+    // whole 16-int runs move as four int4 (the arrays are 256-byte aligned)
+    const bool full = base + kScanItems <= d.ncell1;
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int j = base + k;
-        v[k] = (j < d.ncell1) ? d.count[j] : 0;
-        s += v[k];
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            const int4 q = *(const CVT_G int4*)(d.count + base + 4 * k);
+            v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int j = base + k;
+            v[k] = (j < d.ncell1) ? d.count[j] : 0;
+        }
     }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) s += v[k];
     int total;
     int run = block_excl_scan(s, total) + d.bsum[br.blk];
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        const int j = base + k;
-        if (j < d.ncell1) d.start[j] = run;
-        run += v[k];
+        for (int k = 0; k < kScanItems / 4; ++k) {
+            int4 q;
+            q.x = run; run += v[4 * k];
+            q.y = run; run += v[4 * k + 1];
+            q.z = run; run += v[4 * k + 2];
+            q.w = run; run += v[4 * k + 3];
+            *(CVT_G int4*)(d.start + base + 4 * k) = q;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int j = base + k;
+            if (j < d.ncell1) d.start[j] = run;
+            run += v[k];
+        }
     }
 }
 
