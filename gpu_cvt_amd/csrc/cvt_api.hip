@@ -50,10 +50,26 @@ namespace {
 
 }  // namespace
 
+// Side stream (default priority) and fork/join events of a graph-captured iteration.
+static int side_create(SideStream& sd) {
+    if (hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sd.join, hipEventDisableTiming) != hipSuccess)
+        return -1;
+    return 0;
+}
+static void side_destroy(SideStream& sd) {
+    if (sd.fork) (void)hipEventDestroy(sd.fork);
+    if (sd.join) (void)hipEventDestroy(sd.join);
+    if (sd.s) (void)hipStreamDestroy(sd.s);
+    sd = SideStream{};
+}
+
 struct cvt_job {
     cvt_params p{};
     int nV = 0, nF = 0, nS = 0;
     hipStream_t stream = nullptr;
+    SideStream side;  // the FP64 pass overlaps the projection kernel on it
     std::vector<void*> allocs;
     unsigned long long bytes = 0;
 
@@ -114,6 +130,7 @@ struct cvt_job {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void* d : allocs) (void)hipFree(d);
         if (stream) (void)hipStreamDestroy(stream);
+        side_destroy(side);
     }
 };
 
@@ -374,7 +391,7 @@ void pad_site_blocks(HostTables& h) {
 }
 
 // One iteration of the job's 1-job batch (graph-capturable: no host sync, no memset).
-void enqueue_iteration(cvt_job* j) { launch_batch_iteration(j->tables, j->p.K, j->p.kproj, j->stream); }
+void enqueue_iteration(cvt_job* j) { launch_batch_iteration(j->tables, j->p.K, j->p.kproj, j->stream, &j->side); }
 
 int ensure_graph(cvt_job* j) {
     if (j->graph) return 0;
@@ -539,7 +556,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->sgp = make_grid(glo, ghi, ss * rK, 32L * nS + 4096);
     j->vg = make_grid(glo, ghi, vs * rV, 32L * nV + 4096);
 
-    if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess || side_create(j->side))
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
     if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->FU, nV)) ||
         (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
@@ -1164,6 +1181,7 @@ struct cvt_batch {
     std::vector<cvt_job*> jobs;
     int K = 0, kproj = 0;
     hipStream_t stream = nullptr;
+    SideStream side;
     std::vector<void*> allocs;
     BatchTables tables{};
     hipGraphExec_t graph = nullptr;
@@ -1171,6 +1189,7 @@ struct cvt_batch {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void* p : allocs) (void)hipFree(p);
         if (stream) (void)hipStreamDestroy(stream);
+        side_destroy(side);
     }
 };
 
@@ -1189,7 +1208,7 @@ int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
     b->jobs.assign(jobs, jobs + njobs);
     b->K = K;
     b->kproj = kp;
-    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || side_create(b->side))
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
     std::vector<JobDesc> descs(njobs);
     HostTables ht;
@@ -1218,7 +1237,7 @@ int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bail(fail(CVT_EHIP, "capture begin failed"));
-    launch_batch_iteration(b->tables, K, kp, b->stream);
+    launch_batch_iteration(b->tables, K, kp, b->stream, &b->side);
     if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return bail(fail(CVT_EHIP, "capture end failed"));
     hipError_t e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);

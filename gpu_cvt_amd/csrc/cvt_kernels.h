@@ -13,6 +13,10 @@ constexpr int kStatWords = 12;  // mirrors cvt_iter_stats (int fields)
 // largest site displacement (float bits).
 constexpr int kStatDeferred = 9;
 constexpr int kStatMaxDisp = 10;
+// Word 11: sites the projection phase deferred (a second list, filled from the
+// back of defer_list, so that the FP64 pass over the first list can run while
+// the projection kernel still appends); folded into word 9 at the iteration end.
+constexpr int kStatDeferredB = 11;
 // Blocks per job of the precise pass (they stride over the job's deferred list).
 constexpr int kPreciseBlocks = 4;
 
@@ -147,13 +151,23 @@ void launch_normals(const float4* V, const int* F, const int* vf_off, const int*
 bool lloyd_supported(int K, int kproj);
 int scan_tile();
 // One Lloyd iteration of every job of a batch (all jobs share K and kproj).
-void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s);
+// A second stream with fork/join events: the FP64 pass over the sites that the
+// frame and cell phases deferred runs on it concurrently with the projection
+// kernel (it touches only those sites, which the projection kernel skips).
+struct SideStream {
+    hipStream_t s = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s,
+                            const SideStream* side = nullptr);
 // The fused step alone over a batch, without state writes (energy / debug evaluation).
 void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s);
 // Pieces of an iteration, for per-stage timing.
 void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s);
 // which: 1 = the FP32 pass, 2 = the FP64-resolving pass over its deferred sites, 3 = both
-void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3);
+// (with a side stream, the first deferred list overlaps the projection kernel)
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3,
+                        const SideStream* side = nullptr);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
                          float4* site_n, hipStream_t s);
@@ -163,8 +177,10 @@ void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsig
 void launch_site_tiers(const JobDesc* desc, int nS, int K, int scheme, int* nn_tmp, unsigned char* tier,
                        double* nv_out, double* L_out, hipStream_t s);
 // Per-K launchers, defined in cvt_lloyd.hip (one explicit instantiation per object).
+// part 0: the cell kernel; 1: the FP64 pass over the frame/cell deferrals;
+// 2: the FP64 pass over the projection deferrals
 template <int K>
-void launch_lloyd_k(const BatchTables& bt, int eval_only, int which, hipStream_t s);
+void launch_lloyd_k(const BatchTables& bt, int eval_only, int part, hipStream_t s);
 template <int K>
 void launch_remesh_cells_k(const JobDesc* desc, int nS, int* knn_tmp, unsigned long long* keys, float4* site_n,
                            hipStream_t s);

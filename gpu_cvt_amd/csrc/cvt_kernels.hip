@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ desc
     SiteCtx c = site_ctx(a, eval_only != 0, t, true);
     const bool ok = phase_frame<KP, false>(a, eval_only != 0, c);
     a.fnn[c.i] = ok ? c.nn : kFrameDeferred;
-    if (!ok) defer_site(a, t, false, false);
+    if (!ok) defer_site(a, t, false, false, false);
 }
 
 // FP32 pass, phase 3 (not run for an evaluation-only step).
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
             cr.kind = (flags >> kCentKindShift) & 3;
             cr.area = cr.moment = 0.f;
             phase_project<KP, false>(a, par, false, c, cr, f);
-            if (!f.done) defer_site(a, t, f.row_written, f.same);
+            if (!f.done) defer_site(a, t, f.row_written, f.same, true);
         }
     }
     count_site(a, par, false, f, live);
@@ -396,6 +396,8 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     const int c = *a.hist_count;
     const int slot = c < kHistCap ? c : kHistCap - 1;
     a.stats[kStatMaxDisp] = __float_as_int(a.dmax01[*a.parity]);
+    a.stats[kStatDeferred] += a.stats[kStatDeferredB];
+    a.stats[kStatDeferredB] = 0;
     for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
     *a.hist_count = c + 1;
     *a.has_prev = 1;
@@ -597,24 +599,40 @@ static void launch_lloyd_part(const BatchTables& bt, int K, int eval_only, int w
     }
 }
 
-void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which) {
+void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which,
+                        const SideStream* side) {
     (void)kproj;
     if (which & 1) {
         const dim3 g(bt.n_site_blocks), b(256);
         hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
-        launch_lloyd_part(bt, K, eval_only, 1, s);
+        launch_lloyd_part(bt, K, eval_only, 0, s);
+        const bool fork = side && side->s && (which & 2);
+        if (fork) {
+            (void)hipEventRecord(side->fork, s);
+            (void)hipStreamWaitEvent(side->s, side->fork, 0);
+            launch_lloyd_part(bt, K, eval_only, 1, side->s);
+            (void)hipEventRecord(side->join, side->s);
+        }
         if (!eval_only) hipLaunchKernelGGL((kb_project<5>), g, b, 0, s, bt.descs, bt.site_blocks);
+        if (fork) {
+            (void)hipStreamWaitEvent(s, side->join, 0);
+            if (!eval_only) launch_lloyd_part(bt, K, eval_only, 2, s);
+            return;
+        }
     }
-    if (which & 2) launch_lloyd_part(bt, K, eval_only, 2, s);
+    if (which & 2) {
+        launch_lloyd_part(bt, K, eval_only, 1, s);
+        if (!eval_only) launch_lloyd_part(bt, K, eval_only, 2, s);
+    }
 }
 
 void launch_batch_end(const BatchTables& bt, hipStream_t s) {
     hipLaunchKernelGGL(kb_end, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 }
 
-void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s) {
+void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s, const SideStream* side) {
     launch_batch_grid(bt, 0, s);
-    launch_batch_lloyd(bt, K, kproj, 0, s);
+    launch_batch_lloyd(bt, K, kproj, 0, s, 3, side);
     launch_batch_end(bt, s);
 }
 

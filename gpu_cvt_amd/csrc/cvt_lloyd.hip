@@ -38,7 +38,7 @@ __global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs
     CellRes cr;
     if (!phase_cell<K, false>(a, par, eval_only != 0, t, c, -1, lds + threadIdx.x, cr)) {
         a.cent[c.i] = make_float4(0.f, 0.f, 0.f, 0.f);
-        defer_site(a, t, false, false);
+        defer_site(a, t, false, false, false);
         return;
     }
     const int flags = kCentDone | (cr.same ? kCentSame : 0) | (cr.cell_ok ? 0 : kCentFail) |
@@ -49,15 +49,16 @@ __global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs
 
 // Precise pass over each job's deferred sites (FP64 where FP32 was
 // inconclusive); kPreciseBlocks blocks per job stride over its list.
+// back: the list the projection phase filled (from the back of defer_list).
 template <int K, int KP>
-__global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restrict__ descs, int eval_only) {
+__global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restrict__ descs, int eval_only, int back) {
     const StepArgs& a = descs[blockIdx.x / kPreciseBlocks].a;
     const int par = *a.parity;
-    const int n = a.stats[kStatDeferred];
+    const int n = a.stats[back ? kStatDeferredB : kStatDeferred];
     for (int base = (blockIdx.x % kPreciseBlocks) * 256; base < n; base += kPreciseBlocks * 256) {
         const int idx = base + (int)threadIdx.x;
         const bool live = idx < n;
-        const int e = live ? a.defer_list[idx] : 0;
+        const int e = live ? a.defer_list[back ? a.nS - 1 - idx : idx] : 0;
         const int t = e & ((1 << 29) - 1);
         const int carried = (e & (1 << 30)) ? ((e >> 29) & 1) : -1;
         const SiteFlags f = lloyd_site_precise<K, KP>(a, par, eval_only != 0, t, live, carried);
@@ -247,13 +248,13 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
 
 // ------------------------------------------------------------------ launches
 template <int K>
-void launch_lloyd_k(const BatchTables& bt, int eval_only, int which, hipStream_t s) {
-    if (which & 1)
+void launch_lloyd_k(const BatchTables& bt, int eval_only, int part, hipStream_t s) {
+    if (part == 0)
         hipLaunchKernelGGL((kb_cell<K>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
                            eval_only);
-    if (which & 2)
+    else
         hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), dim3(bt.njobs * kPreciseBlocks), dim3(256), 0, s, bt.descs,
-                           eval_only);
+                           eval_only, part == 2 ? 1 : 0);
 }
 
 template <int K>

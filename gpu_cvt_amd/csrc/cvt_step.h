@@ -439,9 +439,11 @@ __device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval
     wave_count(a.stats, 8, d && f.nonfinite);
 }
 
-__device__ __forceinline__ void defer_site(const StepArgs& a, int t, bool row_written, bool same) {
-    const int pos = atomicAdd(&a.stats[kStatDeferred], 1);
-    a.defer_list[pos] = t | (row_written ? (1 << 30) : 0) | (same ? (1 << 29) : 0);
+// Hand site t to the FP64 pass: the frame and cell phases append to the front
+// of defer_list, the projection phase to the back (each site at most once).
+__device__ __forceinline__ void defer_site(const StepArgs& a, int t, bool row_written, bool same, bool back) {
+    const int pos = atomicAdd(&a.stats[back ? kStatDeferredB : kStatDeferred], 1);
+    a.defer_list[back ? a.nS - 1 - pos : pos] = t | (row_written ? (1 << 30) : 0) | (same ? (1 << 29) : 0);
 }
 
 }  // namespace cvt
