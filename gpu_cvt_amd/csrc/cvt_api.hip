@@ -566,7 +566,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->S[1], nS)) || (rc = j->alloc(&j->knn, (size_t)nS * p.K)) ||
         (rc = j->alloc(&j->knnV, (size_t)nS * p.kproj)) || (rc = j->alloc(&j->frozen, nS)) ||
         (rc = j->alloc(&j->streak, nS)) || (rc = j->alloc(&j->tier, nS)) || (rc = j->alloc(&j->has_prev, 1)) ||
-        (rc = j->alloc(&j->stats, kStatWords)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
+        (rc = j->alloc(&j->stats, kStatCopies * kStatStride)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
         (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
@@ -626,7 +626,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill((int*)j->gaps, 4L * nS, (int)0xbf800000, s);  // gaps -1: never reusable
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
-    launch_fill(j->stats, kStatWords, 0, s);
+    launch_fill(j->stats, kStatCopies * kStatStride, 0, s);
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
     {
         MeshDev m{};

@@ -17,6 +17,13 @@ constexpr int kStatMaxDisp = 10;
 // back of defer_list, so that the FP64 pass over the first list can run while
 // the projection kernel still appends); folded into word 9 at the iteration end.
 constexpr int kStatDeferredB = 11;
+// The per-site counters (words 0-8) and the largest displacement (word 10, float
+// bits, atomic max) are accumulated per block into one of kStatCopies copies of
+// the stat words, each on its own 128-byte line (copy c at stats[c * kStatStride]),
+// so that the atomics of all the waves do not queue on one line; the iteration
+// end folds the copies into copy 0, which also holds the deferred-list lengths.
+constexpr int kStatCopies = 8;
+constexpr int kStatStride = 32;
 // Blocks per job of the precise pass (they stride over the job's deferred list).
 constexpr int kPreciseBlocks = 4;
 
@@ -69,7 +76,7 @@ struct StepArgs {
     CVT_G unsigned char* streak;
     const CVT_G unsigned char* tier;
     CVT_G int* has_prev;        // device flag
-    CVT_G int* stats;           // kStatWords counters of this iteration
+    CVT_G int* stats;           // kStatCopies x kStatStride words of this iteration (copy 0 holds the totals)
     CVT_G int* hist;            // kHistCap * kStatWords history of stats
     CVT_G int* hist_count;
     CVT_G int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass

@@ -239,7 +239,7 @@ __global__ void kb_begin(const JobDesc* __restrict__ descs, int njobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= njobs) return;
     const StepArgs& a = descs[j].a;
-    for (int k = 0; k < kStatWords; ++k) a.stats[k] = 0;
+    for (int k = 0; k < kStatCopies * kStatStride; ++k) a.stats[k] = 0;
     a.dmax01[*a.parity] = 0.f;
 }
 
@@ -395,7 +395,14 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     const StepArgs& a = descs[j].a;
     const int c = *a.hist_count;
     const int slot = c < kHistCap ? c : kHistCap - 1;
-    a.stats[kStatMaxDisp] = __float_as_int(a.dmax01[*a.parity]);
+    // fold the per-block stat copies into copy 0
+    unsigned mx = (unsigned)a.stats[kStatMaxDisp];
+    for (int c = 1; c < kStatCopies; ++c) {
+        for (int k = 0; k < 9; ++k) a.stats[k] += a.stats[c * kStatStride + k];
+        mx = max(mx, (unsigned)a.stats[c * kStatStride + kStatMaxDisp]);
+    }
+    a.dmax01[*a.parity] = __uint_as_float(mx);
+    a.stats[kStatMaxDisp] = (int)mx;
     a.stats[kStatDeferred] += a.stats[kStatDeferredB];
     a.stats[kStatDeferredB] = 0;
     for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];

@@ -9,10 +9,6 @@
 
 namespace cvt {
 
-__device__ __forceinline__ void wave_count(int* __restrict__ stats, int idx, bool pred) {
-    const unsigned long long b = __ballot(pred);
-    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&stats[idx], (int)__popcll(b));
-}
 
 struct SiteFlags {
     float disp;     // |S' - S| (0 when not done or not finite)
@@ -419,24 +415,44 @@ __device__ __forceinline__ SiteFlags lloyd_site_precise(const StepArgs& a, int p
     return r;
 }
 
+// Per-iteration counters of a block's sites: each wave counts with ballots,
+// the block sums its waves in LDS, and one thread per word adds the sum to the
+// block's stat copy.  Every thread of the block must call it.
 __device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval_only, const SiteFlags& f, bool live) {
     if (eval_only) return;
+    (void)par;
+    __shared__ int red[4][10];
     const bool hp = *a.has_prev != 0;
     const bool d = live && f.done;
     // largest displacement of this iteration (seeds the next iteration's searches)
     float m = (d && isfinite(f.disp)) ? f.disp : 0.f;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax((unsigned*)(a.dmax01 + par), __float_as_uint(m));
-    wave_count(a.stats, 0, d && hp && f.same);
-    wave_count(a.stats, 1, d && hp && f.low);
-    wave_count(a.stats, 2, d && hp && f.both);
-    wave_count(a.stats, 3, d && f.newly);
-    wave_count(a.stats, 4, d && f.frozen_after);
-    wave_count(a.stats, 5, d && f.kind == 2);
-    wave_count(a.stats, 6, d && f.kind == 1);
-    wave_count(a.stats, 7, d && f.cell_fail);
-    wave_count(a.stats, 8, d && f.nonfinite);
+    const bool pred[9] = {d && hp && f.same, d && hp && f.low, d && hp && f.both, d && f.newly, d && f.frozen_after,
+                          d && f.kind == 2, d && f.kind == 1, d && f.cell_fail, d && f.nonfinite};
+    const int w = (int)threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const int c = (int)__popcll(__ballot(pred[k]));
+        if ((threadIdx.x & 63) == 0) red[w][k] = c;
+    }
+    if ((threadIdx.x & 63) == 0) red[w][9] = __float_as_int(m);
+    __syncthreads();
+    const int k = (int)threadIdx.x;
+    if (k < 10) {
+        const int nw = ((int)blockDim.x + 63) >> 6;
+        CVT_G int* st = a.stats + (blockIdx.x % kStatCopies) * kStatStride;
+        if (k < 9) {
+            int sum = 0;
+            for (int q = 0; q < nw; ++q) sum += red[q][k];
+            if (sum) atomicAdd(&st[k], sum);
+        } else {
+            unsigned mx = 0;  // non-negative float bits order as unsigned
+            for (int q = 0; q < nw; ++q) mx = max(mx, (unsigned)red[q][9]);
+            if (mx) atomicMax((CVT_G unsigned*)&st[kStatMaxDisp], mx);
+        }
+    }
+    __syncthreads();
 }
 
 // Hand site t to the FP64 pass: the frame and cell phases append to the front
