@@ -303,8 +303,8 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const CVT_G int* __res
 
 // Collect-then-select variant of grid_topl for the FP32 pass: every point of
 // the 3x3x3 block closer than tau is appended to this thread's LDS column
-// (lds[k * 256], k < CAP; conflict-free since the stride is a multiple of the
-// bank count), so divergent lanes pay a 4-op append per candidate.  The L
+// (lds[k * 256], k <= CAP; conflict-free since the stride is a multiple of the
+// bank count), so divergent lanes pay a short append per candidate.  The L
 // nearest of the collected points are then selected into a sorted list of
 // 32-bit keys: the FP32 distance bits with the low 6 bits replaced by the
 // candidate number k (non-negative floats order like their bits, and k makes
@@ -337,20 +337,24 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
         r0[k] = keep ? start[row + x0] : 0;
         r1[k] = keep ? start[row + x1 + 1] : 0;
     }
+    // Branch-free append, four points per round trip: every scanned point is
+    // written at row min(cnt, CAP) (row CAP is a spill row) and cnt advances
+    // only for a point within tau, so rows below cnt hold the candidates.
     int cnt = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        int s = r0[k];
-        for (; s + 1 < r1[k]; s += 2) {
-            const float4 p0 = pts[s], p1 = pts[s + 1];
-            const bool a0 = d2f(qx, qy, qz, p0) < tau && s != self_slot;
-            const bool a1 = d2f(qx, qy, qz, p1) < tau && s + 1 != self_slot;
-            if (a0) { if (cnt < CAP) lds[cnt * 256] = s; ++cnt; }
-            if (a1) { if (cnt < CAP) lds[cnt * 256] = s + 1; ++cnt; }
-        }
-        if (s < r1[k]) {
-            const float4 p0 = pts[s];
-            if (d2f(qx, qy, qz, p0) < tau && s != self_slot) { if (cnt < CAP) lds[cnt * 256] = s; ++cnt; }
+        const int e = r1[k];
+        for (int s = r0[k]; s < e; s += 4) {
+            float4 p[4];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) p[b] = pts[min(s + b, e - 1)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int sb = s + b;
+                const bool acc = (sb < e) & (d2f(qx, qy, qz, p[b]) < tau) & (sb != self_slot);
+                lds[min(cnt, CAP) * 256] = sb;
+                cnt += acc ? 1 : 0;
+            }
         }
     }
     if (cnt < L || cnt > CAP) return false;

@@ -21,7 +21,7 @@ static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 template <int K>
 __global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
                                                int eval_only) {
-    __shared__ int lds[kCollectCap * 256];
+    __shared__ int lds[(kCollectCap + 1) * 256];  // collect rows + a spill row
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
