@@ -96,6 +96,7 @@ struct cvt_job {
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
     float4 *fcache = nullptr, *vcache = nullptr, *gaps = nullptr;  // per-site vertex-search caches
+    int4* vpool = nullptr;  // candidate pool of the centroid vertex search
     int* fnn = nullptr;       // FP32-pass hand-offs: frame vertex per site,
     float4* cent = nullptr;   // centroid and cell flags per site
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
@@ -335,6 +336,7 @@ JobDesc make_desc(cvt_job* j) {
     a.fcache = to_g(j->fcache);
     a.vcache = to_g(j->vcache);
     a.gaps = to_g(j->gaps);
+    a.vpool = to_g(j->vpool);
     a.fnn = to_g(j->fnn);
     a.cent = to_g(j->cent);
     a.dmax01 = to_g(j->dmax);
@@ -569,7 +571,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->stats, kStatCopies * kStatStride)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
-        (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) ||
+        (rc = j->alloc(&j->vpool, 2L * nS)) || (rc = j->alloc(&j->dmax, 2)) ||
         (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->cent, nS)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);

@@ -83,8 +83,9 @@ struct StepArgs {
     CVT_G float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
     CVT_G int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
     CVT_G float4* fcache;       // nS: (site position at its last vertex search, 3rd-nearest distance there; +inf: none)
-    CVT_G float4* vcache;       // nS: (centroid at its last kproj search, 7th-nearest distance there; +inf: none)
-    CVT_G float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, 5th/6th gap at vcache, 0)
+    CVT_G float4* vcache;       // nS: (centroid at its last kproj search, 8th-nearest distance there; +inf: none)
+    CVT_G int4* vpool;          // 2 nS: the 7 nearest vertices there, 9th-nearest distance (float bits)
+    CVT_G float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, -, -)
     CVT_G float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     CVT_G int* fnn;             // nS: frame vertex of this iteration (hand-off of the FP32 pass; -2: deferred)
     CVT_G float4* cent;         // nS: centroid and cell-phase flags (hand-off of the FP32 pass)

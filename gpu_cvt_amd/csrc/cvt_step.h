@@ -120,7 +120,7 @@ __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, S
             ok = k1 >= 0;
             c.nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
             if (ok && !eval_only) {
-                if (!c.use_cache) gp = a.gaps[c.i];  // keep the centroid-search fields
+                if (!c.use_cache) gp = a.gaps[c.i];
                 gp.x = c.nn >= 0 ? sqrtf(d1[1]) - sqrtf(d1[0]) : -1.f;
                 gp.y = __int_as_float(c.nn);
                 a.fcache[c.i] = make_float4(q.x, q.y, q.z, d1[2] < inf ? sqrtf(d1[2]) : inf);
@@ -255,9 +255,14 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
         for (int k = 0; k < KP; ++k) vl[k] = -1;
         if (!c.fz) {
             if (finite3(c3.x, c3.y, c3.z)) {
-                // the 5 nearest vertices keep their set while 2|c3 - c_s| is below
-                // the 5th/6th gap (gaps.z) at the cached centroid c_s (vcache);
-                // only their order is recomputed (exact FP64 keys, index ties)
+                // pool cache: vcache = (centroid at the last kproj search, the 8th-
+                // nearest distance there), vpool = the 7 nearest vertices there and
+                // the 9th-nearest distance (float bits).  After a move by d every
+                // vertex outside the pool is at least (8th - d) away, so when the 5th
+                // nearest pooled vertex (exact FP64 keys, index ties) is closer, the
+                // pool's first 5 are the list; otherwise the 9th distance plus d
+                // seeds the search (triangle inequality).
+                static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
                 float tauv = inf;
                 bool have = false;
                 if (c.use_cache) {
@@ -265,51 +270,58 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
                     if (vc.w < inf) {
                         const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
                                                (c3.z - vc.z) * (c3.z - vc.z));
-                        if (a.gaps[i].z - 2.f * dc > 2e-5f * vc.w + 4.f * a.m.vg.eps_abs) {
-                            const int* row = a.knnV + (size_t)i * KP;
-                            double key[KP];
+                        const int4 p0 = a.vpool[2 * (size_t)i], p1 = a.vpool[2 * (size_t)i + 1];
+                        int id[KP + 2] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+                        double key[KP + 2];
 #pragma unroll
-                            for (int k = 0; k < KP; ++k) {
-                                vl[k] = row[k];
-                                const float4 pv = a.m.V[vl[k]];
-                                key[k] = key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z);
+                        for (int k = 0; k < KP + 2; ++k) {
+                            const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
+                            key[k] = id[k] >= 0 ? key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
+                            id[k] = id[k] >= 0 ? id[k] : 0x7fffffff;
+                        }
+#pragma unroll
+                        for (int p0i = 1; p0i < KP + 2; ++p0i) {  // insertion sort by (key, id)
+#pragma unroll
+                            for (int k = p0i; k >= 1; --k) {
+                                const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && id[k] < id[k - 1]);
+                                const double tk = key[k];
+                                const int tv = id[k];
+                                key[k] = sw ? key[k - 1] : key[k];
+                                id[k] = sw ? id[k - 1] : id[k];
+                                key[k - 1] = sw ? tk : key[k - 1];
+                                id[k - 1] = sw ? tv : id[k - 1];
                             }
+                        }
+                        if (key[KP - 1] < __builtin_huge_val() &&
+                            sqrtf((float)key[KP - 1]) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < vc.w - dc) {
 #pragma unroll
-                            for (int p0 = 1; p0 < KP; ++p0) {  // insertion sort of KP entries by (key, id)
-#pragma unroll
-                                for (int k = p0; k >= 1; --k) {
-                                    const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && vl[k] < vl[k - 1]);
-                                    const double tk = key[k];
-                                    const int tv = vl[k];
-                                    key[k] = sw ? key[k - 1] : key[k];
-                                    vl[k] = sw ? vl[k - 1] : vl[k];
-                                    key[k - 1] = sw ? tk : key[k - 1];
-                                    vl[k - 1] = sw ? tv : vl[k - 1];
-                                }
-                            }
+                            for (int k = 0; k < KP; ++k) vl[k] = id[k];
                             have = true;
                         } else {
-                            const float rr = (vc.w + dc) * (1.f + 1e-4f);
+                            const float rr = (__int_as_float(p1.w) + dc) * (1.f + 1e-4f);
                             tauv = rr * rr;
                         }
                     }
                 }
                 if (!have) {
-                    int sl[KP];
-                    float lv, dv[KP + 2];
-                    const int kk = knn_exact<KP, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1,
-                                                          tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
+                    int sl[KP + 2];
+                    float lv, dv[KP + 4];
+                    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z,
+                                                              -1, tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
                     if (!PRECISE && kk < 0) {
                         r.same = cr.same;
                         r.row_written = true;
                         return;
                     }
+                    int id[KP + 2];
 #pragma unroll
-                    for (int k = 0; k < KP; ++k) vl[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
-                    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 1] < inf ? sqrtf(dv[KP + 1]) : inf);
-                    float4 gp = a.gaps[i];
-                    gp.z = vl[KP - 1] >= 0 ? (dv[KP] < inf ? sqrtf(dv[KP]) : inf) - sqrtf(dv[KP - 1]) : -1.f;
-                    a.gaps[i] = gp;
+                    for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
+#pragma unroll
+                    for (int k = 0; k < KP; ++k) vl[k] = id[k];
+                    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 2] < inf ? sqrtf(dv[KP + 2]) : inf);
+                    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
+                    a.vpool[2 * (size_t)i + 1] =
+                        make_int4(id[4], id[5], id[6], __float_as_int(dv[KP + 3] < inf ? sqrtf(dv[KP + 3]) : inf));
                 }
             }
             int* row = a.knnV + (size_t)i * KP;
