@@ -71,6 +71,11 @@ __device__ __forceinline__ double key64(float qx, float qy, float qz, float px, 
     return s;
 }
 
+// x / y through the hardware reciprocal (v_rcp_f32, 1 ulp) instead of the
+// correctly rounded division sequence (about ten instructions): for the cell
+// geometry and closest points, where a few ulps are far below the tolerances.
+__device__ __forceinline__ float fast_div(float x, float y) { return x * __builtin_amdgcn_rcpf(y); }
+
 __device__ __forceinline__ float d2f(float qx, float qy, float qz, float4 p) {
     float dx = p.x - qx, dy = p.y - qy, dz = p.z - qz;
     return fmaf(dz, dz, fmaf(dy, dy, dx * dx));
@@ -757,8 +762,8 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
         const float det = curx * by - cury * bx;
         inside = inside && det > 0.f;
         const float ex = bx - curx, ey = by - cury;
-        rin2 = fminf(rin2, det * det / (ex * ex + ey * ey));
-        const float inv = 1.0f / det;
+        rin2 = fminf(rin2, fast_div(det * det, ex * ex + ey * ey));
+        const float inv = fast_div(1.0f, det);
         const float vx = (by - cury) * inv, vy = (curx - bx) * inv;
         if (step == 0) {
             fvx = vx; fvy = vy;
@@ -788,7 +793,7 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
     o.moment = fabsf(I12.s) * (1.0f / 12.0f);
     if (o.nverts < 3) { o.ok = false; return o; }
     if (fabsf(A) < 1e-18f) { o.cx = 0.f; o.cy = 0.f; return o; }
-    const float inv6A = 1.0f / (3.0f * A2.s);
+    const float inv6A = fast_div(1.0f, 3.0f * A2.s);
     o.cx = Cx.s * inv6A;
     o.cy = Cy.s * inv6A;
     return o;
@@ -884,7 +889,7 @@ __device__ __forceinline__ float3 closest_point_tri(float3 P, float3 A, float3 B
     if (d3 >= 0.f && d4 <= d3) return B;
     const float vc = d1 * d4 - d3 * d2;
     if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
-        const float t = d1 / (d1 - d3 + 1e-30f);
+        const float t = fast_div(d1, d1 - d3 + 1e-30f);
         return make_float3(A.x + AB.x * t, A.y + AB.y * t, A.z + AB.z * t);
     }
     const float3 CP = make_float3(P.x - C.x, P.y - C.y, P.z - C.z);
@@ -893,15 +898,15 @@ __device__ __forceinline__ float3 closest_point_tri(float3 P, float3 A, float3 B
     if (d6 >= 0.f && d5 <= d6) return C;
     const float vb = d5 * d2 - d1 * d6;
     if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
-        const float w = d2 / (d2 - d6 + 1e-30f);
+        const float w = fast_div(d2, d2 - d6 + 1e-30f);
         return make_float3(A.x + AC.x * w, A.y + AC.y * w, A.z + AC.z * w);
     }
     const float va = d3 * d6 - d5 * d4;
     if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
-        const float w = (d4 - d3) / ((d4 - d3) + (d5 - d6) + 1e-30f);
+        const float w = fast_div(d4 - d3, (d4 - d3) + (d5 - d6) + 1e-30f);
         return make_float3(B.x + (C.x - B.x) * w, B.y + (C.y - B.y) * w, B.z + (C.z - B.z) * w);
     }
-    const float denom = 1.0f / (va + vb + vc + 1e-30f);
+    const float denom = fast_div(1.0f, va + vb + vc + 1e-30f);
     const float vv = vb * denom, ww = vc * denom;
     return make_float3(A.x + AB.x * vv + AC.x * ww, A.y + AB.y * vv + AC.y * ww, A.z + AB.z * vv + AC.z * ww);
 }

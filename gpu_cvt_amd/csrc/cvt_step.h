@@ -199,7 +199,7 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
         const float aa = dx * u.x + dy * u.y + dz * u.z;
         const float bb = dx * w.x + dy * w.y + dz * w.z;
         const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
-        const float ic = cc > 0.f ? 1.0f / cc : 0.f;
+        const float ic = cc > 0.f ? fast_div(1.0f, cc) : 0.f;
         px[k] = aa * ic;
         py[k] = bb * ic;
     }
