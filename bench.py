@@ -8,8 +8,10 @@ projection kernel), launched as HIP graphs; inputs are resident in HBM before
 the timed region.
 
 Prints one JSON line (rank 0).  Also measures, in the same run:
-  * roofline: algorithmic bytes of the fused Lloyd kernel per launch divided by
-    its mean launch duration (HIP events on the job's stream);
+  * roofline: algorithmic bytes of one Lloyd iteration divided by the mean
+    duration of the FP32 Lloyd pass -- the three kernels that do all of the
+    per-site work, kb_frame + kb_cell + kb_project, launched back to back --
+    timed with HIP events on the job's stream;
   * cpu_baseline: the FP64 CPU oracle's Lloyd step on this host, on a bounded
     sample (one iteration of a sub-sampled site set on the same mesh).
 """
@@ -38,13 +40,14 @@ def algorithmic_bytes(nS: int, nV: int, nF: int, K: int) -> int:
 
 
 def traffic_from_profile(workload: str):
-    """HBM bytes per launch of the fused Lloyd kernel, as recorded by the
-    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (each
-    counter scaled by its copy-kernel calibration factor); (None, None) when no
-    summary exists for the workload.  The counters cannot be read from inside
-    this process, so the figure comes from the profile run of the same command."""
+    """HBM bytes per FP32 Lloyd pass (kb_frame + kb_cell + kb_project), as
+    recorded by the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
+    workload (each counter scaled by its copy-kernel calibration factor);
+    (None, None) when no summary exists for the workload.  The counters cannot
+    be read from inside this process, so the figure comes from the profile run
+    of the same command."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        f"kb_lloyd_{workload}_hbm_traffic.md")
+                        f"lloyd_pass_{workload}_hbm_traffic.md")
     if not os.path.exists(path):
         return None, None
     with open(path) as f:
@@ -325,7 +328,8 @@ def main(argv=None) -> int:
     achieved = alg / (prof["lloyd_ms"] * 1e-3) / 1e9
     traffic, traffic_src = traffic_from_profile(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                "kernel": "FP32 Lloyd pass: kb_frame + kb_cell<24> + kb_project"}
     whole_iter_gbs = alg / (ms_per_step * 1e-3) / 1e9
     print(f"[bench] timed {args.steps} steps: {ms:.3f} ms device ({wall * 1e3:.1f} ms wall); "
           f"profile {prof}; kernel {achieved:.1f} GB/s; whole iteration {whole_iter_gbs:.1f} GB/s",
