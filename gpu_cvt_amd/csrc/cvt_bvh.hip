@@ -192,24 +192,50 @@ __global__ void k_leaf_boxes(const float4* __restrict__ tri, int nc, const int* 
     const float4 a = tri[nc * f], b = tri[nc * f + (nc == 3 ? 1 : 0)], c = tri[nc * f + (nc == 3 ? 2 : 0)];
     lo[n - 1 + l] = make_float4(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)), 0.f);
     hi[n - 1 + l] = make_float4(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)), 0.f);
-    if (l < n - 1) flags[l] = 0;
+    if (l < n - 1) {  // internal node l: empty box, no child arrived yet
+        const float inf = __builtin_huge_valf();
+        lo[l] = make_float4(inf, inf, inf, 0.f);
+        hi[l] = make_float4(-inf, -inf, -inf, 0.f);
+        flags[l] = 0;
+    }
 }
 
-// Bottom-up refit: the second child to arrive at a node computes its box.
+// Bottom-up refit without fences: every child merges its box into its parent
+// with atomic min/max (device-coherent read-modify-writes), then counts itself
+// in; the second to arrive -- whose merge and count come after the first's,
+// all at the coherence point -- reads the finished parent box with coherent
+// loads and carries it upward.  (A release/acquire fence per level would
+// write back and invalidate the whole L2 at every step of every thread.)
+__device__ __forceinline__ void box_merge(float4* lo, float4* hi, int node, float4 blo, float4 bhi) {
+    float* L = (float*)(lo + node);
+    float* H = (float*)(hi + node);
+    __hip_atomic_fetch_min(L + 0, blo.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_min(L + 1, blo.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_min(L + 2, blo.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(H + 0, bhi.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(H + 1, bhi.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_fetch_max(H + 2, bhi.z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ float4 box_load(const float4* b, int node) {
+    float* p = (float*)(b + node);
+    return make_float4(__hip_atomic_load(p + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(p + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                       __hip_atomic_load(p + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT), 0.f);
+}
+
 __global__ void k_refit(int n, const int2* __restrict__ child, const int* __restrict__ parent, float4* lo, float4* hi,
                         int* flags) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n) return;
+    (void)child;
+    float4 blo = lo[n - 1 + l], bhi = hi[n - 1 + l];
     int node = parent[n - 1 + l];
     while (node >= 0) {
-        __threadfence();
-        if (atomicAdd(&flags[node], 1) == 0) return;
-        __threadfence();
-        const int2 c = child[node];
-        const float4 la = lo[c.x], lb = lo[c.y];
-        const float4 ha = hi[c.x], hb = hi[c.y];
-        lo[node] = make_float4(fminf(la.x, lb.x), fminf(la.y, lb.y), fminf(la.z, lb.z), 0.f);
-        hi[node] = make_float4(fmaxf(ha.x, hb.x), fmaxf(ha.y, hb.y), fmaxf(ha.z, hb.z), 0.f);
+        box_merge(lo, hi, node, blo, bhi);
+        if (__hip_atomic_fetch_add(flags + node, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+        blo = box_load(lo, node);
+        bhi = box_load(hi, node);
         node = parent[node];
     }
 }

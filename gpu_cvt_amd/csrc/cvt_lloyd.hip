@@ -191,6 +191,7 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
                                                       unsigned long long* __restrict__ keys,
                                                       float4* __restrict__ site_n) {
     constexpr int M = K + 4;
+    __shared__ int lds[(kCollectCap + 1) * 256];
     const StepArgs& a = desc->a;
     const int t = blockIdx.x * 256 + (int)threadIdx.x;
     if (t >= a.nS) return;
@@ -213,10 +214,24 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
         u = make_float3(fu.x, fu.y, fu.z);
         w = make_float3(fw.x, fw.y, fw.z);
     }
+    // site KNN: the LDS collect of the FP32 pass, seeded as the next Lloyd
+    // iteration would be (the last (K+2)-th distance grown by twice the last
+    // largest displacement), with the exact FP64 rescan behind it
+    float tau = inf, ctau = a.tau0;
+    if (*a.seed_ok) {
+        const int par = *a.parity;
+        const float rp = sqrtf(a.rad2[i]);
+        const float r0 = rp + 2.f * a.dmax01[par ^ 1];
+        tau = r0 * r0 * (1.f + 1e-5f);
+        const float4 sp = a.S01[par ^ 1][i];
+        const float di = sqrtf((q.x - sp.x) * (q.x - sp.x) + (q.y - sp.y) * (q.y - sp.y) + (q.z - sp.z) * (q.z - sp.z));
+        const float rc = 1.05f * rp + 2.5f * di;
+        ctau = fminf(rc * rc, tau);
+    }
     int slots[K];
     float lk;
-    knn_exact<K, true>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, inf, slots, lk, nullptr, 0.f, nullptr,
-                       a.sg.cid);
+    knn_exact<K, true, kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, tau, slots, lk,
+                                    lds + threadIdx.x, ctau, nullptr, a.sg.cid);
     int* row = knn_tmp + (size_t)i * K;
     float px[M], py[M];
 #pragma unroll
