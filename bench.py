@@ -168,7 +168,7 @@ def bench_serve(args) -> int:
 
     # served: continuous batching
     with cvt.Server(hbm_budget_bytes=200 << 30, epoch_iters=10, min_check_iters=10, check_disp_rel=1e30,
-                    check_frozen_frac=0.0) as srv:
+                    check_frozen_frac=0.0, ingest_threads=args.ingest_threads) as srv:
         t0 = time.perf_counter()
         tickets = []
         for k, (ta, V, F, S, _tag) in enumerate(trace):
@@ -181,6 +181,9 @@ def bench_serve(args) -> int:
         srv.wait_idle(3000)
         info = srv.info()
         res = [srv.poll(t) for t in tickets]
+    if args.serve_log:
+        with open(args.serve_log, "w") as f:
+            json.dump([{k: r[k] for k in ("nS", "iterations", "state", "t_arrive", "t_admit", "t_done")} for r in res], f)
     lat_s = np.array([r["t_done"] - r["t_arrive"] for r in res])
     span_s = max(r["t_done"] for r in res) - min(r["t_arrive"] for r in res)
     si_s = float(sum(r["nS"] * r["iterations"] for r in res))
@@ -259,6 +262,8 @@ def main(argv=None) -> int:
                     help="plain mode: after the timed steps, symmetric Hausdorff remesh<->input with this many "
                          "area-uniform samples per surface (0 = skip)")
     ap.add_argument("--rate", type=float, default=200.0, help="serve mode: arrivals per second")
+    ap.add_argument("--ingest-threads", type=int, default=4, help="serve mode: server threads creating jobs")
+    ap.add_argument("--serve-log", default="", help="serve mode: write per-request times (JSON) to this path")
     ap.add_argument("--jobs", type=int, default=256, help="batch mode: number of jobs")
     ap.add_argument("--site-cell", type=float, default=0.0, help="site grid cell scale (0 = library default)")
     ap.add_argument("--vert-cell", type=float, default=0.0, help="vertex grid cell scale (0 = library default)")

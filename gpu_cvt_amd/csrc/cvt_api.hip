@@ -37,6 +37,80 @@ int fail(int code, const char* fmt, ...) {
     g_err = buf;
     return code;
 }
+
+DevArena*& current_arena() {
+    static thread_local DevArena* a = nullptr;
+    return a;
+}
+
+void* DevArena::alloc(size_t bytes) {
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> lk(mu);
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < bytes) continue;
+            const uintptr_t p = it->first;
+            const size_t rest = it->second - bytes;
+            free_.erase(it);
+            if (rest) free_[p + bytes] = rest;
+            return (void*)p;
+        }
+        if (attempt) break;
+        const size_t c = std::max(grow_bytes, bytes);
+        void* q = nullptr;
+        if (hipMalloc(&q, c) != hipSuccess) return nullptr;
+        chunks.push_back({(uintptr_t)q, c});
+        reserved += c;
+        free_[(uintptr_t)q] = c;
+    }
+    return nullptr;
+}
+
+void DevArena::free(void* ptr, size_t bytes) {
+    if (!ptr) return;
+    bytes = (std::max<size_t>(bytes, 1) + 255) & ~size_t(255);
+    std::lock_guard<std::mutex> lk(mu);
+    uintptr_t p = (uintptr_t)ptr;
+    // the chunk holding p (ranges never merge across chunks)
+    uintptr_t cb = 0, ce = 0;
+    for (const Chunk& c : chunks)
+        if (p >= c.base && p < c.base + c.size) { cb = c.base; ce = c.base + c.size; break; }
+    auto next = free_.lower_bound(p);
+    if (next != free_.end() && next->first == p + bytes && next->first < ce) {
+        bytes += next->second;
+        next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+        auto prev = std::prev(next);
+        if (prev->first + prev->second == p && prev->first >= cb) {
+            prev->second += bytes;
+            return;
+        }
+    }
+    free_[p] = bytes;
+}
+
+DevArena::~DevArena() {
+    for (const Chunk& c : chunks) (void)hipFree((void*)c.base);
+}
+
+int dev_alloc(DevArena* a, void** p, size_t bytes) {
+    *p = nullptr;
+    if (a) {
+        *p = a->alloc(bytes);
+        if (!*p) return fail(CVT_ENOMEM, "arena allocation of %zu bytes failed", bytes);
+        return 0;
+    }
+    const hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+    if (e != hipSuccess) return fail(CVT_ENOMEM, "hipMalloc(%zu bytes): %s", bytes, hipGetErrorString(e));
+    return 0;
+}
+
+void dev_free(DevArena* a, void* p, size_t bytes) {
+    if (!p) return;
+    if (a) a->free(p, bytes);
+    else (void)hipFree(p);
+}
 }  // namespace cvt
 
 namespace {
@@ -70,6 +144,7 @@ struct cvt_job {
     int nV = 0, nF = 0, nS = 0;
     hipStream_t stream = nullptr;
     SideStream side;  // the FP64 pass overlaps the projection kernel on it
+    TempPool check_pool;  // device temporaries of cvt_job_hausdorff_within, kept between checks
     std::vector<void*> allocs;
     unsigned long long bytes = 0;
 
@@ -115,13 +190,16 @@ struct cvt_job {
     hipGraphExec_t graph = nullptr;
     cvt_bvh* bvh = nullptr;  // LBVH of the input mesh, built on first use
 
+    DevArena* arena = current_arena();  // where the device buffers come from (nullptr: hipMalloc)
+    std::vector<size_t> alloc_bytes;
+
     template <class T>
     int alloc(T** ptr, size_t count) {
         size_t nbytes = std::max<size_t>(count, 1) * sizeof(T);
         void* d = nullptr;
-        hipError_t e = hipMalloc(&d, nbytes);
-        if (e != hipSuccess) return fail(CVT_ENOMEM, "hipMalloc(%zu bytes): %s", nbytes, hipGetErrorString(e));
+        if (int rc = dev_alloc(arena, &d, nbytes)) return rc;
         allocs.push_back(d);
+        alloc_bytes.push_back(nbytes);
         bytes += nbytes;
         *ptr = (T*)d;
         return 0;
@@ -129,7 +207,9 @@ struct cvt_job {
     ~cvt_job();
     void release() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (void* d : allocs) (void)hipFree(d);
+        for (size_t k = 0; k < allocs.size(); ++k) dev_free(arena, allocs[k], alloc_bytes[k]);
+        allocs.clear();
+        alloc_bytes.clear();
         if (stream) (void)hipStreamDestroy(stream);
         side_destroy(side);
     }
@@ -140,10 +220,12 @@ struct cvt_bvh {
     BVHDev d;
     float4* tri = nullptr;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;
+    DevArena* arena = current_arena();
     hipStream_t stream = nullptr;
     int nF = 0;
     ~cvt_bvh() {
-        for (void* p : allocs) (void)hipFree(p);
+        for (size_t k = 0; k < allocs.size(); ++k) dev_free(arena, allocs[k], alloc_bytes[k]);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -156,9 +238,10 @@ cvt_job::~cvt_job() {
 namespace {
 
 int bvh_alloc(cvt_bvh* b, void** p, size_t bytes) {
-    hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 16));
-    if (e != hipSuccess) return fail(CVT_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    bytes = std::max<size_t>(bytes, 16);
+    if (int rc = dev_alloc(b->arena, p, bytes)) return rc;
     b->allocs.push_back(*p);
+    b->alloc_bytes.push_back(bytes);
     return 0;
 }
 
@@ -214,10 +297,10 @@ int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_
     if ((rc = bvh_alloc(b, &p, sizeof(int) * nF))) return bail(rc);
     b->d.leaf_tri = (int*)p;
     void* work = nullptr;
-    if (hipMalloc(&work, bvh_work_bytes(nF)) != hipSuccess) return bail(fail(CVT_ENOMEM, "bvh work alloc"));
+    if ((rc = dev_alloc(b->arena, &work, bvh_work_bytes(nF)))) return bail(rc);
     bvh_build(b->d, b->d.tri, nF, lo, hi, work, b->stream);
     hipError_t e = hipStreamSynchronize(b->stream);
-    (void)hipFree(work);
+    dev_free(b->arena, work, bvh_work_bytes(nF));
     if (e != hipSuccess) return bail(fail(CVT_EHIP, "bvh build: %s", hipGetErrorString(e)));
     if ((e = hipGetLastError()) != hipSuccess) return bail(fail(CVT_EHIP, "bvh build launch: %s", hipGetErrorString(e)));
     *out = b;
@@ -1088,10 +1171,9 @@ int cvt_job_hausdorff(cvt_job* j, double eps, double* h_out, int* nF_out) {
 }
 
 int cvt_job_hausdorff_within(cvt_job* j, double eps, double tol, int* within, double* h_out, int* nF_out) {
-    if (!within || !(tol >= 0)) return fail(CVT_EINVAL, "bad argument");
-    TempPool T;
+    if (!j || !within || !(tol >= 0)) return fail(CVT_EINVAL, "bad argument");
     double h = -1;
-    const int rc = job_hausdorff(j, eps, T, tol, within, false, h_out ? &h : nullptr, nF_out);
+    const int rc = job_hausdorff(j, eps, j->check_pool, tol, within, false, h_out ? &h : nullptr, nF_out);
     if (!rc && h_out) *h_out = h;
     return rc;
 }
@@ -1118,6 +1200,34 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     int rc;
     if ((rc = ensure_bvh(j))) return rc;
     T.reset();
+    if (tol >= 0 && within && !exact_always && j->nS < (1 << 21)) {
+        // The verdict straight from the remesh cells' candidate triples (each
+        // taken once, at its smallest emitting site; no sort, one host sync);
+        // the full remesh below runs only for a job within tolerance, for its
+        // distance.  Both give the same verdict: the same triangles, probes
+        // and bounded queries.
+        const int K = j->p.K, nS = j->nS, M = K + 4;
+        int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
+        auto* keys = (unsigned long long*)T.get(8 * (size_t)nS * M);
+        float4* site_n = (float4*)T.get(sizeof(float4) * nS);
+        int* flag = (int*)T.get(sizeof(int));
+        if (!knn_tmp || !keys || !site_n || !flag) return fail(CVT_ENOMEM, "check temporaries");
+        hipStream_t s = j->stream;
+        launch_batch_grid(j->tables, 1, s);
+        launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+        launch_fill(flag, 1, 0, s);
+        bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), flag, s);
+        int hf = 0;
+        HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if (hf) {
+            *within = 0;
+            if (h_out) *h_out = -1;
+            if (nF_out) *nF_out = -1;
+            return 0;
+        }
+        T.reset();
+    }
     int* dF = nullptr;
     int nkeep = 0;
     if ((rc = remesh_device(j, eps, T, &dF, &nkeep))) return rc;
@@ -1186,11 +1296,13 @@ struct cvt_batch {
     hipStream_t stream = nullptr;
     SideStream side;
     std::vector<void*> allocs;
+    std::vector<size_t> alloc_bytes;
+    DevArena* arena = current_arena();
     BatchTables tables{};
     hipGraphExec_t graph = nullptr;
     ~cvt_batch() {
         if (graph) (void)hipGraphExecDestroy(graph);
-        for (void* p : allocs) (void)hipFree(p);
+        for (size_t k = 0; k < allocs.size(); ++k) dev_free(arena, allocs[k], alloc_bytes[k]);
         if (stream) (void)hipStreamDestroy(stream);
         side_destroy(side);
     }
@@ -1222,8 +1334,10 @@ int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
     pad_site_blocks(ht);
     auto dalloc = [&](size_t bytes) -> void* {
         void* p = nullptr;
-        if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
+        bytes = std::max<size_t>(bytes, 16);
+        if (dev_alloc(b->arena, &p, bytes)) return nullptr;
         b->allocs.push_back(p);
+        b->alloc_bytes.push_back(bytes);
         return p;
     };
     auto* dd = (JobDesc*)dalloc(sizeof(JobDesc) * njobs);
@@ -1299,7 +1413,10 @@ static int ensure_bvh(cvt_job* j) {
             lo[a] = std::nextafter((float)j->lo[a], -INFINITY);
             hi[a] = std::nextafter((float)j->hi[a], INFINITY);
         }
+        DevArena* const prev = current_arena();
+        current_arena() = j->arena;  // the LBVH lives where the job does
         int rc = bvh_make(nullptr, j->nV, nullptr, j->nF, j->tri, &j->bvh, lo, hi);
+        current_arena() = prev;
         if (rc) return rc;
     }
     return 0;

@@ -337,31 +337,24 @@ __global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __rest
     out[i] = best;
 }
 
-// Bounded query: out[i] = the squared distance of the first triangle found
-// within thr2 of P[i] (<= thr2), or +inf when none is.  Only boxes within thr2
-// are entered and the walk stops at the first hit, so a probe near the surface
-// costs a few node visits instead of a proof of its nearest triangle.
-__global__ __launch_bounds__(256) void k_point_within(BVHDev b, const float4* __restrict__ P, int np, float thr2,
-                                                      float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const float4 q4 = P[i];
-    const float3 q = make_float3(q4.x, q4.y, q4.z);
+// Bounded query: the squared distance of the first triangle found within thr2
+// of q (<= thr2), or +inf when none is.  Only boxes within thr2 are entered
+// and the walk stops at the first hit, so a probe near the surface costs a
+// few node visits instead of a proof of its nearest triangle.
+__device__ float point_within(const BVHDev& b, float3 q, float thr2) {
     const int n = b.n;
     const float inf = __builtin_huge_valf();
     if (n == 1) {
         const int f = b.leaf_tri[0];
         const float d = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
-        out[i] = d <= thr2 ? d : inf;
-        return;
+        return d <= thr2 ? d : inf;
     }
-    float hit = inf;
     int cur = 0, last = -1;
     for (;;) {
         if (cur >= n - 1) {  // leaf
             const int f = b.leaf_tri[cur - (n - 1)];
             const float d = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
-            if (d <= thr2) { hit = d; break; }
+            if (d <= thr2) return d;
             last = cur;
             cur = b.parent[cur];
             if (cur < 0) break;
@@ -384,7 +377,82 @@ __global__ __launch_bounds__(256) void k_point_within(BVHDev b, const float4* __
         cur = next;
         if (cur < 0) break;
     }
-    out[i] = hit;
+    return inf;
+}
+
+// out[i] = point_within(P[i]).
+__global__ __launch_bounds__(256) void k_point_within(BVHDev b, const float4* __restrict__ P, int np, float thr2,
+                                                      float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float4 q4 = P[i];
+    out[i] = point_within(b, make_float3(q4.x, q4.y, q4.z), thr2);
+}
+
+// Convergence test straight from the remesh cells' candidate keys: row i
+// (M slots) holds the sorted triples (packed 21:21:21, ~0 unused) that site i
+// emits.  The remeshed surface is the union of all rows (buildmesh.cuh:170-241
+// sorts and de-duplicates them); here each triple is taken once, at the
+// smallest of its sites that emits it (a lookup in the other sites' rows), so
+// neither a sort nor a scan is needed.  A kept, non-degenerate triangle
+// (buildmesh.cuh:225-241) is oriented by the summed site normals and its four
+// probes (centroid, edge midpoints; evaluate.cuh:252-283 without the vertices)
+// are formed with the arithmetic of the remesh path; the first probe with no
+// triangle within thr2 sets *flag (a plain store: every writer stores 1).
+__global__ __launch_bounds__(256) void k_check_keys(BVHDev b, const unsigned long long* __restrict__ keys, int nS,
+                                                    int M, const float4* __restrict__ S,
+                                                    const float4* __restrict__ site_n, float eps, float thr2,
+                                                    int* flag) {
+    const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (long)nS * M) return;
+    const unsigned long long v = keys[k];
+    if (v == ~0ull) return;
+    if (*(volatile int*)flag) return;
+    const int i = (int)(k / M);
+    int ia = (int)(v >> 42), ib = (int)((v >> 21) & 0x1fffff), ic = (int)(v & 0x1fffff);
+    // the smallest emitting site keeps the triple
+    const int others[2] = {ia == i ? ib : ia, ic == i ? ib : ic};
+#pragma unroll
+    for (int o = 0; o < 2; ++o) {
+        const int sidx = others[o];
+        if (sidx >= i) continue;
+        const unsigned long long* row = keys + (size_t)sidx * M;
+        bool found = false;
+        for (int t = 0; t < M && !found; ++t) found = row[t] == v;
+        if (found) return;
+    }
+    const float4 A = S[ia], B = S[ib], C = S[ic];
+    const float ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
+    const float fx = C.x - A.x, fy = C.y - A.y, fz = C.z - A.z;
+    const float gx = C.x - B.x, gy = C.y - B.y, gz = C.z - B.z;
+    const float nx = ey * fz - ez * fy, ny = ez * fx - ex * fz, nz = ex * fy - ey * fx;
+    const float e0 = sqrtf(ex * ex + ey * ey + ez * ez), e1 = sqrtf(gx * gx + gy * gy + gz * gz);
+    const float e2 = sqrtf(fx * fx + fy * fy + fz * fz), ar = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (!(e0 > eps && e1 > eps && e2 > eps && ar > eps)) return;
+    const float4 na = site_n[ia], nb = site_n[ib], nc = site_n[ic];
+    const float dot = nx * (na.x + nb.x + nc.x) + ny * (na.y + nb.y + nc.y) + nz * (na.z + nb.z + nc.z);
+    const float4 a = A, bb = dot < 0.f ? C : B, c = dot < 0.f ? B : C;
+    float3 P[4];
+    P[0] = make_float3((a.x + bb.x + c.x) * (1.f / 3.f), (a.y + bb.y + c.y) * (1.f / 3.f),
+                       (a.z + bb.z + c.z) * (1.f / 3.f));
+    P[1] = make_float3(0.5f * (a.x + bb.x), 0.5f * (a.y + bb.y), 0.5f * (a.z + bb.z));
+    P[2] = make_float3(0.5f * (bb.x + c.x), 0.5f * (bb.y + c.y), 0.5f * (bb.z + c.z));
+    P[3] = make_float3(0.5f * (c.x + a.x), 0.5f * (c.y + a.y), 0.5f * (c.z + a.z));
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (point_within(b, P[q], thr2) > thr2) {
+            *flag = 1;
+            return;
+        }
+    }
+}
+
+void bvh_check_keys(const BVHDev& b, const unsigned long long* keys, int nS, int M, const float4* S,
+                    const float4* site_n, float eps, float thr2, int* flag, hipStream_t s) {
+    const long nk = (long)nS * M;
+    if (nk <= 0) return;
+    hipLaunchKernelGGL(k_check_keys, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, b, keys, nS, M, S, site_n,
+                       eps, thr2, flag);
 }
 
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s) {

@@ -4,6 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "cvt.h"
@@ -12,6 +15,33 @@ namespace cvt {
 
 // Records the calling thread's error message (cvt_last_error) and returns code.
 __attribute__((visibility("hidden"))) int fail(int code, const char* fmt, ...);
+
+// Device memory arena of a serving loop: chunks taken from hipMalloc as it
+// grows, carved first-fit (256-byte aligned) and returned by free() with
+// neighbours coalesced inside a chunk.  Jobs created and retired continuously
+// then never call hipMalloc / hipFree in the steady state (hipFree may wait for
+// the whole device, stalling every other job's work).  Thread-safe.
+struct DevArena {
+    struct Chunk {
+        uintptr_t base;
+        size_t size;
+    };
+    std::mutex mu;
+    std::vector<Chunk> chunks;
+    std::map<uintptr_t, size_t> free_;  // start -> length (never spanning two chunks)
+    size_t grow_bytes = size_t(1) << 31;
+    size_t reserved = 0;  // bytes held from hipMalloc
+    void* alloc(size_t bytes);
+    void free(void* p, size_t bytes);
+    ~DevArena();
+};
+
+// The arena the calling thread allocates new jobs, LBVHs, batches and
+// temporaries from (nullptr: plain hipMalloc).  Objects remember the arena
+// they were created with and return their memory to it.
+__attribute__((visibility("hidden"))) DevArena*& current_arena();
+__attribute__((visibility("hidden"))) int dev_alloc(DevArena* a, void** p, size_t bytes);
+__attribute__((visibility("hidden"))) void dev_free(DevArena* a, void* p, size_t bytes);
 
 // Device temporaries handed out in call order.  reset() starts a new round of
 // get() calls that reuse the previous round's buffers (grown when too small);
@@ -24,17 +54,18 @@ struct TempPool {
     };
     std::vector<Slot> slots;
     size_t next = 0;
+    DevArena* arena = current_arena();
     void reset() { next = 0; }
     void* get(size_t bytes) {
         bytes = std::max<size_t>(bytes, 256);
         if (next == slots.size()) slots.emplace_back();
         Slot& s = slots[next++];
         if (s.cap < bytes) {
-            if (s.p) (void)hipFree(s.p);
+            if (s.p) dev_free(arena, s.p, s.cap);
+            const size_t want = std::max(bytes, s.cap + s.cap / 2);
             s.p = nullptr;
             s.cap = 0;
-            const size_t want = std::max(bytes, s.cap + s.cap / 2);
-            if (hipMalloc(&s.p, want) != hipSuccess) return nullptr;
+            if (dev_alloc(arena, &s.p, want)) return nullptr;
             s.cap = want;
         }
         return s.p;
@@ -46,7 +77,7 @@ struct TempPool {
     }
     ~TempPool() {
         for (Slot& s : slots)
-            if (s.p) (void)hipFree(s.p);
+            if (s.p) dev_free(arena, s.p, s.cap);
     }
 };
 

@@ -56,6 +56,9 @@ double secs(Clock::time_point t0) { return std::chrono::duration<double>(Clock::
 
 struct cvt_server {
     cvt_server_config cfg{};
+    // device memory of every job, LBVH, batch and check temporary of this
+    // server (declared before its users, so it is released after them)
+    cvt::DevArena arena;
     Clock::time_point t0;
     std::mutex mu;
     std::condition_variable cv;
@@ -104,6 +107,7 @@ void cvt_server::finish(Request* r, int status) {
 }
 
 void cvt_server::ingest_loop() {
+    cvt::current_arena() = &arena;
     for (;;) {
         Request* r = nullptr;
         {
@@ -158,6 +162,7 @@ void cvt_server::ingest_loop() {
 }
 
 void cvt_server::sched_loop() {
+    cvt::current_arena() = &arena;
     // One batch per (K, kproj) group; a group's batch is rebuilt only when its
     // membership changes (a job joined or retired).
     struct Group {
@@ -322,6 +327,7 @@ int cvt_server_create(const cvt_server_config* cfg, cvt_server** out) {
     if (s->cfg.epoch_iters < 1) s->cfg.epoch_iters = 1;
     s->t0 = Clock::now();
     s->check_pools.resize(std::max(1, std::min(s->cfg.check_threads, 64)));
+    for (cvt::TempPool& tp : s->check_pools) tp.arena = &s->arena;
     const int nth = std::max(1, std::min(s->cfg.ingest_threads, 64));
     for (int k = 0; k < nth; ++k) s->ingest_th.emplace_back([s] { s->ingest_loop(); });
     s->sched_th = std::thread([s] { s->sched_loop(); });
