@@ -267,9 +267,9 @@ int main(int argc, char** argv) {
         const Quality q = quality(Sf, Fn);
         double dH = 0;
         if (cvt_job_hausdorff(job, 1e-6, &dH, nullptr) != CVT_OK) return fail_msg("hausdorff");
-        // stage columns: the fused kernel computes KNN, frame, cell, projection
-        // and freeze together; its time is reported as centroids_ms, the site
-        // grid rebuild as knn_sites_ms
+        // stage columns: the FP32 pass (frame, cell and projection kernels,
+        // which do KNN, frame, cell, projection and freeze) is reported as
+        // centroids_ms, the site grid rebuild as knn_sites_ms
         const int blocked_same = std::max(0, st.count_low_disp - st.count_both);
         const int blocked_low = std::max(0, st.count_same - st.count_both);
         std::ofstream out(eval_csv, std::ios::app);
