@@ -172,8 +172,11 @@ int cvt_job_hausdorff(cvt_job* job, double eps, double* h_out, int* nF_out);
 /* Convergence test for the same one-sided distance: *within = 1 iff every
  * probe lies within tol of the input mesh (i.e. cvt_job_hausdorff <= tol), by
  * a bounded LBVH query that stops at the first triangle within tol.  h_out
- * (nullable) receives the exact distance when *within is 1, else -1.  Replaces
- * the dH <= tolerance decision of the reference's dump loop (main.cu:1034). */
+ * (nullable) receives the exact distance when *within is 1, else -1.  nF_out
+ * (nullable): the remeshed triangle count, or -1 when the job was found beyond
+ * tolerance straight from the remesh cells' candidate triangles, without
+ * building the surface (jobs under 2^21 sites).  Replaces the dH <= tolerance decision of the
+ * reference's dump loop (main.cu:1034). */
 int cvt_job_hausdorff_within(cvt_job* job, double eps, double tol, int* within, double* h_out, int* nF_out);
 
 /* Squared distance from each query point (n*3, host) to the job's input mesh,
