@@ -134,6 +134,51 @@ __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, S
     return ok;
 }
 
+// KNN rows of K ints (K even, so a row starts 8-byte aligned, and 16-byte
+// aligned when K % 4 == 0) moved with vector loads and stores.
+template <int K>
+__device__ __forceinline__ void load_row(const CVT_G int* __restrict__ row, int (&v)[K]) {
+    static_assert(K % 2 == 0, "rows are moved in pairs");
+    if constexpr (K % 4 == 0) {
+        const CVT_G int4* r4 = reinterpret_cast<const CVT_G int4*>(row);
+#pragma unroll
+        for (int j = 0; j < K / 4; ++j) {
+            const int4 x = r4[j];
+            v[4 * j] = x.x; v[4 * j + 1] = x.y; v[4 * j + 2] = x.z; v[4 * j + 3] = x.w;
+        }
+    } else {
+        const CVT_G int2* r2 = reinterpret_cast<const CVT_G int2*>(row);
+#pragma unroll
+        for (int j = 0; j < K / 2; ++j) {
+            const int2 x = r2[j];
+            v[2 * j] = x.x; v[2 * j + 1] = x.y;
+        }
+    }
+}
+
+// Writes the row; with compare, first reads the old one and returns whether
+// they are equal.
+template <int K>
+__device__ __forceinline__ bool store_row(CVT_G int* __restrict__ row, const int (&v)[K], bool compare) {
+    bool same = true;
+    if (compare) {
+        int old[K];
+        load_row<K>(row, old);
+#pragma unroll
+        for (int k = 0; k < K; ++k) same &= old[k] == v[k];
+    }
+    if constexpr (K % 4 == 0) {
+        CVT_G int4* r4 = reinterpret_cast<CVT_G int4*>(row);
+#pragma unroll
+        for (int j = 0; j < K / 4; ++j) r4[j] = make_int4(v[4 * j], v[4 * j + 1], v[4 * j + 2], v[4 * j + 3]);
+    } else {
+        CVT_G int2* r2 = reinterpret_cast<CVT_G int2*>(row);
+#pragma unroll
+        for (int j = 0; j < K / 2; ++j) r2[j] = make_int2(v[2 * j], v[2 * j + 1]);
+    }
+    return same;
+}
+
 // ---- 2. site KNN (active) or pinned list (frozen), consumed straight into the
 //         dual points p_j = (d.u, d.v) / (|d|^2 / 2) of the cell (2dClip.cuh:112-126);
 // ---- 3. tangent-plane cell (dual hull) and centroid (2dClip.cuh:128-134).
@@ -145,7 +190,7 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
                                            int carried, int* lds, CellRes& r) {
     const float4 q = c.q;
     const float inf = __builtin_huge_valf();
-    const float4* __restrict__ S = a.S01[par];
+    const CVT_G float4* __restrict__ S = a.S01[par];
     r.c3 = make_float3(q.x, q.y, q.z);
     r.area = 0.f;
     r.moment = 0.f;
@@ -179,30 +224,41 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
         if (!eval_only) a.rad2[c.i] = lk;
     }
     const float3 u = c.u, w = c.w;
-    int* row = a.knn + (size_t)c.i * K;
-    bool same = true;
+    CVT_G int* __restrict__ row = a.knn + (size_t)c.i * K;
+    // the K neighbours: the proven row (sites by id) or the new list (grid
+    // slots); all gathers are issued before any row store
+    int idx[K];
+    if (use_row) {
+        load_row<K>(row, idx);
+    } else {
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        float4 pj;
-        if (!use_row) {
-            const int sl = slots[k] >= 0 ? slots[k] : t;
-            pj = a.sg.pts[sl];
-            const int id = __float_as_int(pj.w);
-            if (!eval_only) {
-                same = same && (carried >= 0 || row[k] == id);
-                row[k] = id;
-            }
-        } else {
-            pj = S[row[k]];
-        }
-        const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
-        const float aa = dx * u.x + dy * u.y + dz * u.z;
-        const float bb = dx * w.x + dy * w.y + dz * w.z;
-        const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
-        const float ic = cc > 0.f ? fast_div(1.0f, cc) : 0.f;
-        px[k] = aa * ic;
-        py[k] = bb * ic;
+        for (int k = 0; k < K; ++k) idx[k] = slots[k] >= 0 ? slots[k] : t;
     }
+    const CVT_G float4* __restrict__ src = use_row ? S : a.sg.pts;
+    constexpr int kG = 8;  // gathers in flight per batch
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += kG) {
+        float4 pb[kG];
+#pragma unroll
+        for (int b = 0; b < kG; ++b)
+            if (k0 + b < K) pb[b] = src[idx[k0 + b]];
+#pragma unroll
+        for (int b = 0; b < kG; ++b) {
+            const int k = k0 + b;
+            if (k >= K) break;
+            const float4 pj = pb[b];
+            idx[k] = __float_as_int(pj.w);  // the point's id (new lists)
+            const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
+            const float aa = dx * u.x + dy * u.y + dz * u.z;
+            const float bb = dx * w.x + dy * w.y + dz * w.z;
+            const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
+            const float ic = cc > 0.f ? fast_div(1.0f, cc) : 0.f;
+            px[k] = aa * ic;
+            py[k] = bb * ic;
+        }
+    }
+    bool same = true;
+    if (!use_row && !eval_only) same = store_row<K>(row, idx, carried < 0);
     r.same = carried >= 0 ? carried != 0 : same;
     const float iR = 1.0f / a.R;
     px[K] = iR;   py[K] = 0.f;
