@@ -376,7 +376,7 @@ int alloc_grid(cvt_job* j, GridBufs& b, int n, int ncell) {
     if ((rc = j->alloc(&b.bsum, scan_partials(T2)))) return rc;
     if ((rc = j->alloc(&b.bucket, n))) return rc;
     if ((rc = j->alloc(&b.rank, n))) return rc;
-    if ((rc = j->alloc(&b.pts, n))) return rc;
+    if ((rc = j->alloc(&b.pts, (long)n + kPtsPad))) return rc;  // collects may read past the end
     return 0;
 }
 

@@ -306,10 +306,23 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const CVT_G int* __res
     return T.s[L - 1] >= 0 || tau == __builtin_huge_valf();
 }
 
+// Points past the end of a cell-sorted store that a collect may read (and
+// ignore): stores scanned by grid_collect_topl are allocated this much longer.
+constexpr int kPtsPad = 4;
+
+// LDS ints per thread of grid_collect_topl: CAP + 1 candidate rows, then the
+// 9 + 9 bounds of the block's non-empty rows.
+template <int CAP>
+constexpr int collect_lds_rows() { return CAP + 1 + 18; }
+
 // Collect-then-select variant of grid_topl for the FP32 pass: every point of
 // the 3x3x3 block closer than tau is appended to this thread's LDS column
 // (lds[k * 256], k <= CAP; conflict-free since the stride is a multiple of the
-// bank count), so divergent lanes pay a short append per candidate.  The L
+// bank count), so divergent lanes pay a short append per candidate.  The
+// block's non-empty rows of cells are walked as one stream (their bounds kept
+// in the same LDS column), four points per round trip from one address, so a
+// wave runs as many rounds as its longest stream rather than the sum over
+// rows of the longest row.  The L
 // nearest of the collected points are then selected into a sorted list of
 // 32-bit keys: the FP32 distance bits with the low 6 bits replaced by the
 // candidate number k (non-negative floats order like their bits, and k makes
@@ -342,24 +355,47 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
         r0[k] = keep ? start[row + x0] : 0;
         r1[k] = keep ? start[row + x1 + 1] : 0;
     }
-    // Branch-free append, four points per round trip: every scanned point is
-    // written at row min(cnt, CAP) (row CAP is a spill row) and cnt advances
-    // only for a point within tau, so rows below cnt hold the candidates.
-    int cnt = 0;
+    // the non-empty rows, compacted: bounds of row k at lds[(CAP + 1 + k) * 256]
+    // and lds[(CAP + 10 + k) * 256]
+    int* rlo = lds + (CAP + 1) * 256;
+    int* rhi = lds + (CAP + 10) * 256;
+    int nr = 0;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
-        const int e = r1[k];
-        for (int s = r0[k]; s < e; s += 4) {
-            float4 p[4];
+        rlo[nr * 256] = r0[k];
+        rhi[nr * 256] = r1[k];
+        nr += r1[k] > r0[k] ? 1 : 0;
+    }
+    // Branch-free append: every scanned point is written at row min(cnt, CAP)
+    // (row CAP is a spill row) and cnt advances only for a point within tau, so
+    // rows below cnt hold the candidates.  The four loads of a round may run
+    // up to 3 points past the row's end (into the next cells, or the store's
+    // padding); those points are not taken.
+    int cnt = 0;
+    int s = nr > 0 ? rlo[0] : 0, e = nr > 0 ? rhi[0] : 0;
+    int ns = rlo[256], ne = rhi[256];  // next row's bounds (meaningful while kn < nr)
+    int kn = 1;
+    while (s < e) {
+        const CVT_G float4* __restrict__ ps = pts + s;
+        float4 p[4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) p[b] = pts[min(s + b, e - 1)];
+        for (int b = 0; b < 4; ++b) p[b] = ps[b];
+        const int lim = e - s;  // >= 1
 #pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int sb = s + b;
-                const bool acc = (sb < e) & (d2f(qx, qy, qz, p[b]) < tau) & (sb != self_slot);
-                lds[min(cnt, CAP) * 256] = sb;
-                cnt += acc ? 1 : 0;
-            }
+        for (int b = 0; b < 4; ++b) {
+            const bool acc = (b == 0 || b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (s + b != self_slot);
+            lds[min(cnt, CAP) * 256] = s + b;
+            cnt += acc ? 1 : 0;
+        }
+        s += 4;
+        if (s >= e) {  // next row (read one switch ahead), then read the one after
+            const bool more = kn < nr;
+            s = more ? ns : 0;
+            e = more ? ne : 0;
+            ++kn;
+            const int kc = min(kn, 8);
+            ns = rlo[kc * 256];
+            ne = rhi[kc * 256];
         }
     }
     if (cnt < L || cnt > CAP) return false;

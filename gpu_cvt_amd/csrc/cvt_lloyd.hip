@@ -21,7 +21,7 @@ static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 template <int K>
 __global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
                                                int eval_only) {
-    __shared__ int lds[(kCollectCap + 1) * 256];  // collect rows + a spill row
+    __shared__ int lds[collect_lds_rows<kCollectCap>() * 256];  // collect rows, a spill row, row bounds
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
@@ -191,7 +191,7 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
                                                       unsigned long long* __restrict__ keys,
                                                       float4* __restrict__ site_n) {
     constexpr int M = K + 4;
-    __shared__ int lds[(kCollectCap + 1) * 256];
+    __shared__ int lds[collect_lds_rows<kCollectCap>() * 256];
     const StepArgs& a = desc->a;
     const int t = blockIdx.x * 256 + (int)threadIdx.x;
     if (t >= a.nS) return;

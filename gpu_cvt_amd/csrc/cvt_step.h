@@ -292,99 +292,109 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 
 // ---- 4. centroid -> mesh kproj-NN (active) or pinned list (frozen); 5. projection;
 // ---- 6. the new site and the freeze state machine (freezetest.cuh:270-327, :451-466).
-// Returns r.done = false (r.row_written, r.same set for the precise pass) when
-// the FP32 kproj search could not be proven.
-template <int KP, bool PRECISE>
-__device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
-                                              const CellRes& cr, SiteFlags& r) {
+// In three pieces: pool_lookup (the list from the vertex pool cache, when it
+// proves it), vertex_search (the grid search otherwise) and project_finish
+// (the faces, the new site and the freeze state), run in one thread by
+// phase_project.
+
+// Vertex pool cache: vcache = (centroid at the last kproj search, the 8th-
+// nearest distance there), vpool = the 7 nearest vertices there and the 9th-
+// nearest distance (float bits).  After a move by d every vertex outside the
+// pool is at least (8th - d) away, so when the 5th nearest pooled vertex (exact
+// FP64 keys, index ties) is closer, the pool's first 5 are the list (returns
+// true); otherwise the 9th distance plus d seeds the search in tauv (triangle
+// inequality; +inf without a cache).
+template <int KP>
+__device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const float3& c3, bool use_cache, int (&vl)[KP],
+                                            float& tauv) {
+    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
     const float inf = __builtin_huge_valf();
+    tauv = inf;
+    if (!use_cache) return false;
+    const float4 vc = a.vcache[i];
+    if (!(vc.w < inf)) return false;
+    const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
+                           (c3.z - vc.z) * (c3.z - vc.z));
+    const int4 p0 = a.vpool[2 * (size_t)i], p1 = a.vpool[2 * (size_t)i + 1];
+    int id[KP + 2] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
+    double key[KP + 2];
+#pragma unroll
+    for (int k = 0; k < KP + 2; ++k) {
+        const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
+        key[k] = id[k] >= 0 ? key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
+        id[k] = id[k] >= 0 ? id[k] : 0x7fffffff;
+    }
+#pragma unroll
+    for (int p0i = 1; p0i < KP + 2; ++p0i) {  // insertion sort by (key, id)
+#pragma unroll
+        for (int k = p0i; k >= 1; --k) {
+            const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && id[k] < id[k - 1]);
+            const double tk = key[k];
+            const int tv = id[k];
+            key[k] = sw ? key[k - 1] : key[k];
+            id[k] = sw ? id[k - 1] : id[k];
+            key[k - 1] = sw ? tk : key[k - 1];
+            id[k - 1] = sw ? tv : id[k - 1];
+        }
+    }
+    if (key[KP - 1] < __builtin_huge_val() &&
+        sqrtf((float)key[KP - 1]) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < vc.w - dc) {
+#pragma unroll
+        for (int k = 0; k < KP; ++k) vl[k] = id[k];
+        return true;
+    }
+    const float rr = (__int_as_float(p1.w) + dc) * (1.f + 1e-4f);
+    tauv = rr * rr;
+    return false;
+}
+
+// The kproj nearest mesh vertices of c3 by the vertex grid (seed tauv), and
+// the pool cache refreshed.  Returns false when the FP32 search could not be
+// proven (never for PRECISE).
+template <int KP, bool PRECISE>
+__device__ __forceinline__ bool vertex_search(const StepArgs& a, int i, const float3& c3, float tauv, int (&vl)[KP]) {
+    const float inf = __builtin_huge_valf();
+    int sl[KP + 2];
+    float lv, dv[KP + 4];
+    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, tauv, sl, lv,
+                                              nullptr, 0.f, dv, nullptr, &a.m.vb);
+    if (!PRECISE && kk < 0) return false;
+    int id[KP + 2];
+#pragma unroll
+    for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
+#pragma unroll
+    for (int k = 0; k < KP; ++k) vl[k] = id[k];
+    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 2] < inf ? sqrtf(dv[KP + 2]) : inf);
+    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
+    a.vpool[2 * (size_t)i + 1] =
+        make_int4(id[4], id[5], id[6], __float_as_int(dv[KP + 3] < inf ? sqrtf(dv[KP + 3]) : inf));
+    return true;
+}
+
+// Whether the site needs a kproj list this step (active, valid, finite centroid).
+__device__ __forceinline__ bool needs_list(const SiteCtx& c, const CellRes& cr) {
+    return c.valid && !c.fz && finite3(cr.c3.x, cr.c3.y, cr.c3.z);
+}
+
+// Steps 5 and 6 given the list vl of an active site (ignored for a frozen one,
+// whose pinned list is read back).
+template <int KP>
+__device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
+                                               const CellRes& cr, int (&vl)[KP], SiteFlags& r) {
     const int has_prev = eval_only ? 0 : *a.has_prev;
-    float4* __restrict__ Snew = a.S01[par ^ 1];
+    CVT_G float4* __restrict__ Snew = a.S01[par ^ 1];
     const float4 q = c.q;
     const float3 c3 = cr.c3;
     const int i = c.i;
     r.kind = cr.kind;
     float3 sp = c3;
     if (c.valid && !eval_only) {
-        int vl[KP];
-#pragma unroll
-        for (int k = 0; k < KP; ++k) vl[k] = -1;
         if (!c.fz) {
-            if (finite3(c3.x, c3.y, c3.z)) {
-                // pool cache: vcache = (centroid at the last kproj search, the 8th-
-                // nearest distance there), vpool = the 7 nearest vertices there and
-                // the 9th-nearest distance (float bits).  After a move by d every
-                // vertex outside the pool is at least (8th - d) away, so when the 5th
-                // nearest pooled vertex (exact FP64 keys, index ties) is closer, the
-                // pool's first 5 are the list; otherwise the 9th distance plus d
-                // seeds the search (triangle inequality).
-                static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
-                float tauv = inf;
-                bool have = false;
-                if (c.use_cache) {
-                    const float4 vc = a.vcache[i];
-                    if (vc.w < inf) {
-                        const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
-                                               (c3.z - vc.z) * (c3.z - vc.z));
-                        const int4 p0 = a.vpool[2 * (size_t)i], p1 = a.vpool[2 * (size_t)i + 1];
-                        int id[KP + 2] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
-                        double key[KP + 2];
-#pragma unroll
-                        for (int k = 0; k < KP + 2; ++k) {
-                            const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
-                            key[k] = id[k] >= 0 ? key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
-                            id[k] = id[k] >= 0 ? id[k] : 0x7fffffff;
-                        }
-#pragma unroll
-                        for (int p0i = 1; p0i < KP + 2; ++p0i) {  // insertion sort by (key, id)
-#pragma unroll
-                            for (int k = p0i; k >= 1; --k) {
-                                const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && id[k] < id[k - 1]);
-                                const double tk = key[k];
-                                const int tv = id[k];
-                                key[k] = sw ? key[k - 1] : key[k];
-                                id[k] = sw ? id[k - 1] : id[k];
-                                key[k - 1] = sw ? tk : key[k - 1];
-                                id[k - 1] = sw ? tv : id[k - 1];
-                            }
-                        }
-                        if (key[KP - 1] < __builtin_huge_val() &&
-                            sqrtf((float)key[KP - 1]) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < vc.w - dc) {
-#pragma unroll
-                            for (int k = 0; k < KP; ++k) vl[k] = id[k];
-                            have = true;
-                        } else {
-                            const float rr = (__int_as_float(p1.w) + dc) * (1.f + 1e-4f);
-                            tauv = rr * rr;
-                        }
-                    }
-                }
-                if (!have) {
-                    int sl[KP + 2];
-                    float lv, dv[KP + 4];
-                    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z,
-                                                              -1, tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
-                    if (!PRECISE && kk < 0) {
-                        r.same = cr.same;
-                        r.row_written = true;
-                        return;
-                    }
-                    int id[KP + 2];
-#pragma unroll
-                    for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
-#pragma unroll
-                    for (int k = 0; k < KP; ++k) vl[k] = id[k];
-                    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 2] < inf ? sqrtf(dv[KP + 2]) : inf);
-                    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
-                    a.vpool[2 * (size_t)i + 1] =
-                        make_int4(id[4], id[5], id[6], __float_as_int(dv[KP + 3] < inf ? sqrtf(dv[KP + 3]) : inf));
-                }
-            }
-            int* row = a.knnV + (size_t)i * KP;
+            CVT_G int* row = a.knnV + (size_t)i * KP;
 #pragma unroll
             for (int k = 0; k < KP; ++k) row[k] = vl[k];
         } else {
-            const int* row = a.knnV + (size_t)i * KP;
+            const CVT_G int* row = a.knnV + (size_t)i * KP;
 #pragma unroll
             for (int k = 0; k < KP; ++k) vl[k] = row[k];
         }
@@ -467,6 +477,28 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
     }
     r.cell_fail = c.valid && !cr.cell_ok;
     r.nonfinite = c.live && !finite3(sp.x, sp.y, sp.z);
+}
+
+// The three pieces in one thread.  Returns r.done = false (r.row_written,
+// r.same set for the precise pass) when the FP32 kproj search could not be
+// proven.
+template <int KP, bool PRECISE>
+__device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
+                                              const CellRes& cr, SiteFlags& r) {
+    int vl[KP];
+#pragma unroll
+    for (int k = 0; k < KP; ++k) vl[k] = -1;
+    if (!eval_only && needs_list(c, cr)) {
+        float tauv;
+        if (!pool_lookup<KP>(a, c.i, cr.c3, c.use_cache, vl, tauv) &&
+            !vertex_search<KP, PRECISE>(a, c.i, cr.c3, tauv, vl)) {
+            r.kind = cr.kind;
+            r.same = cr.same;
+            r.row_written = true;
+            return;
+        }
+    }
+    project_finish<KP>(a, par, eval_only, c, cr, vl, r);
 }
 
 // The whole step of one site in one thread (the precise pass).
