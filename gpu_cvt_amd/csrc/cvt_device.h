@@ -310,19 +310,21 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const CVT_G int* __res
 // ignore): stores scanned by grid_collect_topl are allocated this much longer.
 constexpr int kPtsPad = 4;
 
-// LDS ints per thread of grid_collect_topl: CAP + 1 candidate rows, then the
-// 9 + 9 bounds of the block's non-empty rows.
+// LDS of grid_collect_topl for a block of 256 threads, in ints: per thread,
+// the first and end slots of the block's 9 rows of cells (int columns), then
+// CAP + 1 candidate entries of 16 bits (ushort columns).
 template <int CAP>
-constexpr int collect_lds_rows() { return CAP + 1 + 18; }
+constexpr int collect_lds_ints() { return 18 * 256 + (CAP + 1) * 128; }
 
-// Collect-then-select variant of grid_topl for the FP32 pass: every point of
-// the 3x3x3 block closer than tau is appended to this thread's LDS column
-// (lds[k * 256], k <= CAP; conflict-free since the stride is a multiple of the
-// bank count), so divergent lanes pay a short append per candidate.  The
-// block's non-empty rows of cells are walked as one stream (their bounds kept
-// in the same LDS column), four points per round trip from one address, so a
-// wave runs as many rounds as its longest stream rather than the sum over
-// rows of the longest row.  The L
+// Collect-then-select variant of grid_topl for the FP32 pass (blocks of 256
+// threads; lds: the block's collect_lds_ints<CAP>() array): every point of
+// the 3x3x3 block closer than tau is appended to this thread's LDS column as
+// a 16-bit entry (its row, its offset in the row), so divergent lanes pay a
+// short append per candidate; the columns have a stride of 256 entries, so a
+// wave's appends touch distinct banks.  The block's non-empty rows of cells
+// are walked as one stream (their bounds kept in LDS too), four points per
+// round trip from one address, so a wave runs as many rounds as its longest
+// stream rather than the sum over rows of the longest row.  The L
 // nearest of the collected points are then selected into a sorted list of
 // 32-bit keys: the FP32 distance bits with the low 6 bits replaced by the
 // candidate number k (non-negative floats order like their bits, and k makes
@@ -331,8 +333,8 @@ constexpr int collect_lds_rows() { return CAP + 1 + 18; }
 // bound within a factor 1 + 2^-17 of the FP32 distance; the caller orders
 // with kRelKey) and the slot of candidate k; worst_up receives an upper bound
 // of the FP32 L-th distance.  Returns false -- the caller then runs grid_topl
-// -- when fewer than L or more than CAP points lie within tau, or the block
-// is not provably large enough.
+// -- when fewer than L or more than CAP points lie within tau, a row holds
+// more than 4096 points, or the block is not provably large enough.
 template <int L, int CAP>
 __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G int* __restrict__ start,
                                                   const CVT_G float4* __restrict__ pts, float qx, float qy, float qz,
@@ -355,17 +357,21 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
         r0[k] = keep ? start[row + x0] : 0;
         r1[k] = keep ? start[row + x1 + 1] : 0;
     }
-    // the non-empty rows, compacted: bounds of row k at lds[(CAP + 1 + k) * 256]
-    // and lds[(CAP + 10 + k) * 256]
-    int* rlo = lds + (CAP + 1) * 256;
-    int* rhi = lds + (CAP + 10) * 256;
+    // the non-empty rows, compacted: bounds of row k at rlo[k * 256], rhi[k * 256]
+    const int tid = (int)threadIdx.x;
+    int* rlo = lds + tid;
+    int* rhi = lds + 9 * 256 + tid;
+    unsigned short* cand = reinterpret_cast<unsigned short*>(lds + 18 * 256) + tid;
     int nr = 0;
+    bool longrow = false;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         rlo[nr * 256] = r0[k];
         rhi[nr * 256] = r1[k];
         nr += r1[k] > r0[k] ? 1 : 0;
+        longrow |= r1[k] - r0[k] > 4096;
     }
+    if (longrow) return false;
     // Branch-free append: every scanned point is written at row min(cnt, CAP)
     // (row CAP is a spill row) and cnt advances only for a point within tau, so
     // rows below cnt hold the candidates.  The four loads of a round may run
@@ -375,6 +381,7 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
     int s = nr > 0 ? rlo[0] : 0, e = nr > 0 ? rhi[0] : 0;
     int ns = rlo[256], ne = rhi[256];  // next row's bounds (meaningful while kn < nr)
     int kn = 1;
+    int tag = -s;  // (row << 12) - first slot of the row: entry = tag + slot
     while (s < e) {
         const CVT_G float4* __restrict__ ps = pts + s;
         float4 p[4];
@@ -384,7 +391,7 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
             const bool acc = (b == 0 || b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (s + b != self_slot);
-            lds[min(cnt, CAP) * 256] = s + b;
+            cand[min(cnt, CAP) * 256] = (unsigned short)(tag + s + b);
             cnt += acc ? 1 : 0;
         }
         s += 4;
@@ -392,6 +399,7 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
             const bool more = kn < nr;
             s = more ? ns : 0;
             e = more ? ne : 0;
+            tag = (kn << 12) - s;
             ++kn;
             const int kc = min(kn, 8);
             ns = rlo[kc * 256];
@@ -409,7 +417,9 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
         int sl[kB];
         float4 p[kB];
 #pragma unroll
-        for (int b = 0; b < kB; ++b) sl[b] = lds[min(k + b, cnt - 1) * 256];
+        for (int b = 0; b < kB; ++b) sl[b] = cand[min(k + b, cnt - 1) * 256];
+#pragma unroll
+        for (int b = 0; b < kB; ++b) sl[b] = rlo[(sl[b] >> 12) * 256] + (sl[b] & 4095);
 #pragma unroll
         for (int b = 0; b < kB; ++b) p[b] = pts[sl[b]];
 #pragma unroll
@@ -424,7 +434,8 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
 #pragma unroll
     for (int t = 0; t < L; ++t) {
         T.d[t] = __uint_as_float(key[t] & ~63u);
-        T.s[t] = lds[(key[t] & 63u) * 256];
+        const int en = cand[(key[t] & 63u) * 256];
+        T.s[t] = rlo[(en >> 12) * 256] + (en & 4095);
     }
     worst_up = __uint_as_float(key[L - 1] | 63u);
     float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, 1);

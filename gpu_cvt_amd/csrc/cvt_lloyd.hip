@@ -17,11 +17,17 @@ namespace cvt {
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 // ------------------------------------------------------------ Lloyd step (K-dependent kernels)
+// Waves per SIMD the cell kernel is compiled for: 3 (at most 168 VGPRs and
+// three 43.5 KB blocks per CU) up to K = 24, where the kernel spends about half
+// its time waiting on memory; 2 above, where the dual hull needs more registers.
+template <int K>
+constexpr int kCellWaves() { return K <= 24 ? 3 : 2; }
+
 // FP32 pass, phase 2.
 template <int K>
-__global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
-                                               int eval_only) {
-    __shared__ int lds[collect_lds_rows<kCollectCap>() * 256];  // collect rows, a spill row, row bounds
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kCellWaves<K>(), kCellWaves<K>())))
+void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt, int eval_only) {
+    __shared__ int lds[collect_lds_ints<kCollectCap>()];  // row bounds, candidates and a spill entry
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
@@ -36,7 +42,7 @@ __global__ __launch_bounds__(256) void kb_cell(const JobDesc* __restrict__ descs
     }
     set_frame(a, c);
     CellRes cr;
-    if (!phase_cell<K, false>(a, par, eval_only != 0, t, c, -1, lds + threadIdx.x, cr)) {
+    if (!phase_cell<K, false>(a, par, eval_only != 0, t, c, -1, lds, cr)) {
         a.cent[c.i] = make_float4(0.f, 0.f, 0.f, 0.f);
         defer_site(a, t, false, false, false);
         return;
@@ -191,7 +197,7 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
                                                       unsigned long long* __restrict__ keys,
                                                       float4* __restrict__ site_n) {
     constexpr int M = K + 4;
-    __shared__ int lds[collect_lds_rows<kCollectCap>() * 256];
+    __shared__ int lds[collect_lds_ints<kCollectCap>()];
     const StepArgs& a = desc->a;
     const int t = blockIdx.x * 256 + (int)threadIdx.x;
     if (t >= a.nS) return;
@@ -231,7 +237,7 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
     int slots[K];
     float lk;
     knn_exact<K, true, kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t, tau, slots, lk,
-                                    lds + threadIdx.x, ctau, nullptr, a.sg.cid);
+                                    lds, ctau, nullptr, a.sg.cid);
     int* row = knn_tmp + (size_t)i * K;
     float px[M], py[M];
 #pragma unroll
