@@ -128,13 +128,17 @@ namespace {
 static int side_create(SideStream& sd) {
     if (hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sd.join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&sd.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sd.ffork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&sd.fjoin, hipEventDisableTiming) != hipSuccess)
         return -1;
     return 0;
 }
 static void side_destroy(SideStream& sd) {
     if (sd.fork) (void)hipEventDestroy(sd.fork);
     if (sd.join) (void)hipEventDestroy(sd.join);
+    if (sd.ffork) (void)hipEventDestroy(sd.ffork);
+    if (sd.fjoin) (void)hipEventDestroy(sd.fjoin);
     if (sd.s) (void)hipStreamDestroy(sd.s);
     sd = SideStream{};
 }

@@ -164,7 +164,8 @@ int scan_tile();
 // kernel (it touches only those sites, which the projection kernel skips).
 struct SideStream {
     hipStream_t s = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;    // precise pass beside the projection
+    hipEvent_t ffork = nullptr, fjoin = nullptr;  // frame phase beside the grid rebuild
 };
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s,
                             const SideStream* side = nullptr);
@@ -173,7 +174,9 @@ void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s);
 // Pieces of an iteration, for per-stage timing.
 void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s);
 // which: 1 = the FP32 pass, 2 = the FP64-resolving pass over its deferred sites, 3 = both
-// (with a side stream, the first deferred list overlaps the projection kernel)
+// (with a side stream, the first deferred list overlaps the projection kernel);
+// | kLloydFrameDone: the frame phase of the FP32 pass was already launched
+constexpr int kLloydFrameDone = 4;
 void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3,
                         const SideStream* side = nullptr);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);

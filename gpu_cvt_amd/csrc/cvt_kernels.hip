@@ -190,20 +190,23 @@ __global__ void k_vertex_frames(MeshDev m, float4* __restrict__ FU, float4* __re
 }
 
 // ------------------------------------------------------ FP32 Lloyd pass, K-independent phases
-// FP32 pass, phase 1 over all sites of all jobs of a batch (cell-sorted order
-// within a job; the block table is walked XCD-contiguously).
+// FP32 pass, phase 1 over all sites of all jobs of a batch, in internal site
+// order (cell-sorted when the job was made, so still spatially coherent; the
+// block table is walked XCD-contiguously).  It reads the sites themselves,
+// not the site grid, so it can run beside the grid rebuild; a site it cannot
+// prove (fnn = kFrameDeferred) is put on the precise pass's list by kb_cell,
+// which knows its grid slot.
 template <int KP>
 __global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
                                                 int eval_only) {
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
-    const int t = br.blk * 256 + (int)threadIdx.x;
-    if (t >= a.nS) return;
-    SiteCtx c = site_ctx(a, eval_only != 0, t, true);
+    const int i = br.blk * 256 + (int)threadIdx.x;
+    if (i >= a.nS) return;
+    SiteCtx c = site_ctx_by_index(a, eval_only != 0, i);
     const bool ok = phase_frame<KP, false>(a, eval_only != 0, c);
-    a.fnn[c.i] = ok ? c.nn : kFrameDeferred;
-    if (!ok) defer_site(a, t, false, false, false);
+    a.fnn[i] = ok ? c.nn : kFrameDeferred;
 }
 
 // FP32 pass, phase 3 (not run for an evaluation-only step).
@@ -586,14 +589,18 @@ bool lloyd_supported(int K, int kproj) {
     }
 }
 
-void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
-    if (!eval_only) hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+static void launch_grid_rebuild(const BatchTables& bt, hipStream_t s) {
     hipLaunchKernelGGL(kb_fill_count, dim3(bt.n_cell_blocks), dim3(256), 0, s, bt.descs, bt.cell_blocks);
     hipLaunchKernelGGL(kb_grid_count, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
     hipLaunchKernelGGL(kb_scan_reduce, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
     hipLaunchKernelGGL(kb_scan_bsum, dim3(bt.njobs), dim3(256), 0, s, bt.descs);
     hipLaunchKernelGGL(kb_scan_down, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
     hipLaunchKernelGGL(kb_grid_scatter, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
+}
+
+void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
+    if (!eval_only) hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+    launch_grid_rebuild(bt, s);
 }
 
 static void launch_lloyd_part(const BatchTables& bt, int K, int eval_only, int which, hipStream_t s) {
@@ -611,7 +618,7 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
     (void)kproj;
     if (which & 1) {
         const dim3 g(bt.n_site_blocks), b(256);
-        hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
+        if (!(which & kLloydFrameDone)) hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
         launch_lloyd_part(bt, K, eval_only, 0, s);
         const bool fork = side && side->s && (which & 2);
         if (fork) {
@@ -638,8 +645,20 @@ void launch_batch_end(const BatchTables& bt, hipStream_t s) {
 }
 
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s, const SideStream* side) {
-    launch_batch_grid(bt, 0, s);
-    launch_batch_lloyd(bt, K, kproj, 0, s, 3, side);
+    if (side && side->s) {
+        // the frame phase (sites by index, static vertex grid) beside the site grid rebuild
+        hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+        (void)hipEventRecord(side->ffork, s);
+        (void)hipStreamWaitEvent(side->s, side->ffork, 0);
+        hipLaunchKernelGGL((kb_frame<5>), dim3(bt.n_site_blocks), dim3(256), 0, side->s, bt.descs, bt.site_blocks, 0);
+        (void)hipEventRecord(side->fjoin, side->s);
+        launch_grid_rebuild(bt, s);
+        (void)hipStreamWaitEvent(s, side->fjoin, 0);
+        launch_batch_lloyd(bt, K, kproj, 0, s, 3 | kLloydFrameDone, side);
+    } else {
+        launch_batch_grid(bt, 0, s);
+        launch_batch_lloyd(bt, K, kproj, 0, s, 3, side);
+    }
     launch_batch_end(bt, s);
 }
 

@@ -36,8 +36,9 @@ void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
     if (t >= a.nS) return;
     SiteCtx c = site_ctx(a, eval_only != 0, t, true);
     c.nn = a.fnn[c.i];
-    if (c.nn == kFrameDeferred) {  // on the precise pass's list already
+    if (c.nn == kFrameDeferred) {  // the frame phase could not prove it: the precise pass takes it
         a.cent[c.i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        defer_site(a, t, false, false, false);
         return;
     }
     set_frame(a, c);

@@ -70,6 +70,21 @@ __device__ __forceinline__ SiteCtx site_ctx(const StepArgs& a, bool eval_only, i
     return c;
 }
 
+// The same context from the site buffer by internal index (no grid slot).
+__device__ __forceinline__ SiteCtx site_ctx_by_index(const StepArgs& a, bool eval_only, int i) {
+    SiteCtx c;
+    c.live = true;
+    c.i = i;
+    c.q = a.S01[*a.parity][i];
+    c.valid = finite3(c.q.x, c.q.y, c.q.z);
+    c.fz = c.valid && !eval_only && a.frozen[i] != 0;
+    c.use_cache = !eval_only && *a.seed_ok;
+    c.nn = -1;
+    c.u = make_float3(1.f, 0.f, 0.f);
+    c.w = make_float3(0.f, 1.f, 0.f);
+    return c;
+}
+
 __device__ __forceinline__ void set_frame(const StepArgs& a, SiteCtx& c) {
     if (c.nn >= 0 && c.nn < a.m.nV) {
         const float4 fu = a.m.FU[c.nn], fw = a.m.FW[c.nn];
