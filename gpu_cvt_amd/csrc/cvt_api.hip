@@ -177,6 +177,7 @@ struct cvt_job {
     float4 *fcache = nullptr, *vcache = nullptr, *gaps = nullptr;  // per-site vertex-search caches
     int4* vpool = nullptr;  // candidate pool of the centroid vertex search
     int* fnn = nullptr;       // FP32-pass hand-offs: frame vertex per site,
+    int* slot_of = nullptr;   // each site's slot in the current site grid
     float4* cent = nullptr;   // centroid and cell flags per site
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
@@ -425,6 +426,7 @@ JobDesc make_desc(cvt_job* j) {
     a.gaps = to_g(j->gaps);
     a.vpool = to_g(j->vpool);
     a.fnn = to_g(j->fnn);
+    a.slot_of = to_g(j->slot_of);
     a.cent = to_g(j->cent);
     a.dmax01 = to_g(j->dmax);
     a.nS = j->nS;
@@ -660,7 +662,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
         (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) ||
         (rc = j->alloc(&j->vpool, 2L * nS)) || (rc = j->alloc(&j->dmax, 2)) ||
-        (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->cent, nS)) ||
+        (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->slot_of, nS)) || (rc = j->alloc(&j->cent, nS)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {

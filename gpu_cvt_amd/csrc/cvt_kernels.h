@@ -88,6 +88,7 @@ struct StepArgs {
     CVT_G float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, -, -)
     CVT_G float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     CVT_G int* fnn;             // nS: frame vertex of this iteration (hand-off of the FP32 pass; -2: deferred)
+    CVT_G int* slot_of;         // nS: each site's slot in this iteration's site grid (sg.pts)
     CVT_G float4* cent;         // nS: centroid and cell-phase flags (hand-off of the FP32 pass)
     int nS;
     float R;

@@ -209,19 +209,28 @@ __global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ desc
     a.fnn[i] = ok ? c.nn : kFrameDeferred;
 }
 
-// FP32 pass, phase 3 (not run for an evaluation-only step).
+// FP32 pass, phase 3 (not run for an evaluation-only step), in internal site
+// order: every per-site load and store of a wave is contiguous.
 template <int KP>
 __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
     const BlockRef br = bt[xcd_block(blockIdx.x, gridDim.x)];
     if (br.job < 0) return;
     const StepArgs& a = descs[br.job].a;
     const int par = *a.parity;
-    const int t = br.blk * 256 + (int)threadIdx.x;
-    const bool live = t < a.nS;
+    const int i = br.blk * 256 + (int)threadIdx.x;
+    const bool live = i < a.nS;
     SiteFlags f{};
     if (live) {
-        const SiteCtx c = site_ctx(a, false, t, true);
-        const float4 ce = a.cent[c.i];
+        const SiteCtx c = site_ctx_by_index(a, false, i);
+        // the hand-off and the pool cache entry in one round trip
+        const float4 ce = a.cent[i];
+        float4 vc = make_float4(0.f, 0.f, 0.f, __builtin_huge_valf());
+        int4 p0 = make_int4(0, 0, 0, 0), p1 = p0;
+        if (c.valid && !c.fz && c.use_cache) {
+            vc = a.vcache[i];
+            p0 = a.vpool[2 * (size_t)i];
+            p1 = a.vpool[2 * (size_t)i + 1];
+        }
         const int flags = c.valid ? __float_as_int(ce.w) : kCentDone;
         if (flags & kCentDone) {
             CellRes cr;
@@ -230,8 +239,23 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
             cr.cell_ok = (flags & kCentFail) == 0;
             cr.kind = (flags >> kCentKindShift) & 3;
             cr.area = cr.moment = 0.f;
-            phase_project<KP, false>(a, par, false, c, cr, f);
-            if (!f.done) defer_site(a, t, f.row_written, f.same, true);
+            int vl[KP];
+#pragma unroll
+            for (int k = 0; k < KP; ++k) vl[k] = -1;
+            bool ok = true;
+            if (needs_list(c, cr)) {
+                float tauv;
+                ok = pool_lookup_with<KP>(a, cr.c3, vc, p0, p1, vl, tauv) ||
+                     vertex_search<KP, false>(a, i, cr.c3, tauv, vl);
+            }
+            if (ok) {
+                project_finish<KP>(a, par, false, c, cr, vl, f);
+            } else {
+                f.kind = cr.kind;
+                f.same = cr.same;
+                f.row_written = true;
+            }
+            if (!f.done) defer_site(a, a.slot_of[i], f.row_written, f.same, true);
         }
     }
     count_site(a, par, false, f, live);
@@ -389,6 +413,7 @@ __global__ void kb_grid_scatter(const JobDesc* __restrict__ descs, const BlockRe
     const float4 p = a.S01[*a.parity][i];
     const int pos = d.start[d.bucket[i]] + d.rank[i];
     const_cast<CVT_G float4*>(a.sg.pts)[pos] = make_float4(p.x, p.y, p.z, __int_as_float(i));
+    a.slot_of[i] = pos;
 }
 
 // End of an iteration: stats history, flags, buffer parity.

@@ -319,18 +319,16 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 // FP64 keys, index ties) is closer, the pool's first 5 are the list (returns
 // true); otherwise the 9th distance plus d seeds the search in tauv (triangle
 // inequality; +inf without a cache).
+// pool_lookup on a cache entry already loaded (vc: vcache, p0/p1: vpool).
 template <int KP>
-__device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const float3& c3, bool use_cache, int (&vl)[KP],
-                                            float& tauv) {
+__device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3& c3, const float4& vc, const int4& p0,
+                                                 const int4& p1, int (&vl)[KP], float& tauv) {
     static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
     const float inf = __builtin_huge_valf();
     tauv = inf;
-    if (!use_cache) return false;
-    const float4 vc = a.vcache[i];
     if (!(vc.w < inf)) return false;
     const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
                            (c3.z - vc.z) * (c3.z - vc.z));
-    const int4 p0 = a.vpool[2 * (size_t)i], p1 = a.vpool[2 * (size_t)i + 1];
     int id[KP + 2] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
     double key[KP + 2];
 #pragma unroll
@@ -361,6 +359,14 @@ __device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const floa
     const float rr = (__int_as_float(p1.w) + dc) * (1.f + 1e-4f);
     tauv = rr * rr;
     return false;
+}
+
+template <int KP>
+__device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const float3& c3, bool use_cache, int (&vl)[KP],
+                                            float& tauv) {
+    tauv = __builtin_huge_valf();
+    if (!use_cache) return false;
+    return pool_lookup_with<KP>(a, c3, a.vcache[i], a.vpool[2 * (size_t)i], a.vpool[2 * (size_t)i + 1], vl, tauv);
 }
 
 // The kproj nearest mesh vertices of c3 by the vertex grid (seed tauv), and
