@@ -39,6 +39,13 @@ def _case(name):
         S = mg.sample_surface(V, F, 20000, seed=12)
         S = S[S[:, 2] > 0.85][:400]
         return V, F, S, 8
+    if name == "cluster_k24":  # 14000 of 15500 sites on the 10 faces around one pole: cell rows of
+        # thousands of points (collect overflow, 16-bit row offsets exceeded, exact fallbacks)
+        V, F = mg.icosphere(4)
+        C = V[F].mean(axis=1)
+        S = np.concatenate([mg.sample_surface(V, F, 1500, seed=13),
+                            mg.sample_surface(V, F[C[:, 2] > 0.995], 14000, seed=14)])
+        return V, F, S, 24
     if name == "dsphere_k32":
         V, F = mg.displaced_sphere(40, seed=4, crease_seed=5)
         S = mg.sample_surface(V, F, 6000, seed=6)
@@ -92,6 +99,32 @@ def test_stages_teacher_forced(orc, name):
             r = _stage_check(orc, mesh, job, S32, K)
             print(f"  iter {it}: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e} stats {st}", flush=True)
             assert st["cell_failures"] == 0
+
+
+def test_cluster_knn_exact(orc):
+    """Sites packed ~500x denser on a few faces than elsewhere: the site
+    search's LDS collect overflows and rows exceed its 16-bit offsets, so the
+    exact fallbacks run.  KNN lists must stay bit-exact in every iteration.
+    Cells and projections are held to the usual bar in the first iteration
+    only: afterwards open cells at the cluster's edge send sites across the
+    sphere, and cells whose dual points span ~2000:1 in scale are beyond the
+    FP32 accuracy of the dual hull (the reference's FP32 clipper shares it)."""
+    V, F, S, K = _case("cluster_k24")
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=K, debug=1) as job:
+        for it in range(5):
+            S32 = orc.f32r(job.sites())
+            job.step(1)
+            knn_o = orc.knn(S32, S32, K, exclude_self=True)
+            knn_g = job.knn()
+            assert np.array_equal(knn_g, knn_o), f"iter {it}: site KNN mismatch in {np.sum(np.any(knn_g != knn_o, 1))} rows"
+            cent_g32 = orc.f32r(job.debug()["cent"])
+            ok = np.all(np.isfinite(cent_g32), axis=1)
+            knnv_o = orc.knn(mesh.V, cent_g32[ok], 5, exclude_self=False)
+            assert np.array_equal(job.knnv()[ok], knnv_o), f"iter {it}: vertex KNN mismatch"
+            if it == 0:
+                r = _stage_check(orc, mesh, job, S32, K)
+                print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
 
 
 def test_vertex_normals(orc):
