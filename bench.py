@@ -84,8 +84,18 @@ def cpu_baseline(V, F, S, K: int, budget_s: float = 20.0):
         dt = time.perf_counter() - t0
     else:
         n, dt = n0, dt0
-    return {"value": n / dt, "unit": "site-iters/s", "cores": threads, "kind": "port",
-            "sample": f"1 FP64 oracle Lloyd iteration over {n} of the {len(S)} sites on the same mesh "
+    # the whole job in one iteration takes less than the budget: time further
+    # iterations of the same sites (the next Lloyd steps, as on the GPU)
+    iters = 1
+    if n == len(S) and dt < budget_s * 0.5:
+        extra = int(budget_s * 0.5 / dt) - 1
+        t0 = time.perf_counter()
+        for _ in range(extra):
+            orc.lloyd_step(mesh, st, K=K, R=R)
+        dt += time.perf_counter() - t0
+        iters += extra
+    return {"value": n * iters / dt, "unit": "site-iters/s", "cores": threads, "kind": "port",
+            "sample": f"{iters} FP64 oracle Lloyd iteration(s) over {n} of the {len(S)} sites on the same mesh "
                       f"({dt:.1f} s, {threads} threads)"}
 
 
