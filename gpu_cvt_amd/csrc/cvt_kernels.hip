@@ -671,12 +671,13 @@ void launch_batch_end(const BatchTables& bt, hipStream_t s) {
 
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s, const SideStream* side) {
     if (side && side->s) {
-        // the frame phase (sites by index, static vertex grid) beside the site grid rebuild
-        hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+        // the frame phase (sites by index, static vertex grid; it touches none
+        // of what kb_begin clears) beside the stats reset and site grid rebuild
         (void)hipEventRecord(side->ffork, s);
         (void)hipStreamWaitEvent(side->s, side->ffork, 0);
         hipLaunchKernelGGL((kb_frame<5>), dim3(bt.n_site_blocks), dim3(256), 0, side->s, bt.descs, bt.site_blocks, 0);
         (void)hipEventRecord(side->fjoin, side->s);
+        hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
         launch_grid_rebuild(bt, s);
         (void)hipStreamWaitEvent(s, side->fjoin, 0);
         launch_batch_lloyd(bt, K, kproj, 0, s, 3 | kLloydFrameDone, side);
