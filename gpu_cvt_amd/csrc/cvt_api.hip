@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -502,6 +503,63 @@ int check_finite(const double* a, long n) {
     return 1;
 }
 
+// f(t, begin, end) over [0, n) split into contiguous chunks on up to 8 host
+// threads (one below 64k items); chunk t is [n t / T, n (t + 1) / T).
+template <class Fn>
+void parallel_chunks(size_t n, Fn&& f) {
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
+                                                             n / 65536 + 1}));
+    std::vector<std::thread> th;
+    for (int t = 1; t < T; ++t) th.emplace_back([&, t] { f(t, n * t / T, n * (t + 1) / T); });
+    f(0, 0, n / T);
+    for (auto& x : th) x.join();
+}
+
+// Sorts keys whose low `lo_bit` bits are an index in ascending order (as
+// built) by the bits above, stably, 8 bits per pass (LSD radix, the passes
+// split over up to 8 host threads): the same order as a full sort of the
+// unique keys, in O(n) passes instead of O(n log n) comparisons.
+void sort_indexed_keys(std::vector<unsigned long long>& k, int lo_bit) {
+    const size_t n = k.size();
+    std::vector<unsigned long long> tmp(n);
+    unsigned long long all = 0;
+    for (const unsigned long long x : k) all |= x;
+    const int T = (int)std::max<size_t>(1, std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
+                                                             n / 65536 + 1}));
+    std::vector<size_t> cnt((size_t)T * 256);
+    auto chunk = [&](int t, size_t& b, size_t& e) { b = n * t / T; e = n * (t + 1) / T; };
+    auto run = [&](auto&& f) {
+        std::vector<std::thread> th;
+        for (int t = 1; t < T; ++t) th.emplace_back(f, t);
+        f(0);
+        for (auto& x : th) x.join();
+    };
+    for (int sh = lo_bit; sh < 64 && (all >> sh) != 0; sh += 8) {
+        std::fill(cnt.begin(), cnt.end(), 0);
+        run([&](int t) {
+            size_t b, e;
+            chunk(t, b, e);
+            size_t* c = &cnt[(size_t)t * 256];
+            for (size_t i = b; i < e; ++i) c[(k[i] >> sh) & 255]++;
+        });
+        // exclusive offsets in (digit, thread) order keep the pass stable
+        size_t run_sum = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int t = 0; t < T; ++t) {
+                const size_t c = cnt[(size_t)t * 256 + d];
+                cnt[(size_t)t * 256 + d] = run_sum;
+                run_sum += c;
+            }
+        run([&](int t) {
+            size_t b, e;
+            chunk(t, b, e);
+            size_t* c = &cnt[(size_t)t * 256];
+            for (size_t i = b; i < e; ++i) tmp[c[(k[i] >> sh) & 255]++] = k[i];
+        });
+        k.swap(tmp);
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -589,37 +647,45 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
             v = (v | (v << 2)) & 0x9249249ull;
             return v;
         };
-        for (int f = 0; f < nF; ++f) {
-            unsigned q[3];
-            for (int a = 0; a < 3; ++a) {
-                const double c = ((double)(&V4[Fh[3 * f]].x)[a] + (&V4[Fh[3 * f + 1]].x)[a] + (&V4[Fh[3 * f + 2]].x)[a]) / 3.0;
-                const double t = e[a] > 0 ? (c - j->lo[a]) / e[a] : 0.0;
-                q[a] = (unsigned)std::min(1023.0, std::max(0.0, t * 1024.0));
+        parallel_chunks(nF, [&](int, size_t b, size_t e2) {
+            for (size_t f = b; f < e2; ++f) {
+                unsigned q[3];
+                for (int a = 0; a < 3; ++a) {
+                    const double c = ((double)(&V4[Fh[3 * f]].x)[a] + (&V4[Fh[3 * f + 1]].x)[a] + (&V4[Fh[3 * f + 2]].x)[a]) / 3.0;
+                    const double t = e[a] > 0 ? (c - j->lo[a]) / e[a] : 0.0;
+                    q[a] = (unsigned)std::min(1023.0, std::max(0.0, t * 1024.0));
+                }
+                key[f] = ((spread(q[0]) << 2 | spread(q[1]) << 1 | spread(q[2])) << 32) | (unsigned)f;
             }
-            key[f] = ((spread(q[0]) << 2 | spread(q[1]) << 1 | spread(q[2])) << 32) | (unsigned)f;
-        }
-        std::sort(key.begin(), key.end());
-        for (int k = 0; k < nF; ++k) {
-            const int f = (int)(key[k] & 0xffffffffu);
-            for (int c = 0; c < 3; ++c) Fi[3L * k + c] = Fh[3L * f + c];
-        }
+        });
+        sort_indexed_keys(key, 32);
+        parallel_chunks(nF, [&](int, size_t b, size_t e2) {
+            for (size_t k = b; k < e2; ++k) {
+                const size_t f = (size_t)(key[k] & 0xffffffffu);
+                for (int c = 0; c < 3; ++c) Fi[3 * k + c] = Fh[3 * f + c];
+            }
+        });
     }
     std::vector<float4> tri((size_t)3 * nF);
-    double area = 0;
-    for (int f = 0; f < nF; ++f) {
-        const float4 A = V4[Fi[3 * f]], B = V4[Fi[3 * f + 1]], C = V4[Fi[3 * f + 2]];
-        for (int k = 0; k < 3; ++k) {
-            const float4 P = V4[Fi[3 * f + k]];
-            int id = Fi[3 * f + k];
-            float w;
-            std::memcpy(&w, &id, 4);
-            tri[(size_t)3 * f + k] = make_float4(P.x, P.y, P.z, w);
+    std::vector<double> farea(nF);
+    parallel_chunks(nF, [&](int, size_t b, size_t e2) {
+        for (size_t f = b; f < e2; ++f) {
+            const float4 A = V4[Fi[3 * f]], B = V4[Fi[3 * f + 1]], C = V4[Fi[3 * f + 2]];
+            for (int k = 0; k < 3; ++k) {
+                const float4 P = V4[Fi[3 * f + k]];
+                int id = Fi[3 * f + k];
+                float w;
+                std::memcpy(&w, &id, 4);
+                tri[3 * f + k] = make_float4(P.x, P.y, P.z, w);
+            }
+            double ux = (double)B.x - A.x, uy = (double)B.y - A.y, uz = (double)B.z - A.z;
+            double vx = (double)C.x - A.x, vy = (double)C.y - A.y, vz = (double)C.z - A.z;
+            double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+            farea[f] = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
         }
-        double ux = (double)B.x - A.x, uy = (double)B.y - A.y, uz = (double)B.z - A.z;
-        double vx = (double)C.x - A.x, vy = (double)C.y - A.y, vz = (double)C.z - A.z;
-        double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
-        area += 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-    }
+    });
+    double area = 0;
+    for (int f = 0; f < nF; ++f) area += farea[f];  // in face order, as one thread would
     j->area = area > 0 ? area : emax * emax;
     // vertex -> incident-face CSR in face order (main.cu:259-285)
     std::vector<int> vf_off(nV + 1, 0), vf_faces((size_t)3 * nF);
@@ -673,7 +739,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     // internal numbering: sites stably sorted by their cell in the site grid
     {
         const GridP& g = j->sgp;
-        std::vector<long long> key(nS);
+        std::vector<unsigned long long> key(nS);
         for (int i = 0; i < nS; ++i) {
             const float x = (float)S0[3 * i], y = (float)S0[3 * i + 1], z = (float)S0[3 * i + 2];
             auto cl = [](float v, float o, float ih, int d) {
@@ -682,9 +748,9 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
             };
             const long c = ((long)cl(z, g.oz, g.inv_h, g.dz) * g.dy + cl(y, g.oy, g.inv_h, g.dy)) * g.dx +
                            cl(x, g.ox, g.inv_h, g.dx);
-            key[i] = (c << 31) | i;
+            key[i] = ((unsigned long long)c << 31) | (unsigned)i;
         }
-        std::sort(key.begin(), key.end());
+        sort_indexed_keys(key, 31);
         j->perm.resize(nS);
         j->inv.resize(nS);
         for (int k = 0; k < nS; ++k) {
