@@ -1282,17 +1282,23 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
         int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
         auto* keys = (unsigned long long*)T.get(8 * (size_t)nS * M);
         float4* site_n = (float4*)T.get(sizeof(float4) * nS);
-        int* flag = (int*)T.get(sizeof(int));
-        if (!knn_tmp || !keys || !site_n || !flag) return fail(CVT_ENOMEM, "check temporaries");
+        const int nst = bvh_check_state_ints();
+        int* flag = (int*)T.get(sizeof(int) * nst);  // verdict, probe counters
+        // probes of ~2 triangles per site (a closed surface), with slack; an
+        // overflow leaves the verdict to the full remesh below
+        const int cap = 9 * nS + 1024;
+        float4* probes = (float4*)T.get(sizeof(float4) * (size_t)cap);
+        if (!knn_tmp || !keys || !site_n || !flag || !probes) return fail(CVT_ENOMEM, "check temporaries");
         hipStream_t s = j->stream;
         launch_batch_grid(j->tables, 1, s);
         launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
-        launch_fill(flag, 1, 0, s);
-        bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), flag, s);
+        launch_fill(flag, nst, 0, s);
+        bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), probes, cap, flag,
+                       s);
         int hf = 0;
         HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        if (hf) {
+        if (hf == 1) {
             *within = 0;
             if (h_out) *h_out = -1;
             if (nF_out) *nF_out = -1;

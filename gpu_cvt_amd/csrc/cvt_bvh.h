@@ -44,10 +44,13 @@ void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const 
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s);
 // out[i] <= thr2 iff some triangle lies within thr2 of P[i] (+inf otherwise)
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s);
-// *flag = 1 iff a probe of the remeshed surface -- the union of the remesh
+// st[0] = 1 iff a probe of the remeshed surface -- the union of the remesh
 // cells' candidate triples keys[nS][M] (~0 unused) -- has no triangle within
-// thr2; *flag is not cleared here.
+// thr2, or 2 when its probes overflow probes[cap] (no verdict).  st holds
+// bvh_check_state_ints() ints (the verdict and per-segment probe counters),
+// all zero on entry.
+int bvh_check_state_ints();
 void bvh_check_keys(const BVHDev& b, const unsigned long long* keys, int nS, int M, const float4* S,
-                    const float4* site_n, float eps, float thr2, int* flag, hipStream_t s);
+                    const float4* site_n, float eps, float thr2, float4* probes, int cap, int* st, hipStream_t s);
 
 }  // namespace cvt

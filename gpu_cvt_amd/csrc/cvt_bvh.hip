@@ -389,38 +389,53 @@ __global__ __launch_bounds__(256) void k_point_within(BVHDev b, const float4* __
     out[i] = point_within(b, make_float3(q4.x, q4.y, q4.z), thr2);
 }
 
-// Convergence test straight from the remesh cells' candidate keys: row i
-// (M slots) holds the sorted triples (packed 21:21:21, ~0 unused) that site i
-// emits.  The remeshed surface is the union of all rows (buildmesh.cuh:170-241
-// sorts and de-duplicates them); here each triple is taken once, at the
-// smallest of its sites that emits it (a lookup in the other sites' rows), so
-// neither a sort nor a scan is needed.  A kept, non-degenerate triangle
-// (buildmesh.cuh:225-241) is oriented by the summed site normals and its four
-// probes (centroid, edge midpoints; evaluate.cuh:252-283 without the vertices)
-// are formed with the arithmetic of the remesh path; the first probe with no
-// triangle within thr2 sets *flag (a plain store: every writer stores 1).
-__global__ __launch_bounds__(256) void k_check_keys(BVHDev b, const unsigned long long* __restrict__ keys, int nS,
-                                                    int M, const float4* __restrict__ S,
-                                                    const float4* __restrict__ site_n, float eps, float thr2,
-                                                    int* flag) {
+// Convergence test straight from the remesh cells' candidate keys, in two
+// kernels.  k_check_emit: row i (M slots) holds the sorted triples (packed
+// 21:21:21, ~0 unused) that site i emits.  The remeshed surface is the union
+// of all rows (buildmesh.cuh:170-241 sorts and de-duplicates them); here each
+// triple is taken once, at the smallest of its sites that emits it (a lookup
+// in the other sites' rows), so neither a sort nor a scan is needed.  A kept,
+// non-degenerate triangle (buildmesh.cuh:225-241) is oriented by the summed
+// site normals and its four probes (centroid, edge midpoints;
+// evaluate.cuh:252-283 without the vertices) are formed with the arithmetic of
+// the remesh path and appended to a dense probe list: kCheckSegs segments of
+// cap / kCheckSegs probes, block b appending to segment b % kCheckSegs with
+// one atomic per wave on that segment's own counter line (a single counter
+// serialised ~400k atomics).  k_check_probes then runs one bounded query per
+// probe, so every lane of a wave walks the LBVH (most key slots are empty or
+// duplicates, and a thread per slot would leave most lanes idle during the
+// walks).  st[0]: 1 when a probe has no triangle within thr2, 2 when a
+// segment overflowed (the verdict is then left to the full remesh);
+// st[kCheckStride * (1 + g)]: segment g's probe count.  Every writer of st[0]
+// stores the same value for the same event (plain stores).
+constexpr int kCheckSegs = 64, kCheckStride = 32;
+
+__global__ __launch_bounds__(256) void k_check_emit(const unsigned long long* __restrict__ keys, int nS, int M,
+                                                    const float4* __restrict__ S, const float4* __restrict__ site_n,
+                                                    float eps, float4* __restrict__ probes, int cap, int* st) {
     const long k = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= (long)nS * M) return;
     const unsigned long long v = keys[k];
     if (v == ~0ull) return;
-    if (*(volatile int*)flag) return;
     const int i = (int)(k / M);
     int ia = (int)(v >> 42), ib = (int)((v >> 21) & 0x1fffff), ic = (int)(v & 0x1fffff);
-    // the smallest emitting site keeps the triple
+    // the smallest emitting site keeps the triple; both rows are compared
+    // whole, without an early exit, so their loads go out together (rows are
+    // M keys with M even: 16-byte loads)
     const int others[2] = {ia == i ? ib : ia, ic == i ? ib : ic};
+    bool found = false;
 #pragma unroll
     for (int o = 0; o < 2; ++o) {
         const int sidx = others[o];
         if (sidx >= i) continue;
-        const unsigned long long* row = keys + (size_t)sidx * M;
-        bool found = false;
-        for (int t = 0; t < M && !found; ++t) found = row[t] == v;
-        if (found) return;
+        const ulonglong2* row = reinterpret_cast<const ulonglong2*>(keys + (size_t)sidx * M);
+#pragma unroll 4
+        for (int t = 0; t < M / 2; ++t) {
+            const ulonglong2 r = row[t];
+            found |= (r.x == v) | (r.y == v);
+        }
     }
+    if (found) return;
     const float4 A = S[ia], B = S[ib], C = S[ic];
     const float ex = B.x - A.x, ey = B.y - A.y, ez = B.z - A.z;
     const float fx = C.x - A.x, fy = C.y - A.y, fz = C.z - A.z;
@@ -432,27 +447,50 @@ __global__ __launch_bounds__(256) void k_check_keys(BVHDev b, const unsigned lon
     const float4 na = site_n[ia], nb = site_n[ib], nc = site_n[ic];
     const float dot = nx * (na.x + nb.x + nc.x) + ny * (na.y + nb.y + nc.y) + nz * (na.z + nb.z + nc.z);
     const float4 a = A, bb = dot < 0.f ? C : B, c = dot < 0.f ? B : C;
-    float3 P[4];
-    P[0] = make_float3((a.x + bb.x + c.x) * (1.f / 3.f), (a.y + bb.y + c.y) * (1.f / 3.f),
-                       (a.z + bb.z + c.z) * (1.f / 3.f));
-    P[1] = make_float3(0.5f * (a.x + bb.x), 0.5f * (a.y + bb.y), 0.5f * (a.z + bb.z));
-    P[2] = make_float3(0.5f * (bb.x + c.x), 0.5f * (bb.y + c.y), 0.5f * (bb.z + c.z));
-    P[3] = make_float3(0.5f * (c.x + a.x), 0.5f * (c.y + a.y), 0.5f * (c.z + a.z));
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (point_within(b, P[q], thr2) > thr2) {
-            *flag = 1;
-            return;
-        }
+    float4 P[4];
+    P[0] = make_float4((a.x + bb.x + c.x) * (1.f / 3.f), (a.y + bb.y + c.y) * (1.f / 3.f),
+                       (a.z + bb.z + c.z) * (1.f / 3.f), 0.f);
+    P[1] = make_float4(0.5f * (a.x + bb.x), 0.5f * (a.y + bb.y), 0.5f * (a.z + bb.z), 0.f);
+    P[2] = make_float4(0.5f * (bb.x + c.x), 0.5f * (bb.y + c.y), 0.5f * (bb.z + c.z), 0.f);
+    P[3] = make_float4(0.5f * (c.x + a.x), 0.5f * (c.y + a.y), 0.5f * (c.z + a.z), 0.f);
+    // append to this block's segment: one atomic per wave, slots in lane order
+    const int seg = (int)(blockIdx.x % kCheckSegs), seg_cap = cap / kCheckSegs;
+    const unsigned long long m = __ballot(1);
+    const int lane = (int)(threadIdx.x & 63);
+    const int leader = __ffsll((long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&st[kCheckStride * (1 + seg)], 4 * (int)__popcll(m));
+    base = __shfl(base, leader, 64);
+    const int pos = base + 4 * (int)__popcll(m & ((1ull << lane) - 1ull));
+    if (pos + 4 > seg_cap) {
+        st[0] = 2;
+        return;
     }
+    float4* out = probes + (size_t)seg * seg_cap + pos;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[q] = P[q];
 }
 
+__global__ __launch_bounds__(256) void k_check_probes(BVHDev b, const float4* __restrict__ probes, int cap,
+                                                      float thr2, int* st) {
+    const int seg_cap = cap / kCheckSegs;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int seg = i / seg_cap, j = i - seg * seg_cap;
+    if (seg >= kCheckSegs || j >= st[kCheckStride * (1 + seg)]) return;
+    if (*(volatile int*)st != 0) return;
+    const float4 q = probes[i];
+    if (point_within(b, make_float3(q.x, q.y, q.z), thr2) > thr2) st[0] = 1;
+}
+
+int bvh_check_state_ints() { return kCheckStride * (1 + kCheckSegs); }
+
 void bvh_check_keys(const BVHDev& b, const unsigned long long* keys, int nS, int M, const float4* S,
-                    const float4* site_n, float eps, float thr2, int* flag, hipStream_t s) {
+                    const float4* site_n, float eps, float thr2, float4* probes, int cap, int* st, hipStream_t s) {
     const long nk = (long)nS * M;
-    if (nk <= 0) return;
-    hipLaunchKernelGGL(k_check_keys, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, b, keys, nS, M, S, site_n,
-                       eps, thr2, flag);
+    if (nk <= 0 || cap <= 0) return;
+    hipLaunchKernelGGL(k_check_emit, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, keys, nS, M, S, site_n, eps,
+                       probes, cap, st);
+    hipLaunchKernelGGL(k_check_probes, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, b, probes, cap, thr2, st);
 }
 
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s) {
