@@ -1272,6 +1272,7 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     int rc;
     if ((rc = ensure_bvh(j))) return rc;
     T.reset();
+    bool fast_within = false;  // the fast path below proved the job within tol
     if (tol >= 0 && within && !exact_always && j->nS < (1 << 21)) {
         // The verdict straight from the remesh cells' candidate triples (each
         // taken once, at its smallest emitting site; no sort, one host sync);
@@ -1304,6 +1305,7 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
             if (nF_out) *nF_out = -1;
             return 0;
         }
+        fast_within = hf == 0;
         T.reset();
     }
     int* dF = nullptr;
@@ -1317,7 +1319,7 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     }
     float4* P = (float4*)T.get(sizeof(float4) * 4 * (size_t)nkeep);
     float* d2 = (float*)T.get(sizeof(float) * 4 * (size_t)nkeep);
-    unsigned* mx = (unsigned*)T.get(sizeof(unsigned));
+    unsigned* mx = (unsigned*)T.get(2 * sizeof(unsigned));
     if (!P || !d2 || !mx) return fail(CVT_ENOMEM, "hausdorff temporaries");
     hipStream_t s = j->stream;
     launch_tri_probes(dF, nkeep, j->S[j->cur], P, s);
@@ -1331,7 +1333,9 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
         return 0;
     };
     bool in = true;
-    if (tol >= 0 && within) {
+    if (fast_within) {
+        *within = 1;  // the fast path's verdict: same triangles, probes and queries
+    } else if (tol >= 0 && within) {
         const float thr2 = sq_threshold(tol);
         bvh_point_within(j->bvh->d, P, 4 * nkeep, thr2, d2, s);
         float m;
@@ -1341,9 +1345,14 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     }
     if (h_out) {
         if (in || exact_always) {
-            bvh_point_d2(j->bvh->d, P, 4 * nkeep, d2, s);
+            launch_fill((int*)mx, 2, 0, s);
+            bvh_max_d2(j->bvh->d, P, 4 * nkeep, mx, s);
+            unsigned hb[2] = {0, 0};
+            HIPCHK(hipMemcpyAsync(hb, mx, 2 * sizeof(unsigned), hipMemcpyDeviceToHost, s));
+            HIPCHK(hipStreamSynchronize(s));
             float m;
-            if ((rc = max_of(m))) return rc;
+            const unsigned mb = std::max(hb[0], hb[1]);  // non-negative floats order like their bits
+            std::memcpy(&m, &mb, 4);
             *h_out = std::sqrt((double)m);
         } else {
             *h_out = -1;

@@ -42,6 +42,10 @@ void bvh_build(BVHDev& out, const float4* tri, int nF, const float lo[3], const 
                hipStream_t s, int ncorner = 3);
 // Squared distance of each point (float4 xyz) to the mesh.
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s);
+// max(mx[0], mx[1]) (float bits; both zero on entry) = the largest squared
+// distance of P[0, n) to the mesh, the same value as the maximum of
+// bvh_point_d2's outputs, found by branch and bound.
+void bvh_max_d2(const BVHDev& b, const float4* P, int n, unsigned* mx, hipStream_t s);
 // out[i] <= thr2 iff some triangle lies within thr2 of P[i] (+inf otherwise)
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s);
 // st[0] = 1 iff a probe of the remeshed surface -- the union of the remesh

@@ -294,18 +294,12 @@ __device__ __forceinline__ float tri_d2(float3 p, float4 A, float4 B, float4 C) 
 // Stackless traversal with parent links: children are visited near-first
 // (by box distance, ties left-first); returning from a child the same order
 // is recomputed, so no stack is needed.
-__global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __restrict__ P, int np,
-                                                  float* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= np) return;
-    const float4 q4 = P[i];
-    const float3 q = make_float3(q4.x, q4.y, q4.z);
+__device__ float point_d2(const BVHDev& b, float3 q) {
     const int n = b.n;
     float best = __builtin_huge_valf();
     if (n == 1) {
         const int f = b.leaf_tri[0];
-        out[i] = tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
-        return;
+        return tri_d2(q, b.tri[3 * f], b.tri[3 * f + 1], b.tri[3 * f + 2]);
     }
     int cur = 0, last = -1;
     for (;;) {
@@ -334,7 +328,53 @@ __global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __rest
         cur = next;
         if (cur < 0) break;
     }
-    out[i] = best;
+    return best;
+}
+
+__global__ __launch_bounds__(256) void k_point_d2(BVHDev b, const float4* __restrict__ P, int np,
+                                                  float* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float4 q4 = P[i];
+    out[i] = point_d2(b, make_float3(q4.x, q4.y, q4.z));
+}
+
+// Largest squared distance of the points P[0, np) to the mesh, by branch and
+// bound: every kMaxStride-th point exactly (k_max_d2_sample, into mx[0]),
+// then each other point only when a bounded query finds no triangle within
+// that lower bound (k_max_d2_rest, into mx[1]); max(mx[0], mx[1]) is then
+// the maximum of the exact per-point distances.  One atomic per wave.
+constexpr int kMaxStride = 64;
+
+__device__ float point_within(const BVHDev& b, float3 q, float thr2);
+
+__device__ __forceinline__ void wave_max_atomic(unsigned* dst, float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.f) atomicMax(dst, __float_as_uint(v));
+}
+
+__global__ __launch_bounds__(256) void k_max_d2_sample(BVHDev b, const float4* __restrict__ P, int np,
+                                                       unsigned* mx) {
+    const long i = ((long)blockIdx.x * blockDim.x + threadIdx.x) * kMaxStride;
+    float d = 0.f;
+    if (i < np) {
+        const float4 q4 = P[i];
+        d = point_d2(b, make_float3(q4.x, q4.y, q4.z));
+    }
+    wave_max_atomic(&mx[0], d);
+}
+
+__global__ __launch_bounds__(256) void k_max_d2_rest(BVHDev b, const float4* __restrict__ P, int np, unsigned* mx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float d = 0.f;
+    if (i < np && i % kMaxStride != 0) {
+        const float thr2 = __uint_as_float(mx[0]);
+        const float4 q4 = P[i];
+        const float3 q = make_float3(q4.x, q4.y, q4.z);
+        if (point_within(b, q, thr2) > thr2) d = point_d2(b, q);
+    }
+    wave_max_atomic(&mx[1], d);
 }
 
 // Bounded query: the squared distance of the first triangle found within thr2
@@ -491,6 +531,13 @@ void bvh_check_keys(const BVHDev& b, const unsigned long long* keys, int nS, int
     hipLaunchKernelGGL(k_check_emit, dim3((unsigned)((nk + 255) / 256)), dim3(256), 0, s, keys, nS, M, S, site_n, eps,
                        probes, cap, st);
     hipLaunchKernelGGL(k_check_probes, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, s, b, probes, cap, thr2, st);
+}
+
+void bvh_max_d2(const BVHDev& b, const float4* P, int n, unsigned* mx, hipStream_t s) {
+    if (n <= 0) return;
+    const long ns = (n + kMaxStride - 1) / kMaxStride;
+    hipLaunchKernelGGL(k_max_d2_sample, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, s, b, P, n, mx);
+    hipLaunchKernelGGL(k_max_d2_rest, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, b, P, n, mx);
 }
 
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s) {
