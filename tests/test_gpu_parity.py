@@ -46,6 +46,11 @@ def _case(name):
         S = np.concatenate([mg.sample_surface(V, F, 1500, seed=13),
                             mg.sample_surface(V, F[C[:, 2] > 0.995], 14000, seed=14)])
         return V, F, S, 24
+    if name.startswith("kcov_k"):  # every compiled K (4, 6, 12, 16 are not covered above)
+        K = int(name[len("kcov_k"):])
+        V, F = mg.icosphere(4)
+        S = mg.sample_surface(V, F, 1500, seed=20 + K)
+        return V, F, S, K
     if name == "dsphere_k32":
         V, F = mg.displaced_sphere(40, seed=4, crease_seed=5)
         S = mg.sample_surface(V, F, 6000, seed=6)
@@ -53,8 +58,14 @@ def _case(name):
     raise KeyError(name)
 
 
-def _stage_check(orc, mesh, job, S32, K, prev_state=None):
-    """Run one GPU step from S32 and compare every stage with the oracle."""
+def _stage_check(orc, mesh, job, S32, K, prev_state=None, far_rel=None):
+    """Run one GPU step from S32 and compare every stage with the oracle.
+
+    far_rel: with few neighbours (K <= 6) some cells stay open and their
+    centroids land far off the surface (~0.5-1 x the radius), where the
+    nearest point over the fan faces is ill-conditioned in FP32 (near-equal
+    face distances).  Sites whose centroid lies more than 0.05 x diag off the
+    surface are then held to err <= far_rel x that distance instead."""
     diag = mesh.bbox_diag
     knn_g = job.knn()
     dbg = job.debug()
@@ -82,12 +93,19 @@ def _stage_check(orc, mesh, job, S32, K, prev_state=None):
     assert np.array_equal(knnv_g, knnv_o), f"vertex KNN mismatch in {np.sum(np.any(knnv_g != knnv_o, 1))} rows"
     # 5. projection from the same centroids and lists
     proj_o = orc.project(cent_g32, knnv_o, mesh.V, mesh.F, mesh.vf)
-    err_p = np.linalg.norm(Snew_g - proj_o, axis=1).max()
+    err = np.linalg.norm(Snew_g - proj_o, axis=1)
+    if far_rel is not None:
+        off = np.linalg.norm(cent_g32 - proj_o, axis=1)
+        far = off > 0.05 * diag
+        assert np.all(err[far] <= far_rel * off[far]), f"far-centroid projection error {np.max(err[far] / off[far]):.3e} x"
+        err = err[~far]
+    err_p = err.max() if err.size else 0.0
     assert err_p <= TOL_REL_DIAG * diag, f"projection error {err_p:.3e}"
     return dict(knn=knn_g, cent=dbg["cent"], Snew=Snew_g, err_c=err_c, err_p=err_p)
 
 
-@pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32", "cap_far"])
+@pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32", "cap_far",
+                                  "kcov_k12", "kcov_k16"])
 def test_stages_teacher_forced(orc, name):
     V, F, S, K = _case(name)
     print(f"[parity] case={name} nV={len(V)} nF={len(F)} nS={len(S)} K={K}", flush=True)
@@ -99,6 +117,31 @@ def test_stages_teacher_forced(orc, name):
             r = _stage_check(orc, mesh, job, S32, K)
             print(f"  iter {it}: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e} stats {st}", flush=True)
             assert st["cell_failures"] == 0
+
+
+@pytest.mark.parametrize("K", [4, 6])
+def test_small_k_knn_exact(orc, K):
+    """K = 4 and 6: most cells stay open, so sites jump across the sphere and
+    later iterations hold cells whose dual points span extreme scales (beyond
+    the FP32 dual hull's accuracy, as for the reference's FP32 clipper).  KNN
+    lists must stay bit-exact in every iteration; the first iteration gets the
+    full stage check, with far-off centroids held to a relative bar."""
+    V, F, S, K = _case(f"kcov_k{K}")
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=K, debug=1) as job:
+        for it in range(5):
+            S32 = orc.f32r(job.sites())
+            job.step(1)
+            if it == 0:
+                r = _stage_check(orc, mesh, job, S32, K, far_rel=1e-3)
+                print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
+                continue
+            knn_o = orc.knn(S32, S32, K, exclude_self=True)
+            assert np.array_equal(job.knn(), knn_o), f"iter {it}: site KNN mismatch"
+            cent_g32 = orc.f32r(job.debug()["cent"])
+            ok = np.all(np.isfinite(cent_g32), axis=1)
+            knnv_o = orc.knn(mesh.V, cent_g32[ok], 5, exclude_self=False)
+            assert np.array_equal(job.knnv()[ok], knnv_o), f"iter {it}: vertex KNN mismatch"
 
 
 def test_cluster_knn_exact(orc):
