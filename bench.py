@@ -282,8 +282,8 @@ def main(argv=None) -> int:
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU only: this path has no multi-GPU form in this build "
-              "(jobs shard across GPUs with no communication; see the design notes).", file=sys.stderr)
+        print("bench.py measures one GPU only: the remesher has no multi-GPU form "
+              "(jobs would shard across GPUs with no communication between them).", file=sys.stderr)
         return 2
 
     from gpu_cvt_amd import cvt, meshgen as mg
