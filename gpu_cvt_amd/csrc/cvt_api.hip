@@ -687,15 +687,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     double area = 0;
     for (int f = 0; f < nF; ++f) area += farea[f];  // in face order, as one thread would
     j->area = area > 0 ? area : emax * emax;
-    // vertex -> incident-face CSR in face order (main.cu:259-285)
-    std::vector<int> vf_off(nV + 1, 0), vf_faces((size_t)3 * nF);
-    for (long f = 0; f < 3L * nF; ++f) vf_off[Fi[f] + 1]++;
-    for (int v = 0; v < nV; ++v) vf_off[v + 1] += vf_off[v];
-    {
-        std::vector<int> cur(vf_off.begin(), vf_off.end() - 1);
-        for (int f = 0; f < nF; ++f)
-            for (int k = 0; k < 3; ++k) vf_faces[cur[Fi[3 * f + k]]++] = f;
-    }
+    // the vertex -> incident-face CSR (main.cu:259-285) is built on the device below
     // padded grid bbox
     double glo[3], ghi[3];
     for (int a = 0; a < 3; ++a) {
@@ -767,8 +759,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     hipStream_t s = j->stream;
     auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
     if (up(j->V, V4.data(), sizeof(float4) * nV) || up(j->tri, tri.data(), sizeof(float4) * tri.size()) ||
-        up(j->F, Fv.data(), sizeof(int) * Fv.size()) || up(j->vf_off, vf_off.data(), sizeof(int) * vf_off.size()) ||
-        up(j->vf_faces, vf_faces.data(), sizeof(int) * vf_faces.size()) ||
+        up(j->F, Fv.data(), sizeof(int) * Fv.size()) ||
         up(j->S[0], S4.data(), sizeof(float4) * nS) || up(j->cid, j->perm.data(), sizeof(int) * nS))
         return bail(fail(CVT_EHIP, "upload failed"));
     launch_fill(j->knn, (long)nS * p.K, 0, s);
@@ -785,6 +776,12 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatCopies * kStatStride, 0, s);
+    TempPool build_tmp;  // lives until the set-up synchronises below
+    {
+        void* csr_tmp = build_tmp.get((size_t)csr_temp_bytes(nV, nF));
+        if (!csr_tmp) return bail(fail(CVT_ENOMEM, "CSR temporaries"));
+        launch_build_csr(j->F, nF, nV, j->vf_off, j->vf_faces, csr_tmp, s);
+    }
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
     {
         MeshDev m{};
@@ -793,7 +790,6 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         launch_frames(m, j->FU, j->FW, s);
     }
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
-    TempPool build_tmp;  // lives until the set-up synchronises below
     {
         // point LBVH over the vertex grid's cell-sorted points (leaf = slot)
         BVHDev bd;

@@ -155,6 +155,10 @@ void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipS
 // Per-vertex tangent frames: FP32 from N, or from the FP64 normal where the
 // axis switch |n_z| > 0.9 is within FP32 error (or N is degenerate).
 void launch_frames(const MeshDev& m, float4* FU, float4* FW, hipStream_t s);
+// Vertex -> incident-face CSR of F (nF faces) on the device, into vf_off
+// (nV + 1) and vf_faces (3 nF); tmp holds csr_temp_bytes(nV, nF) bytes.
+long csr_temp_bytes(int nV, int nF);
+void launch_build_csr(const int* F, int nF, int nV, int* vf_off, int* vf_faces, void* tmp, hipStream_t s);
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
 bool lloyd_supported(int K, int kproj);

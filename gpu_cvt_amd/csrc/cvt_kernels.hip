@@ -11,6 +11,7 @@
 
 #include <algorithm>
 
+#include "cvt_bvh.h"
 #include "cvt_device.h"
 #include "cvt_kernels.h"
 #include "cvt_step.h"
@@ -590,6 +591,52 @@ void launch_grid_build(const float4* P, int n, const GridP& g, GridBufs& b, hipS
     hipLaunchKernelGGL(k_grid_count, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, g, b.count, b.bucket, b.rank);
     launch_scan(b.count, T1, b.start, b.bsum, s);
     hipLaunchKernelGGL(k_grid_scatter, dim3(nblk(n, 256)), dim3(256), 0, s, P, n, b.bucket, b.rank, b.start, b.pts);
+}
+
+// Vertex -> incident-face CSR on the device (main.cu:259-285: faces in face
+// order within each vertex's fan): the keys (vertex << 32 | face) of the 3nF
+// corners, sorted, give vf_faces; the per-vertex counts, scanned, give vf_off.
+__global__ void k_csr_keys(const int* __restrict__ F, long n3, unsigned long long* __restrict__ keys,
+                           int* __restrict__ cnt) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n3) return;
+    const int v = F[e];
+    keys[e] = ((unsigned long long)(unsigned)v << 32) | (unsigned)(e / 3);
+    atomicAdd(&cnt[v], 1);
+}
+
+__global__ void k_csr_unpack(const unsigned long long* __restrict__ keys, long n3, int* __restrict__ vf_faces) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n3) vf_faces[e] = (int)(keys[e] & 0xffffffffull);
+}
+
+long csr_temp_bytes(int nV, int nF) {
+    const long n3 = 3L * nF;
+    return 2 * 8 * n3 + 4 * (radix_hist_words((int)n3) + 1 + scan_partials(radix_hist_words((int)n3))) +
+           4 * ((long)nV + 1 + scan_partials(nV + 1)) + 6 * 256;
+}
+
+void launch_build_csr(const int* F, int nF, int nV, int* vf_off, int* vf_faces, void* tmp, hipStream_t s) {
+    const long n3 = 3L * nF;
+    auto take = [&](size_t bytes) {
+        void* p = tmp;
+        tmp = (char*)tmp + ((bytes + 255) & ~size_t(255));
+        return p;
+    };
+    auto* keys = (unsigned long long*)take(8 * n3);
+    auto* alt = (unsigned long long*)take(8 * n3);
+    const long hw = radix_hist_words((int)n3);
+    int* hist = (int*)take(4 * (hw + 1));
+    int* hsum = (int*)take(4 * scan_partials(hw));
+    int* cnt = (int*)take(4 * ((long)nV + 1));
+    int* bsum = (int*)take(4 * scan_partials(nV + 1));
+    int bits = 32;
+    while (bits < 64 && ((unsigned long long)(nV - 1) >> (bits - 32)) != 0) ++bits;
+    launch_fill(cnt, nV, 0, s);
+    hipLaunchKernelGGL(k_csr_keys, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, F, n3, keys, cnt);
+    radix_sort_u64(keys, (int)n3, bits, alt, hist, hsum, s);
+    hipLaunchKernelGGL(k_csr_unpack, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, keys, n3, vf_faces);
+    launch_scan(cnt, nV, vf_off, bsum, s);
 }
 
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
