@@ -1262,6 +1262,33 @@ static float sq_threshold(double T) {
     return f;
 }
 
+int check_fast_enqueue(cvt_job* j, double eps, TempPool& T, double tol, int* flag_host, int* enqueued) {
+    *enqueued = 0;
+    if (!(tol >= 0) || j->nS >= (1 << 21)) return 0;
+    int rc;
+    if ((rc = ensure_bvh(j))) return rc;
+    T.reset();
+    const int K = j->p.K, nS = j->nS, M = K + 4;
+    int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
+    auto* keys = (unsigned long long*)T.get(8 * (size_t)nS * M);
+    float4* site_n = (float4*)T.get(sizeof(float4) * nS);
+    const int nst = bvh_check_state_ints();
+    int* flag = (int*)T.get(sizeof(int) * nst);  // verdict, probe counters
+    // probes of ~2 triangles per site (a closed surface), with slack; an
+    // overflow leaves the verdict undecided (2)
+    const int cap = 9 * nS + 1024;
+    float4* probes = (float4*)T.get(sizeof(float4) * (size_t)cap);
+    if (!knn_tmp || !keys || !site_n || !flag || !probes) return fail(CVT_ENOMEM, "check temporaries");
+    hipStream_t s = j->stream;
+    launch_batch_grid(j->tables, 1, s);
+    launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+    launch_fill(flag, nst, 0, s);
+    bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), probes, cap, flag, s);
+    HIPCHK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    *enqueued = 1;
+    return 0;
+}
+
 int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, bool exact_always, double* h_out,
                   int* nF_out) {
     if (!j) return fail(CVT_EINVAL, "null argument");
@@ -1269,40 +1296,25 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     if ((rc = ensure_bvh(j))) return rc;
     T.reset();
     bool fast_within = false;  // the fast path below proved the job within tol
-    if (tol >= 0 && within && !exact_always && j->nS < (1 << 21)) {
+    if (within && !exact_always) {
         // The verdict straight from the remesh cells' candidate triples (each
         // taken once, at its smallest emitting site; no sort, one host sync);
         // the full remesh below runs only for a job within tolerance, for its
         // distance.  Both give the same verdict: the same triangles, probes
         // and bounded queries.
-        const int K = j->p.K, nS = j->nS, M = K + 4;
-        int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
-        auto* keys = (unsigned long long*)T.get(8 * (size_t)nS * M);
-        float4* site_n = (float4*)T.get(sizeof(float4) * nS);
-        const int nst = bvh_check_state_ints();
-        int* flag = (int*)T.get(sizeof(int) * nst);  // verdict, probe counters
-        // probes of ~2 triangles per site (a closed surface), with slack; an
-        // overflow leaves the verdict to the full remesh below
-        const int cap = 9 * nS + 1024;
-        float4* probes = (float4*)T.get(sizeof(float4) * (size_t)cap);
-        if (!knn_tmp || !keys || !site_n || !flag || !probes) return fail(CVT_ENOMEM, "check temporaries");
-        hipStream_t s = j->stream;
-        launch_batch_grid(j->tables, 1, s);
-        launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
-        launch_fill(flag, nst, 0, s);
-        bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), probes, cap, flag,
-                       s);
-        int hf = 0;
-        HIPCHK(hipMemcpyAsync(&hf, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (hf == 1) {
-            *within = 0;
-            if (h_out) *h_out = -1;
-            if (nF_out) *nF_out = -1;
-            return 0;
+        int hf = 0, enq = 0;
+        if ((rc = check_fast_enqueue(j, eps, T, tol, &hf, &enq))) return rc;
+        if (enq) {
+            HIPCHK(hipStreamSynchronize(j->stream));
+            if (hf == 1) {
+                *within = 0;
+                if (h_out) *h_out = -1;
+                if (nF_out) *nF_out = -1;
+                return 0;
+            }
+            fast_within = hf == 0;
+            T.reset();
         }
-        fast_within = hf == 0;
-        T.reset();
     }
     int* dF = nullptr;
     int nkeep = 0;

@@ -55,6 +55,15 @@ struct TempPool {
     std::vector<Slot> slots;
     size_t next = 0;
     DevArena* arena = current_arena();
+    TempPool() = default;
+    // one owner per buffer: a copy would free the buffers the other still uses
+    TempPool(const TempPool&) = delete;
+    TempPool& operator=(const TempPool&) = delete;
+    TempPool(TempPool&& o) noexcept : slots(std::move(o.slots)), next(o.next), arena(o.arena) {
+        o.slots.clear();
+        o.next = 0;
+    }
+    TempPool& operator=(TempPool&&) = delete;
     void reset() { next = 0; }
     void* get(size_t bytes) {
         bytes = std::max<size_t>(bytes, 256);
@@ -88,5 +97,12 @@ struct TempPool {
 // it does or exact_always is set (else *h_out = -1).  tol < 0: exact only.
 __attribute__((visibility("hidden"))) int job_hausdorff(cvt_job* j, double eps, TempPool& pool, double tol,
                                                         int* within, bool exact_always, double* h_out, int* nF_out);
+// The fast convergence verdict of job_hausdorff, enqueued on the job's stream
+// with pool-held temporaries: *enqueued = 1, and *flag_host receives 0 (every
+// probe within tol), 1 (not) or 2 (undecided) once that stream is
+// synchronised (the pool must not be reused before then); *enqueued = 0 when
+// the fast path does not apply (no tolerance, or nS >= 2^21).
+__attribute__((visibility("hidden"))) int check_fast_enqueue(cvt_job* j, double eps, TempPool& pool, double tol,
+                                                             int* flag_host, int* enqueued);
 
 }  // namespace cvt
