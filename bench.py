@@ -3,9 +3,9 @@
 Workload (default): displaced cube-sphere with 1,997,568 triangles (998,786
 vertices), 1,000,000 area-uniform sites, K = 24, freezing off, FP32 -- the
 large-job shape the headline metric is quoted on.  A "step" is one Lloyd
-iteration over all sites (site-grid rebuild + fused KNN / frame / cell /
-projection kernel), launched as HIP graphs; inputs are resident in HBM before
-the timed region.
+iteration over all sites (site-grid rebuild, then the frame, cell and
+projection kernels), launched as one HIP graph; inputs are resident in HBM
+before the timed region.
 
 Prints one JSON line (rank 0).  Also measures, in the same run:
   * roofline: algorithmic bytes of one Lloyd iteration divided by the mean
@@ -13,7 +13,9 @@ Prints one JSON line (rank 0).  Also measures, in the same run:
     per-site work, kb_frame + kb_cell + kb_project, launched back to back --
     timed with HIP events on the job's stream;
   * cpu_baseline: the FP64 CPU oracle's Lloyd step on this host, on a bounded
-    sample (one iteration of a sub-sampled site set on the same mesh).
+    sample (a sub-sampled site set on the same mesh, or the whole job over as
+    many iterations as fill about half of --cpu-budget seconds).
+Other modes: batch, serve, check (see --mode).
 """
 from __future__ import annotations
 
@@ -254,6 +256,41 @@ def bench_serve(args) -> int:
     return 0
 
 
+def bench_check(args) -> int:
+    """Latency of the convergence check (cvt_job_hausdorff_within) on a
+    workload after --warmup iterations: the bounded verdict alone (a tolerance
+    the job does not meet yet) and the verdict plus the exact distance (a
+    tolerance it meets), each the median of --steps calls."""
+    from gpu_cvt_amd import cvt, meshgen as mg
+
+    V, F, S, K, _iters, _fz = mg.workload(args.workload)
+    t0 = time.perf_counter()
+    job = cvt.Job(V, F, S, cvt.default_params(K=K, freeze_mode=0))
+    t_create = time.perf_counter() - t0
+    job.step(args.warmup)
+    _ok, h = job.hausdorff_within(1e30)  # builds the LBVH; the distance now
+    out = {}
+    for name, tol in [("beyond", 0.5 * h), ("within", 2.0 * h)]:
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t1 = time.perf_counter()
+            ok, _h = job.hausdorff_within(tol)
+            ts.append(time.perf_counter() - t1)
+            assert ok == (name == "within")
+        out[name] = 1e3 * sorted(ts)[len(ts) // 2]
+        print(f"[bench] check {name} tolerance: median {out[name]:.2f} ms over {len(ts)} calls", file=sys.stderr,
+              flush=True)
+    job.close()
+    print(json.dumps({"metric": "convergence check latency (p50 of cvt_job_hausdorff_within)",
+                      "value": out["beyond"], "unit": "ms", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+                      "ms_per_step": None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+                      "dtype": "f32", "data": "synthetic",
+                      "config": {"workload": args.workload, "sites": len(S), "triangles": len(F), "K": K},
+                      "within_with_distance_ms": out["within"], "job_create_ms": 1e3 * t_create,
+                      "hausdorff": h}))
+    return 0
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -264,9 +301,10 @@ def main(argv=None) -> int:
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--profile-iters", type=int, default=5)
-    ap.add_argument("--mode", choices=["plain", "batch", "serve"], default="plain",
+    ap.add_argument("--mode", choices=["plain", "batch", "serve", "check"], default="plain",
                     help="plain: the headline 1M-site job; batch: 256 mixed jobs, batched vs one at a time; "
-                         "serve: Poisson request trace through the serving loop vs FIFO plain path")
+                         "serve: Poisson request trace through the serving loop vs FIFO plain path; "
+                         "check: convergence-check latency on the workload")
     ap.add_argument("--arrivals", type=int, default=2000, help="serve mode: requests in the trace")
     ap.add_argument("--hausdorff-samples", type=int, default=0,
                     help="plain mode: after the timed steps, symmetric Hausdorff remesh<->input with this many "
@@ -305,6 +343,8 @@ def main(argv=None) -> int:
         return bench_batch(args)
     if args.mode == "serve":
         return bench_serve(args)
+    if args.mode == "check":
+        return bench_check(args)
 
     t_gen = time.perf_counter()
     V, F, S, K, _iters, _fz = mg.workload(args.workload)
