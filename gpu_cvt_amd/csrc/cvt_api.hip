@@ -1167,17 +1167,20 @@ int cvt_job_energy(cvt_job* j, double* E) {
 // reusing them): dF receives 3 * nkeep vertex ids.
 using DevTemps = TempPool;
 
-static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int* nkeep_out) {
+// keys_pre/site_n_pre (nullable): the remesh cells' keys and site normals of
+// the current sites, already computed (by the fast convergence verdict).
+static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int* nkeep_out,
+                         unsigned long long* keys_pre = nullptr, float4* site_n_pre = nullptr) {
     if (j->nS >= (1 << 21)) return fail(CVT_EINVAL, "remesh keys pack site ids in 21 bits (nS < 2097152)");
     const int K = j->p.K, M = K + 4, nS = j->nS;
     const long nk = (long)nS * M;
     if (nk > (1L << 31) - 1) return fail(CVT_EINVAL, "too many candidate triangles");
     hipStream_t s = j->stream;
     const long hw = radix_hist_words((int)nk);
-    int* knn_tmp = (int*)T.get(sizeof(int) * (size_t)nS * K);
-    auto* keys = (unsigned long long*)T.get(8 * (size_t)nk);
+    int* knn_tmp = keys_pre ? nullptr : (int*)T.get(sizeof(int) * (size_t)nS * K);
+    auto* keys = keys_pre ? keys_pre : (unsigned long long*)T.get(8 * (size_t)nk);
     auto* keys_alt = (unsigned long long*)T.get(8 * (size_t)nk);
-    float4* site_n = (float4*)T.get(sizeof(float4) * nS);
+    float4* site_n = keys_pre ? site_n_pre : (float4*)T.get(sizeof(float4) * nS);
     int* hist = (int*)T.get(4 * (size_t)(hw + 1));
     int* hsum = (int*)T.get(4 * (size_t)scan_partials(std::max(hw, nk) + 1));
     int* flag = (int*)T.get(4 * (size_t)nk);
@@ -1185,10 +1188,13 @@ static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int*
     int4* tri = (int4*)T.get(sizeof(int4) * (size_t)nk);
     int* keep = (int*)T.get(4 * (size_t)nk);
     int* pos2 = (int*)T.get(4 * (size_t)(nk + 1));
-    if (!knn_tmp || !keys || !keys_alt || !site_n || !hist || !hsum || !flag || !pos || !tri || !keep || !pos2)
+    if ((!keys_pre && !knn_tmp) || !keys || !keys_alt || !site_n || !hist || !hsum || !flag || !pos || !tri ||
+        !keep || !pos2)
         return fail(CVT_ENOMEM, "remesh temporaries");
-    launch_batch_grid(j->tables, 1, s);
-    launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+    if (!keys_pre) {
+        launch_batch_grid(j->tables, 1, s);
+        launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+    }
     radix_sort_u64(keys, (int)nk, 64, keys_alt, hist, hsum, s);
     launch_unique_mark(keys, (int)nk, flag, s);
     launch_scan(flag, (int)nk, pos, hsum, s);
@@ -1262,7 +1268,8 @@ static float sq_threshold(double T) {
     return f;
 }
 
-int check_fast_enqueue(cvt_job* j, double eps, TempPool& T, double tol, int* flag_host, int* enqueued) {
+int check_fast_enqueue(cvt_job* j, double eps, TempPool& T, double tol, int* flag_host, int* enqueued,
+                       unsigned long long** keys_out, float4** site_n_out) {
     *enqueued = 0;
     if (!(tol >= 0) || j->nS >= (1 << 21)) return 0;
     int rc;
@@ -1286,6 +1293,8 @@ int check_fast_enqueue(cvt_job* j, double eps, TempPool& T, double tol, int* fla
     bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), probes, cap, flag, s);
     HIPCHK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));
     *enqueued = 1;
+    if (keys_out) *keys_out = keys;
+    if (site_n_out) *site_n_out = site_n;
     return 0;
 }
 
@@ -1296,6 +1305,8 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
     if ((rc = ensure_bvh(j))) return rc;
     T.reset();
     bool fast_within = false;  // the fast path below proved the job within tol
+    unsigned long long* fast_keys = nullptr;
+    float4* fast_site_n = nullptr;
     if (within && !exact_always) {
         // The verdict straight from the remesh cells' candidate triples (each
         // taken once, at its smallest emitting site; no sort, one host sync);
@@ -1303,7 +1314,7 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
         // distance.  Both give the same verdict: the same triangles, probes
         // and bounded queries.
         int hf = 0, enq = 0;
-        if ((rc = check_fast_enqueue(j, eps, T, tol, &hf, &enq))) return rc;
+        if ((rc = check_fast_enqueue(j, eps, T, tol, &hf, &enq, &fast_keys, &fast_site_n))) return rc;
         if (enq) {
             HIPCHK(hipStreamSynchronize(j->stream));
             if (hf == 1) {
@@ -1313,12 +1324,16 @@ int job_hausdorff(cvt_job* j, double eps, TempPool& T, double tol, int* within, 
                 return 0;
             }
             fast_within = hf == 0;
-            T.reset();
+            // the remesh below reuses its keys and normals (the pool is not reset)
+            if (!fast_within) fast_keys = nullptr;
+            if (!fast_within) T.reset();
+        } else {
+            fast_keys = nullptr;
         }
     }
     int* dF = nullptr;
     int nkeep = 0;
-    if ((rc = remesh_device(j, eps, T, &dF, &nkeep))) return rc;
+    if ((rc = remesh_device(j, eps, T, &dF, &nkeep, fast_keys, fast_site_n))) return rc;
     if (nF_out) *nF_out = nkeep;
     if (nkeep == 0) {
         if (within) *within = 1;

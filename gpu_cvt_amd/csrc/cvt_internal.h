@@ -102,7 +102,11 @@ __attribute__((visibility("hidden"))) int job_hausdorff(cvt_job* j, double eps, 
 // probe within tol), 1 (not) or 2 (undecided) once that stream is
 // synchronised (the pool must not be reused before then); *enqueued = 0 when
 // the fast path does not apply (no tolerance, or nS >= 2^21).
+// keys_out/site_n_out (nullable) receive its remesh-cell keys and site normals
+// (pool buffers), which a remesh of the same sites can reuse.
 __attribute__((visibility("hidden"))) int check_fast_enqueue(cvt_job* j, double eps, TempPool& pool, double tol,
-                                                             int* flag_host, int* enqueued);
+                                                             int* flag_host, int* enqueued,
+                                                             unsigned long long** keys_out = nullptr,
+                                                             float4** site_n_out = nullptr);
 
 }  // namespace cvt
