@@ -697,9 +697,12 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     }
     const double kPi = 3.14159265358979323846;
     // cell = scale x the radius that holds K+2 (kproj+2) points at the mean
-    // surface density, so the 3x3x3 block usually covers the search
-    const double ss = p.site_cell_scale > 0 ? p.site_cell_scale : 1.2;
-    const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.4;
+    // surface density, so the 3x3x3 block usually covers the search.  The
+    // scales are measured optima on the 1M-site displaced sphere (K = 24):
+    // site 1.3 (1.2: +4 %, 1.1: +26 %), vertex 1.2 (flat over 0.9-1.25, +4 %
+    // at 1.7)
+    const double ss = p.site_cell_scale > 0 ? p.site_cell_scale : 1.3;
+    const double vs = p.vert_cell_scale > 0 ? p.vert_cell_scale : 1.2;
     const double rK = std::sqrt((p.K + 2) / (kPi * (nS / j->area)));
     const double rV = std::sqrt((p.kproj + 2) / (kPi * (nV / j->area)));
     j->sgp = make_grid(glo, ghi, ss * rK, 32L * nS + 4096);
