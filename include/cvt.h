@@ -38,8 +38,8 @@ typedef struct cvt_params {
     int nan_policy;     /* 0 keep the old site (scripts/testfreeze.py:413-425), 1 propagate NaN (2dClip.cuh:129-132) */
     int clip_R_mode;    /* 0: R = mesh bbox max edge (main.cu:226,711), 1: R = clip_R */
     double clip_R;
-    double site_cell_scale;  /* site grid cell size in units of the expected K-NN radius (0 = default) */
-    double vert_cell_scale;  /* vertex grid cell size in units of the expected kproj-NN radius (0 = default) */
+    double site_cell_scale;  /* site grid cell size in units of the expected (K+2)-NN radius (0 = default, 1.3) */
+    double vert_cell_scale;  /* vertex grid cell size in units of the expected (kproj+2)-NN radius (0 = default, 1.2) */
     int debug;          /* 1: record per-site intermediates of the last step (cvt_job_get_debug) */
     int reserved[7];
 } cvt_params;
