@@ -170,6 +170,35 @@ def test_cluster_knn_exact(orc):
                 print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
 
 
+@pytest.mark.parametrize("scales", [(0.7, 0.7), (2.2, 2.6)])
+def test_grid_cell_size_invariance(orc, scales):
+    """The site and vertex grids only accelerate exact searches: cells far
+    smaller than the neighbour radius (the 3x3x3 block rarely holds the K+2
+    nearest, so the ring and LBVH fallbacks run) or far larger (long rows)
+    must give the same lists as the default grids, and the same sites up to
+    the FP32 bar (a site proven in FP32 on one grid may be deferred to the
+    FP64 pass on another)."""
+    V, F, S, K = _case("torus_20k_k24")
+    mesh = orc.Mesh(V, F)
+    with cvt.Job(V, F, S, K=K) as ref:
+        ref.step(3)
+        S_ref = ref.sites()
+    with cvt.Job(V, F, S, K=K, debug=1, site_cell_scale=scales[0], vert_cell_scale=scales[1]) as job:
+        for it in range(3):
+            S32 = orc.f32r(job.sites())
+            job.step(1)
+            if it == 0:
+                r = _stage_check(orc, mesh, job, S32, K)
+                print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
+            else:
+                assert np.array_equal(job.knn(), orc.knn(S32, S32, K, exclude_self=True)), f"iter {it}: site KNN"
+        S_job = job.sites()
+    err = np.linalg.norm(S_job - S_ref, axis=1)
+    print(f"  vs default grids after 3 iterations: {np.mean(err == 0):.4f} identical, max err {err.max():.2e}",
+          flush=True)
+    assert err.max() <= TOL_REL_DIAG * mesh.bbox_diag
+
+
 def test_vertex_normals(orc):
     V, F = mg.torus(1.0, 0.3, 60, 24, jitter=1e-3, seed=3)
     mesh = orc.Mesh(V, F)
