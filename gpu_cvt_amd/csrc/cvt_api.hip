@@ -608,6 +608,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     for (long f = 0; f < 3L * nF; ++f)
         if (Fh[f] < 0 || Fh[f] >= nV) return fail(CVT_EINVAL, "face index out of range at %ld", f);
     if ((long)nS * p.K > (1L << 31) - 1) return fail(CVT_EINVAL, "nS*K too large");
+    // the vertex->face CSR, its radix sort and the face LBVH index 3 nF corners with int
+    if (3L * nF > (1L << 31) - 1) return fail(CVT_EINVAL, "3*nF too large (nF=%d)", nF);
 
     cvt_job* j = new cvt_job();
     j->p = p;
@@ -1544,6 +1546,7 @@ int cvt_job_point_mesh_d2(cvt_job* j, const double* P, int n, double* d2) {
 
 int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out) {
     if (!out || !V || !F || nV < 3 || nF < 1) return fail(CVT_EINVAL, "bad argument");
+    if (3L * nF > (1L << 31) - 1) return fail(CVT_EINVAL, "3*nF too large (nF=%d)", nF);
     *out = nullptr;
     if (!check_finite(V, 3L * nV)) return fail(CVT_ENONFINITE, "non-finite vertex coordinate");
     for (long f = 0; f < 3L * nF; ++f)

@@ -17,10 +17,12 @@ long radix_hist_words(int n);
 // Sorts keys/vals in place (ping-pongs through the alt buffers, copies back if
 // the pass count is odd).  `bits` = number of significant low key bits.
 void radix_sort_pairs(unsigned* keys, int* vals, int n, int bits, RadixBufs& b, hipStream_t s);
-// Keys-only stable LSD sort of 64-bit keys; hist needs radix_hist_words(n)+1
-// ints, hsum scan_partials(radix_hist_words(n)) ints.
+// Keys-only stable LSD sort of 64-bit keys by their bits [lo_bit, bits)
+// (keys already ordered by the bits below lo_bit keep that order among
+// equals); hist needs radix_hist_words(n)+1 ints, hsum
+// scan_partials(radix_hist_words(n)) ints.
 void radix_sort_u64(unsigned long long* keys, int n, int bits, unsigned long long* keys_alt, int* hist, int* hsum,
-                    hipStream_t s);
+                    hipStream_t s, int lo_bit = 0);
 
 // LBVH: internal nodes [0, n-1), leaves [n-1, 2n-1); one triangle per leaf.
 struct BVHDev {

@@ -90,7 +90,7 @@ __global__ void k_copy_pairs(const KeyT* __restrict__ ks, const int* __restrict_
 
 template <class KeyT>
 static void radix_sort_impl(KeyT* keys, int* vals, int n, int bits, KeyT* keys_alt, int* vals_alt, int* hist,
-                            int* hsum, hipStream_t s) {
+                            int* hsum, hipStream_t s, int lo_bit = 0) {
     if (n <= 1) return;
     const int nb = nblk(n, kRadixTile);
     KeyT* ka = keys;
@@ -98,7 +98,7 @@ static void radix_sort_impl(KeyT* keys, int* vals, int n, int bits, KeyT* keys_a
     KeyT* kb = keys_alt;
     int* vb = vals ? vals_alt : nullptr;
     int passes = 0;
-    for (int shift = 0; shift < bits; shift += 8, ++passes) {
+    for (int shift = lo_bit; shift < bits; shift += 8, ++passes) {
         hipLaunchKernelGGL(k_radix_hist<KeyT>, dim3(nb), dim3(256), 0, s, ka, n, shift, hist);
         launch_scan(hist, 256 * nb, hist, hsum, s);
         hipLaunchKernelGGL(k_radix_scatter<KeyT>, dim3(nb), dim3(256), 0, s, ka, va, n, shift, hist, kb, vb);
@@ -114,8 +114,8 @@ void radix_sort_pairs(unsigned* keys, int* vals, int n, int bits, RadixBufs& b, 
 }
 
 void radix_sort_u64(unsigned long long* keys, int n, int bits, unsigned long long* keys_alt, int* hist, int* hsum,
-                    hipStream_t s) {
-    radix_sort_impl<unsigned long long>(keys, nullptr, n, bits, keys_alt, nullptr, hist, hsum, s);
+                    hipStream_t s, int lo_bit) {
+    radix_sort_impl<unsigned long long>(keys, nullptr, n, bits, keys_alt, nullptr, hist, hsum, s, lo_bit);
 }
 
 // ------------------------------------------------------------------ LBVH

@@ -634,7 +634,9 @@ void launch_build_csr(const int* F, int nF, int nV, int* vf_off, int* vf_faces, 
     while (bits < 64 && ((unsigned long long)(nV - 1) >> (bits - 32)) != 0) ++bits;
     launch_fill(cnt, nV, 0, s);
     hipLaunchKernelGGL(k_csr_keys, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, F, n3, keys, cnt);
-    radix_sort_u64(keys, (int)n3, bits, alt, hist, hsum, s);
+    // the keys are written in ascending face order and the sort is stable, so
+    // sorting by the vertex bits alone keeps each fan in face order
+    radix_sort_u64(keys, (int)n3, bits, alt, hist, hsum, s, 32);
     hipLaunchKernelGGL(k_csr_unpack, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, keys, n3, vf_faces);
     launch_scan(cnt, nV, vf_off, bsum, s);
 }
