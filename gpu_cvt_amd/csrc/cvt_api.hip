@@ -224,6 +224,7 @@ struct cvt_job {
 // A device LBVH over one triangle mesh (owned buffers).
 struct cvt_bvh {
     BVHDev d;
+    int device = 0;  // the GPU the LBVH lives on (current at creation)
     float4* tri = nullptr;
     std::vector<void*> allocs;
     std::vector<size_t> alloc_bytes;
@@ -257,6 +258,7 @@ int bvh_make(const double* Vh, int nV, const int* Fh, int nF, const float4* dev_
     cvt_bvh* b = new cvt_bvh();
     b->nF = nF;
     auto bail = [&](int rc) { delete b; return rc; };
+    if (hipGetDevice(&b->device) != hipSuccess) return bail(fail(CVT_EHIP, "hipGetDevice failed"));
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
@@ -1557,6 +1559,30 @@ int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out)
 int cvt_bvh_point_d2(cvt_bvh* b, const double* P, int n, double* d2) {
     if (!b || n < 0 || (n > 0 && (!P || !d2))) return fail(CVT_EINVAL, "bad argument");
     return bvh_query(b, P, n, d2);
+}
+
+// Allocation-free variant of cvt_bvh_point_d2: the points and distances are
+// staged in the library's fixed per-GPU device area (noalloc_scratch) instead
+// of per-call hipMalloc buffers; the same kernel arithmetic, so the same values.
+int cvt_bvh_point_d2_noalloc(cvt_bvh* b, const double* P, int n, double* d2) {
+    if (!b || n < 0 || (n > 0 && (!P || !d2))) return fail(CVT_EINVAL, "bad argument");
+    const size_t cap = kNoAllocScratchBytes / (4 * sizeof(double));
+    if ((size_t)n > cap)
+        return fail(CVT_EINVAL, "n=%d points exceed the %zu one allocation-free call can stage", n, cap);
+    if (n == 0) return 0;
+    int dev = -1;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev != b->device) return fail(CVT_ESTATE, "the LBVH lives on device %d, device %d is current", b->device, dev);
+    void* base = nullptr;
+    if (noalloc_scratch(&base)) return fail(CVT_EHIP, "device staging area address lookup failed");
+    double* dP = (double*)base;   // 3 cap doubles of points,
+    double* dD = dP + 3 * cap;    // then cap doubles of distances
+    HIPCHK(hipMemcpyAsync(dP, P, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, b->stream));
+    bvh_point_d2_f64(b->d, dP, n, dD, b->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(d2, dD, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
 }
 
 void cvt_bvh_destroy(cvt_bvh* b) { delete b; }

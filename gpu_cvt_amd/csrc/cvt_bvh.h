@@ -48,6 +48,14 @@ void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream
 // distance of P[0, n) to the mesh, the same value as the maximum of
 // bvh_point_d2's outputs, found by branch and bound.
 void bvh_max_d2(const BVHDev& b, const float4* P, int n, unsigned* mx, hipStream_t s);
+// bvh_point_d2 on points given as n*3 doubles (device), distances as doubles
+// (device): the same FP32 traversal, so the same values.
+void bvh_point_d2_f64(const BVHDev& b, const double* P, int n, double* out, hipStream_t s);
+// Size of the library's fixed device staging area for allocation-free point
+// queries, and its address on the current device (0 on success).  The area
+// holds 3 doubles of input and 1 of output per point.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+int noalloc_scratch(void** out);
 // out[i] <= thr2 iff some triangle lies within thr2 of P[i] (+inf otherwise)
 void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float* out, hipStream_t s);
 // st[0] = 1 iff a probe of the remeshed surface -- the union of the remesh

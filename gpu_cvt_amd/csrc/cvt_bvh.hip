@@ -4,6 +4,8 @@
 // (evaluate.cuh:206-250).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "cvt_bvh.h"
 #include "cvt_kernels.h"
 
@@ -548,6 +550,49 @@ void bvh_point_within(const BVHDev& b, const float4* P, int n, float thr2, float
 void bvh_point_d2(const BVHDev& b, const float4* P, int n, float* out, hipStream_t s) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_point_d2, dim3(nblk(n, 256)), dim3(256), 0, s, b, P, n, out);
+}
+
+// ------------------------------------------------- allocation-free point queries
+// Staging memory of bvh_point_d2_f64's callers that may not allocate: the
+// query points as the caller's doubles (3 per point) in the first 3/4, their
+// squared distances (1 double per point) in the last 1/4.  Every GPU holds its
+// own copy at its own address for the life of the process; host code reaches
+// it only through noalloc_scratch(), never through this name (which on the host
+// is a stand-in, not a device address).  Calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Same traversal and FP32 distance as k_point_d2, on points given as doubles
+// (rounded to FP32 as the host path rounds them), distances widened to double.
+__global__ __launch_bounds__(256) void k_point_d2_f64(BVHDev b, const double* __restrict__ P, int np,
+                                                      double* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= np) return;
+    const float3 q = make_float3((float)P[3 * i], (float)P[3 * i + 1], (float)P[3 * i + 2]);
+    out[i] = (double)point_d2(b, q);
+}
+
+void bvh_point_d2_f64(const BVHDev& b, const double* P, int n, double* out, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_point_d2_f64, dim3(nblk(n, 256)), dim3(256), 0, s, b, P, n, out);
+}
+
+int noalloc_scratch(void** out) {
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static void* addr[kMaxDevices] = {};
+    *out = nullptr;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        // once per device, with that device current: each GPU's copy of the
+        // array lives at its own address
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        addr[dev] = p;
+    }
+    *out = addr[dev];
+    return 0;
 }
 
 }  // namespace cvt

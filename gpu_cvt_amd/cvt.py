@@ -29,7 +29,7 @@ EXPORTS = [
     "cvt_batch_create", "cvt_batch_step", "cvt_batch_step_timed", "cvt_batch_destroy", "cvt_job_hausdorff",
     "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
     "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
-    "cvt_job_compute_tiers", "cvt_job_hausdorff_within",
+    "cvt_job_compute_tiers", "cvt_job_hausdorff_within", "cvt_bvh_point_d2_noalloc",
 ]
 
 
@@ -175,6 +175,7 @@ def lib():
     L.cvt_bvh_create.argtypes = [_dp, C.c_int, _ip, C.c_int, C.POINTER(C.c_void_p)]
     L.cvt_bvh_point_d2.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_bvh_destroy.argtypes = [C.c_void_p]
+    L.cvt_bvh_point_d2_noalloc.argtypes = [C.c_void_p, _dp, C.c_int, _dp]
     L.cvt_server_config_default.argtypes = [C.POINTER(ServerConfig)]
     L.cvt_server_config_default.restype = None
     L.cvt_server_create.argtypes = [C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]
@@ -402,6 +403,14 @@ class BVH:
         P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
         out = np.zeros(len(P))
         _check("cvt_bvh_point_d2", lib().cvt_bvh_point_d2(self._h, P.reshape(-1), len(P), out))
+        return out
+
+    def point_d2_noalloc(self, P) -> np.ndarray:
+        """point_d2 staged in the library's fixed device area (no device
+        allocation; at most 1,048,576 points per call)."""
+        P = np.ascontiguousarray(P, dtype=np.float64).reshape(-1, 3)
+        out = np.zeros(len(P))
+        _check("cvt_bvh_point_d2_noalloc", lib().cvt_bvh_point_d2_noalloc(self._h, P.reshape(-1), len(P), out))
         return out
 
     def close(self) -> None:

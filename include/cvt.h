@@ -209,6 +209,17 @@ int cvt_bvh_create(const double* V, int nV, const int* F, int nF, cvt_bvh** out)
 int cvt_bvh_point_d2(cvt_bvh* bvh, const double* P, int n, double* d2_out);
 void cvt_bvh_destroy(cvt_bvh* bvh);
 
+/* The same query without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * serving loops, harnesses that forbid hipMalloc).  The points and distances
+ * are staged in a fixed 32 MiB device area that the library holds on each GPU
+ * for the life of the process, so n is at most CVT_NOALLOC_MAX_POINTS per call
+ * (CVT_EINVAL beyond).  Every call on one GPU shares that area: calls that
+ * overlap on one GPU race.  The LBVH's device must be current (CVT_ESTATE
+ * otherwise).  Results are identical to cvt_bvh_point_d2's. */
+#define CVT_NOALLOC_MAX_POINTS 1048576
+int cvt_bvh_point_d2_noalloc(cvt_bvh* bvh, const double* P, int n, double* d2_out);
+
 /* Serving form: a continuous-batching request loop.  The reference runs one
  * mesh per process invocation (main.cu:371-402) and checks convergence every
  * DUMP_STRIDE = 10 iterations with a Hausdorff evaluation (main.cu:395,
