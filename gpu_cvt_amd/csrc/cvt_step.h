@@ -475,8 +475,12 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
         if (has_prev) {
             const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
             const float ddx = sp.x - q.x, ddy = sp.y - q.y, ddz = sp.z - q.z;
+            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz, thr = a.thr2[tr];
             r.same = cr.same;
-            r.low = (ddx * ddx + ddy * ddy + ddz * ddz) <= a.thr2[tr];
+            r.low = d2 <= thr;
+            // within the FP32 error of the threshold: the FP64 test on the same
+            // FP32 sites (freezetest.cuh:190 compares |S' - S|^2 <= threshold)
+            if (fabsf(d2 - thr) <= 1e-6f * thr) r.low = key64(q.x, q.y, q.z, sp.x, sp.y, sp.z) <= (double)thr;
             r.both = r.same && r.low;
             if (a.freeze_mode != 0) {
                 int st = a.streak[i];

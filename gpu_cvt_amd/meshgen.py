@@ -126,6 +126,37 @@ def cube_sphere(n: int):
     return P, F.astype(np.int32)
 
 
+def cube(n: int):
+    """Closed cube surface [-1, 1]^3, n x n quads per face (two triangles
+    each), outward-oriented, edges welded: sharp edges and corners for the
+    curvature tiers."""
+    g = np.linspace(-1.0, 1.0, n + 1)
+    pts, idx, faces = [], {}, []
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in idx:
+            idx[key] = len(pts)
+            pts.append(p)
+        return idx[key]
+
+    for axis in range(3):
+        for sgn in (-1.0, 1.0):
+            u, v = [a for a in range(3) if a != axis]
+            for i in range(n):
+                for j in range(n):
+                    q = []
+                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
+                        p = np.zeros(3)
+                        p[axis], p[u], p[v] = sgn, g[i + di], g[j + dj]
+                        q.append(vid(p))
+                    if (sgn > 0) == (axis != 1):
+                        faces += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
+                    else:
+                        faces += [[q[0], q[2], q[1]], [q[0], q[3], q[2]]]
+    return np.array(pts), np.array(faces, np.int32)
+
+
 def _legendre_table(ls, ct: np.ndarray) -> dict:
     """Orthonormal associated Legendre values N_lm P_l^m(ct) for every l in
     ``ls`` and 0 <= m <= l, by the standard three-term recurrence in m-columns

@@ -302,35 +302,6 @@ def test_energy_decreases_and_matches_oracle(orc):
     assert np.isfinite(Sg).all()
 
 
-def _cube(n):
-    """Closed cube surface [-1, 1]^3, n x n quads per face, welded edges."""
-    g = np.linspace(-1.0, 1.0, n + 1)
-    pts, idx, faces = [], {}, []
-
-    def vid(p):
-        key = tuple(np.round(p, 12))
-        if key not in idx:
-            idx[key] = len(pts)
-            pts.append(p)
-        return idx[key]
-
-    for axis in range(3):
-        for sgn in (-1.0, 1.0):
-            u, v = [a for a in range(3) if a != axis]
-            for i in range(n):
-                for j in range(n):
-                    q = []
-                    for di, dj in ((0, 0), (1, 0), (1, 1), (0, 1)):
-                        p = np.zeros(3)
-                        p[axis], p[u], p[v] = sgn, g[i + di], g[j + dj]
-                        q.append(vid(p))
-                    if (sgn > 0) == (axis != 1):
-                        faces += [[q[0], q[1], q[2]], [q[0], q[2], q[3]]]
-                    else:
-                        faces += [[q[0], q[2], q[1]], [q[0], q[3], q[2]]]
-    return np.array(pts), np.array(faces, np.int32)
-
-
 @pytest.mark.parametrize("scheme,shape", [(1, "dsphere"), (2, "dsphere"), (1, "cube"), (2, "cube")])
 def test_device_tiers_match_oracle(orc, scheme, shape):
     """Freeze tiers computed on the device equal the oracle's from the same
@@ -339,7 +310,7 @@ def test_device_tiers_match_oracle(orc, scheme, shape):
     if shape == "dsphere":
         V, F = mg.displaced_sphere(30, seed=4, crease_seed=5)
     else:
-        V, F = _cube(24)
+        V, F = mg.cube(24)
     S = mg.sample_surface(V, F, 6000, seed=6)
     K = 24
     print(f"[tiers] scheme={scheme} shape={shape} nS={len(S)} K={K}", flush=True)
