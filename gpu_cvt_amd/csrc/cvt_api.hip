@@ -180,10 +180,11 @@ struct cvt_job {
     int* fnn = nullptr;       // FP32-pass hand-offs: frame vertex per site,
     int* slot_of = nullptr;   // each site's slot in the current site grid
     float4* cent = nullptr;   // centroid and cell flags per site
+    int* vs_list = nullptr;   // FP64 pass: sites waiting for a wave-wide vertex search
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
-    BlockRef *site_blocks = nullptr, *cell_blocks = nullptr, *tile_blocks = nullptr;
+    BlockRef *site_blocks = nullptr, *cell_blocks = nullptr, *tile_blocks = nullptr, *prec_blocks = nullptr;
     BatchTables tables{};
     // debug
     int* dbg_nn = nullptr;
@@ -191,7 +192,7 @@ struct cvt_job {
     float2* dbg_am = nullptr;
     // host-side facts
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    double diag = 0, R = 0, area = 0;
+    double diag = 0, R = 0, area = 0, max_edge = 0;
     int iters_done = 0;
     hipGraphExec_t graph = nullptr;
     cvt_bvh* bvh = nullptr;  // LBVH of the input mesh, built on first use
@@ -401,6 +402,7 @@ JobDesc make_desc(cvt_job* j) {
     a.m.vf_faces = to_g(j->vf_faces);
     a.m.nV = j->nV;
     a.m.nF = j->nF;
+    a.m.max_edge = (float)(j->max_edge * (1.0 + 1e-6));
     a.m.vg = j->vg;
     a.m.vb = j->vb;
     a.m.vg_start = to_g(j->vgb.start);
@@ -431,6 +433,7 @@ JobDesc make_desc(cvt_job* j) {
     a.fnn = to_g(j->fnn);
     a.slot_of = to_g(j->slot_of);
     a.cent = to_g(j->cent);
+    a.vs_list = to_g(j->vs_list);
     a.dmax01 = to_g(j->dmax);
     a.nS = j->nS;
     a.R = (float)j->R;
@@ -453,6 +456,7 @@ JobDesc make_desc(cvt_job* j) {
         }
     }
     a.nan_policy = j->p.nan_policy;
+    a.tol_abs = (float)(0.25e-5 * j->diag);
     if (j->p.debug) {
         a.dbg_nn = to_g(j->dbg_nn);
         a.dbg_U = to_g(j->dbg_U);
@@ -471,11 +475,12 @@ JobDesc make_desc(cvt_job* j) {
 
 // Block tables of a set of jobs (job index k refers to descs[k]).
 struct HostTables {
-    std::vector<BlockRef> site, cell, tile;
+    std::vector<BlockRef> site, cell, tile, prec;
 };
 
 void add_job_blocks(HostTables& h, int k, int nS, int ncell1) {
     for (int b = 0; b < (nS + 255) / 256; ++b) h.site.push_back({k, b});
+    for (int b = 0; b < precise_blocks(nS); ++b) h.prec.push_back({k, b});
     for (int b = 0; b < (ncell1 + 1023) / 1024; ++b) h.cell.push_back({k, b});  // 1024 counters per block
     for (int b = 0; b < (ncell1 + scan_tile() - 1) / scan_tile(); ++b) h.tile.push_back({k, b});
 }
@@ -610,6 +615,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     for (long f = 0; f < 3L * nF; ++f)
         if (Fh[f] < 0 || Fh[f] >= nV) return fail(CVT_EINVAL, "face index out of range at %ld", f);
     if ((long)nS * p.K > (1L << 31) - 1) return fail(CVT_EINVAL, "nS*K too large");
+    // the FP64 pass's work list packs a site's slot in 28 bits
+    if (nS >= (1 << 28)) return fail(CVT_EINVAL, "nS must be below 2^28 (nS=%d)", nS);
     // the vertex->face CSR, its radix sort and the face LBVH index 3 nF corners with int
     if (3L * nF > (1L << 31) - 1) return fail(CVT_EINVAL, "3*nF too large (nF=%d)", nF);
 
@@ -671,7 +678,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         });
     }
     std::vector<float4> tri((size_t)3 * nF);
-    std::vector<double> farea(nF);
+    std::vector<double> farea(nF), fedge(nF);
     parallel_chunks(nF, [&](int, size_t b, size_t e2) {
         for (size_t f = b; f < e2; ++f) {
             const float4 A = V4[Fi[3 * f]], B = V4[Fi[3 * f + 1]], C = V4[Fi[3 * f + 2]];
@@ -686,11 +693,18 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
             double vx = (double)C.x - A.x, vy = (double)C.y - A.y, vz = (double)C.z - A.z;
             double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
             farea[f] = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+            const double wx = (double)C.x - B.x, wy = (double)C.y - B.y, wz = (double)C.z - B.z;
+            fedge[f] = std::sqrt(std::max({ux * ux + uy * uy + uz * uz, vx * vx + vy * vy + vz * vz,
+                                           wx * wx + wy * wy + wz * wz}));
         }
     });
-    double area = 0;
-    for (int f = 0; f < nF; ++f) area += farea[f];  // in face order, as one thread would
+    double area = 0, max_edge = 0;
+    for (int f = 0; f < nF; ++f) {
+        area += farea[f];  // in face order, as one thread would
+        max_edge = std::max(max_edge, fedge[f]);
+    }
     j->area = area > 0 ? area : emax * emax;
+    j->max_edge = max_edge;
     // the vertex -> incident-face CSR (main.cu:259-285) is built on the device below
     // padded grid bbox
     double glo[3], ghi[3];
@@ -728,6 +742,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) ||
         (rc = j->alloc(&j->vpool, 2L * nS)) || (rc = j->alloc(&j->dmax, 2)) ||
         (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->slot_of, nS)) || (rc = j->alloc(&j->cent, nS)) ||
+        (rc = j->alloc(&j->vs_list, nS)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
         return bail(rc);
     if (p.debug) {
@@ -822,14 +837,16 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         add_job_blocks(ht, 0, nS, j->sgp.ncell + 1);
         pad_site_blocks(ht);
         if ((rc = j->alloc(&j->site_blocks, ht.site.size())) || (rc = j->alloc(&j->cell_blocks, ht.cell.size())) ||
-            (rc = j->alloc(&j->tile_blocks, ht.tile.size())))
+            (rc = j->alloc(&j->tile_blocks, ht.tile.size())) || (rc = j->alloc(&j->prec_blocks, ht.prec.size())))
             return bail(rc);
         if (up(j->desc, &dh, sizeof dh) || up(j->site_blocks, ht.site.data(), sizeof(BlockRef) * ht.site.size()) ||
             up(j->cell_blocks, ht.cell.data(), sizeof(BlockRef) * ht.cell.size()) ||
-            up(j->tile_blocks, ht.tile.data(), sizeof(BlockRef) * ht.tile.size()))
+            up(j->tile_blocks, ht.tile.data(), sizeof(BlockRef) * ht.tile.size()) ||
+            up(j->prec_blocks, ht.prec.data(), sizeof(BlockRef) * ht.prec.size()))
             return bail(fail(CVT_EHIP, "upload failed"));
         j->tables = BatchTables{j->desc, 1, j->site_blocks, (int)ht.site.size(), j->cell_blocks,
-                                (int)ht.cell.size(), j->tile_blocks, (int)ht.tile.size()};
+                                (int)ht.cell.size(), j->tile_blocks, (int)ht.tile.size(), j->prec_blocks,
+                                (int)ht.prec.size()};
     }
     hipError_t e2 = hipStreamSynchronize(s);
     if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup: %s", hipGetErrorString(e2)));
@@ -886,6 +903,8 @@ static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
         o.nonfinite_sites = w[8];
         o.deferred_sites = w[9];
         std::memcpy(&o.max_disp, &w[10], 4);
+        o.deferred_projections = w[11];
+        o.fp64_cells = w[12];
     }
     return 0;
 }
@@ -1461,13 +1480,16 @@ int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
     auto* sb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.site.size());
     auto* cb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.cell.size());
     auto* tb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.tile.size());
-    if (!dd || !sb || !cb || !tb) return bail(fail(CVT_ENOMEM, "batch tables"));
+    auto* pb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.prec.size());
+    if (!dd || !sb || !cb || !tb || !pb) return bail(fail(CVT_ENOMEM, "batch tables"));
     if (hipMemcpy(dd, descs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice) ||
         hipMemcpy(sb, ht.site.data(), sizeof(BlockRef) * ht.site.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(cb, ht.cell.data(), sizeof(BlockRef) * ht.cell.size(), hipMemcpyHostToDevice) ||
-        hipMemcpy(tb, ht.tile.data(), sizeof(BlockRef) * ht.tile.size(), hipMemcpyHostToDevice))
+        hipMemcpy(tb, ht.tile.data(), sizeof(BlockRef) * ht.tile.size(), hipMemcpyHostToDevice) ||
+        hipMemcpy(pb, ht.prec.data(), sizeof(BlockRef) * ht.prec.size(), hipMemcpyHostToDevice))
         return bail(fail(CVT_EHIP, "batch table upload failed"));
-    b->tables = BatchTables{dd, njobs, sb, (int)ht.site.size(), cb, (int)ht.cell.size(), tb, (int)ht.tile.size()};
+    b->tables = BatchTables{dd, njobs, sb, (int)ht.site.size(), cb, (int)ht.cell.size(), tb, (int)ht.tile.size(),
+                            pb, (int)ht.prec.size()};
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bail(fail(CVT_EHIP, "capture begin failed"));

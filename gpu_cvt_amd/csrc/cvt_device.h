@@ -195,8 +195,8 @@ __device__ __forceinline__ float box_gap2(float qx, float qy, float qz, float4 l
 // may hold a point closer than the current L-th (with a 1e-6 slack for the
 // box distance's different rounding).
 template <int L>
-__device__ __noinline__ void bvh_topl(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy, float qz,
-                                      int self_slot, TopL<L>& T) {
+__device__ __forceinline__ void bvh_topl_body(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx,
+                                              float qy, float qz, int self_slot, TopL<L>& T) {
     T.init(__builtin_huge_valf());
     const int n = b.n;
     if (n <= 0) return;
@@ -236,6 +236,15 @@ __device__ __noinline__ void bvh_topl(const PointBVH& b, const CVT_G float4* __r
         cur = next;
         if (cur < 0) break;
     }
+}
+
+// Out of line for the FP32 kernels, whose registers would otherwise hold the
+// walk's state on every path (the FP64 pass inlines the body: there the walk
+// is common, and a list in scratch memory made each step a memory round trip).
+template <int L>
+__device__ __noinline__ void bvh_topl(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy,
+                                      float qz, int self_slot, TopL<L>& T) {
+    bvh_topl_body<L>(b, pts, qx, qy, qz, self_slot, T);
 }
 
 // FP32 top-L of the whole point set (self_slot excluded), seeded with radius
@@ -514,7 +523,7 @@ __device__ __forceinline__ void exact_insert(double (&kd)[K], int (&kid)[K], int
 // The exact rescan over a point LBVH: every point within tau (boxes within
 // tau, with slack) enters the FP64-keyed top-K.
 template <int K>
-__device__ __noinline__ void bvh_exact64(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy,
+__device__ __forceinline__ void bvh_exact64(const PointBVH& b, const CVT_G float4* __restrict__ pts, float qx, float qy,
                                          float qz, int self_slot, float tau, int* __restrict__ out_slot,
                                          const CVT_G int* __restrict__ cid) {
     double kd[K];
@@ -560,7 +569,7 @@ __device__ __noinline__ void bvh_exact64(const PointBVH& b, const CVT_G float4* 
 // Exact K nearest under (key64, id) by FP64 insertion over every point with
 // FP32 distance <= tau.  Only the FP64-resolving pass uses it.
 template <int K>
-__device__ __noinline__ void grid_exact64(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
+__device__ __forceinline__ void grid_exact64(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
                                           float qx, float qy, float qz, int self_slot, float tau,
                                           int* __restrict__ out_slot, const CVT_G int* __restrict__ cid) {
     double kd[K];
@@ -611,8 +620,9 @@ __device__ __noinline__ void grid_exact64(const GridP& g, const CVT_G int* __res
 }
 
 // K nearest (in exact (key64, id) order) of q as slots.  Returns 0 when FP32
-// (plus isolated FP64 pair checks) proved the list, 1 when the FP64 rescan
-// was needed, -1 (only !PRECISE) when the query must go to the FP64 pass.
+// proved the list, 1 when isolated near-tie pairs were ordered by FP64 keys,
+// 2 when the FP64 rescan was needed, -1 (only !PRECISE) when the query must go
+// to the FP64 pass.
 // tau: a provable upper bound of the (K+2)-th distance^2, or inf.
 // l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
 // lds (optional, with collect_tau finite): try grid_collect_topl first.
@@ -639,11 +649,15 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __rest
         if (!grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, tau, T, max_r, &limited) && !limited)
             grid_topl<L>(g, start, pts, qx, qy, qz, self_slot, __builtin_huge_valf(), T, max_r, &limited);
         if (limited) {
-            // the out-of-line walk gets a copy: a list whose address reaches a
-            // call would live in scratch memory on every path, not only this one
-            TopL<L> Tb = T;
-            bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, Tb);
-            T = Tb;
+            if constexpr (PRECISE) {
+                bvh_topl_body<L>(*pbvh, pts, qx, qy, qz, self_slot, T);
+            } else {
+                // the out-of-line walk gets a copy: a list whose address reaches a
+                // call would live in scratch memory on every path, not only this one
+                TopL<L> Tb = T;
+                bvh_topl<L>(*pbvh, pts, qx, qy, qz, self_slot, Tb);
+                T = Tb;
+            }
         }
     }
     l_out = T.s[L - 1] >= 0 ? (keyed ? worst_up : T.d[L - 1]) : __builtin_huge_valf();
@@ -655,21 +669,315 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __rest
 #pragma unroll
         for (int t = 0; t < L; ++t) d_out[t] = T.s[t] >= 0 ? T.d[t] : __builtin_huge_valf();
     }
-    if (ok) return 0;
+    if (ok) return pairs > 0 ? 1 : 0;
     if constexpr (!PRECISE) {
         return -1;
     } else {
-        int sb[K];  // out-of-line rescans write a copy (see above)
-        if (limited) bvh_exact64<K>(*pbvh, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), sb, cid);
-        else grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, T.d[L - 1] * (1.f + 1e-5f), sb, cid);
+        int sb[K];
+        // the grid rescan sweeps O((r / h)^3) cells for a list radius r: past
+        // two cells (a query far off the points) the LBVH rescan is used
+        const float tau = T.d[L - 1] * (1.f + 1e-5f);
+        if (pbvh && (limited || tau > 4.f * g.h * g.h)) bvh_exact64<K>(*pbvh, pts, qx, qy, qz, self_slot, tau, sb, cid);
+        else grid_exact64<K>(g, start, pts, qx, qy, qz, self_slot, tau, sb, cid);
 #pragma unroll
         for (int t = 0; t < K; ++t) slot_out[t] = sb[t];
         if (d_out) {  // FP32 distances no longer follow the rescanned list: report no margin
 #pragma unroll
             for (int t = 0; t < L; ++t) d_out[t] = 0.f;
         }
-        return 1;
+        return 2;
     }
+}
+
+// ------------------------------------------------------- wave-wide exact KNN
+// The L nearest points of q (self_slot excluded) under (FP64 key, tie id),
+// found by one whole wave (q and every argument wave-uniform).  The points of
+// the block of Chebyshev radius r around q's cell are keyed in FP64 by the 64
+// lanes together, each lane keeping its share in its own LDS column (CAPL
+// entries; when a lane's share does not fit, every round re-reads the block);
+// then each of L rounds takes the smallest (key, tie) above the previous
+// round's by a wave-wide minimum, so round k yields the k-th nearest.  The
+// block grows until the L-th key lies inside its boundary (every point outside
+// is farther) or it covers the grid.  Lane k (< L) returns the k-th slot in
+// my_slot and its key in my_key (-1 / +inf past the points that exist); the
+// return value is the number found.  lds: CAPL * 64 doubles (keys) followed by
+// CAPL * 64 ints (ties) and CAPL * 64 ints (slots), private to the wave.
+template <int L, int CAPL>
+__device__ __forceinline__ int wave_exact_topl(const GridP& g, const CVT_G int* __restrict__ start,
+                                               const CVT_G float4* __restrict__ pts, const CVT_G int* __restrict__ cid,
+                                               float qx, float qy, float qz, int self_slot, char* lds, int& my_slot,
+                                               double& my_key) {
+    static_assert(L <= 64, "one result per lane");
+    const int lane = (int)threadIdx.x & 63;
+    double* ck = reinterpret_cast<double*>(lds) + lane;
+    int* ct = reinterpret_cast<int*>(lds + 8 * CAPL * 64) + lane;
+    int* cs = reinterpret_cast<int*>(lds + 12 * CAPL * 64) + lane;
+    int cx, cy, cz;
+    cell_of(g, qx, qy, qz, cx, cy, cz);
+    const int maxr = max(g.dx, max(g.dy, g.dz));
+    int found = 0;
+    for (int r = 1; r <= maxr; ++r) {
+        const int x0 = max(cx - r, 0), x1 = min(cx + r, g.dx - 1);
+        // calls f(valid, slot) for every point of the block, 64 at a time
+        auto for_block = [&](auto&& f) {
+            for (int dz = -r; dz <= r; ++dz) {
+                const int z = cz + dz;
+                if (z < 0 || z >= g.dz) continue;
+                for (int dy = -r; dy <= r; ++dy) {
+                    const int y = cy + dy;
+                    if (y < 0 || y >= g.dy) continue;
+                    const int row = (z * g.dy + y) * g.dx;
+                    const int s0 = start[row + x0], s1 = start[row + x1 + 1];
+                    for (int base = s0; base < s1; base += 64) {
+                        const int sl = base + lane;
+                        f(sl < s1 && sl != self_slot, sl);
+                    }
+                }
+            }
+        };
+        int cnt = 0;
+        for_block([&](bool v, int sl) {
+            if (v) {
+                if (cnt < CAPL) {
+                    const float4 p = pts[sl];
+                    ck[cnt * 64] = key64(qx, qy, qz, p.x, p.y, p.z);
+                    ct[cnt * 64] = tie_id(p, cid);
+                    cs[cnt * 64] = sl;
+                }
+                ++cnt;
+            }
+        });
+        const bool staged = __ballot(cnt > CAPL) == 0;
+        my_slot = -1;
+        my_key = __builtin_huge_val();
+        double pk = -1.0;
+        int pt = -1;
+        found = 0;
+        for (int round = 0; round < L; ++round) {
+            double bk = __builtin_huge_val();
+            int bt = 0x7fffffff, bs = -1;
+            auto consider = [&](double k, int t, int sl) {
+                const bool above = k > pk || (k == pk && t > pt);
+                const bool better = k < bk || (k == bk && t < bt);
+                if (above && better) { bk = k; bt = t; bs = sl; }
+            };
+            if (staged) {
+                for (int j = 0; j < cnt; ++j) consider(ck[j * 64], ct[j * 64], cs[j * 64]);
+            } else {
+                for_block([&](bool v, int sl) {
+                    if (v) {
+                        const float4 p = pts[sl];
+                        consider(key64(qx, qy, qz, p.x, p.y, p.z), tie_id(p, cid), sl);
+                    }
+                });
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const double ok = __shfl_xor(bk, o, 64);
+                const int ot = __shfl_xor(bt, o, 64), os = __shfl_xor(bs, o, 64);
+                const bool take = ok < bk || (ok == bk && ot < bt);
+                bk = take ? ok : bk;
+                bt = take ? ot : bt;
+                bs = take ? os : bs;
+            }
+            if (bs < 0) break;  // wave-uniform: the block holds no further point
+            if (lane == round) { my_slot = bs; my_key = bk; }
+            pk = bk;
+            pt = bt;
+            ++found;
+        }
+        float bnd = block_bound(g, qx, qy, qz, cx, cy, cz, r);
+        if (bnd == __builtin_huge_valf()) break;  // the block covers the grid
+        bnd -= g.eps_abs;
+        if (found == L && bnd > 0.f && pk < (double)bnd * (double)bnd * (1.0 - 1e-5)) break;
+    }
+    return found;
+}
+
+// Orders one wave's LDS writes before its later LDS reads by other lanes.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Appends v at the wave's running count for every lane with pred (ballot
+// compaction); returns false when the list would pass cap (nothing beyond
+// cap is written; cnt still advances, so every lane sees the overflow).
+template <class T>
+__device__ __forceinline__ void wave_append(bool pred, T v, T* arr, int cap, int& cnt) {
+    const unsigned long long m = __ballot(pred);
+    const int lane = (int)threadIdx.x & 63;
+    const int pos = cnt + (int)__popcll(m & ((1ull << lane) - 1ull));
+    if (pred && pos < cap) arr[pos] = v;
+    cnt += (int)__popcll(m);
+}
+
+// The L nearest points of q over a point LBVH under (FP64 key, tie id) by one
+// whole wave (q wave-uniform), for queries far from the points, where a
+// per-lane walk turns into a chain of thousands of dependent node visits:
+//  1. a first bound B of the L-th FP32 distance: the wave descends to q's
+//     nearest leaf (near-first), then takes the L-th smallest FP32 distance
+//     of the 64 leaves around it in Morton order (one per lane);
+//  2. a depth-first sweep of the hierarchy that expands up to 64 nodes of its
+//     stack per round (near children on top), keeps every point within the
+//     bound (with the FP32 slack of box and point distances), and, whenever
+//     the kept points fill their LDS list, tightens the bound to their L-th
+//     smallest distance and drops the points beyond it;
+//  3. the L smallest (key, tie) of the kept points, one per round by a
+//     wave-wide minimum (as wave_exact_topl).
+// Lane k (< L) returns the k-th slot and key (-1 / +inf past the points that
+// exist); *ok is false when the stack or the kept points still exceed their
+// LDS capacity (the caller then searches per lane).  lds: CAPS ints (stack),
+// then CAPC floats (distances) and CAPC ints (slots), private to the wave.
+template <int L, int CAPS, int CAPC>
+__device__ __forceinline__ int wave_bvh_topl(const PointBVH& b, const CVT_G float4* __restrict__ pts,
+                                             const CVT_G int* __restrict__ cid, float qx, float qy, float qz,
+                                             char* lds, int& my_slot, double& my_key, bool& ok) {
+    static_assert(L <= 64 && CAPC >= 128, "one result per lane; room for a round of kept points");
+    const int lane = (int)threadIdx.x & 63;
+    int* stk = reinterpret_cast<int*>(lds);
+    float* cd = reinterpret_cast<float*>(lds + 4 * CAPS);
+    int* cs = reinterpret_cast<int*>(lds + 4 * CAPS + 4 * CAPC);
+    const int n = b.n;
+    ok = true;
+    my_slot = -1;
+    my_key = __builtin_huge_val();
+    int cnt = 0;
+    // the L-th smallest of the kept distances (every lane; +inf when fewer)
+    auto kth_kept = [&]() {
+        float prev = -1.f;
+        int below = 0;
+        float kth = __builtin_huge_valf();
+        while (below < L) {
+            float m = __builtin_huge_valf();
+            for (int j = lane; j < cnt; j += 64)
+                if (cd[j] > prev) m = fminf(m, cd[j]);
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) m = fminf(m, __shfl_xor(m, o, 64));
+            if (!(m < __builtin_huge_valf())) break;
+            int eq = 0;
+            for (int j = lane; j < cnt; j += 64) eq += cd[j] == m ? 1 : 0;
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) eq += __shfl_xor(eq, o, 64);
+            below += eq;
+            prev = m;
+            kth = m;
+        }
+        return below >= L ? kth : __builtin_huge_valf();
+    };
+    if (n <= 64) {
+        const bool v = lane < n;
+        const int sl = v ? b.leaf[lane] : 0;
+        wave_append<float>(v, 0.f, cd, CAPC, cnt);
+        cnt = 0;
+        wave_append<int>(v, sl, cs, CAPC, cnt);
+    } else {
+        // 1. the first bound
+        int cur = 0;
+        while (cur < n - 1) {
+            const int2 c = b.child[cur];
+            const float dl = box_gap2(qx, qy, qz, b.lo[c.x], b.hi[c.x]);
+            const float dr = box_gap2(qx, qy, qz, b.lo[c.y], b.hi[c.y]);
+            cur = dl <= dr ? c.x : c.y;
+        }
+        const int lo = min(max(cur - (n - 1) - 32, 0), n - 64);
+        const float dme = d2f(qx, qy, qz, pts[b.leaf[lo + lane]]);
+        int rank = 0;
+        for (int t = 0; t < 64; ++t) {
+            const float dt = __shfl(dme, t, 64);
+            rank += (dt < dme || (dt == dme && t < lane)) ? 1 : 0;
+        }
+        const unsigned long long at = __ballot(rank == L - 1);
+        float lim = __shfl(dme, (int)__ffsll((long long)at) - 1, 64) * (1.f + 1e-5f);
+        // 2. the sweep
+        int sp = 1;
+        if (lane == 0) stk[0] = 0;
+        wave_lds_sync();
+        while (sp > 0) {
+            const int take = min(sp, 64);
+            sp -= take;
+            const int node = lane < take ? stk[sp + lane] : -1;
+            wave_lds_sync();
+            const float blim = lim * (1.f + 2e-5f);
+            const bool leaf = node >= n - 1;
+            float d = 0.f;
+            int sl = 0;
+            if (leaf) {
+                sl = b.leaf[node - (n - 1)];
+                d = d2f(qx, qy, qz, pts[sl]);
+            }
+            int nearc = 0, farc = 0;
+            bool pn = false, pf = false;
+            if (node >= 0 && !leaf) {
+                const int2 c = b.child[node];
+                const float gl = box_gap2(qx, qy, qz, b.lo[c.x], b.hi[c.x]);
+                const float gr = box_gap2(qx, qy, qz, b.lo[c.y], b.hi[c.y]);
+                nearc = gl <= gr ? c.x : c.y;
+                farc = gl <= gr ? c.y : c.x;
+                pn = fminf(gl, gr) <= blim;
+                pf = fmaxf(gl, gr) <= blim;
+            }
+            const bool keep = leaf && d <= lim;
+            int c0 = cnt;
+            wave_append<float>(keep, d, cd, CAPC, c0);
+            wave_append<int>(keep, sl, cs, CAPC, cnt);
+            wave_append<int>(pf, farc, stk, CAPS, sp);  // far children below,
+            wave_append<int>(pn, nearc, stk, CAPS, sp);  // near children on top
+            if (sp > CAPS || cnt > CAPC) { ok = false; return 0; }
+            wave_lds_sync();
+            if (cnt > CAPC - 64) {
+                // tighten: the L-th kept distance bounds the L-th nearest
+                const float kth = kth_kept();
+                if (kth < __builtin_huge_valf()) lim = fminf(lim, kth * (1.f + 1e-5f));
+                int nc = 0;
+                for (int base = 0; base < cnt; base += 64) {
+                    const int j = base + lane;
+                    const float dj = j < cnt ? cd[j] : 0.f;
+                    const int sj = j < cnt ? cs[j] : 0;
+                    wave_lds_sync();
+                    int c1 = nc;
+                    wave_append<float>(j < cnt && dj <= lim, dj, cd, CAPC, c1);
+                    wave_append<int>(j < cnt && dj <= lim, sj, cs, CAPC, nc);
+                    wave_lds_sync();
+                }
+                cnt = nc;
+                if (cnt > CAPC - 64) { ok = false; return 0; }
+            }
+        }
+    }
+    wave_lds_sync();
+    // 3. the selection, by FP64 keys
+    double pk = -1.0;
+    int pt = -1, found = 0;
+    for (int round = 0; round < L; ++round) {
+        double bk = __builtin_huge_val();
+        int bt = 0x7fffffff, bs = -1;
+        for (int j = lane; j < cnt; j += 64) {
+            const int sl = cs[j];
+            const float4 p = pts[sl];
+            const double k = key64(qx, qy, qz, p.x, p.y, p.z);
+            const int t = tie_id(p, cid);
+            const bool above = k > pk || (k == pk && t > pt);
+            const bool better = k < bk || (k == bk && t < bt);
+            if (above && better) { bk = k; bt = t; bs = sl; }
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            const double okk = __shfl_xor(bk, o, 64);
+            const int ot = __shfl_xor(bt, o, 64), os = __shfl_xor(bs, o, 64);
+            const bool tk = okk < bk || (okk == bk && ot < bt);
+            bk = tk ? okk : bk;
+            bt = tk ? ot : bt;
+            bs = tk ? os : bs;
+        }
+        if (bs < 0) break;
+        if (lane == round) { my_slot = bs; my_key = bk; }
+        pk = bk;
+        pt = bt;
+        ++found;
+    }
+    return found;
 }
 
 // ------------------------------------------------------------------- frames
@@ -726,6 +1034,7 @@ struct Kahan {
 // FP32 cannot decide are redone in FP64 on the same FP32 inputs.
 struct CellOut {
     float cx, cy, area, moment;
+    float err;   // FP32 error allowance of the cell vertices (see hull_walk)
     int nverts;
     bool ok;
     bool defer;  // an orientation test FP32 could not decide (!PRECISE only)
@@ -741,13 +1050,26 @@ struct CellOut {
 // in which a test is within FP32 error or exactly collinear is redone with
 // exact decisions (FP64 orientation on the same FP32 inputs, the farther
 // point on a tie), which is what every step would compute.
+// o.err bounds the FP32 error of every cell vertex v = (Q - P)^perp /
+// (P x Q) of consecutive hull points P, Q whose coordinates carry an absolute
+// error of at most eta each (eta = 8u M with M the largest dual-point
+// magnitude 2 / |d|): perturbing them moves v by <= eta (2 + |v| (|P| + |Q|)) /
+// |P x Q|, and rounding det adds ~ 2u |v| |P| |Q| / |P x Q| <= 0.2 eta |v|
+// (|P| + |Q|) / |P x Q| since |P|, |Q| <= M (L1 norms, which bound the
+// Euclidean ones; the u |v| rounding of the numerator is far below any
+// tolerance of 1e-6 of the diagonal and is left out).  The centroid, an
+// area-weighted mean of the vertices with compensated sums, moves by at most
+// about twice the largest vertex error; a cell whose allowance exceeds the
+// caller's tolerance is recomputed in FP64.
 template <int M, int NB, bool SQ>
-__device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (&py)[M], float& rin2, bool& inside) {
+__device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (&py)[M], float eta, float& rin2,
+                                             bool& inside) {
     constexpr int NP = NB + (SQ ? 4 : 0);
     CellOut o;
     o.ok = false;
     o.defer = false;
     o.nverts = 0;
+    o.err = 0.f;
     rin2 = __builtin_huge_valf();
     inside = true;
     // start: max x, then max y -- always a hull vertex
@@ -812,6 +1134,8 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
         rin2 = fminf(rin2, fast_div(det * det, ex * ex + ey * ey));
         const float inv = fast_div(1.0f, det);
         const float vx = (by - cury) * inv, vy = (curx - bx) * inv;
+        o.err = fmaxf(o.err, eta * (2.f + 2.2f * (fabsf(vx) + fabsf(vy)) *
+                                              (fabsf(curx) + fabsf(cury) + fabsf(bx) + fabsf(by))) * fabsf(inv));
         if (step == 0) {
             fvx = vx; fvy = vy;
         } else {
@@ -855,13 +1179,14 @@ __device__ __forceinline__ CellOut hull_walk(const float (&px)[M], const float (
 // distance 1/R -- lies strictly inside its inscribed circle about the origin,
 // the full hull is the same polygon, started at the same point, and the walk
 // is done; otherwise it is repeated over all points.
+// eta: the absolute error allowance of every FP32 dual point (hull_walk).
 template <int M>
-__device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M]) {
+__device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const float (&py)[M], float eta) {
     constexpr int kNear = 12;
     float rin2;
     bool inside;
     if constexpr (M - 4 > kNear + 4) {
-        const CellOut o = hull_walk<M, kNear, false>(px, py, rin2, inside);
+        const CellOut o = hull_walk<M, kNear, false>(px, py, eta, rin2, inside);
         if (o.ok && inside) {
             const float lim = rin2 * (1.f - 1e-4f);
             bool in = px[M - 4] * px[M - 4] < lim;
@@ -870,7 +1195,81 @@ __device__ __forceinline__ CellOut dual_hull_cell(const float (&px)[M], const fl
             if (in) return o;
         }
     }
-    return hull_walk<M, M - 4, true>(px, py, rin2, inside);
+    return hull_walk<M, M - 4, true>(px, py, eta, rin2, inside);
+}
+
+// The same cell in FP64 (the FP64-resolving pass): dual points of FP64
+// coordinates, gift wrapping with FP64 orientation tests (a collinear tie takes
+// the farther point), and the polygon's area, centroid and polar second moment
+// about the origin as FP64 sums.  px/py: M points, the last 4 the clip square.
+struct CellOut64 {
+    double cx, cy, area, moment;
+    bool ok;
+};
+
+template <int M>
+__device__ __forceinline__ CellOut64 dual_hull_cell64(const double (&px)[M], const double (&py)[M]) {
+#pragma clang fp contract(off)
+    CellOut64 o;
+    o.ok = false;
+    o.cx = o.cy = o.area = o.moment = 0.0;
+    double sx = px[0], sy = py[0];
+#pragma unroll
+    for (int j = 1; j < M; ++j) {
+        const bool b = px[j] > sx || (px[j] == sx && py[j] > sy);
+        sx = b ? px[j] : sx;
+        sy = b ? py[j] : sy;
+    }
+    double curx = sx, cury = sy;
+    double A2 = 0.0, Cx = 0.0, Cy = 0.0, I12 = 0.0;
+    double fvx = 0.0, fvy = 0.0, pvx = 0.0, pvy = 0.0;
+    int nv = 0;
+    for (int step = 0; step <= M; ++step) {
+        double bx = curx, by = cury;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            const double wx = px[j] - curx, wy = py[j] - cury;
+            const double ex = bx - curx, ey = by - cury;
+            const bool isnew = (wx != 0.0) || (wy != 0.0);
+            const bool none = (ex == 0.0) && (ey == 0.0);
+            const double cr = ex * wy - ey * wx;
+            const bool farther = (wx * wx + wy * wy) > (ex * ex + ey * ey);
+            const bool take = isnew && (none || cr < 0.0 || (cr == 0.0 && farther));
+            bx = take ? px[j] : bx;
+            by = take ? py[j] : by;
+        }
+        if (bx == curx && by == cury) return o;
+        const double det = curx * by - cury * bx;
+        const double vx = (by - cury) / det, vy = (curx - bx) / det;
+        if (step == 0) {
+            fvx = vx; fvy = vy;
+        } else {
+            const double cr = pvx * vy - vx * pvy;
+            A2 += cr;
+            Cx += (pvx + vx) * cr;
+            Cy += (pvy + vy) * cr;
+            I12 += cr * (pvx * pvx + pvx * vx + vx * vx + pvy * pvy + pvy * vy + vy * vy);
+        }
+        pvx = vx; pvy = vy;
+        ++nv;
+        curx = bx; cury = by;
+        if (curx == sx && cury == sy) {
+            const double cr = pvx * fvy - fvx * pvy;
+            A2 += cr;
+            Cx += (pvx + fvx) * cr;
+            Cy += (pvy + fvy) * cr;
+            I12 += cr * (pvx * pvx + pvx * fvx + fvx * fvx + pvy * pvy + pvy * fvy + fvy * fvy);
+            o.ok = nv >= 3;
+            break;
+        }
+    }
+    if (!o.ok) return o;
+    o.area = 0.5 * A2;
+    o.moment = fabs(I12) / 12.0;
+    if (fabs(o.area) < 1e-300) return o;
+    o.cx = Cx / (3.0 * A2);
+    o.cy = Cy / (3.0 * A2);
+    return o;
 }
 
 // Labelled walk of the same dual hull for the remesh stage: calls
@@ -922,6 +1321,83 @@ __device__ __forceinline__ int dual_hull_emit(const float (&px)[M], const float 
 }
 
 // ---------------------------------------------------- closest point (Ericson)
+// The FP64 form for the FP64-resolving pass (same regions, same guards).
+__device__ __forceinline__ double3 closest_point_tri64(double3 P, double3 A, double3 B, double3 C) {
+#pragma clang fp contract(off)
+    const double3 AB = make_double3(B.x - A.x, B.y - A.y, B.z - A.z);
+    const double3 AC = make_double3(C.x - A.x, C.y - A.y, C.z - A.z);
+    const double3 AP = make_double3(P.x - A.x, P.y - A.y, P.z - A.z);
+    const double d1 = AB.x * AP.x + AB.y * AP.y + AB.z * AP.z;
+    const double d2 = AC.x * AP.x + AC.y * AP.y + AC.z * AP.z;
+    if (d1 <= 0.0 && d2 <= 0.0) return A;
+    const double3 BP = make_double3(P.x - B.x, P.y - B.y, P.z - B.z);
+    const double d3 = AB.x * BP.x + AB.y * BP.y + AB.z * BP.z;
+    const double d4 = AC.x * BP.x + AC.y * BP.y + AC.z * BP.z;
+    if (d3 >= 0.0 && d4 <= d3) return B;
+    const double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        const double t = d1 / (d1 - d3 + 1e-30);
+        return make_double3(A.x + AB.x * t, A.y + AB.y * t, A.z + AB.z * t);
+    }
+    const double3 CP = make_double3(P.x - C.x, P.y - C.y, P.z - C.z);
+    const double d5 = AB.x * CP.x + AB.y * CP.y + AB.z * CP.z;
+    const double d6 = AC.x * CP.x + AC.y * CP.y + AC.z * CP.z;
+    if (d6 >= 0.0 && d5 <= d6) return C;
+    const double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        const double w = d2 / (d2 - d6 + 1e-30);
+        return make_double3(A.x + AC.x * w, A.y + AC.y * w, A.z + AC.z * w);
+    }
+    const double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        const double w = (d4 - d3) / ((d4 - d3) + (d5 - d6) + 1e-30);
+        return make_double3(B.x + (C.x - B.x) * w, B.y + (C.y - B.y) * w, B.z + (C.z - B.z) * w);
+    }
+    const double denom = 1.0 / (va + vb + vc + 1e-30);
+    const double vv = vb * denom, ww = vc * denom;
+    return make_double3(A.x + AB.x * vv + AC.x * ww, A.y + AB.y * vv + AC.y * ww, A.z + AB.z * vv + AC.z * ww);
+}
+
+// The offset P - closest point, for the same regions and guards, formed from
+// the corner offsets AP, BP, CP (e.g. AP - AB v - AC w in the interior), so
+// its rounding error is relative to the triangle's size and distance, not to
+// the coordinates' magnitude: FP32 distances of nearby faces compare to
+// ~u (|AP| + |AB| + |AC|) instead of ~u |P|.
+__device__ __forceinline__ float3 closest_offset_tri(float3 P, float3 A, float3 B, float3 C) {
+    const float3 AB = make_float3(B.x - A.x, B.y - A.y, B.z - A.z);
+    const float3 AC = make_float3(C.x - A.x, C.y - A.y, C.z - A.z);
+    const float3 AP = make_float3(P.x - A.x, P.y - A.y, P.z - A.z);
+    const float d1 = AB.x * AP.x + AB.y * AP.y + AB.z * AP.z;
+    const float d2 = AC.x * AP.x + AC.y * AP.y + AC.z * AP.z;
+    if (d1 <= 0.f && d2 <= 0.f) return AP;
+    const float3 BP = make_float3(P.x - B.x, P.y - B.y, P.z - B.z);
+    const float d3 = AB.x * BP.x + AB.y * BP.y + AB.z * BP.z;
+    const float d4 = AC.x * BP.x + AC.y * BP.y + AC.z * BP.z;
+    if (d3 >= 0.f && d4 <= d3) return BP;
+    const float vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.f && d1 >= 0.f && d3 <= 0.f) {
+        const float t = fast_div(d1, d1 - d3 + 1e-30f);
+        return make_float3(AP.x - AB.x * t, AP.y - AB.y * t, AP.z - AB.z * t);
+    }
+    const float3 CP = make_float3(P.x - C.x, P.y - C.y, P.z - C.z);
+    const float d5 = AB.x * CP.x + AB.y * CP.y + AB.z * CP.z;
+    const float d6 = AC.x * CP.x + AC.y * CP.y + AC.z * CP.z;
+    if (d6 >= 0.f && d5 <= d6) return CP;
+    const float vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.f && d2 >= 0.f && d6 <= 0.f) {
+        const float w = fast_div(d2, d2 - d6 + 1e-30f);
+        return make_float3(AP.x - AC.x * w, AP.y - AC.y * w, AP.z - AC.z * w);
+    }
+    const float va = d3 * d6 - d5 * d4;
+    if (va <= 0.f && (d4 - d3) >= 0.f && (d5 - d6) >= 0.f) {
+        const float w = fast_div(d4 - d3, (d4 - d3) + (d5 - d6) + 1e-30f);
+        return make_float3(BP.x - (C.x - B.x) * w, BP.y - (C.y - B.y) * w, BP.z - (C.z - B.z) * w);
+    }
+    const float denom = fast_div(1.0f, va + vb + vc + 1e-30f);
+    const float vv = vb * denom, ww = vc * denom;
+    return make_float3(AP.x - AB.x * vv - AC.x * ww, AP.y - AB.y * vv - AC.y * ww, AP.z - AB.z * vv - AC.z * ww);
+}
+
 // near_project.cuh:12-53, including the reference's +1e-30 guards.
 __device__ __forceinline__ float3 closest_point_tri(float3 P, float3 A, float3 B, float3 C) {
     const float3 AB = make_float3(B.x - A.x, B.y - A.y, B.z - A.z);

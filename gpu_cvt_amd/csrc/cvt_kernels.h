@@ -6,7 +6,7 @@
 
 namespace cvt {
 
-constexpr int kStatWords = 12;  // mirrors cvt_iter_stats (int fields)
+constexpr int kStatWords = 14;  // words recorded per iteration (cvt_iter_stats maps them)
 // Stat word layout (cvt_iter_stats): 0 same, 1 low, 2 both, 3 newly frozen,
 // 4 frozen total, 5 exact rescans, 6 reranks, 7 cell failures, 8 non-finite,
 // 9 deferred to the precise pass (also that pass's work-list length), 10 the
@@ -17,6 +17,11 @@ constexpr int kStatMaxDisp = 10;
 // back of defer_list, so that the FP64 pass over the first list can run while
 // the projection kernel still appends); folded into word 9 at the iteration end.
 constexpr int kStatDeferredB = 11;
+// Word 12: cells the FP32 pass handed to the FP64 pass for their error allowance.
+constexpr int kStatFp64Cells = 12;
+// Word 13: front-list sites of the FP64 pass waiting for a wave-wide vertex
+// search (their list in vs_list).
+constexpr int kStatVsearch = 13;
 // The per-site counters (words 0-8) and the largest displacement (word 10, float
 // bits, atomic max) are accumulated per block into one of kStatCopies copies of
 // the stat words, each on its own 128-byte line (copy c at stats[c * kStatStride]),
@@ -24,8 +29,14 @@ constexpr int kStatDeferredB = 11;
 // end folds the copies into copy 0, which also holds the deferred-list lengths.
 constexpr int kStatCopies = 8;
 constexpr int kStatStride = 32;
-// Blocks per job of the precise pass (they stride over the job's deferred list).
-constexpr int kPreciseBlocks = 4;
+// Blocks per job of the FP64 pass: one per 256 sites up to kPreciseMaxBlocks
+// (about two per CU), striding over the job's deferred list, so that a list
+// of any length is spread over the whole device.
+constexpr int kPreciseMaxBlocks = 512;
+__host__ __device__ inline int precise_blocks(int nS) {
+    const int b = (nS + 255) / 256;
+    return b < kPreciseMaxBlocks ? b : kPreciseMaxBlocks;
+}
 
 // Neighbour counts K with compiled Lloyd kernels; each has its own object
 // (cvt_lloyd.hip built with -DCVT_K=k).
@@ -50,6 +61,7 @@ struct MeshDev {
     const CVT_G int* vf_off;    // nV+1
     const CVT_G int* vf_faces;  // 3*nF
     int nV, nF;
+    float max_edge;       // longest edge of the mesh (rounding allowance of the projection)
     GridP vg;             // static vertex grid (pts: x, y, z, vertex id bits)
     PointBVH vb;          // LBVH over the vertex grid's points (far-query fallback)
     const CVT_G int* vg_start;
@@ -90,6 +102,7 @@ struct StepArgs {
     CVT_G int* fnn;             // nS: frame vertex of this iteration (hand-off of the FP32 pass; -2: deferred)
     CVT_G int* slot_of;         // nS: each site's slot in this iteration's site grid (sg.pts)
     CVT_G float4* cent;         // nS: centroid and cell-phase flags (hand-off of the FP32 pass)
+    CVT_G int* vs_list;         // nS: sites whose FP64 vertex search a whole wave runs
     int nS;
     float R;
     float tau0;           // first-iteration candidate radius^2 guess (mean-density estimate)
@@ -97,6 +110,8 @@ struct StepArgs {
     float thr2[6];
     int need[6];
     int nan_policy;
+    float tol_abs;        // a quarter of the parity tolerance (1e-5 of the bbox diagonal): FP32 results
+                          // whose error allowance exceeds it are recomputed by the FP64 pass
     // optional per-site debug outputs (null when off)
     CVT_G int* dbg_nn;
     CVT_G float4* dbg_U;
@@ -144,6 +159,8 @@ struct BatchTables {
     int n_cell_blocks;
     const BlockRef* tile_blocks;   // scan tiles of kScanTile entries
     int n_tile_blocks;
+    const BlockRef* prec_blocks;   // precise_blocks(nS) per job (the FP64 pass)
+    int n_prec_blocks;
 };
 
 void launch_fill(int* p, long n, int v, hipStream_t s);

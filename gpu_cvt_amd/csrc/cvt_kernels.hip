@@ -433,7 +433,6 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     a.dmax01[*a.parity] = __uint_as_float(mx);
     a.stats[kStatMaxDisp] = (int)mx;
     a.stats[kStatDeferred] += a.stats[kStatDeferredB];
-    a.stats[kStatDeferredB] = 0;
     for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
     *a.hist_count = c + 1;
     *a.has_prev = 1;

@@ -48,28 +48,179 @@ void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
         defer_site(a, t, false, false, false);
         return;
     }
+    if (cr.ill) {
+        // the FP32 cell's error allowance exceeds the tolerance: the FP64 pass
+        // recomputes it from the row just written (an evaluation-only pass
+        // writes no row, so it repeats the search there)
+        a.cent[c.i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        defer_site(a, t, eval_only == 0, cr.same, false);
+        atomicAdd(&a.stats[kStatFp64Cells], 1);
+        return;
+    }
     const int flags = kCentDone | (cr.same ? kCentSame : 0) | (cr.cell_ok ? 0 : kCentFail) |
                       (cr.kind << kCentKindShift);
     a.cent[c.i] = make_float4(cr.c3.x, cr.c3.y, cr.c3.z, __int_as_float(flags));
     write_debug(a, c, cr);
 }
 
-// Precise pass over each job's deferred sites (FP64 where FP32 was
-// inconclusive); kPreciseBlocks blocks per job stride over its list.
-// back: the list the projection phase filled (from the back of defer_list).
+// FP64 pass, part 1: one wave per front-list site that has no proven KNN row
+// yet (the FP32 search could not prove its order, or its frame was deferred):
+// the exact K nearest by the wave-wide FP64 selection, written as its row,
+// with the (K+2)-th distance as its next search seed; the entry is then marked
+// as carrying a row.  Waves of the job's precise_blocks(nS) blocks stride over
+// the list.
+constexpr int kWaveCapl = 8;  // LDS entries per lane of the wave-wide search
+// The wave-per-site kernels run kWaveBlocks blocks per entry of the FP64
+// pass's block table (16 waves), so that a long list is spread over more waves.
+constexpr int kWaveBlocks = 4;
+
+template <int K>
+__global__ __launch_bounds__(256) void kb_precise_knn(const JobDesc* __restrict__ descs,
+                                                      const BlockRef* __restrict__ bt) {
+    constexpr int L = K + 2;
+    __shared__ double lds_d[4 * 2 * kWaveCapl * 64];  // per wave: keys, then ties and slots
+    const BlockRef br = bt[blockIdx.x / kWaveBlocks];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int n = a.stats[kStatDeferred];
+    const int w = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    char* lds = reinterpret_cast<char*>(lds_d + w * 2 * kWaveCapl * 64);
+    const int nwaves = precise_blocks(a.nS) * 4 * kWaveBlocks;
+    for (int e = (br.blk * kWaveBlocks + (int)blockIdx.x % kWaveBlocks) * 4 + w; e < n; e += nwaves) {
+        const int ent = a.defer_list[e];
+        if (ent & kDeferRow) continue;
+        const int t = ent & kDeferSlot;
+        const float4 q = a.sg.pts[t];
+        if (!finite3(q.x, q.y, q.z)) continue;
+        const int i = __float_as_int(q.w);
+        int my_slot;
+        double my_key;
+        const int found = wave_exact_topl<L, kWaveCapl>(a.sg.g, a.sg.start, a.sg.pts, a.sg.cid, q.x, q.y, q.z, t,
+                                                        lds, my_slot, my_key);
+        // lane k < K writes the k-th neighbour's id (the site itself pads a
+        // list of fewer than K, as in the FP32 pass)
+        bool differs = false;
+        if (lane < K) {
+            const int id = my_slot >= 0 ? __float_as_int(a.sg.pts[my_slot].w) : i;
+            CVT_G int* row = a.knn + (size_t)i * K;
+            differs = row[lane] != id;
+            row[lane] = id;
+        }
+        const bool same = __ballot(differs) == 0;
+        const double kL = __shfl(my_key, L - 1, 64);
+        if (lane == 0) {
+            float r2 = __builtin_huge_valf();
+            if (found == L) {
+                r2 = (float)kL;
+                if ((double)r2 < kL) r2 = __int_as_float(__float_as_int(r2) + 1);  // next float up (r2 >= 0)
+            }
+            a.rad2[i] = r2;
+            a.defer_list[e] = ent | kDeferRow | kDeferExact | (same ? kDeferSame : 0);
+        }
+    }
+}
+
+// FP64 pass, part 2: one thread per deferred site, the whole step with FP64
+// cells and closest points (its KNN row already proven, except in an
+// evaluation-only pass, which searches per thread).  back: the list the
+// projection phase filled (from the back of defer_list).
 template <int K, int KP>
-__global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restrict__ descs, int eval_only, int back) {
-    const StepArgs& a = descs[blockIdx.x / kPreciseBlocks].a;
+__global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restrict__ descs,
+                                                        const BlockRef* __restrict__ bt, int eval_only, int back) {
+    __shared__ double vs_lds[4 * kVsLdsBytes / 8];  // the wave-wide vertex searches
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
     const int par = *a.parity;
     const int n = a.stats[back ? kStatDeferredB : kStatDeferred];
-    for (int base = (blockIdx.x % kPreciseBlocks) * 256; base < n; base += kPreciseBlocks * 256) {
+    const int stride = precise_blocks(a.nS) * 256;
+    char* wave_lds = reinterpret_cast<char*>(vs_lds) + ((int)threadIdx.x >> 6) * kVsLdsBytes;
+    for (int base = br.blk * 256; base < n; base += stride) {
         const int idx = base + (int)threadIdx.x;
         const bool live = idx < n;
         const int e = live ? a.defer_list[back ? a.nS - 1 - idx : idx] : 0;
-        const int t = e & ((1 << 29) - 1);
-        const int carried = (e & (1 << 30)) ? ((e >> 29) & 1) : -1;
-        const SiteFlags f = lloyd_site_precise<K, KP>(a, par, eval_only != 0, t, live, carried);
+        const int t = e & kDeferSlot;
+        const int carried = (e & kDeferRow) ? ((e & kDeferSame) ? 1 : 0) : -1;
+        const int kind = (e & kDeferExact) ? 2 : 0;
+        const SiteFlags f =
+            lloyd_site_precise<K, KP>(a, par, eval_only != 0, t, live, carried, kind, wave_lds, back == 0);
         count_site(a, par, eval_only != 0, f, live);
+    }
+}
+
+// FP64 pass, part 3: one wave per front-list site left waiting by part 2:
+// the exact kproj + 4 nearest mesh vertices of its centroid by the wave-wide
+// LBVH search, into the site's pool cache (the list is its first kproj).
+template <int KP>
+__global__ __launch_bounds__(256) void kb_precise_vsearch(const JobDesc* __restrict__ descs,
+                                                          const BlockRef* __restrict__ bt) {
+    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
+    __shared__ double vs_lds[4 * kVsLdsBytes / 8];
+    const BlockRef br = bt[blockIdx.x / kWaveBlocks];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int n = a.stats[kStatVsearch];
+    const int w = (int)threadIdx.x >> 6;
+    char* lds = reinterpret_cast<char*>(vs_lds) + w * kVsLdsBytes;
+    const int nwaves = precise_blocks(a.nS) * 4 * kWaveBlocks;
+    for (int e = (br.blk * kWaveBlocks + (int)blockIdx.x % kWaveBlocks) * 4 + w; e < n; e += nwaves) {
+        const int i = a.vs_list[e];
+        const float4 c4 = a.cent[i];
+        int ms;
+        double mk;
+        bool ok;
+        wave_bvh_topl<KP + 4, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, c4.x, c4.y, c4.z, lds, ms, mk, ok);
+        int id[KP + 2];
+#pragma unroll
+        for (int k = 0; k < KP + 2; ++k) {
+            const int sl = __shfl(ms, k, 64);
+            id[k] = sl >= 0 ? __float_as_int(a.m.vg_pts[sl].w) : -1;
+        }
+        const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
+        if (((int)threadIdx.x & 63) == 0) {
+            if (ok) {
+                store_vertex_pool(a, i, c4.x, c4.y, c4.z, id, k8, k9);
+            } else {
+                // beyond the wave's LDS: the per-lane search (rare)
+                int vl[KP];
+                vertex_search<KP, true>(a, i, make_float3(c4.x, c4.y, c4.z), __builtin_huge_valf(), vl);
+            }
+        }
+    }
+}
+
+// FP64 pass, part 4: the projection and freeze step of the sites part 3
+// searched, per thread, from their stored centroids and fresh pool lists.
+template <int KP>
+__global__ __launch_bounds__(256) void kb_precise_finish(const JobDesc* __restrict__ descs,
+                                                         const BlockRef* __restrict__ bt) {
+    const BlockRef br = bt[blockIdx.x];
+    if (br.job < 0) return;
+    const StepArgs& a = descs[br.job].a;
+    const int par = *a.parity;
+    const int n = a.stats[kStatVsearch];
+    const int stride = precise_blocks(a.nS) * 256;
+    for (int base = br.blk * 256; base < n; base += stride) {
+        const int e = base + (int)threadIdx.x;
+        const bool live = e < n;
+        SiteFlags f{};
+        if (live) {
+            const int i = a.vs_list[e];
+            const SiteCtx c = site_ctx_by_index(a, false, i);
+            const float4 ce = a.cent[i];
+            const int flags = __float_as_int(ce.w);
+            CellRes cr;
+            cr.c3 = make_float3(ce.x, ce.y, ce.z);
+            cr.same = (flags & kCentSame) != 0;
+            cr.cell_ok = (flags & kCentFail) == 0;
+            cr.ill = false;
+            cr.kind = (flags >> kCentKindShift) & 3;
+            cr.area = cr.moment = 0.f;
+            const int4 p0 = a.vpool[2 * (size_t)i];
+            int vl[KP] = {p0.x, p0.y, p0.z, p0.w, a.vpool[2 * (size_t)i + 1].x};
+            project_finish<KP, true>(a, par, false, c, cr, vl, f);
+        }
+        count_site(a, par, false, f, live);
     }
 }
 
@@ -271,12 +422,18 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
 // ------------------------------------------------------------------ launches
 template <int K>
 void launch_lloyd_k(const BatchTables& bt, int eval_only, int part, hipStream_t s) {
-    if (part == 0)
+    if (part == 0) {
         hipLaunchKernelGGL((kb_cell<K>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
                            eval_only);
-    else
-        hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), dim3(bt.njobs * kPreciseBlocks), dim3(256), 0, s, bt.descs,
-                           eval_only, part == 2 ? 1 : 0);
+        return;
+    }
+    const dim3 g(bt.n_prec_blocks), gw(bt.n_prec_blocks * kWaveBlocks), b(256);
+    if (part == 1 && !eval_only) hipLaunchKernelGGL((kb_precise_knn<K>), gw, b, 0, s, bt.descs, bt.prec_blocks);
+    hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), g, b, 0, s, bt.descs, bt.prec_blocks, eval_only, part == 2 ? 1 : 0);
+    if (part == 1 && !eval_only) {
+        hipLaunchKernelGGL((kb_precise_vsearch<5>), gw, b, 0, s, bt.descs, bt.prec_blocks);
+        hipLaunchKernelGGL((kb_precise_finish<5>), g, b, 0, s, bt.descs, bt.prec_blocks);
+    }
 }
 
 template <int K>

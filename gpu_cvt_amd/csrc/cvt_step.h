@@ -42,6 +42,7 @@ struct CellRes {
     float3 c3;      // centroid (NaN when the cell failed)
     float area, moment;
     bool same, cell_ok;
+    bool ill;       // FP32 pass: the cell's error allowance exceeds the tolerance (recomputed in FP64)
     int kind;
 };
 
@@ -202,7 +203,7 @@ __device__ __forceinline__ bool store_row(CVT_G int* __restrict__ row, const int
 // Returns false (before writing the row) when the FP32 KNN could not be proven.
 template <int K, bool PRECISE>
 __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval_only, int t, const SiteCtx& c,
-                                           int carried, int* lds, CellRes& r) {
+                                           int carried, int* lds, CellRes& r, int kind_carried = 0) {
     const float4 q = c.q;
     const float inf = __builtin_huge_valf();
     const CVT_G float4* __restrict__ S = a.S01[par];
@@ -211,9 +212,9 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
     r.moment = 0.f;
     r.same = true;
     r.cell_ok = false;
-    r.kind = 0;
+    r.ill = false;
+    r.kind = carried >= 0 ? kind_carried : 0;
     if (!c.valid) return true;
-    float px[K + 4], py[K + 4];
     int slots[K];
     // the precise pass reuses a row the fast pass already proved and wrote
     const bool use_row = c.fz || (PRECISE && carried >= 0);
@@ -251,48 +252,96 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
     }
     const CVT_G float4* __restrict__ src = use_row ? S : a.sg.pts;
     constexpr int kG = 8;  // gathers in flight per batch
+    if constexpr (PRECISE) {
+        // FP64 dual points of the FP32 positions, FP64 hull and centroid
+        double px[K + 4], py[K + 4];
+        const double ux = u.x, uy = u.y, uz = u.z, wx = w.x, wy = w.y, wz = w.z;
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += kG) {
-        float4 pb[kG];
-#pragma unroll
-        for (int b = 0; b < kG; ++b)
-            if (k0 + b < K) pb[b] = src[idx[k0 + b]];
-#pragma unroll
-        for (int b = 0; b < kG; ++b) {
-            const int k = k0 + b;
-            if (k >= K) break;
-            const float4 pj = pb[b];
-            idx[k] = __float_as_int(pj.w);  // the point's id (new lists)
-            const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
-            const float aa = dx * u.x + dy * u.y + dz * u.z;
-            const float bb = dx * w.x + dy * w.y + dz * w.z;
-            const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
-            const float ic = cc > 0.f ? fast_div(1.0f, cc) : 0.f;
-            px[k] = aa * ic;
-            py[k] = bb * ic;
+        for (int k = 0; k < K; ++k) {
+#pragma clang fp contract(off)
+            const float4 pj = src[idx[k]];
+            idx[k] = __float_as_int(pj.w);
+            const double dx = (double)pj.x - (double)q.x, dy = (double)pj.y - (double)q.y,
+                         dz = (double)pj.z - (double)q.z;
+            const double aa = dx * ux + dy * uy + dz * uz;
+            const double bb = dx * wx + dy * wy + dz * wz;
+            const double cc = 0.5 * (dx * dx + dy * dy + dz * dz);
+            px[k] = cc > 0.0 ? aa / cc : 0.0;
+            py[k] = cc > 0.0 ? bb / cc : 0.0;
         }
-    }
-    bool same = true;
-    if (!use_row && !eval_only) same = store_row<K>(row, idx, carried < 0);
-    r.same = carried >= 0 ? carried != 0 : same;
-    const float iR = 1.0f / a.R;
-    px[K] = iR;   py[K] = 0.f;
-    px[K + 1] = 0.f; py[K + 1] = iR;
-    px[K + 2] = -iR; py[K + 2] = 0.f;
-    px[K + 3] = 0.f; py[K + 3] = -iR;
-    // near-collinear orientation tests are decided in FP64 inline (both passes)
-    const CellOut co = dual_hull_cell<K + 4>(px, py);
-    r.cell_ok = co.ok;
-    r.area = co.area;
-    r.moment = co.moment;
-    if (co.ok) {
-        r.c3 = make_float3(q.x + co.cx * u.x + co.cy * w.x, q.y + co.cx * u.y + co.cy * w.y,
-                           q.z + co.cx * u.z + co.cy * w.z);
+        bool same = true;
+        if (!use_row && !eval_only) same = store_row<K>(row, idx, carried < 0);
+        r.same = carried >= 0 ? carried != 0 : same;
+        const double iR = 1.0 / (double)a.R;
+        px[K] = iR;   py[K] = 0.0;
+        px[K + 1] = 0.0; py[K + 1] = iR;
+        px[K + 2] = -iR; py[K + 2] = 0.0;
+        px[K + 3] = 0.0; py[K + 3] = -iR;
+        const CellOut64 co = dual_hull_cell64<K + 4>(px, py);
+        r.cell_ok = co.ok;
+        r.area = (float)co.area;
+        r.moment = (float)co.moment;
+        if (co.ok) {
+#pragma clang fp contract(off)
+            r.c3 = make_float3((float)((double)q.x + co.cx * ux + co.cy * wx),
+                               (float)((double)q.y + co.cx * uy + co.cy * wy),
+                               (float)((double)q.z + co.cx * uz + co.cy * wz));
+        } else {
+            const float nan = __builtin_nanf("");
+            r.c3 = make_float3(nan, nan, nan);
+        }
+        return true;
     } else {
-        const float nan = __builtin_nanf("");
-        r.c3 = make_float3(nan, nan, nan);
+        float px[K + 4], py[K + 4];
+        float icmax = 0.f;  // 1 / (|d|^2 / 2) of the nearest neighbour
+#pragma unroll
+        for (int k0 = 0; k0 < K; k0 += kG) {
+            float4 pb[kG];
+#pragma unroll
+            for (int b = 0; b < kG; ++b)
+                if (k0 + b < K) pb[b] = src[idx[k0 + b]];
+#pragma unroll
+            for (int b = 0; b < kG; ++b) {
+                const int k = k0 + b;
+                if (k >= K) break;
+                const float4 pj = pb[b];
+                idx[k] = __float_as_int(pj.w);  // the point's id (new lists)
+                const float dx = pj.x - q.x, dy = pj.y - q.y, dz = pj.z - q.z;
+                const float aa = dx * u.x + dy * u.y + dz * u.z;
+                const float bb = dx * w.x + dy * w.y + dz * w.z;
+                const float cc = 0.5f * (dx * dx + dy * dy + dz * dz);
+                const float ic = cc > 0.f ? fast_div(1.0f, cc) : 0.f;
+                icmax = fmaxf(icmax, ic);
+                px[k] = aa * ic;
+                py[k] = bb * ic;
+            }
+        }
+        bool same = true;
+        if (!use_row && !eval_only) same = store_row<K>(row, idx, carried < 0);
+        r.same = carried >= 0 ? carried != 0 : same;
+        const float iR = 1.0f / a.R;
+        px[K] = iR;   py[K] = 0.f;
+        px[K + 1] = 0.f; py[K + 1] = iR;
+        px[K + 2] = -iR; py[K + 2] = 0.f;
+        px[K + 3] = 0.f; py[K + 3] = -iR;
+        // every dual point (a, b) / c is within ~5 u (2 / |d|) of its exact
+        // value; 2 / |d| is largest for the nearest neighbour
+        const float eta = 4.8e-7f * sqrtf(2.f * icmax);
+        // near-collinear orientation tests are decided in FP64 inline (both passes)
+        const CellOut co = dual_hull_cell<K + 4>(px, py, eta);
+        r.cell_ok = co.ok;
+        r.area = co.area;
+        r.moment = co.moment;
+        r.ill = co.ok && 2.f * co.err > a.tol_abs;
+        if (co.ok) {
+            r.c3 = make_float3(q.x + co.cx * u.x + co.cy * w.x, q.y + co.cx * u.y + co.cy * w.y,
+                               q.z + co.cx * u.z + co.cy * w.z);
+        } else {
+            const float nan = __builtin_nanf("");
+            r.c3 = make_float3(nan, nan, nan);
+        }
+        return true;
     }
-    return true;
 }
 
 __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c, const CellRes& cr) {
@@ -397,9 +446,48 @@ __device__ __forceinline__ bool needs_list(const SiteCtx& c, const CellRes& cr) 
     return c.valid && !c.fz && finite3(cr.c3.x, cr.c3.y, cr.c3.z);
 }
 
-// Steps 5 and 6 given the list vl of an active site (ignored for a frozen one,
-// whose pinned list is read back).
+// The nearest point to c over the faces incident to the list vertices vl, in
+// FP64: faces in list order and vf-CSR order, a face incident to an earlier
+// list vertex skipped (already visited), the first strict minimum kept
+// (near_project.cuh:169-186).  c itself when no face qualifies.
 template <int KP>
+__device__ __forceinline__ float3 nearest_on_faces64(const StepArgs& a, float3 c, const int (&vl)[KP]) {
+#pragma clang fp contract(off)
+    const double3 P = make_double3(c.x, c.y, c.z);
+    double best = 1e30;
+    double3 bp = P;
+    for (int k = 0; k < KP; ++k) {
+        if ((unsigned)vl[k] >= (unsigned)a.m.nV) continue;
+        const int o1 = a.m.vf_off[vl[k] + 1];
+        for (int ii = a.m.vf_off[vl[k]]; ii < o1; ++ii) {
+            const int f = a.m.vf_faces[ii];
+            const float4 A = a.m.tri[3 * f], B = a.m.tri[3 * f + 1], C = a.m.tri[3 * f + 2];
+            bool seen = false;
+            for (int s2 = 0; s2 < k; ++s2) {
+                const float vs = __int_as_float(vl[s2]);
+                seen = seen || (A.w == vs) || (B.w == vs) || (C.w == vs);
+            }
+            if (seen) continue;
+            const double3 p = closest_point_tri64(P, make_double3(A.x, A.y, A.z), make_double3(B.x, B.y, B.z),
+                                                  make_double3(C.x, C.y, C.z));
+            const double dx = p.x - P.x, dy = p.y - P.y, dz = p.z - P.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < best) { best = d2; bp = p; }
+        }
+    }
+    return make_float3((float)bp.x, (float)bp.y, (float)bp.z);
+}
+
+// Steps 5 and 6 given the list vl of an active site (ignored for a frozen one,
+// whose pinned list is read back).  The nearest point over the faces is the
+// first strict minimum in visiting order (near_project.cuh:169-186).
+// !PRECISE: FP32 offsets to the closest points; when a face's FP32 distance
+// lies within the FP32 error of the best so far while the two closest points
+// are more than a quarter of the tolerance apart (a centroid near the medial
+// axis of its faces, whose choice FP32 cannot make), the walk is repeated in
+// FP64 after the FP32 one, so the choice is the FP64 one.  PRECISE: the FP64
+// walk alone.
+template <int KP, bool PRECISE = false>
 __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
                                                const CellRes& cr, int (&vl)[KP], SiteFlags& r) {
     const int has_prev = eval_only ? 0 : *a.has_prev;
@@ -420,49 +508,70 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
             for (int k = 0; k < KP; ++k) vl[k] = row[k];
         }
         // faces incident to the list vertices, in list order; a face incident
-        // to an earlier list vertex was already visited.  The fan bounds are
-        // loaded together and faces go kFaceBatch at a time, so that more
-        // loads are in flight per round trip.
-        float best = 1e30f;
-        int o0[KP], o1[KP];
+        // to an earlier list vertex was already visited
+        if constexpr (PRECISE) {
+            sp = nearest_on_faces64<KP>(a, c3, vl);
+        } else {
+            // The FP32 offsets o carry an error of a few roundings of |AP| +
+            // |AB| + |AC| <= sqrt(3) (d + 3 E) (E: the mesh's longest edge, d:
+            // the distance), so |d2 - exact| <= 2 d 6u sqrt(3) (d + 3E) + 3u d2.
+            float best = 1e30f;
+            float3 bo = make_float3(0.f, 0.f, 0.f);
+            bool amb = false;
+            const float tol2 = a.tol_abs * a.tol_abs;
+            const float e3 = 3.f * a.m.max_edge;
+            int o0[KP], o1[KP];
 #pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
-            o0[k] = ok ? a.m.vf_off[vl[k]] : 0;
-            o1[k] = ok ? a.m.vf_off[vl[k] + 1] : 0;
-        }
-        auto visit = [&](const float4& A, const float4& B, const float4& C, int k) {
-            bool seen = false;
-#pragma unroll
-            for (int s2 = 0; s2 < KP - 1; ++s2) {
-                const float vs = __int_as_float(vl[s2]);
-                seen = seen || (s2 < k && ((A.w == vs) || (B.w == vs) || (C.w == vs)));
+            for (int k = 0; k < KP; ++k) {
+                const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
+                o0[k] = ok ? a.m.vf_off[vl[k]] : 0;
+                o1[k] = ok ? a.m.vf_off[vl[k] + 1] : 0;
             }
-            if (seen) return;
-            const float3 p = closest_point_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
-                                               make_float3(C.x, C.y, C.z));
-            const float ddx = p.x - c3.x, ddy = p.y - c3.y, ddz = p.z - c3.z;
-            const float d2 = ddx * ddx + ddy * ddy + ddz * ddz;
-            if (d2 < best) { best = d2; sp = p; }
-        };
-        constexpr int kFaceBatch = 6;
+            auto visit = [&](const float4& A, const float4& B, const float4& C, int k) {
+                bool seen = false;
 #pragma unroll
-        for (int k = 0; k < KP; ++k) {
-            for (int ii = o0[k]; ii < o1[k]; ii += kFaceBatch) {
-                int f[kFaceBatch];
-                float4 A[kFaceBatch], B[kFaceBatch], C[kFaceBatch];
-#pragma unroll
-                for (int b = 0; b < kFaceBatch; ++b) f[b] = a.m.vf_faces[min(ii + b, o1[k] - 1)];
-#pragma unroll
-                for (int b = 0; b < kFaceBatch; ++b) {
-                    A[b] = a.m.tri[3 * f[b]];
-                    B[b] = a.m.tri[3 * f[b] + 1];
-                    C[b] = a.m.tri[3 * f[b] + 2];
+                for (int s2 = 0; s2 < KP - 1; ++s2) {
+                    const float vs = __int_as_float(vl[s2]);
+                    seen = seen || (s2 < k && ((A.w == vs) || (B.w == vs) || (C.w == vs)));
                 }
+                if (seen) return;
+                const float3 o = closest_offset_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                                                    make_float3(C.x, C.y, C.z));
+                const float d2 = o.x * o.x + o.y * o.y + o.z * o.z;
+                const float mx = fmaxf(d2, best), dd = fabsf(d2 - best);
+                // the bound with d e3 <= (d^2 + e3^2) / 2 first (no square root)
+                if (dd <= 2.2e-6f * mx + 6.5e-7f * e3 * e3) {
+                    const float dm = sqrtf(mx);
+                    if (dd <= 1.3e-6f * dm * (dm + e3) + 2e-7f * mx) {
+                        const float sx = o.x - bo.x, sy = o.y - bo.y, sz = o.z - bo.z;
+                        amb = amb || (sx * sx + sy * sy + sz * sz > tol2);
+                    }
+                }
+                if (d2 < best) { best = d2; bo = o; }
+            };
+            // the fan bounds are loaded together and faces go kFaceBatch at a
+            // time, so that more loads are in flight per round trip
+            constexpr int kFaceBatch = 6;
 #pragma unroll
-                for (int b = 0; b < kFaceBatch; ++b)
-                    if (ii + b < o1[k]) visit(A[b], B[b], C[b], k);
+            for (int k = 0; k < KP; ++k) {
+                for (int ii = o0[k]; ii < o1[k]; ii += kFaceBatch) {
+                    int f[kFaceBatch];
+                    float4 A[kFaceBatch], B[kFaceBatch], C[kFaceBatch];
+#pragma unroll
+                    for (int b = 0; b < kFaceBatch; ++b) f[b] = a.m.vf_faces[min(ii + b, o1[k] - 1)];
+#pragma unroll
+                    for (int b = 0; b < kFaceBatch; ++b) {
+                        A[b] = a.m.tri[3 * f[b]];
+                        B[b] = a.m.tri[3 * f[b] + 1];
+                        C[b] = a.m.tri[3 * f[b] + 2];
+                    }
+#pragma unroll
+                    for (int b = 0; b < kFaceBatch; ++b)
+                        if (ii + b < o1[k]) visit(A[b], B[b], C[b], k);
+                }
             }
+            if (best < 1e30f) sp = make_float3(c3.x - bo.x, c3.y - bo.y, c3.z - bo.z);
+            if (amb) sp = nearest_on_faces64<KP>(a, c3, vl);
         }
         if (!finite3(sp.x, sp.y, sp.z) && a.nan_policy == 0) sp = make_float3(q.x, q.y, q.z);
     }
@@ -504,15 +613,103 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
     r.nonfinite = c.live && !finite3(sp.x, sp.y, sp.z);
 }
 
+// The kproj nearest mesh vertices of c3 (and the pool cache refreshed) for
+// every lane of the wave with want set, one lane at a time by the whole wave
+// (wave_bvh_topl over the vertex LBVH: the FP64 pass's centroids are often far
+// off the surface, where a per-lane search is a long chain of dependent node
+// visits).  Every lane of the wave must call it.  Lanes whose search did not
+// fit the wave's LDS keep want set (the caller searches per lane).
+// The pool cache of site i after an exact search at (qx, qy, qz): its 7
+// nearest vertices, the 8th distance (rounded down: it bounds every vertex
+// outside the pool from below) and the 9th (rounded up: it seeds the next
+// search from above), from FP64 keys.
+__device__ __forceinline__ void store_vertex_pool(const StepArgs& a, int i, float qx, float qy, float qz,
+                                                  const int (&id)[7], double k8, double k9) {
+    const float inf = __builtin_huge_valf();
+    float r8 = k8 < 1e300 ? (float)sqrt(k8) : inf, r9 = k9 < 1e300 ? (float)sqrt(k9) : inf;
+    if (r8 < inf && (double)r8 > sqrt(k8)) r8 = __int_as_float(__float_as_int(r8) - 1);
+    if (r9 < inf && (double)r9 < sqrt(k9)) r9 = __int_as_float(__float_as_int(r9) + 1);
+    a.vcache[i] = make_float4(qx, qy, qz, r8);
+    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
+    a.vpool[2 * (size_t)i + 1] = make_int4(id[4], id[5], id[6], __float_as_int(r9));
+}
+
+constexpr int kVsStack = 2048, kVsCand = 256;
+constexpr int kVsLdsBytes = 4 * kVsStack + 8 * kVsCand;  // per wave
+
+template <int KP>
+__device__ __forceinline__ void wave_vertex_searches(const StepArgs& a, int i, float3 c3, bool& want, int (&vl)[KP],
+                                                     char* lds) {
+    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
+    constexpr int L = KP + 4;  // the pool's 7, its 8th distance (bound) and 9th (seed)
+    const int lane = (int)threadIdx.x & 63;
+    unsigned long long need = __ballot(want);
+    while (need) {
+        const int l = (int)__ffsll((long long)need) - 1;
+        need &= need - 1ull;
+        const float qx = __shfl(c3.x, l, 64), qy = __shfl(c3.y, l, 64), qz = __shfl(c3.z, l, 64);
+        const int qi = __shfl(i, l, 64);
+        int ms;
+        double mk;
+        bool ok;
+        wave_bvh_topl<L, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, qx, qy, qz, lds, ms, mk, ok);
+        if (!ok) continue;  // want stays set: lane l searches itself
+        int id[KP + 2];
+#pragma unroll
+        for (int k = 0; k < KP + 2; ++k) {
+            const int sl = __shfl(ms, k, 64);
+            id[k] = sl >= 0 ? __float_as_int(a.m.vg_pts[sl].w) : -1;
+        }
+        const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
+        if (lane == l) {
+#pragma unroll
+            for (int k = 0; k < KP; ++k) vl[k] = id[k];
+            want = false;
+        }
+        if (lane == 0) store_vertex_pool(a, qi, qx, qy, qz, id, k8, k9);
+    }
+}
+
 // The three pieces in one thread.  Returns r.done = false (r.row_written,
 // r.same set for the precise pass) when the FP32 kproj search could not be
-// proven.
+// proven.  PRECISE (wave_lds: kVsLdsBytes of LDS for each wave of the block):
+// every lane of the wave must call it.
+// post_search (PRECISE): a site whose vertex list the pool cannot prove is not
+// projected here: its centroid and cell flags go to cent and the site to
+// vs_list, and r.done stays false (one wave per such site searches, then the
+// projection finishes per thread).
 template <int KP, bool PRECISE>
 __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
-                                              const CellRes& cr, SiteFlags& r) {
+                                              const CellRes& cr, SiteFlags& r, char* wave_lds = nullptr,
+                                              bool post_search = false) {
     int vl[KP];
 #pragma unroll
     for (int k = 0; k < KP; ++k) vl[k] = -1;
+    if constexpr (PRECISE) {
+        bool want = false;
+        float tauv = __builtin_huge_valf();
+        if (!eval_only && needs_list(c, cr)) want = !pool_lookup<KP>(a, c.i, cr.c3, c.use_cache, vl, tauv);
+        // a seed within two vertex cells: the per-lane grid search stays short
+        const float h2 = 4.f * a.m.vg.h * a.m.vg.h;
+        if (want && tauv <= h2) {
+            vertex_search<KP, true>(a, c.i, cr.c3, tauv, vl);
+            want = false;
+        }
+        if (post_search) {
+            if (want) {
+                const int flags = kCentDone | (cr.same ? kCentSame : 0) | (cr.cell_ok ? 0 : kCentFail) |
+                                  (cr.kind << kCentKindShift);
+                a.cent[c.i] = make_float4(cr.c3.x, cr.c3.y, cr.c3.z, __int_as_float(flags));
+                a.vs_list[atomicAdd(&a.stats[kStatVsearch], 1)] = c.i;
+                return;  // r.done stays false
+            }
+        } else {
+            wave_vertex_searches<KP>(a, c.i, cr.c3, want, vl, wave_lds);
+        }
+        if (want) vertex_search<KP, true>(a, c.i, cr.c3, tauv, vl);
+        project_finish<KP, true>(a, par, eval_only, c, cr, vl, r);
+        return;
+    }
     if (!eval_only && needs_list(c, cr)) {
         float tauv;
         if (!pool_lookup<KP>(a, c.i, cr.c3, c.use_cache, vl, tauv) &&
@@ -523,19 +720,20 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
             return;
         }
     }
-    project_finish<KP>(a, par, eval_only, c, cr, vl, r);
+    project_finish<KP, PRECISE>(a, par, eval_only, c, cr, vl, r);
 }
 
 // The whole step of one site in one thread (the precise pass).
 template <int K, int KP>
 __device__ __forceinline__ SiteFlags lloyd_site_precise(const StepArgs& a, int par, bool eval_only, int t, bool live,
-                                                        int carried) {
+                                                        int carried, int kind_carried, char* wave_lds,
+                                                        bool post_search) {
     SiteFlags r{};
     SiteCtx c = site_ctx(a, eval_only, t, live);
     phase_frame<KP, true>(a, eval_only, c);
     CellRes cr;
-    phase_cell<K, true>(a, par, eval_only, t, c, carried, nullptr, cr);
-    phase_project<KP, true>(a, par, eval_only, c, cr, r);
+    phase_cell<K, true>(a, par, eval_only, t, c, carried, nullptr, cr, kind_carried);
+    phase_project<KP, true>(a, par, eval_only, c, cr, r, wave_lds, post_search);
     write_debug(a, c, cr);
     return r;
 }
@@ -582,9 +780,15 @@ __device__ __forceinline__ void count_site(const StepArgs& a, int par, bool eval
 
 // Hand site t to the FP64 pass: the frame and cell phases append to the front
 // of defer_list, the projection phase to the back (each site at most once).
+// Entry of defer_list: the site's grid slot and three flags.
+constexpr int kDeferRow = 1 << 30;    // its KNN row is written (proven) ...
+constexpr int kDeferSame = 1 << 29;   // ... and equal to the previous one
+constexpr int kDeferExact = 1 << 28;  // the row came from the FP64 wave search
+constexpr int kDeferSlot = (1 << 28) - 1;
+
 __device__ __forceinline__ void defer_site(const StepArgs& a, int t, bool row_written, bool same, bool back) {
     const int pos = atomicAdd(&a.stats[back ? kStatDeferredB : kStatDeferred], 1);
-    a.defer_list[back ? a.nS - 1 - pos : pos] = t | (row_written ? (1 << 30) : 0) | (same ? (1 << 29) : 0);
+    a.defer_list[back ? a.nS - 1 - pos : pos] = t | (row_written ? kDeferRow : 0) | (same ? kDeferSame : 0);
 }
 
 }  // namespace cvt

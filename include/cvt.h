@@ -58,6 +58,8 @@ typedef struct cvt_iter_stats {
     int nonfinite_sites;   /* sites with non-finite coordinates after the step */
     int deferred_sites;    /* sites the FP32 pass handed to the FP64-resolving pass */
     float max_disp;        /* largest |S' - S| of the iteration */
+    int deferred_projections; /* of deferred_sites: nearest face not decidable in FP32 */
+    int fp64_cells;        /* of deferred_sites: cells whose FP32 error allowance exceeded the tolerance */
     int reserved[1];
 } cvt_iter_stats;
 

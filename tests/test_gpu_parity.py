@@ -58,14 +58,8 @@ def _case(name):
     raise KeyError(name)
 
 
-def _stage_check(orc, mesh, job, S32, K, prev_state=None, far_rel=None):
-    """Run one GPU step from S32 and compare every stage with the oracle.
-
-    far_rel: with few neighbours (K <= 6) some cells stay open and their
-    centroids land far off the surface (~0.5-1 x the radius), where the
-    nearest point over the fan faces is ill-conditioned in FP32 (near-equal
-    face distances).  Sites whose centroid lies more than 0.05 x diag off the
-    surface are then held to err <= far_rel x that distance instead."""
+def _stage_check(orc, mesh, job, S32, K):
+    """Compare every stage of the GPU step just run from S32 with the oracle."""
     diag = mesh.bbox_diag
     knn_g = job.knn()
     dbg = job.debug()
@@ -94,11 +88,6 @@ def _stage_check(orc, mesh, job, S32, K, prev_state=None, far_rel=None):
     # 5. projection from the same centroids and lists
     proj_o = orc.project(cent_g32, knnv_o, mesh.V, mesh.F, mesh.vf)
     err = np.linalg.norm(Snew_g - proj_o, axis=1)
-    if far_rel is not None:
-        off = np.linalg.norm(cent_g32 - proj_o, axis=1)
-        far = off > 0.05 * diag
-        assert np.all(err[far] <= far_rel * off[far]), f"far-centroid projection error {np.max(err[far] / off[far]):.3e} x"
-        err = err[~far]
     err_p = err.max() if err.size else 0.0
     assert err_p <= TOL_REL_DIAG * diag, f"projection error {err_p:.3e}"
     return dict(knn=knn_g, cent=dbg["cent"], Snew=Snew_g, err_c=err_c, err_p=err_p)
@@ -119,55 +108,38 @@ def test_stages_teacher_forced(orc, name):
             assert st["cell_failures"] == 0
 
 
-@pytest.mark.parametrize("K", [4, 6])
-def test_small_k_knn_exact(orc, K):
-    """K = 4 and 6: most cells stay open, so sites jump across the sphere and
-    later iterations hold cells whose dual points span extreme scales (beyond
-    the FP32 dual hull's accuracy, as for the reference's FP32 clipper).  KNN
-    lists must stay bit-exact in every iteration; the first iteration gets the
-    full stage check, with far-off centroids held to a relative bar."""
+@pytest.mark.parametrize("K", [4, 6, 8])
+def test_small_k_every_stage(orc, K):
+    """K = 4, 6 and 8: most cells stay open, so centroids land far off the
+    surface and sites jump across the sphere; later iterations hold cells whose
+    dual points span extreme scales.  Every stage of every iteration is held to
+    the usual bars: such cells and projections go to the FP64 pass."""
     V, F, S, K = _case(f"kcov_k{K}")
     mesh = orc.Mesh(V, F)
     with cvt.Job(V, F, S, K=K, debug=1) as job:
-        for it in range(5):
+        for it in range(6):
             S32 = orc.f32r(job.sites())
-            job.step(1)
-            if it == 0:
-                r = _stage_check(orc, mesh, job, S32, K, far_rel=1e-3)
-                print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
-                continue
-            knn_o = orc.knn(S32, S32, K, exclude_self=True)
-            assert np.array_equal(job.knn(), knn_o), f"iter {it}: site KNN mismatch"
-            cent_g32 = orc.f32r(job.debug()["cent"])
-            ok = np.all(np.isfinite(cent_g32), axis=1)
-            knnv_o = orc.knn(mesh.V, cent_g32[ok], 5, exclude_self=False)
-            assert np.array_equal(job.knnv()[ok], knnv_o), f"iter {it}: vertex KNN mismatch"
+            st = job.step(1, stats=True)[0]
+            r = _stage_check(orc, mesh, job, S32, K)
+            print(f"  iter {it}: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e} "
+                  f"deferred {st['deferred_sites']}", flush=True)
 
 
-def test_cluster_knn_exact(orc):
+def test_cluster_every_stage(orc):
     """Sites packed ~500x denser on a few faces than elsewhere: the site
     search's LDS collect overflows and rows exceed its 16-bit offsets, so the
-    exact fallbacks run.  KNN lists must stay bit-exact in every iteration.
-    Cells and projections are held to the usual bar in the first iteration
-    only: afterwards open cells at the cluster's edge send sites across the
-    sphere, and cells whose dual points span ~2000:1 in scale are beyond the
-    FP32 accuracy of the dual hull (the reference's FP32 clipper shares it)."""
+    exact fallbacks run; cells at the cluster's edge have dual points spanning
+    ~2000:1 in scale after the first iteration.  Every stage of every
+    iteration is held to the usual bars."""
     V, F, S, K = _case("cluster_k24")
     mesh = orc.Mesh(V, F)
     with cvt.Job(V, F, S, K=K, debug=1) as job:
-        for it in range(5):
+        for it in range(6):
             S32 = orc.f32r(job.sites())
-            job.step(1)
-            knn_o = orc.knn(S32, S32, K, exclude_self=True)
-            knn_g = job.knn()
-            assert np.array_equal(knn_g, knn_o), f"iter {it}: site KNN mismatch in {np.sum(np.any(knn_g != knn_o, 1))} rows"
-            cent_g32 = orc.f32r(job.debug()["cent"])
-            ok = np.all(np.isfinite(cent_g32), axis=1)
-            knnv_o = orc.knn(mesh.V, cent_g32[ok], 5, exclude_self=False)
-            assert np.array_equal(job.knnv()[ok], knnv_o), f"iter {it}: vertex KNN mismatch"
-            if it == 0:
-                r = _stage_check(orc, mesh, job, S32, K)
-                print(f"  iter 0: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e}", flush=True)
+            st = job.step(1, stats=True)[0]
+            r = _stage_check(orc, mesh, job, S32, K)
+            print(f"  iter {it}: centroid err {r['err_c']:.2e} proj err {r['err_p']:.2e} "
+                  f"deferred {st['deferred_sites']}", flush=True)
 
 
 @pytest.mark.parametrize("scales", [(0.7, 0.7), (2.2, 2.6)])
