@@ -59,49 +59,70 @@ def traffic_from_profile(workload: str):
     return None, None
 
 
+def cpu_threads() -> tuple[int, int]:
+    """(threads used, hardware threads): every CPU this process may run on,
+    capped at the machine's CPU share for this process when the environment
+    states one (OMP_NUM_THREADS)."""
+    hw = os.cpu_count() or 1
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else hw
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return max(1, min(allowed, share) if share > 0 else allowed), hw
+
+
+def _time_cpu(step, n_total: int, budget_s: float, sub_sizes):
+    """Times step(n, iters) -> seconds on growing site samples until one run
+    fills about half the budget; returns (sites, iterations, seconds)."""
+    n, it, dt = sub_sizes[0], 1, step(sub_sizes[0], 1)
+    for n2 in sub_sizes[1:]:
+        if dt > 0.25 * budget_s:
+            break
+        n, dt = n2, step(n2, 1)
+    if n == n_total and dt < 0.5 * budget_s:
+        it = max(1, int(0.5 * budget_s / dt))
+        if it > 1:
+            dt = step(n, it)
+    return n, it, dt
+
+
 def cpu_baseline(V, F, S, K: int, budget_s: float = 20.0):
-    """Time the FP64 oracle's full Lloyd step on host threads (bounded sample)."""
+    """The CPU Lloyd step on this host, on bounded samples of the same job:
+    the FP32 speed variant (oracle/cvt_cpu32.cpp) and the FP64 oracle, each on
+    every CPU this process may use (pinned), about budget_s / 2 each."""
     from oracle import oracle as orc
 
     orc.build()
-    threads = min(16, orc.hw_threads())
+    threads, hw = cpu_threads()
     orc.set_threads(threads)
     mesh = orc.Mesh(V, F)
     R = float(np.float32(mesh.bbox_max_edge))
-    # calibrate on a small subsample, then size the sample to the budget
     rng = np.random.Generator(np.random.PCG64(123))
-    n0 = min(len(S), 20000)
-    sub = S[np.sort(rng.choice(len(S), n0, replace=False))]
-    st = orc.LloydState(sub, K)
-    t0 = time.perf_counter()
-    orc.lloyd_step(mesh, st, K=K, R=R)
-    dt0 = time.perf_counter() - t0
-    rate0 = n0 / dt0
-    n = int(min(len(S), max(n0, rate0 * budget_s * 0.5)))
-    if n > n0:
-        sub = S[np.sort(rng.choice(len(S), n, replace=False))]
-        st = orc.LloydState(sub, K)
+    sizes = sorted({min(len(S), x) for x in (20_000, 100_000, 300_000, len(S))})
+    subs = {n: S[np.sort(rng.choice(len(S), n, replace=False))] if n < len(S) else S for n in sizes}
+
+    def fp32(n, iters):
         t0 = time.perf_counter()
-        orc.lloyd_step(mesh, st, K=K, R=R)
-        dt = time.perf_counter() - t0
-    else:
-        n, dt = n0, dt0
-    # the whole job in one iteration takes less than the budget: time further
-    # iterations of the same sites (the next Lloyd steps, as on the GPU)
-    iters = 1
-    if n == len(S) and dt < budget_s * 0.5:
-        extra = int(budget_s * 0.5 / dt) - 1
+        orc.cpu32_lloyd(mesh, subs[n], K, R, iters, threads)
+        return time.perf_counter() - t0
+
+    def fp64(n, iters):
+        st = orc.LloydState(subs[n], K)
         t0 = time.perf_counter()
-        for _ in range(extra):
+        for _ in range(iters):
             orc.lloyd_step(mesh, st, K=K, R=R)
-        dt += time.perf_counter() - t0
-        iters += extra
-    return {"value": n * iters / dt, "unit": "site-iters/s", "cores": threads, "kind": "port",
-            "sample": f"{iters} FP64 oracle Lloyd iteration(s) over {n} of the {len(S)} sites on the same mesh "
-                      f"({dt:.1f} s, {threads} threads)"}
+        return time.perf_counter() - t0
+
+    n32, it32, dt32 = _time_cpu(fp32, len(S), 0.5 * budget_s, sizes)
+    n64, it64, dt64 = _time_cpu(fp64, len(S), 0.5 * budget_s, sizes)
+    v32, v64 = n32 * it32 / dt32, n64 * it64 / dt64
+    model = orc.cpu_model()
+    return {"value": v32, "unit": "site-iters/s", "cores": threads, "kind": "port",
+            "sample": f"FP32 C++ Lloyd step: {it32} iteration(s) over {n32} of the {len(S)} sites ({dt32:.1f} s); "
+                      f"FP64 oracle step: {it64} iteration(s) over {n64} sites ({dt64:.1f} s); same mesh, "
+                      f"{threads} pinned threads",
+            "threads": threads, "hardware_threads": hw, "model": model, "fp32_value": v32, "fp64_value": v64}
 
 
-def bench_batch(args) -> int:
+def run_batch(jobs: int, iters: int, warmup: int) -> dict:
     """Mixed-size jobs (log-uniform triangles 5k-400k, sites 1k-100k capped at
     triangles/2, K = 24, freezing on): site-iters/s of the batched form (all
     jobs in one graph launch per iteration) against the plain path (each job's
@@ -111,18 +132,17 @@ def bench_batch(args) -> int:
     from gpu_cvt_amd import cvt, meshgen as mg
 
     t_gen = time.perf_counter()
-    work = mg.batch_workload(args.jobs, seed=7)
+    work = mg.batch_workload(jobs, seed=7)
     t_gen = time.perf_counter() - t_gen
     total_sites = sum(len(w[2]) for w in work)
     kw = dict(K=24, freeze_mode=1, freeze_need=5, freeze_disp_rel=1e-5)
     print(f"[bench] batch: {len(work)} jobs, {total_sites} sites, "
           f"{sum(len(w[1]) for w in work)} triangles, gen {t_gen:.1f}s", file=sys.stderr, flush=True)
-    iters = args.steps
     # plain path: one job at a time, each job's own graph
     plain_ms = 0.0
     for k, (V, F, S, _tag) in enumerate(work):
         with cvt.Job(V, F, S, **kw) as job:
-            job.step(args.warmup)
+            job.step(warmup)
             plain_ms += job.step_timed(iters)
         if k % 32 == 31:
             print(f"[bench] plain: {k + 1} jobs done", file=sys.stderr, flush=True)
@@ -130,7 +150,7 @@ def bench_batch(args) -> int:
     # batched: all jobs resident, one graph launch per iteration
     jobs = [cvt.Job(V, F, S, **kw) for V, F, S, _tag in work]
     with cvt.Batch(jobs) as b:
-        b.step(args.warmup)
+        b.step(warmup)
         batch_ms = b.step_timed(iters)
     frozen = sum(int(j.freeze_state()[0].sum()) for j in jobs)
     for j in jobs:
@@ -138,37 +158,47 @@ def bench_batch(args) -> int:
     batch_rate = total_sites * iters / (batch_ms / 1e3)
     print(f"[bench] plain {plain_ms:.1f} ms, batched {batch_ms:.1f} ms for {iters} iterations; "
           f"frozen at end {frozen}/{total_sites}", file=sys.stderr, flush=True)
+    return {"jobs": len(work), "sites": total_sites, "triangles": int(sum(len(w[1]) for w in work)), "K": 24,
+            "freezing": True, "iterations": iters, "warmup": warmup, "batched_site_iters_per_s": batch_rate,
+            "plain_site_iters_per_s": plain_rate, "batched_ms_per_iteration": batch_ms / iters,
+            "batched_over_plain": batch_rate / plain_rate, "frozen_at_end": frozen}
+
+
+def bench_batch(args) -> int:
+    """Batched form (the cfg4 shape), printed as its own JSON line."""
+    r = run_batch(args.jobs, args.steps, args.warmup)
     out = {
         "metric": "batched Lloyd site-iters/s, 256 mixed jobs (vs one job at a time)",
-        "value": batch_rate, "unit": "site-iters/s", "n_gpus": 1, "steps": iters, "warmup": args.warmup,
-        "ms_per_step": batch_ms / iters, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded icospheres, tori, displaced spheres)",
-        "config": {"workload": "batch_mixed", "jobs": len(work), "sites": total_sites, "K": 24, "freezing": True},
-        "plain_value": plain_rate, "batched_over_plain": batch_rate / plain_rate,
+        "value": r["batched_site_iters_per_s"], "unit": "site-iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["batched_ms_per_iteration"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded icospheres, tori, displaced spheres)",
+        "config": {"workload": "batch_mixed", "jobs": r["jobs"], "sites": r["sites"], "K": 24, "freezing": True},
+        "plain_value": r["plain_site_iters_per_s"], "batched_over_plain": r["batched_over_plain"],
     }
     print(json.dumps(out), flush=True)
     return 0
 
 
-def bench_serve(args) -> int:
-    """Serving form: a seeded Poisson request trace (arrivals at --rate jobs/s,
+def run_serve(arrivals: int, rate: float, ingest_threads: int = 4, serve_log: str = "") -> dict:
+    """Serving form: a seeded Poisson request trace (arrivals at `rate` jobs/s,
     batched-form sizes plus 5% large 1M-site jobs) submitted in real time to
     the continuous-batching loop (cvt_server_*) under a 200 GB admission
     budget; each job runs to a one-sided Hausdorff distance <= 1e-3 x bbox
     diagonal, checked every 10 iterations, or 100 iterations.  The same trace
     is then run first-in-first-out, one job at a time, on the plain path
-    (cvt_job_step + cvt_job_hausdorff_within) with the same checks.  Both paths give
-    every job bit-identical iterates, so the final Hausdorff values agree
-    exactly; the report gives p50/p99 time-to-convergence (arrival -> result)
-    and aggregate site-iters/s of both."""
+    (cvt_job_step + cvt_job_hausdorff_within) with the same checks.  Both paths
+    give every job bit-identical iterates, so the final Hausdorff values agree
+    exactly.  Returns p50/p99 time-to-convergence (arrival -> result) over the
+    converged jobs and over all jobs, and aggregate site-iters/s, of both."""
     from gpu_cvt_amd import cvt, meshgen as mg
 
     t_gen = time.perf_counter()
-    trace = mg.serve_trace(args.arrivals, args.rate, seed=8)
+    trace = mg.serve_trace(arrivals, rate, seed=8)
     t_gen = time.perf_counter() - t_gen
     kw = dict(K=24, freeze_mode=1, freeze_need=5, freeze_disp_rel=1e-5)
     tol, max_iters = 1e-3, 100
-    n_large = sum(1 for x in trace if x[4] == "large")
+    n_large = sum(1 for x in trace if x[4].startswith("large"))
     print(f"[bench] serve: {len(trace)} requests over {trace[-1][0]:.1f} s ({n_large} large), "
           f"{sum(len(x[3]) for x in trace)} sites, gen {t_gen:.1f}s", file=sys.stderr, flush=True)
 
@@ -180,7 +210,7 @@ def bench_serve(args) -> int:
 
     # served: continuous batching
     with cvt.Server(hbm_budget_bytes=200 << 30, epoch_iters=10, min_check_iters=10, check_disp_rel=1e30,
-                    check_frozen_frac=0.0, ingest_threads=args.ingest_threads) as srv:
+                    check_frozen_frac=0.0, ingest_threads=ingest_threads) as srv:
         t0 = time.perf_counter()
         tickets = []
         for k, (ta, V, F, S, _tag) in enumerate(trace):
@@ -193,16 +223,16 @@ def bench_serve(args) -> int:
         srv.wait_idle(3000)
         info = srv.info()
         res = [srv.poll(t) for t in tickets]
-    if args.serve_log:
-        with open(args.serve_log, "w") as f:
+    if serve_log:
+        with open(serve_log, "w") as f:
             json.dump([{k: r[k] for k in ("nS", "iterations", "state", "t_arrive", "t_admit", "t_done")} for r in res], f)
     lat_s = np.array([r["t_done"] - r["t_arrive"] for r in res])
     span_s = max(r["t_done"] for r in res) - min(r["t_arrive"] for r in res)
     si_s = float(sum(r["nS"] * r["iterations"] for r in res))
     h_s = np.array([r["hausdorff"] for r in res])
-    conv_s = sum(r["state"] == "converged" for r in res)
+    conv = np.array([r["state"] == "converged" for r in res])
     print(f"[bench] served: span {span_s:.2f} s, p50 {np.percentile(lat_s, 50):.3f} s, "
-          f"p99 {np.percentile(lat_s, 99):.3f} s, converged {conv_s}/{len(res)}, info {info}",
+          f"p99 {np.percentile(lat_s, 99):.3f} s, converged {int(conv.sum())}/{len(res)}, info {info}",
           file=sys.stderr, flush=True)
 
     # plain path, FIFO, one job at a time
@@ -218,8 +248,8 @@ def bench_serve(args) -> int:
             while True:
                 job.step(10)
                 it += 10
-                conv, h = job.hausdorff_within(tol * diag)
-                if conv:
+                ok, h = job.hausdorff_within(tol * diag)
+                if ok:
                     break
                 if it >= max_iters:
                     h = job.hausdorff()[0]
@@ -237,20 +267,38 @@ def bench_serve(args) -> int:
     print(f"[bench] fifo: span {span_f:.2f} s, p50 {np.percentile(lat_f, 50):.3f} s, "
           f"p99 {np.percentile(lat_f, 99):.3f} s; same iteration counts {same_iters}, max |h diff| {h_diff:.3e}",
           file=sys.stderr, flush=True)
-    out = {
-        "metric": "served Lloyd site-iters/s on a Poisson request trace; p50/p99 time-to-convergence",
-        "value": si_s / span_s, "unit": "site-iters/s", "n_gpus": 1, "steps": int(sum(r["iterations"] for r in res)),
-        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic (seeded icospheres, tori, displaced spheres; Poisson arrivals)",
-        "config": {"workload": "serve_trace", "requests": len(trace), "rate_per_s": args.rate, "large": n_large,
-                   "K": 24, "freezing": True, "tol_hausdorff_rel": tol, "check_every": 10, "max_iters": max_iters,
-                   "hbm_budget_gb": 200},
-        "p50_s": float(np.percentile(lat_s, 50)), "p99_s": float(np.percentile(lat_s, 99)),
-        "converged": int(conv_s), "server": info,
-        "fifo": {"value": si_f / span_f, "p50_s": float(np.percentile(lat_f, 50)),
-                 "p99_s": float(np.percentile(lat_f, 99)), "span_s": span_f},
+
+    def pct(x, q):
+        return float(np.percentile(x, q)) if len(x) else None
+
+    return {
+        "requests": len(trace), "rate_per_s": rate, "large": n_large, "converged": int(conv.sum()),
+        "reached_max_iters": int(len(res) - conv.sum()), "K": 24, "freezing": True, "tol_hausdorff_rel": tol,
+        "check_every": 10, "max_iters": max_iters, "hbm_budget_gb": 200,
+        "site_iters_per_s": si_s / span_s, "span_s": span_s,
+        "p50_s": pct(lat_s, 50), "p99_s": pct(lat_s, 99),
+        "p50_converged_s": pct(lat_s[conv], 50), "p99_converged_s": pct(lat_s[conv], 99),
+        "fifo": {"site_iters_per_s": si_f / span_f, "span_s": span_f, "p50_s": pct(lat_f, 50),
+                 "p99_s": pct(lat_f, 99), "p50_converged_s": pct(lat_f[conv], 50),
+                 "p99_converged_s": pct(lat_f[conv], 99)},
         "served_over_fifo": (si_s / span_s) / (si_f / span_f),
         "same_final_hausdorff": bool(same_iters and h_diff == 0.0),
+        "server": {k: info[k] for k in ("epochs", "rebuilds", "hausdorff_checks", "peak_bytes", "busy_seconds",
+                                         "step_seconds", "check_seconds", "rebuild_seconds") if k in info},
+    }
+
+
+def bench_serve(args) -> int:
+    """Serving form, printed as its own JSON line."""
+    r = run_serve(args.arrivals, args.rate, args.ingest_threads, args.serve_log)
+    out = {
+        "metric": "served Lloyd site-iters/s on a Poisson request trace; p50/p99 time-to-convergence",
+        "value": r["site_iters_per_s"], "unit": "site-iters/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic (seeded icospheres, tori, displaced spheres; Poisson arrivals)",
+        "config": {"workload": "serve_trace", "requests": r["requests"], "rate_per_s": args.rate,
+                   "large": r["large"], "K": 24, "freezing": True},
+        "serve": r,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -317,6 +365,10 @@ def main(argv=None) -> int:
     ap.add_argument("--vert-cell", type=float, default=0.0, help="vertex grid cell scale (0 = library default)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (torch), to calibrate HBM counters")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="plain mode: skip the freezing-on, K = 8, batched and served measurements")
+    ap.add_argument("--extra-arrivals", type=int, default=400,
+                    help="plain mode: requests of the served-form measurement folded into the line")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -391,6 +443,42 @@ def main(argv=None) -> int:
           file=sys.stderr, flush=True)
     job.close()
 
+    extra = {}
+    if not args.no_extra:
+        # the same job with freezing on (the headline's second variant):
+        # algorithmic bytes from each timed iteration's active and frozen sites
+        fz = cvt.Job(V, F, S, cvt.default_params(K=K, freeze_mode=1, freeze_need=5, freeze_disp_rel=1e-5))
+        fz.step(args.warmup)
+        ms_f = fz.step_timed(args.steps)
+        st = fz.last_stats(args.steps + 1)
+        frozen_before = [s_["frozen_total"] for s_ in st[:-1]]
+        fz.close()
+        b_site = [(len(S) - f) * (29 + 8 * K) + f * (25 + 4 * K) for f in frozen_before]
+        b_iter = float(np.mean(b_site)) + 28 * len(V) + 24 * len(F) + 4
+        extra["freezing_on"] = {
+            "site_iters_per_s": len(S) * args.steps / (ms_f / 1e3), "ms_per_step": ms_f / args.steps,
+            "frozen_fraction_end": st[-1]["frozen_total"] / len(S), "algorithmic_bytes_per_iteration": b_iter,
+            "whole_iteration_hbm_frac": round(b_iter / (ms_f / args.steps * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)}
+        print(f"[bench] freezing on: {extra['freezing_on']}", file=sys.stderr, flush=True)
+        # K = 8 on the same mesh and sites: few neighbours leave many cells open
+        # and pile sites onto the same mesh points, which the FP64 pass takes
+        with cvt.Job(V, F, S, K=8) as j8:
+            j8.step(args.warmup)
+            ms_8 = j8.step_timed(args.steps)
+            st8 = j8.last_stats(1)[0]
+        extra["small_k"] = {"K": 8, "site_iters_per_s": len(S) * args.steps / (ms_8 / 1e3),
+                            "ms_per_step": ms_8 / args.steps, "over_headline_ms": (ms_8 / args.steps) / ms_per_step,
+                            "deferred_sites_last": st8["deferred_sites"]}
+        print(f"[bench] K = 8: {extra['small_k']}", file=sys.stderr, flush=True)
+        try:
+            extra["batch"] = run_batch(256, args.steps, args.warmup)
+        except Exception as e:
+            print(f"[bench] batch measurement failed: {e!r}", file=sys.stderr)
+        try:
+            extra["serve"] = run_serve(args.extra_arrivals, 200.0)
+        except Exception as e:
+            print(f"[bench] serve measurement failed: {e!r}", file=sys.stderr)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -421,6 +509,7 @@ def main(argv=None) -> int:
         "stage_ms": prof,
         "quality": quality,
     }
+    out.update(extra)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -697,7 +697,9 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
         }
         if (post_search) {
             if (want) {
-                const int flags = kCentDone | (cr.same ? kCentSame : 0) | (cr.cell_ok ? 0 : kCentFail) |
+                // without kCentDone: the projection kernel, which may still be
+                // running beside this pass, keeps skipping the site
+                const int flags = (cr.same ? kCentSame : 0) | (cr.cell_ok ? 0 : kCentFail) |
                                   (cr.kind << kCentKindShift);
                 a.cent[c.i] = make_float4(cr.c3.x, cr.c3.y, cr.c3.z, __int_as_float(flags));
                 a.vs_list[atomicAdd(&a.stats[kStatVsearch], 1)] = c.i;

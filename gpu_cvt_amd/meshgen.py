@@ -309,23 +309,30 @@ def serve_trace(n_arrivals: int = 2000, rate: float = 200.0, seed: int = 8, larg
     each request is drawn from the batched-form size distribution (mesh type
     uniform over icosphere / torus / displaced sphere, triangles log-uniform in
     nf_range, sites log-uniform in ns_range capped at triangles/2), or with
-    probability large_frac is a large job (the 1,997,568-triangle displaced
-    sphere with 1M sites).  Meshes are shared by requests whose size rounds to
-    the same tessellation and site sets by requests whose site count rounds to
-    the same of 16 log-spaced buckets per decade, so generation stays bounded.
+    probability large_frac is a large job: 1M sites on a 1,997,568-triangle
+    displaced sphere, cycling through four variants (two displacement seeds x
+    two site seeds).  Meshes are shared by requests whose size rounds to the
+    same tessellation and site sets by requests whose site count rounds to the
+    same of 16 log-spaced buckets per decade, so generation stays bounded.
     Returns a list of (t_arrive_s, V, F, S, tag), ordered by arrival."""
     rng = _rng(seed)
     t = 0.0
     mcache, scache = {}, {}
-    large = None
+    large, lmesh = {}, {}
+    n_large = 0
     out = []
     for k in range(n_arrivals):
         t += float(rng.exponential(1.0 / rate))
         if rng.uniform() < large_frac:
-            if large is None:
-                V, F, S, _K, _it, _fz = workload("dsphere_2m_1m")
-                large = (V, F, S)
-            out.append((t, *large, "large"))
+            v = n_large % 4
+            n_large += 1
+            if v not in large:
+                m = v % 2
+                if m not in lmesh:
+                    lmesh[m] = displaced_sphere(408, seed=9 + 2 * m, crease_seed=9 + 3 * m)
+                V, F = lmesh[m]
+                large[v] = (V, F, sample_surface(V, F, 1_000_000, seed=10 + v))
+            out.append((t, *large[v], f"large{v}"))
             continue
         typ = int(rng.integers(0, 3))
         nf = float(np.exp(rng.uniform(np.log(nf_range[0]), np.log(nf_range[1]))))

@@ -1,4 +1,5 @@
-"""ctypes wrapper over the FP64 CPU oracle (``oracle/liboracle_cvt.so``).
+"""ctypes wrapper over the FP64 CPU oracle (``oracle/liboracle_cvt.so``) and the
+FP32 CPU baseline (``oracle/libcpu32_cvt.so``).
 
 TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
 and ``bench.py``'s cpu_baseline leg, as the checker.  The product package
@@ -18,7 +19,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle_cvt.so")
+_CPU32_PATH = os.path.join(_HERE, "libcpu32_cvt.so")
 _lib = None
+_cpu32 = None
 
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
@@ -26,10 +29,10 @@ _up = np.ctypeslib.ndpointer(dtype=np.uint8, flags="C_CONTIGUOUS")
 
 
 def build(force: bool = False) -> str:
-    """Compile the oracle with its Makefile (g++ only)."""
-    if force or not os.path.exists(_LIB_PATH) or (
-        os.path.getmtime(_LIB_PATH) < os.path.getmtime(os.path.join(_HERE, "cvt_oracle.cpp"))
-    ):
+    """Compile the oracle and the FP32 CPU baseline with their Makefile (g++ only)."""
+    stale = any(not os.path.exists(lib) or os.path.getmtime(lib) < os.path.getmtime(os.path.join(_HERE, src))
+                for lib, src in ((_LIB_PATH, "cvt_oracle.cpp"), (_CPU32_PATH, "cvt_cpu32.cpp")))
+    if force or stale:
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
     return _LIB_PATH
 
@@ -268,3 +271,40 @@ def tiers(knn_, nn, N, scheme: int):
     L = np.zeros(n)
     lib().orc_tiers(n, K, knn_.reshape(-1), _i(nn), N.reshape(-1), len(N), int(scheme), tier, nv, L)
     return tier, nv, L
+
+
+def cpu32_lloyd(mesh: Mesh, S, K: int, R: float, iters: int, threads: int) -> np.ndarray:
+    """`iters` FP32 Lloyd iterations of the sites S on `threads` pinned host
+    threads (cvt_cpu32.cpp, the CPU speed baseline); returns the new sites."""
+    global _cpu32
+    if _cpu32 is None:
+        if not os.path.exists(_CPU32_PATH):
+            build()
+        L = C.CDLL(_CPU32_PATH)
+        fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.cpu32_lloyd.argtypes = [C.c_int, fp, C.c_int, _ip, fp, _ip, _ip, C.c_double, C.c_int, fp, C.c_int,
+                                  C.c_float, C.c_int, C.c_int]
+        _cpu32 = L
+    V = np.ascontiguousarray(mesh.V, np.float32)
+    N = np.ascontiguousarray(mesh.N, np.float32)
+    off, faces = mesh.vf
+    S32 = np.ascontiguousarray(np.asarray(S, np.float32)).copy()
+    e = mesh.V[mesh.F]
+    area = float(0.5 * np.linalg.norm(np.cross(e[:, 1] - e[:, 0], e[:, 2] - e[:, 0]), axis=1).sum())
+    rc = _cpu32.cpu32_lloyd(len(V), V.reshape(-1), len(mesh.F), mesh.F.reshape(-1), N.reshape(-1), off, faces, area,
+                            len(S32), S32.reshape(-1), int(K), float(R), int(iters), int(threads))
+    if rc != 0:
+        raise RuntimeError(f"cpu32_lloyd failed ({rc})")
+    return S32.astype(np.float64)
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or '' when unknown."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return ""

@@ -69,3 +69,63 @@ def test_batch_rejects_mixed_k():
         cvt.Batch([a, b])
     a.close()
     b.close()
+
+
+def test_batch_of_32_workload_jobs_matches_solo():
+    """32 jobs of the batched-form distribution (icospheres, tori, displaced
+    spheres; 5k-400k triangles, 1k-100k sites; K = 24, freezing on): the batch
+    gives every job the state it reaches alone."""
+    work = mg.batch_workload(32, seed=7)
+    kw = dict(K=24, freeze_mode=1, freeze_need=5, freeze_disp_rel=1e-5)
+    print(f"[batch32] sites {sum(len(w[2]) for w in work)}", flush=True)
+    ref = []
+    for V, F, S, _tag in work:
+        with cvt.Job(V, F, S, **kw) as j:
+            j.step(12)
+            ref.append(_state(j))
+    jobs = [cvt.Job(V, F, S, **kw) for V, F, S, _tag in work]
+    try:
+        with cvt.Batch(jobs) as b:
+            b.step(12)
+        for k, (j, r) in enumerate(zip(jobs, ref)):
+            got = _state(j)
+            for a, e in zip(got, r):
+                assert np.array_equal(a, e), f"job {k} ({work[k][3]}) differs from its solo run"
+    finally:
+        for j in jobs:
+            j.close()
+
+
+def test_fp64_pass_runs_are_deterministic():
+    """Jobs whose FP64 pass is busy (K = 8: open cells, far centroids, piles of
+    coincident sites; it runs beside the projection kernel): two solo runs and
+    a batched run reach bit-identical states."""
+    V, F = mg.icosphere(4)
+    ins = [(V, F, mg.sample_surface(V, F, 1500, seed=28)), (V, F, mg.sample_surface(V, F, 40000, seed=29))]
+    V6, F6 = mg.icosphere(6)
+    ins.append((V6, F6, mg.sample_surface(V6, F6, 40960, seed=30)))
+
+    def run():
+        out = []
+        for V_, F_, S_ in ins:
+            with cvt.Job(V_, F_, S_, K=8, freeze_mode=1, freeze_disp_rel=1e-4) as j:
+                st = j.step(15, stats=True)
+                out.append((_state(j), sum(s["deferred_sites"] for s in st)))
+        return out
+
+    a, b = run(), run()
+    print(f"[determinism] deferred sites over 15 iterations: {[x[1] for x in a]}", flush=True)
+    assert any(x[1] > 0 for x in a)
+    for k, (x, y) in enumerate(zip(a, b)):
+        for p, q in zip(x[0], y[0]):
+            assert np.array_equal(p, q), f"job {k}: two solo runs differ"
+    jobs = [cvt.Job(V_, F_, S_, K=8, freeze_mode=1, freeze_disp_rel=1e-4) for V_, F_, S_ in ins]
+    try:
+        with cvt.Batch(jobs) as bt:
+            bt.step(15)
+        for k, (j, x) in enumerate(zip(jobs, a)):
+            for p, q in zip(_state(j), x[0]):
+                assert np.array_equal(p, q), f"job {k}: batched run differs from solo"
+    finally:
+        for j in jobs:
+            j.close()
