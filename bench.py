@@ -283,7 +283,8 @@ def run_serve(arrivals: int, rate: float, ingest_threads: int = 4, serve_log: st
                  "p99_converged_s": pct(lat_f[conv], 99)},
         "served_over_fifo": (si_s / span_s) / (si_f / span_f),
         "same_final_hausdorff": bool(same_iters and h_diff == 0.0),
-        "server": {k: info[k] for k in ("epochs", "rebuilds", "hausdorff_checks", "peak_bytes", "busy_seconds",
+        "server": {k: info[k] for k in ("epochs", "rebuilds", "rebuilds_grow", "compactions", "extent_updates",
+                                         "hausdorff_checks", "peak_bytes", "busy_seconds",
                                          "step_seconds", "check_seconds", "rebuild_seconds") if k in info},
     }
 

@@ -1543,6 +1543,408 @@ void cvt_batch_destroy(cvt_batch* b) {
     delete b;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------- slot batches
+namespace cvt {
+
+// First-fit allocator of runs of entries in a block table, with a host mirror
+// of the table (empty entries {-1, 0}).
+struct TableRuns {
+    std::vector<BlockRef> mirror;
+    std::map<int, int> free_;  // start -> length
+    int used = 0;
+    void reset(int cap) {
+        mirror.assign(cap, BlockRef{-1, 0});
+        free_.clear();
+        if (cap > 0) free_[0] = cap;
+        used = 0;
+    }
+    int alloc(int len) {
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < len) continue;
+            const int st = it->first, rest = it->second - len;
+            free_.erase(it);
+            if (rest) free_[st + len] = rest;
+            used += len;
+            return st;
+        }
+        return -1;
+    }
+    void release(int st, int len) {
+        for (int k = st; k < st + len; ++k) mirror[k] = BlockRef{-1, 0};
+        used -= len;
+        auto next = free_.lower_bound(st);
+        if (next != free_.end() && next->first == st + len) {
+            len += next->second;
+            next = free_.erase(next);
+        }
+        if (next != free_.begin()) {
+            auto prev = std::prev(next);
+            if (prev->first + prev->second == st) {
+                prev->second += len;
+                return;
+            }
+        }
+        free_[st] = len;
+    }
+    // One past the last entry in use.
+    int extent() const {
+        const int cap = (int)mirror.size();
+        if (free_.empty()) return cap;
+        const auto last = std::prev(free_.end());
+        return last->first + last->second == cap ? last->first : cap;
+    }
+};
+
+// Tables: site, counter (cell), scan tile, FP64 pass, and the job slots.
+enum { kTabSite, kTabCell, kTabTile, kTabPrec, kTabSlot, kNumTab };
+
+struct SlotBatch {
+    int K = 0, kproj = 0;
+    hipStream_t stream = nullptr;
+    SideStream side;
+    DevArena* arena = current_arena();
+    std::vector<cvt_job*> slots;  // nullptr: empty
+    struct Place {
+        int slot, run[4], len[4];
+    };
+    std::map<cvt_job*, Place> place;
+    TableRuns tab[4];
+    // device copies (capacity-sized)
+    JobDesc* d_descs = nullptr;
+    BlockRef* d_tab[4] = {nullptr, nullptr, nullptr, nullptr};
+    std::vector<std::pair<void*, size_t>> dev;
+    // the captured graph (kept: its kernel nodes are edited through it)
+    hipGraph_t tmpl = nullptr;
+    hipGraphExec_t graph = nullptr;
+    // kernel nodes whose grid spans a table: grid = extent(table) * mult
+    struct KNode {
+        hipGraphNode_t node;
+        hipKernelNodeParams p;
+        int table, mult;
+    };
+    std::vector<KNode> knodes;
+    int launched[kNumTab] = {0, 0, 0, 0, 0};  // extents the graph's grids cover now
+    SlotBatchCounters n{};
+
+    int cap(int t) const { return t == kTabSlot ? (int)slots.size() : (int)tab[t].mirror.size(); }
+    void free_dev() {
+        for (auto& p : dev) dev_free(arena, p.first, p.second);
+        dev.clear();
+    }
+    void drop_graph() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        if (tmpl) (void)hipGraphDestroy(tmpl);
+        graph = nullptr;
+        tmpl = nullptr;
+        knodes.clear();
+    }
+    ~SlotBatch() {
+        drop_graph();
+        free_dev();
+        if (stream) (void)hipStreamDestroy(stream);
+        side_destroy(side);
+    }
+    void* dalloc(size_t bytes) {
+        void* p = nullptr;
+        bytes = std::max<size_t>(bytes, 16);
+        if (dev_alloc(arena, &p, bytes)) return nullptr;
+        dev.push_back({p, bytes});
+        return p;
+    }
+};
+
+// Table capacities.  A captured kernel node is matched to the table its grid
+// spans by the grid's size, so the capacities are kept pairwise distinct by
+// their residues: site = 0 (mod 16) (the Lloyd launches also need a multiple
+// of 8), FP64 pass = 3 (mod 4) (its wave kernels launch kWaveBlocks = 4 times
+// that, = 12 (mod 16)), counters = 1 (mod 4), scan tiles = 2 (mod 4), slots
+// = 4 (mod 16).  The per-slot kernels of 64-thread blocks launch at most
+// kSlotMaxJobs / 64 = 256 blocks, below every table's smallest capacity.
+static_assert(kWaveBlocks == 4, "capacity residues assume 4 wave blocks per entry");
+constexpr int kSlotMaxJobs = 16384;
+constexpr int kCapFloor[kNumTab] = {512, 260, 260, 260, 4};
+constexpr int kCapMod[kNumTab] = {16, 4, 4, 4, 16};
+constexpr int kCapRes[kNumTab] = {0, 1, 2, 3, 4};
+static int cap_fit(int t, long want) {
+    long c = std::max<long>(want, kCapFloor[t]);
+    c += ((kCapRes[t] - c % kCapMod[t]) % kCapMod[t] + kCapMod[t]) % kCapMod[t];
+    return (int)c;
+}
+// First capacities for jobs0 jobs and site_blocks0 blocks of 256 sites (0:
+// 1028 jobs and 65536 blocks, 16.7M sites); the other tables in the
+// proportions of a job whose site grid has about one cell per site.
+static void caps_initial(int jobs0, int site_blocks0, int cap[kNumTab]) {
+    const long sb = site_blocks0 > 0 ? site_blocks0 : 65536;
+    cap[kTabSite] = cap_fit(kTabSite, sb);
+    cap[kTabCell] = cap_fit(kTabCell, sb / 4);
+    cap[kTabTile] = cap_fit(kTabTile, sb / 16);
+    cap[kTabPrec] = cap_fit(kTabPrec, std::min<long>(sb, kPreciseMaxBlocks * 64L));
+    cap[kTabSlot] = cap_fit(kTabSlot, std::min(jobs0 > 0 ? jobs0 : 1028, kSlotMaxJobs - 15));
+}
+
+// Block counts of a job in the four tables (as add_job_blocks lays them out).
+static void job_block_counts(cvt_job* j, int len[4]) {
+    const int ncell1 = j->sgp.ncell + 1;
+    len[kTabSite] = (j->nS + 255) / 256;
+    len[kTabCell] = (ncell1 + 1023) / 1024;
+    len[kTabTile] = (ncell1 + scan_tile() - 1) / scan_tile();
+    len[kTabPrec] = precise_blocks(j->nS);
+}
+
+// Launch extent of a table: the entries up to the last one in use (the site
+// table's rounded up to a multiple of 8 for the XCD mapping).
+static int table_extent(const SlotBatch* b, int t) {
+    if (t == kTabSlot) {
+        int hi = 0;
+        for (int k = 0; k < (int)b->slots.size(); ++k)
+            if (b->slots[k]) hi = k + 1;
+        return std::max(hi, 1);
+    }
+    const int e = std::max(b->tab[t].extent(), 1);
+    return t == kTabSite ? std::min((e + 7) & ~7, b->cap(t)) : e;
+}
+
+// Points every captured kernel node that spans a table at the table's
+// current extent (the graph then launches no block past the last job).
+static int slot_batch_fit_grids(SlotBatch* b) {
+    int ext[kNumTab];
+    bool change = false;
+    for (int t = 0; t < kNumTab; ++t) {
+        ext[t] = table_extent(b, t);
+        change |= ext[t] != b->launched[t];
+    }
+    if (!change) return 0;
+    for (SlotBatch::KNode& kn : b->knodes) {
+        if (ext[kn.table] == b->launched[kn.table]) continue;
+        hipKernelNodeParams p = kn.p;
+        p.gridDim.x = (unsigned)(ext[kn.table] * kn.mult);
+        HIPCHK(hipGraphExecKernelNodeSetParams(b->graph, kn.node, &p));
+    }
+    for (int t = 0; t < kNumTab; ++t) b->launched[t] = ext[t];
+    ++b->n.extent_updates;
+    return 0;
+}
+
+// Uploads the descriptors and the four tables whole.
+static int slot_batch_upload(SlotBatch* b) {
+    const int nslot = b->cap(kTabSlot);
+    std::vector<JobDesc> descs(nslot);
+    std::memset((void*)descs.data(), 0, sizeof(JobDesc) * nslot);
+    for (int k = 0; k < nslot; ++k)
+        if (b->slots[k]) descs[k] = make_desc(b->slots[k]);
+    HIPCHK(hipMemcpyAsync(b->d_descs, descs.data(), sizeof(JobDesc) * nslot, hipMemcpyHostToDevice, b->stream));
+    for (int t = 0; t < 4; ++t)
+        HIPCHK(hipMemcpyAsync(b->d_tab[t], b->tab[t].mirror.data(), sizeof(BlockRef) * b->cap(t),
+                              hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
+// (Re)allocates the device tables at the mirrors' capacities, uploads them,
+// captures the iteration graph at full capacity, records which kernel nodes
+// span which table, and fits their grids to the extents in use.
+static int slot_batch_capture(SlotBatch* b, bool growth) {
+    (void)hipStreamSynchronize(b->stream);
+    b->drop_graph();
+    b->free_dev();
+    b->d_descs = (JobDesc*)b->dalloc(sizeof(JobDesc) * b->cap(kTabSlot));
+    for (int t = 0; t < 4; ++t) b->d_tab[t] = (BlockRef*)b->dalloc(sizeof(BlockRef) * b->cap(t));
+    if (!b->d_descs || !b->d_tab[0] || !b->d_tab[1] || !b->d_tab[2] || !b->d_tab[3])
+        return fail(CVT_ENOMEM, "slot batch tables");
+    int rc = slot_batch_upload(b);
+    if (rc) return rc;
+    const BatchTables bt{b->d_descs,          b->cap(kTabSlot), b->d_tab[kTabSite], b->cap(kTabSite),
+                         b->d_tab[kTabCell],  b->cap(kTabCell), b->d_tab[kTabTile], b->cap(kTabTile),
+                         b->d_tab[kTabPrec],  b->cap(kTabPrec)};
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    launch_batch_iteration(bt, b->K, b->kproj, b->stream, &b->side);
+    HIPCHK(hipStreamEndCapture(b->stream, &b->tmpl));
+    size_t nn = 0;
+    HIPCHK(hipGraphGetNodes(b->tmpl, nullptr, &nn));
+    std::vector<hipGraphNode_t> nodes(nn);
+    HIPCHK(hipGraphGetNodes(b->tmpl, nodes.data(), &nn));
+    for (hipGraphNode_t nd : nodes) {
+        hipGraphNodeType ty;
+        HIPCHK(hipGraphNodeGetType(nd, &ty));
+        if (ty != hipGraphNodeTypeKernel) continue;
+        SlotBatch::KNode kn{nd, {}, -1, 1};
+        HIPCHK(hipGraphKernelNodeGetParams(nd, &kn.p));
+        const int g = (int)kn.p.gridDim.x;
+        for (int t = 0; t < kNumTab; ++t) {
+            if (g == b->cap(t)) kn.table = t;
+            if (t == kTabPrec && g == kWaveBlocks * b->cap(t)) kn.table = t, kn.mult = kWaveBlocks;
+        }
+        if (kn.table >= 0) b->knodes.push_back(kn);
+    }
+    const hipError_t e = hipGraphInstantiate(&b->graph, b->tmpl, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        b->drop_graph();
+        return fail(CVT_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    for (int t = 0; t < kNumTab; ++t) b->launched[t] = b->cap(t);
+    ++b->n.captures;
+    ++(growth ? b->n.grow : b->n.first);
+    return slot_batch_fit_grids(b);
+}
+
+// Lays every member out again, in slot order, from empty tables of the given
+// capacities (slots first, then each job's runs at the low end of each table).
+static void slot_batch_layout(SlotBatch* b, const int cap[kNumTab]) {
+    std::vector<cvt_job*> members;
+    for (cvt_job* j : b->slots)
+        if (j) members.push_back(j);
+    b->slots.assign(std::max<int>(cap[kTabSlot], (int)members.size()), nullptr);
+    for (int t = 0; t < 4; ++t) b->tab[t].reset(cap[t]);
+    b->place.clear();
+    for (size_t k = 0; k < members.size(); ++k) {
+        cvt_job* j = members[k];
+        SlotBatch::Place pl{(int)k, {}, {}};
+        job_block_counts(j, pl.len);
+        b->slots[k] = j;
+        for (int t = 0; t < 4; ++t) {
+            pl.run[t] = b->tab[t].alloc(pl.len[t]);
+            for (int q = 0; q < pl.len[t]; ++q) b->tab[t].mirror[pl.run[t] + q] = BlockRef{(int)k, q};
+        }
+        b->place[j] = pl;
+    }
+}
+
+int slot_batch_create(int K, int kproj, int jobs0, int site_blocks0, SlotBatch** out) {
+    *out = nullptr;
+    if (!lloyd_supported(K, kproj)) return fail(CVT_EINVAL, "unsupported K=%d / kproj=%d", K, kproj);
+    SlotBatch* b = new SlotBatch();
+    b->K = K;
+    b->kproj = kproj;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess || side_create(b->side)) {
+        delete b;
+        return fail(CVT_EHIP, "hipStreamCreate failed");
+    }
+    int cap[kNumTab];
+    caps_initial(jobs0, site_blocks0, cap);
+    slot_batch_layout(b, cap);
+    *out = b;
+    return 0;
+}
+
+// Lays the members out again and uploads the tables whole, within the
+// current capacities (no capture; the grids are fitted to the new extents).
+static int slot_batch_compact(SlotBatch* b) {
+    int cap[kNumTab];
+    for (int t = 0; t < kNumTab; ++t) cap[t] = b->cap(t);
+    slot_batch_layout(b, cap);
+    ++b->n.compactions;
+    if (!b->graph) return 0;
+    int rc = slot_batch_upload(b);
+    return rc ? rc : slot_batch_fit_grids(b);
+}
+
+int slot_batch_add(SlotBatch* b, cvt_job* j) {
+    if (!j || j->p.K != b->K || j->p.kproj != b->kproj) return fail(CVT_EINVAL, "job does not fit the slot batch");
+    if (b->place.count(j)) return fail(CVT_EINVAL, "job already in the slot batch");
+    HIPCHK(hipStreamSynchronize(j->stream));
+    int len[4];
+    job_block_counts(j, len);
+    // room to spare in every table (else grow), contiguous (else compact)
+    long need[kNumTab] = {0, 0, 0, 0, (long)b->place.size() + 1};
+    for (int t = 0; t < 4; ++t) need[t] = (long)b->tab[t].used + len[t];
+    need[kTabSite] += 8;  // the site extent rounds up to a multiple of 8
+    bool grow = false;
+    for (int t = 0; t < kNumTab; ++t) grow |= need[t] > b->cap(t);
+    if (grow) {
+        // every table grows to four times what it would hold (tables grow
+        // together, so a rising load captures about once per factor of four)
+        if (need[kTabSlot] > kSlotMaxJobs) return fail(CVT_ENOMEM, "slot batch holds %d jobs at most", kSlotMaxJobs);
+        int cap[kNumTab];
+        for (int t = 0; t < kNumTab; ++t) cap[t] = std::max(b->cap(t), cap_fit(t, 4 * need[t]));
+        cap[kTabSlot] = std::min(cap[kTabSlot], cap_fit(kTabSlot, kSlotMaxJobs - 15));
+        b->slots.push_back(j);  // joins the layout
+        slot_batch_layout(b, cap);
+        return slot_batch_capture(b, true);
+    }
+    int slot = 0;
+    while (b->slots[slot]) ++slot;
+    SlotBatch::Place pl{slot, {}, {}};
+    bool fits = true;
+    for (int t = 0; t < 4; ++t) {
+        pl.len[t] = len[t];
+        pl.run[t] = fits ? b->tab[t].alloc(len[t]) : -1;
+        fits &= pl.run[t] >= 0;
+    }
+    if (!fits) {
+        // no contiguous room (there is room in total): compact, the new job
+        // in its free slot
+        for (int t = 0; t < 4; ++t)
+            if (pl.run[t] >= 0) b->tab[t].release(pl.run[t], len[t]);
+        b->slots[slot] = j;
+        int rc = slot_batch_compact(b);
+        return rc ? rc : (b->graph ? 0 : slot_batch_capture(b, false));
+    }
+    b->slots[slot] = j;
+    b->place[j] = pl;
+    for (int t = 0; t < 4; ++t)
+        for (int q = 0; q < len[t]; ++q) b->tab[t].mirror[pl.run[t] + q] = BlockRef{slot, q};
+    if (!b->graph) return slot_batch_capture(b, false);
+    // in place: the descriptor and the new block references, then the grids
+    const JobDesc d = make_desc(j);
+    HIPCHK(hipMemcpyAsync(b->d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream));
+    for (int t = 0; t < 4; ++t)
+        HIPCHK(hipMemcpyAsync(b->d_tab[t] + pl.run[t], &b->tab[t].mirror[pl.run[t]], sizeof(BlockRef) * len[t],
+                              hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));  // the host sources must outlive the copies
+    return slot_batch_fit_grids(b);
+}
+
+int slot_batch_remove(SlotBatch* b, cvt_job* j) {
+    auto it = b->place.find(j);
+    if (it == b->place.end()) return fail(CVT_EINVAL, "job not in the slot batch");
+    const SlotBatch::Place pl = it->second;
+    b->place.erase(it);
+    b->slots[pl.slot] = nullptr;
+    for (int t = 0; t < 4; ++t) b->tab[t].release(pl.run[t], pl.len[t]);
+    if (!b->graph) return 0;
+    // holes below the extent are launched as empty blocks: compact when the
+    // site table's extent is over twice what it holds
+    if (table_extent(b, kTabSite) > 2 * b->tab[kTabSite].used + 64) return slot_batch_compact(b);
+    JobDesc d;
+    std::memset((void*)&d, 0, sizeof d);
+    HIPCHK(hipMemcpyAsync(b->d_descs + pl.slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream));
+    for (int t = 0; t < 4; ++t)
+        HIPCHK(hipMemcpyAsync(b->d_tab[t] + pl.run[t], &b->tab[t].mirror[pl.run[t]], sizeof(BlockRef) * pl.len[t],
+                              hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return slot_batch_fit_grids(b);
+}
+
+int slot_batch_step(SlotBatch* b, int iters) {
+    if (iters <= 0 || b->place.empty()) return 0;
+    for (auto& kv : b->place) HIPCHK(hipStreamSynchronize(kv.first->stream));
+    if (!b->graph) {
+        int rc = slot_batch_capture(b, false);
+        if (rc) return rc;
+    }
+    for (int it = 0; it < iters; ++it) HIPCHK(hipGraphLaunch(b->graph, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    for (auto& kv : b->place) {
+        kv.first->cur ^= (iters & 1);
+        kv.first->iters_done += iters;
+    }
+    return 0;
+}
+
+SlotBatchCounters slot_batch_counters(const SlotBatch* b) { return b->n; }
+int slot_batch_size(const SlotBatch* b) { return (int)b->place.size(); }
+void slot_batch_destroy(SlotBatch* b) {
+    if (!b) return;
+    (void)hipStreamSynchronize(b->stream);
+    delete b;
+}
+
+}  // namespace cvt
+
+extern "C" {
+
 static int ensure_bvh(cvt_job* j) {
     if (!j->bvh) {
         // the job's face table is already on the device; its vertex bbox

@@ -109,4 +109,34 @@ __attribute__((visibility("hidden"))) int check_fast_enqueue(cvt_job* j, double 
                                                              unsigned long long** keys_out = nullptr,
                                                              float4** site_n_out = nullptr);
 
+// A batch of jobs for a serving loop, over job slots and block tables of
+// fixed capacity: jobs join and leave by editing the device slot and block
+// tables in place (a joining job's descriptor and block references are
+// written, a leaving job's reset to empty), and the captured iteration graph
+// is kept, with the grids of its table-spanning kernel nodes set to the
+// extent of the tables in use (hipGraphExecKernelNodeSetParams), so a steady
+// stream of jobs runs without capturing the graph again and without
+// launching blocks past the last job.  The graph is captured again only when
+// a table outgrows its capacity (all then grow to four times what they hold);
+// a joining job that finds no contiguous room, or a site table whose extent
+// is over twice what it holds, is compacted in place.  Jobs must share K and
+// kproj, must not run elsewhere while in the batch, and are added and removed
+// only while no step of the batch is in flight.
+struct SlotBatch;
+struct SlotBatchCounters {
+    long captures, first, grow;  // graph captures: total, first, for growth
+    long compactions;            // tables laid out again in place
+    long extent_updates;         // grid edits of the instantiated graph
+};
+// First capacities: jobs0 jobs, site_blocks0 blocks of 256 sites (0: defaults).
+__attribute__((visibility("hidden"))) int slot_batch_create(int K, int kproj, int jobs0, int site_blocks0,
+                                                            SlotBatch** out);
+__attribute__((visibility("hidden"))) int slot_batch_add(SlotBatch* b, cvt_job* job);
+__attribute__((visibility("hidden"))) int slot_batch_remove(SlotBatch* b, cvt_job* job);
+// Runs iters iterations of every job in the batch and waits for them.
+__attribute__((visibility("hidden"))) int slot_batch_step(SlotBatch* b, int iters);
+__attribute__((visibility("hidden"))) SlotBatchCounters slot_batch_counters(const SlotBatch* b);
+__attribute__((visibility("hidden"))) int slot_batch_size(const SlotBatch* b);
+__attribute__((visibility("hidden"))) void slot_batch_destroy(SlotBatch* b);
+
 }  // namespace cvt

@@ -33,6 +33,9 @@ constexpr int kStatStride = 32;
 // (about two per CU), striding over the job's deferred list, so that a list
 // of any length is spread over the whole device.
 constexpr int kPreciseMaxBlocks = 512;
+// The wave-per-site kernels of the FP64 pass run kWaveBlocks blocks per entry
+// of its block table (16 waves), so that a long list is spread over more waves.
+constexpr int kWaveBlocks = 4;
 __host__ __device__ inline int precise_blocks(int nS) {
     const int b = (nS + 255) / 256;
     return b < kPreciseMaxBlocks ? b : kPreciseMaxBlocks;

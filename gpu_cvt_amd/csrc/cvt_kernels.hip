@@ -263,10 +263,13 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
 }
 
 // ------------------------------------------------------ batched grid rebuild
+// Per-job kernels skip empty slots (a zeroed descriptor, nS = 0) of a
+// batch with a fixed launch shape.
 __global__ void kb_begin(const JobDesc* __restrict__ descs, int njobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= njobs) return;
     const StepArgs& a = descs[j].a;
+    if (a.nS == 0) return;
     for (int k = 0; k < kStatCopies * kStatStride; ++k) a.stats[k] = 0;
     a.dmax01[*a.parity] = 0.f;
 }
@@ -344,6 +347,7 @@ __global__ __launch_bounds__(256) void kb_scan_reduce(const JobDesc* __restrict_
 
 __global__ __launch_bounds__(256) void kb_scan_bsum(const JobDesc* __restrict__ descs) {
     const JobDesc& d = descs[blockIdx.x];
+    if (d.ncell1 == 0) return;  // empty slot
     const int nb = (d.ncell1 + kScanTile - 1) / kScanTile;
     int carry = 0;
     for (int c = 0; c < nb; c += 256) {
@@ -422,6 +426,7 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= njobs) return;
     const StepArgs& a = descs[j].a;
+    if (a.nS == 0) return;  // empty slot
     const int c = *a.hist_count;
     const int slot = c < kHistCap ? c : kHistCap - 1;
     // fold the per-block stat copies into copy 0

@@ -70,9 +70,6 @@ void kb_cell(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
 // as carrying a row.  Waves of the job's precise_blocks(nS) blocks stride over
 // the list.
 constexpr int kWaveCapl = 8;  // LDS entries per lane of the wave-wide search
-// The wave-per-site kernels run kWaveBlocks blocks per entry of the FP64
-// pass's block table (16 waves), so that a long list is spread over more waves.
-constexpr int kWaveBlocks = 4;
 
 template <int K>
 __global__ __launch_bounds__(256) void kb_precise_knn(const JobDesc* __restrict__ descs,

@@ -9,7 +9,10 @@
 // Hausdorff distance of the remeshed surface (cvt_job_hausdorff), the check
 // the reference runs every DUMP_STRIDE = 10 iterations (main.cu:395,1034) --
 // retires finished jobs (freeing their memory) and admits jobs that became
-// ready, rebuilding the batch only when its membership changed.
+// ready.  A group's batch has a fixed launch shape over job slots
+// (cvt::SlotBatch): joining and retiring jobs edit its device slot table in
+// place, and its graph is captured again only when the tables outgrow their
+// capacity or shrink to a quarter of it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,6 +50,7 @@ struct Request {
     int next_check = 0;
     double checked_h = -1;  // this epoch's Hausdorff distance (-1: not computed)
     int checked_in = 0;     // this epoch's check found every probe within tolerance
+    bool in_batch = false;  // stepped by its group's slot batch (else on its own)
     std::vector<double> S_out;
 };
 
@@ -72,9 +76,11 @@ struct cvt_server {
     std::vector<std::thread> ingest_th;
     std::thread sched_th;
     std::vector<cvt::TempPool> check_pools;  // device temporaries, one set per check worker
+    unsigned long long pool_bytes = 0;       // their size, counted in used
     // counters
     double site_iters = 0, busy_s = 0, rebuild_s = 0, step_s = 0, check_s = 0;
     long epochs = 0, rebuilds = 0, checks = 0;
+    cvt::SlotBatchCounters batch_n{};  // summed over the groups' batches
     int failures = 0;
 
     void ingest_loop();
@@ -118,7 +124,7 @@ void cvt_server::ingest_loop() {
                 Request* f = pending.front();
                 const unsigned long long est = estimate_bytes((int)f->V.size() / 3, (int)f->F.size() / 3,
                                                               (int)f->S.size() / 3, f->p.K);
-                return used + est <= cfg.hbm_budget_bytes || (active.empty() && ready.empty() && used == 0);
+                return used + est <= cfg.hbm_budget_bytes || (active.empty() && ready.empty() && used == pool_bytes);
             });
             if (stop) return;
             r = pending.front();
@@ -163,14 +169,26 @@ void cvt_server::ingest_loop() {
 
 void cvt_server::sched_loop() {
     cvt::current_arena() = &arena;
-    // One batch per (K, kproj) group; a group's batch is rebuilt only when its
-    // membership changes (a job joined or retired).
+    // One slot batch per (K, kproj) group, edited in place as jobs join and retire.
     struct Group {
-        std::vector<Request*> members;
-        cvt_batch* batch = nullptr;
-        bool dirty = true;
+        std::vector<Request*> members;  // in joining order
+        cvt::SlotBatch* batch = nullptr;
     };
     std::map<std::pair<int, int>, Group> groups;
+    // (a group's batch lives until the server stops)
+    auto counters_now = [&] {
+        cvt::SlotBatchCounters c{};
+        for (auto& kv : groups)
+            if (kv.second.batch) {
+                const cvt::SlotBatchCounters g = cvt::slot_batch_counters(kv.second.batch);
+                c.captures += g.captures;
+                c.first += g.first;
+                c.grow += g.grow;
+                c.compactions += g.compactions;
+                c.extent_updates += g.extent_updates;
+            }
+        return c;
+    };
     for (;;) {
         std::vector<Request*> joined;
         {
@@ -180,39 +198,28 @@ void cvt_server::sched_loop() {
             joined.swap(ready);
             active.insert(active.end(), joined.begin(), joined.end());
         }
+        const auto e0 = Clock::now();
+        double t_rebuild = 0, t_step = 0, t_check = 0;
         for (Request* r : joined) {
             Group& g = groups[{r->p.K, r->p.kproj}];
             g.members.push_back(r);
-            g.dirty = true;
+            const auto r0 = Clock::now();
+            if (!g.batch && cvt::slot_batch_create(r->p.K, r->p.kproj, cfg.batch_jobs0, cfg.batch_site_blocks0, &g.batch) != CVT_OK) g.batch = nullptr;
+            r->in_batch = g.batch && cvt::slot_batch_add(g.batch, r->job) == CVT_OK;
+            t_rebuild += secs(r0);
         }
-        const auto e0 = Clock::now();
         double si = 0;
-        long nrebuild = 0, ncheck = 0;
-        double t_rebuild = 0, t_step = 0, t_check = 0;
+        long ncheck = 0;
         std::vector<Request*> done;
         std::vector<int> status;
         for (auto& kv : groups) {
             Group& g = kv.second;
             if (g.members.empty()) continue;
-            std::vector<cvt_job*> jobs;
-            for (Request* r : g.members) jobs.push_back(r->job);
-            const auto r0 = Clock::now();
-            if (g.dirty) {
-                if (g.batch) cvt_batch_destroy(g.batch);
-                g.batch = nullptr;
-                if (cvt_batch_create(jobs.data(), (int)jobs.size(), &g.batch) != CVT_OK) g.batch = nullptr;
-                g.dirty = false;
-                ++nrebuild;
-            }
             const auto r1 = Clock::now();
-            if (g.batch) {
-                cvt_batch_step(g.batch, cfg.epoch_iters);
-            } else {
-                for (cvt_job* j : jobs) cvt_job_step(j, cfg.epoch_iters, nullptr);
-            }
-            const auto r2 = Clock::now();
-            t_rebuild += std::chrono::duration<double>(r1 - r0).count();
-            t_step += std::chrono::duration<double>(r2 - r1).count();
+            if (g.batch) cvt::slot_batch_step(g.batch, cfg.epoch_iters);
+            for (Request* r : g.members)
+                if (!r->in_batch) cvt_job_step(r->job, cfg.epoch_iters, nullptr);
+            t_step += secs(r1);
             // convergence at the epoch boundary: gate each job, then run the
             // Hausdorff checks of the gated jobs in parallel host threads (each
             // on its job's own stream, with its own device temporaries)
@@ -275,35 +282,42 @@ void cvt_server::sched_loop() {
                 if (st_done) {
                     done.push_back(r);
                     status.push_back(st_done);
-                    g.dirty = true;
+                    // leaves the slot table before finish() destroys the job
+                    const auto r2 = Clock::now();
+                    if (r->in_batch) cvt::slot_batch_remove(g.batch, r->job);
+                    r->in_batch = false;
+                    t_rebuild += secs(r2);
                 } else {
                     keep.push_back(r);
                 }
             }
-            if (g.dirty && g.batch) {
-                // the retiring jobs are destroyed below: drop the batch that names them first
-                cvt_batch_destroy(g.batch);
-                g.batch = nullptr;
-            }
             g.members.swap(keep);
         }
+        unsigned long long pb = 0;
+        for (const cvt::TempPool& tp : check_pools) pb += tp.bytes();
+        const cvt::SlotBatchCounters bn = counters_now();
         const double dt = secs(e0);
         {
             std::lock_guard<std::mutex> lk(mu);
             site_iters += si;
             busy_s += dt;
             ++epochs;
-            rebuilds += nrebuild;
+            rebuilds = bn.captures;
+            batch_n = bn;
             checks += ncheck;
             rebuild_s += t_rebuild;
             step_s += t_step;
             check_s += t_check;
+            // the check workers' temporaries count against the budget as well
+            used = used - pool_bytes + pb;
+            pool_bytes = pb;
+            peak = std::max(peak, used);
             for (Request* r : done) active.erase(std::find(active.begin(), active.end(), r));
         }
         for (size_t k = 0; k < done.size(); ++k) finish(done[k], status[k]);
     }
     for (auto& kv : groups)
-        if (kv.second.batch) cvt_batch_destroy(kv.second.batch);
+        if (kv.second.batch) cvt::slot_batch_destroy(kv.second.batch);
 }
 
 extern "C" {
@@ -383,7 +397,7 @@ int cvt_server_wait_idle(cvt_server* s, double timeout_s) {
     if (!s) return fail(CVT_EINVAL, "null argument");
     std::unique_lock<std::mutex> lk(s->mu);
     const bool ok = s->cv.wait_for(lk, std::chrono::duration<double>(timeout_s), [&] {
-        return s->pending.empty() && s->ready.empty() && s->active.empty() && s->used == 0;
+        return s->pending.empty() && s->ready.empty() && s->active.empty() && s->used == s->pool_bytes;
     });
     return ok ? CVT_OK : fail(CVT_ESTATE, "timed out waiting for the server to drain");
 }
@@ -398,6 +412,9 @@ int cvt_server_info_get(cvt_server* s, cvt_server_info* out) {
     out->used_bytes = s->used;
     out->epochs = s->epochs;
     out->rebuilds = s->rebuilds;
+    out->rebuilds_grow = (int)s->batch_n.grow;
+    out->compactions = (int)s->batch_n.compactions;
+    out->extent_updates = (int)s->batch_n.extent_updates;
     out->hausdorff_checks = s->checks;
     out->pending = (int)s->pending.size();
     out->active = (int)(s->active.size() + s->ready.size());

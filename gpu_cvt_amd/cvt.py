@@ -73,7 +73,8 @@ class ServerConfig(C.Structure):
     _fields_ = [
         ("hbm_budget_bytes", C.c_ulonglong), ("epoch_iters", C.c_int), ("min_check_iters", C.c_int),
         ("check_disp_rel", C.c_double), ("check_frozen_frac", C.c_double), ("ingest_threads", C.c_int),
-        ("check_threads", C.c_int), ("reserved", C.c_int * 6),
+        ("check_threads", C.c_int), ("batch_jobs0", C.c_int), ("batch_site_blocks0", C.c_int),
+        ("reserved", C.c_int * 4),
     ]
 
 
@@ -93,7 +94,9 @@ class ServerInfo(C.Structure):
         ("site_iters", C.c_double), ("busy_seconds", C.c_double),
         ("peak_bytes", C.c_ulonglong), ("used_bytes", C.c_ulonglong),
         ("epochs", C.c_longlong), ("rebuilds", C.c_longlong), ("hausdorff_checks", C.c_longlong),
-        ("pending", C.c_int), ("active", C.c_int), ("failures", C.c_int), ("reserved", C.c_int * 5),
+        ("pending", C.c_int), ("active", C.c_int), ("failures", C.c_int),
+        ("rebuilds_grow", C.c_int), ("compactions", C.c_int), ("extent_updates", C.c_int),
+        ("reserved", C.c_int * 2),
         ("step_seconds", C.c_double), ("check_seconds", C.c_double), ("rebuild_seconds", C.c_double),
     ]
 

@@ -241,7 +241,11 @@ typedef struct cvt_server_config {
     double check_frozen_frac; /* gate: frozen_total >= this * nS (default 0.5) */
     int ingest_threads;       /* threads creating device jobs from requests (default 4) */
     int check_threads;        /* threads running one epoch's convergence checks (default 8) */
-    int reserved[6];
+    /* first capacity of a (K, kproj) group's batch tables: jobs, and blocks
+       of 256 sites (0: 1028 jobs, 65536 blocks); the tables grow fourfold
+       when outgrown, which captures the batch graph again */
+    int batch_jobs0, batch_site_blocks0;
+    int reserved[4];
 } cvt_server_config;
 
 /* status: 0 queued, 1 running, 2 converged, 3 reached max_iters, 4 failed.
@@ -260,13 +264,17 @@ typedef struct cvt_result {
 typedef struct cvt_server_info {
     double site_iters;        /* sum over epochs of (active sites x iterations) */
     double busy_seconds;      /* wall time of epochs incl. convergence checks */
-    unsigned long long peak_bytes, used_bytes;
-    long long epochs, rebuilds, hausdorff_checks;
+    unsigned long long peak_bytes, used_bytes;  /* jobs, their LBVHs and the check temporaries */
+    long long epochs, rebuilds, hausdorff_checks; /* rebuilds: batch graph captures */
     int pending, active, failures;
-    int reserved[5];
+    /* of rebuilds, those for tables that outgrew their capacity (the rest
+       are the first capture of each (K, kproj) group); batch tables laid out
+       again in place (no capture); launch-extent edits of captured graphs */
+    int rebuilds_grow, compactions, extent_updates;
+    int reserved[2];
     double step_seconds;      /* of busy_seconds: batch steps */
     double check_seconds;     /* convergence checks */
-    double rebuild_seconds;   /* batch graph rebuilds after membership changes */
+    double rebuild_seconds;   /* jobs joining and leaving batches, captures included */
 } cvt_server_info;
 
 typedef struct cvt_server cvt_server;
