@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libcvt_hip.so")
 CLI = os.path.join(HERE, "gpu_cvt")
 SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip", "cvt_server.hip"]
-# cvt_lloyd.hip is compiled once per neighbour count K (-DCVT_K=k)
+# cvt_lloyd.hip is compiled once per neighbour count K (-DCVT_K=k) and kproj (-DCVT_KP=p)
 PER_K_SOURCE = "cvt_lloyd.hip"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CVT_OFFLOAD_ARCH", "gfx950")
@@ -44,6 +44,8 @@ def build(verbose: bool = False) -> str:
     os.makedirs(objdir, exist_ok=True)
     units = [(src, src.replace(".hip", ".o"), []) for src in SOURCES]
     units += [(PER_K_SOURCE, f"cvt_lloyd_k{k}.o", [f"-DCVT_K={k}"]) for k in k_values()]
+    # the FP64 pass with kproj = 10 lists, one object per K
+    units += [(PER_K_SOURCE, f"cvt_lloyd_k{k}_p10.o", [f"-DCVT_K={k}", "-DCVT_KP=10"]) for k in k_values()]
 
     def compile_one(unit) -> str:
         src, oname, defs = unit

@@ -181,6 +181,7 @@ struct cvt_job {
     int* slot_of = nullptr;   // each site's slot in the current site grid
     float4* cent = nullptr;   // centroid and cell flags per site
     int* vs_list = nullptr;   // FP64 pass: sites waiting for a wave-wide vertex search
+    int* excl_vslot = nullptr;  // exclude_self_vertex: per site, the vertex-grid slot its kproj search skips
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
@@ -434,6 +435,7 @@ JobDesc make_desc(cvt_job* j) {
     a.slot_of = to_g(j->slot_of);
     a.cent = to_g(j->cent);
     a.vs_list = to_g(j->vs_list);
+    a.excl_vslot = j->p.exclude_self_vertex ? to_g(j->excl_vslot) : nullptr;
     a.dmax01 = to_g(j->dmax);
     a.nS = j->nS;
     a.R = (float)j->R;
@@ -606,9 +608,12 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     if (!Vh || !Fh || !S0) return fail(CVT_EINVAL, "null input array");
     if (nV < 3 || nF < 1) return fail(CVT_EINVAL, "empty mesh (nV=%d nF=%d)", nV, nF);
     if (!lloyd_supported(p.K, p.kproj))
-        return fail(CVT_EINVAL, "unsupported K=%d / kproj=%d (K in {4,6,8,12,16,20,24,32}, kproj=5)", p.K, p.kproj);
+        return fail(CVT_EINVAL, "unsupported K=%d / kproj=%d (K in {4,6,8,12,16,20,24,32}, kproj 5 or 10)", p.K,
+                    p.kproj);
+    if (p.exclude_self_vertex != 0 && p.exclude_self_vertex != 1) return fail(CVT_EINVAL, "exclude_self_vertex: 0 or 1");
+    if (p.remesh_R_mode != 0 && p.remesh_R_mode != 1) return fail(CVT_EINVAL, "remesh_R_mode: 0 or 1");
+    if (nV < p.kproj + p.exclude_self_vertex) return fail(CVT_EINVAL, "need nV >= kproj (+1 with exclude_self_vertex)");
     if (nS < p.K + 1) return fail(CVT_EINVAL, "need nS >= K+1 (nS=%d K=%d)", nS, p.K);
-    if (nV < p.kproj) return fail(CVT_EINVAL, "need nV >= kproj");
     if (p.freeze_mode < 0 || p.freeze_mode > 2) return fail(CVT_EINVAL, "freeze_mode must be 0, 1 or 2");
     if (!check_finite(Vh, 3L * nV)) return fail(CVT_ENONFINITE, "non-finite vertex coordinate");
     if (!check_finite(S0, 3L * nS)) return fail(CVT_ENONFINITE, "non-finite site coordinate");
@@ -812,6 +817,12 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         launch_frames(m, j->FU, j->FW, s);
     }
     launch_grid_build(j->V, nV, j->vg, j->vgb, s);
+    if (p.exclude_self_vertex) {
+        int* vslot = (int*)build_tmp.get(sizeof(int) * (size_t)nV);
+        if ((rc = j->alloc(&j->excl_vslot, nS))) return bail(rc);
+        if (!vslot) return bail(fail(CVT_ENOMEM, "vertex slot temporaries"));
+        launch_excl_slots(j->vgb.pts, nV, j->cid, nS, vslot, j->excl_vslot, s);
+    }
     {
         // point LBVH over the vertex grid's cell-sorted points (leaf = slot)
         BVHDev bd;
@@ -1193,6 +1204,19 @@ int cvt_job_energy(cvt_job* j, double* E) {
 // reusing them): dF receives 3 * nkeep vertex ids.
 using DevTemps = TempPool;
 
+// The remesh cells' clip half-size when it is not the job's R: the max bbox
+// edge of the current sites (remesh_R_mode 1, main.cu:899-909), on the device
+// from the site grid just built on stream s (nullptr: the job's R; also when
+// the temporaries are short, which the caller's next allocation reports).
+static const float* remesh_R_dev(cvt_job* j, TempPool& T, hipStream_t s) {
+    if (j->p.remesh_R_mode != 1) return nullptr;
+    float* partial = (float*)T.get(sizeof(float) * 6 * kBboxBlocks);
+    float* out = (float*)T.get(sizeof(float));
+    if (!partial || !out) return nullptr;
+    launch_site_bbox_edge(j->desc, j->nS, partial, out, s);
+    return out;
+}
+
 // keys_pre/site_n_pre (nullable): the remesh cells' keys and site normals of
 // the current sites, already computed (by the fast convergence verdict).
 static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int* nkeep_out,
@@ -1219,7 +1243,7 @@ static int remesh_device(cvt_job* j, double eps, DevTemps& T, int** dF_out, int*
         return fail(CVT_ENOMEM, "remesh temporaries");
     if (!keys_pre) {
         launch_batch_grid(j->tables, 1, s);
-        launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+        launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, remesh_R_dev(j, T, s), s);
     }
     radix_sort_u64(keys, (int)nk, 64, keys_alt, hist, hsum, s);
     launch_unique_mark(keys, (int)nk, flag, s);
@@ -1314,7 +1338,7 @@ int check_fast_enqueue(cvt_job* j, double eps, TempPool& T, double tol, int* fla
     if (!knn_tmp || !keys || !site_n || !flag || !probes) return fail(CVT_ENOMEM, "check temporaries");
     hipStream_t s = j->stream;
     launch_batch_grid(j->tables, 1, s);
-    launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, s);
+    launch_remesh_cells(j->desc, nS, K, knn_tmp, keys, site_n, remesh_R_dev(j, T, s), s);
     launch_fill(flag, nst, 0, s);
     bvh_check_keys(j->bvh->d, keys, nS, M, j->S[j->cur], site_n, (float)eps, sq_threshold(tol), probes, cap, flag, s);
     HIPCHK(hipMemcpyAsync(flag_host, flag, sizeof(int), hipMemcpyDeviceToHost, s));

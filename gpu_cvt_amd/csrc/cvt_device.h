@@ -833,7 +833,7 @@ __device__ __forceinline__ void wave_append(bool pred, T v, T* arr, int cap, int
 template <int L, int CAPS, int CAPC>
 __device__ __forceinline__ int wave_bvh_topl(const PointBVH& b, const CVT_G float4* __restrict__ pts,
                                              const CVT_G int* __restrict__ cid, float qx, float qy, float qz,
-                                             char* lds, int& my_slot, double& my_key, bool& ok) {
+                                             char* lds, int& my_slot, double& my_key, bool& ok, int excl = -1) {
     static_assert(L <= 64 && CAPC >= 128, "one result per lane; room for a round of kept points");
     const int lane = (int)threadIdx.x & 63;
     int* stk = reinterpret_cast<int*>(lds);
@@ -867,8 +867,8 @@ __device__ __forceinline__ int wave_bvh_topl(const PointBVH& b, const CVT_G floa
         return below >= L ? kth : __builtin_huge_valf();
     };
     if (n <= 64) {
-        const bool v = lane < n;
-        const int sl = v ? b.leaf[lane] : 0;
+        const int sl = lane < n ? b.leaf[lane] : 0;
+        const bool v = lane < n && sl != excl;
         wave_append<float>(v, 0.f, cd, CAPC, cnt);
         cnt = 0;
         wave_append<int>(v, sl, cs, CAPC, cnt);
@@ -882,7 +882,9 @@ __device__ __forceinline__ int wave_bvh_topl(const PointBVH& b, const CVT_G floa
             cur = dl <= dr ? c.x : c.y;
         }
         const int lo = min(max(cur - (n - 1) - 32, 0), n - 64);
-        const float dme = d2f(qx, qy, qz, pts[b.leaf[lo + lane]]);
+        const int mine = b.leaf[lo + lane];
+        // an excluded point ranks last, so the bound comes from L others
+        const float dme = mine == excl ? __builtin_huge_valf() : d2f(qx, qy, qz, pts[mine]);
         int rank = 0;
         for (int t = 0; t < 64; ++t) {
             const float dt = __shfl(dme, t, 64);
@@ -918,7 +920,7 @@ __device__ __forceinline__ int wave_bvh_topl(const PointBVH& b, const CVT_G floa
                 pn = fminf(gl, gr) <= blim;
                 pf = fmaxf(gl, gr) <= blim;
             }
-            const bool keep = leaf && d <= lim;
+            const bool keep = leaf && d <= lim && sl != excl;
             int c0 = cnt;
             wave_append<float>(keep, d, cd, CAPC, c0);
             wave_append<int>(keep, sl, cs, CAPC, cnt);

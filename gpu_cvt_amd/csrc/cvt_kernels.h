@@ -106,6 +106,8 @@ struct StepArgs {
     CVT_G int* slot_of;         // nS: each site's slot in this iteration's site grid (sg.pts)
     CVT_G float4* cent;         // nS: centroid and cell-phase flags (hand-off of the FP32 pass)
     CVT_G int* vs_list;         // nS: sites whose FP64 vertex search a whole wave runs
+    const CVT_G int* excl_vslot; // nS: the vertex-grid slot each site's kproj search skips, -1 none
+                                 // (nullptr: no site skips one)
     int nS;
     float R;
     float tau0;           // first-iteration candidate radius^2 guess (mean-density estimate)
@@ -205,8 +207,16 @@ constexpr int kLloydFrameDone = 4;
 void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3,
                         const SideStream* side = nullptr);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
+// R_dev (nullable): a device float that replaces the job's R as the clip half-size.
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
-                         float4* site_n, hipStream_t s);
+                         float4* site_n, const float* R_dev, hipStream_t s);
+// Max bbox edge of the current sites (FP32 min/max, main.cu:899-909) into
+// *out; partial: 2 * 3 * kBboxBlocks floats of scratch.
+constexpr int kBboxBlocks = 256;
+void launch_site_bbox_edge(const JobDesc* desc, int nS, float* partial, float* out, hipStream_t s);
+// Per site: the vertex-grid slot of the mesh vertex whose index equals the
+// site's caller index (-1 when there is none); vslot: nV ints of scratch.
+void launch_excl_slots(const float4* vg_pts, int nV, const int* cid, int nS, int* vslot, int* excl, hipStream_t s);
 // Per-site freeze tiers (scheme 1: normal variation, 5 tiers; scheme 2: plus
 // the covariance split, 6 tiers) into tier[nS]; nv_out / L_out (nullable)
 // receive the per-site scores.  The site grid must be current.
@@ -215,11 +225,11 @@ void launch_site_tiers(const JobDesc* desc, int nS, int K, int scheme, int* nn_t
 // Per-K launchers, defined in cvt_lloyd.hip (one explicit instantiation per object).
 // part 0: the cell kernel; 1: the FP64 pass over the frame/cell deferrals;
 // 2: the FP64 pass over the projection deferrals
-template <int K>
+template <int K, int KP>
 void launch_lloyd_k(const BatchTables& bt, int eval_only, int part, hipStream_t s);
 template <int K>
 void launch_remesh_cells_k(const JobDesc* desc, int nS, int* knn_tmp, unsigned long long* keys, float4* site_n,
-                           hipStream_t s);
+                           const float* R_dev, hipStream_t s);
 template <int K>
 void launch_site_tiers_k(const JobDesc* desc, int nS, int scheme, const int* nn, unsigned char* tier,
                          double* nv_out, double* L_out, hipStream_t s);

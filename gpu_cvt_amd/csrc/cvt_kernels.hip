@@ -493,6 +493,75 @@ void launch_max_float(const float* v, int n, unsigned* out, hipStream_t s) {
     if (n > 0) hipLaunchKernelGGL(k_max_float, dim3(std::min(nblk(n, 256), 1024)), dim3(256), 0, s, v, n, out);
 }
 
+// FP32 min / max of the sites' coordinates per block (NaN coordinates are
+// ignored, as std::min / std::max with the running value first do), then the
+// largest bbox edge.  Min and max are exact in any order.
+__global__ __launch_bounds__(256) void k_site_bbox_partial(const JobDesc* __restrict__ desc, int nS,
+                                                           float* __restrict__ partial) {
+    const float4* __restrict__ P = desc->a.sg.pts;
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int i = blockIdx.x * 256 + (int)threadIdx.x; i < nS; i += gridDim.x * 256) {
+        const float4 p = P[i];
+        lo[0] = fminf(lo[0], p.x); lo[1] = fminf(lo[1], p.y); lo[2] = fminf(lo[2], p.z);
+        hi[0] = fmaxf(hi[0], p.x); hi[1] = fmaxf(hi[1], p.y); hi[2] = fmaxf(hi[2], p.z);
+    }
+    __shared__ float red[6][4];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            lo[c] = fminf(lo[c], __shfl_xor(lo[c], o, 64));
+            hi[c] = fmaxf(hi[c], __shfl_xor(hi[c], o, 64));
+        }
+    }
+    const int w = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int c = 0; c < 3; ++c) { red[c][w] = lo[c]; red[3 + c][w] = hi[c]; }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+        const int c = (int)threadIdx.x;
+        float v = red[c][0];
+        for (int k = 1; k < 4; ++k) v = c < 3 ? fminf(v, red[c][k]) : fmaxf(v, red[c][k]);
+        partial[blockIdx.x * 6 + c] = v;
+    }
+}
+
+__global__ void k_site_bbox_final(const float* __restrict__ partial, int nb, float* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    for (int b = 0; b < nb; ++b)
+        for (int c = 0; c < 3; ++c) {
+            lo[c] = fminf(lo[c], partial[b * 6 + c]);
+            hi[c] = fmaxf(hi[c], partial[b * 6 + 3 + c]);
+        }
+    const float bx = hi[0] - lo[0], by = hi[1] - lo[1], bz = hi[2] - lo[2];
+    *out = fmaxf(bx, fmaxf(by, bz));
+}
+
+void launch_site_bbox_edge(const JobDesc* desc, int nS, float* partial, float* out, hipStream_t s) {
+    const int nb = std::max(1, std::min(kBboxBlocks, nblk(nS, 256)));
+    hipLaunchKernelGGL(k_site_bbox_partial, dim3(nb), dim3(256), 0, s, desc, nS, partial);
+    hipLaunchKernelGGL(k_site_bbox_final, dim3(1), dim3(64), 0, s, partial, nb, out);
+}
+
+__global__ void k_vertex_slots(const float4* __restrict__ pts, int nV, int* __restrict__ vslot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nV) vslot[__float_as_int(pts[t].w)] = t;
+}
+
+__global__ void k_site_excl(const int* __restrict__ cid, int nS, int nV, const int* __restrict__ vslot,
+                            int* __restrict__ excl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const int c = cid[i];
+    excl[i] = (unsigned)c < (unsigned)nV ? vslot[c] : -1;
+}
+
+void launch_excl_slots(const float4* vg_pts, int nV, const int* cid, int nS, int* vslot, int* excl, hipStream_t s) {
+    hipLaunchKernelGGL(k_vertex_slots, dim3(nblk(nV, 256)), dim3(256), 0, s, vg_pts, nV, vslot);
+    hipLaunchKernelGGL(k_site_excl, dim3(nblk(nS, 256)), dim3(256), 0, s, cid, nS, nV, vslot, excl);
+}
+
 __global__ void k_unique_mark(const unsigned long long* __restrict__ keys, int n, int* __restrict__ flag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
@@ -538,10 +607,10 @@ __global__ void k_tri_compact(const int4* __restrict__ tri, const int* __restric
 }
 
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
-                         float4* site_n, hipStream_t s) {
+                         float4* site_n, const float* R_dev, hipStream_t s) {
     switch (K) {
 #define CVT_CASE(k) \
-    case k: launch_remesh_cells_k<k>(desc, nS, knn_tmp, keys, site_n, s); break;
+    case k: launch_remesh_cells_k<k>(desc, nS, knn_tmp, keys, site_n, R_dev, s); break;
         CVT_FOR_EACH_K(CVT_CASE)
 #undef CVT_CASE
         default: break;
@@ -657,7 +726,7 @@ void launch_frames(const MeshDev& m, float4* FU, float4* FW, hipStream_t s) {
 int scan_tile() { return kScanTile; }
 
 bool lloyd_supported(int K, int kproj) {
-    if (kproj != 5) return false;
+    if (kproj != 5 && kproj != 10) return false;
     switch (K) {
 #define CVT_CASE(k) case k:
         CVT_FOR_EACH_K(CVT_CASE)
@@ -681,40 +750,56 @@ void launch_batch_grid(const BatchTables& bt, int eval_only, hipStream_t s) {
     launch_grid_rebuild(bt, s);
 }
 
-static void launch_lloyd_part(const BatchTables& bt, int K, int eval_only, int which, hipStream_t s) {
+// part 0 (the cell kernel) does not depend on kproj
+static void launch_lloyd_part(const BatchTables& bt, int K, int kproj, int eval_only, int part, hipStream_t s) {
     switch (K) {
-#define CVT_CASE(k) \
-    case k: launch_lloyd_k<k>(bt, eval_only, which, s); break;
+#define CVT_CASE(k)                                                                             \
+    case k:                                                                                     \
+        if (kproj == 10 && part != 0) launch_lloyd_k<k, 10>(bt, eval_only, part, s);            \
+        else launch_lloyd_k<k, 5>(bt, eval_only, part, s);                                      \
+        break;
         CVT_FOR_EACH_K(CVT_CASE)
 #undef CVT_CASE
         default: break;
     }
 }
 
+// The frame and projection kernels per kproj (the frame reads a frozen
+// site's pinned vertex list, kproj wide).
+static void launch_frame(const BatchTables& bt, int kproj, int eval_only, hipStream_t s) {
+    const dim3 g(bt.n_site_blocks), b(256);
+    if (kproj == 10) hipLaunchKernelGGL((kb_frame<10>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
+    else hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
+}
+
+static void launch_project(const BatchTables& bt, int kproj, hipStream_t s) {
+    const dim3 g(bt.n_site_blocks), b(256);
+    if (kproj == 10) hipLaunchKernelGGL((kb_project<10>), g, b, 0, s, bt.descs, bt.site_blocks);
+    else hipLaunchKernelGGL((kb_project<5>), g, b, 0, s, bt.descs, bt.site_blocks);
+}
+
 void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which,
                         const SideStream* side) {
-    (void)kproj;
     if (which & 1) {
-        const dim3 g(bt.n_site_blocks), b(256);
-        if (!(which & kLloydFrameDone)) hipLaunchKernelGGL((kb_frame<5>), g, b, 0, s, bt.descs, bt.site_blocks, eval_only);
-        launch_lloyd_part(bt, K, eval_only, 0, s);
+        if (!(which & kLloydFrameDone)) launch_frame(bt, kproj, eval_only, s);
+        launch_lloyd_part(bt, K, kproj, eval_only, 0, s);
         const bool fork = side && side->s && (which & 2);
         if (fork) {
             (void)hipEventRecord(side->fork, s);
             (void)hipStreamWaitEvent(side->s, side->fork, 0);
-            launch_lloyd_part(bt, K, eval_only, 1, side->s);
+            launch_lloyd_part(bt, K, kproj, eval_only, 1, side->s);
             (void)hipEventRecord(side->join, side->s);
         }
-        if (!eval_only) hipLaunchKernelGGL((kb_project<5>), g, b, 0, s, bt.descs, bt.site_blocks);
+        if (!eval_only) launch_project(bt, kproj, s);
         if (fork) {
             (void)hipStreamWaitEvent(s, side->join, 0);
-            if (!eval_only) launch_lloyd_part(bt, K, eval_only, 2, s);
+            if (!eval_only) launch_lloyd_part(bt, K, kproj, eval_only, 2, s);
             return;
         }
     }
     if (which & 2) {
-        launch_lloyd_part(bt, K, eval_only, 1, s);
-        if (!eval_only) launch_lloyd_part(bt, K, eval_only, 2, s);
+        launch_lloyd_part(bt, K, kproj, eval_only, 1, s);
+        if (!eval_only) launch_lloyd_part(bt, K, kproj, eval_only, 2, s);
     }
 }
 
@@ -728,7 +813,7 @@ void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t
         // of what kb_begin clears) beside the stats reset and site grid rebuild
         (void)hipEventRecord(side->ffork, s);
         (void)hipStreamWaitEvent(side->s, side->ffork, 0);
-        hipLaunchKernelGGL((kb_frame<5>), dim3(bt.n_site_blocks), dim3(256), 0, side->s, bt.descs, bt.site_blocks, 0);
+        launch_frame(bt, kproj, 0, side->s);
         (void)hipEventRecord(side->fjoin, side->s);
         hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
         launch_grid_rebuild(bt, s);

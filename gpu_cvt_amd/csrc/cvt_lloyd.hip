@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restric
 template <int KP>
 __global__ __launch_bounds__(256) void kb_precise_vsearch(const JobDesc* __restrict__ descs,
                                                           const BlockRef* __restrict__ bt) {
-    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
+    static_assert(KP == 5, "the searches fill the vertex pool, which holds the kproj + 2 nearest");
     __shared__ double vs_lds[4 * kVsLdsBytes / 8];
     const BlockRef br = bt[blockIdx.x / kWaveBlocks];
     if (br.job < 0) return;
@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void kb_precise_vsearch(const JobDesc* __restr
         int ms;
         double mk;
         bool ok;
-        wave_bvh_topl<KP + 4, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, c4.x, c4.y, c4.z, lds, ms, mk, ok);
+        wave_bvh_topl<KP + 4, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, c4.x, c4.y, c4.z, lds, ms, mk, ok,
+                                                 vertex_excl(a, i));
         int id[KP + 2];
 #pragma unroll
         for (int k = 0; k < KP + 2; ++k) {
@@ -344,7 +345,7 @@ __device__ __forceinline__ unsigned long long pack3(int a, int b, int c) {
 template <int K>
 __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict__ desc, int* __restrict__ knn_tmp,
                                                       unsigned long long* __restrict__ keys,
-                                                      float4* __restrict__ site_n) {
+                                                      float4* __restrict__ site_n, const float* __restrict__ R_dev) {
     constexpr int M = K + 4;
     __shared__ int lds[collect_lds_ints<kCollectCap>()];
     const StepArgs& a = desc->a;
@@ -402,7 +403,7 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
         px[k] = aa * ic;
         py[k] = bb * ic;
     }
-    const float iR = 1.0f / a.R;
+    const float iR = 1.0f / (R_dev ? *R_dev : a.R);
     px[K] = iR;   py[K] = 0.f;
     px[K + 1] = 0.f; py[K + 1] = iR;
     px[K + 2] = -iR; py[K + 2] = 0.f;
@@ -417,26 +418,34 @@ __global__ __launch_bounds__(256) void k_remesh_cells(const JobDesc* __restrict_
 }
 
 // ------------------------------------------------------------------ launches
-template <int K>
+// The cell kernel does not depend on kproj: part 0 is launched through the
+// kproj = 5 instantiation only.  With another kproj the FP64 pass searches
+// vertex lists inline (the post-pass searches fill the kproj = 5 pool).
+template <int K, int KP>
 void launch_lloyd_k(const BatchTables& bt, int eval_only, int part, hipStream_t s) {
-    if (part == 0) {
-        hipLaunchKernelGGL((kb_cell<K>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
-                           eval_only);
-        return;
+    if constexpr (KP == 5) {
+        if (part == 0) {
+            hipLaunchKernelGGL((kb_cell<K>), dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks,
+                               eval_only);
+            return;
+        }
     }
+    if (part == 0) return;
     const dim3 g(bt.n_prec_blocks), gw(bt.n_prec_blocks * kWaveBlocks), b(256);
     if (part == 1 && !eval_only) hipLaunchKernelGGL((kb_precise_knn<K>), gw, b, 0, s, bt.descs, bt.prec_blocks);
-    hipLaunchKernelGGL((kb_lloyd_precise<K, 5>), g, b, 0, s, bt.descs, bt.prec_blocks, eval_only, part == 2 ? 1 : 0);
-    if (part == 1 && !eval_only) {
-        hipLaunchKernelGGL((kb_precise_vsearch<5>), gw, b, 0, s, bt.descs, bt.prec_blocks);
-        hipLaunchKernelGGL((kb_precise_finish<5>), g, b, 0, s, bt.descs, bt.prec_blocks);
+    hipLaunchKernelGGL((kb_lloyd_precise<K, KP>), g, b, 0, s, bt.descs, bt.prec_blocks, eval_only, part == 2 ? 1 : 0);
+    if constexpr (KP == 5) {
+        if (part == 1 && !eval_only) {
+            hipLaunchKernelGGL((kb_precise_vsearch<5>), gw, b, 0, s, bt.descs, bt.prec_blocks);
+            hipLaunchKernelGGL((kb_precise_finish<5>), g, b, 0, s, bt.descs, bt.prec_blocks);
+        }
     }
 }
 
 template <int K>
 void launch_remesh_cells_k(const JobDesc* desc, int nS, int* knn_tmp, unsigned long long* keys, float4* site_n,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_remesh_cells<K>, dim3(nblk(nS, 256)), dim3(256), 0, s, desc, knn_tmp, keys, site_n);
+                           const float* R_dev, hipStream_t s) {
+    hipLaunchKernelGGL(k_remesh_cells<K>, dim3(nblk(nS, 256)), dim3(256), 0, s, desc, knn_tmp, keys, site_n, R_dev);
 }
 
 template <int K>
@@ -445,9 +454,17 @@ void launch_site_tiers_k(const JobDesc* desc, int nS, int scheme, const int* nn,
     hipLaunchKernelGGL(k_site_tiers<K>, dim3(nblk(nS, 256)), dim3(256), 0, s, desc, nn, scheme, tier, nv_out, L_out);
 }
 
-template void launch_lloyd_k<CVT_K>(const BatchTables&, int, int, hipStream_t);
-template void launch_remesh_cells_k<CVT_K>(const JobDesc*, int, int*, unsigned long long*, float4*, hipStream_t);
+// One object per (K, kproj): CVT_KP (default 5) selects the kproj; the
+// kproj = 5 object also carries the K-only launchers.
+#ifndef CVT_KP
+#define CVT_KP 5
+#endif
+template void launch_lloyd_k<CVT_K, CVT_KP>(const BatchTables&, int, int, hipStream_t);
+#if CVT_KP == 5
+template void launch_remesh_cells_k<CVT_K>(const JobDesc*, int, int*, unsigned long long*, float4*, const float*,
+                                           hipStream_t);
 template void launch_site_tiers_k<CVT_K>(const JobDesc*, int, int, const int*, unsigned char*, double*, double*,
                                          hipStream_t);
+#endif
 
 }  // namespace cvt

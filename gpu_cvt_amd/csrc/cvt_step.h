@@ -369,12 +369,14 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 // true); otherwise the 9th distance plus d seeds the search in tauv (triangle
 // inequality; +inf without a cache).
 // pool_lookup on a cache entry already loaded (vc: vcache, p0/p1: vpool).
+// The pool holds kproj + 2 = 7 vertices: with another kproj there is no
+// cache (every list is searched).
 template <int KP>
 __device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3& c3, const float4& vc, const int4& p0,
                                                  const int4& p1, int (&vl)[KP], float& tauv) {
-    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
     const float inf = __builtin_huge_valf();
     tauv = inf;
+    if constexpr (KP != 5) return false;
     if (!(vc.w < inf)) return false;
     const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
                            (c3.z - vc.z) * (c3.z - vc.z));
@@ -418,6 +420,9 @@ __device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const floa
     return pool_lookup_with<KP>(a, c3, a.vcache[i], a.vpool[2 * (size_t)i], a.vpool[2 * (size_t)i + 1], vl, tauv);
 }
 
+// The vertex-grid slot that site i's kproj search skips (-1: none).
+__device__ __forceinline__ int vertex_excl(const StepArgs& a, int i) { return a.excl_vslot ? a.excl_vslot[i] : -1; }
+
 // The kproj nearest mesh vertices of c3 by the vertex grid (seed tauv), and
 // the pool cache refreshed.  Returns false when the FP32 search could not be
 // proven (never for PRECISE).
@@ -426,14 +431,15 @@ __device__ __forceinline__ bool vertex_search(const StepArgs& a, int i, const fl
     const float inf = __builtin_huge_valf();
     int sl[KP + 2];
     float lv, dv[KP + 4];
-    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, -1, tauv, sl, lv,
-                                              nullptr, 0.f, dv, nullptr, &a.m.vb);
+    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, vertex_excl(a, i),
+                                              tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
     if (!PRECISE && kk < 0) return false;
     int id[KP + 2];
 #pragma unroll
     for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
 #pragma unroll
     for (int k = 0; k < KP; ++k) vl[k] = id[k];
+    if constexpr (KP != 5) return true;  // no pool cache
     a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 2] < inf ? sqrtf(dv[KP + 2]) : inf);
     a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
     a.vpool[2 * (size_t)i + 1] =
@@ -640,33 +646,38 @@ constexpr int kVsLdsBytes = 4 * kVsStack + 8 * kVsCand;  // per wave
 template <int KP>
 __device__ __forceinline__ void wave_vertex_searches(const StepArgs& a, int i, float3 c3, bool& want, int (&vl)[KP],
                                                      char* lds) {
-    static_assert(KP == 5, "the vertex pool holds the kproj + 2 nearest");
-    constexpr int L = KP + 4;  // the pool's 7, its 8th distance (bound) and 9th (seed)
+    // with the pool (kproj 5): its 7, its 8th distance (bound) and 9th (seed)
+    constexpr bool kPool = KP == 5;
+    constexpr int L = kPool ? KP + 4 : KP;
     const int lane = (int)threadIdx.x & 63;
+    const int my_excl = vertex_excl(a, i);
     unsigned long long need = __ballot(want);
     while (need) {
         const int l = (int)__ffsll((long long)need) - 1;
         need &= need - 1ull;
         const float qx = __shfl(c3.x, l, 64), qy = __shfl(c3.y, l, 64), qz = __shfl(c3.z, l, 64);
-        const int qi = __shfl(i, l, 64);
+        const int qi = __shfl(i, l, 64), qe = __shfl(my_excl, l, 64);
         int ms;
         double mk;
         bool ok;
-        wave_bvh_topl<L, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, qx, qy, qz, lds, ms, mk, ok);
+        wave_bvh_topl<L, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, qx, qy, qz, lds, ms, mk, ok, qe);
         if (!ok) continue;  // want stays set: lane l searches itself
-        int id[KP + 2];
+        constexpr int NI = kPool ? KP + 2 : KP;
+        int id[NI];
 #pragma unroll
-        for (int k = 0; k < KP + 2; ++k) {
+        for (int k = 0; k < NI; ++k) {
             const int sl = __shfl(ms, k, 64);
             id[k] = sl >= 0 ? __float_as_int(a.m.vg_pts[sl].w) : -1;
         }
-        const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
         if (lane == l) {
 #pragma unroll
             for (int k = 0; k < KP; ++k) vl[k] = id[k];
             want = false;
         }
-        if (lane == 0) store_vertex_pool(a, qi, qx, qy, qz, id, k8, k9);
+        if constexpr (kPool) {
+            const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
+            if (lane == 0) store_vertex_pool(a, qi, qx, qy, qz, id, k8, k9);
+        }
     }
 }
 
@@ -695,7 +706,7 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
             vertex_search<KP, true>(a, c.i, cr.c3, tauv, vl);
             want = false;
         }
-        if (post_search) {
+        if (post_search && KP == 5) {  // the post-pass searches fill the pool
             if (want) {
                 // without kCentDone: the projection kernel, which may still be
                 // running beside this pass, keeps skipping the site

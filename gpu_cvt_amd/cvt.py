@@ -40,7 +40,8 @@ class Params(C.Structure):
         ("tier_streak", C.c_int * 6), ("tier_disp_abs2", C.c_double * 6),
         ("nan_policy", C.c_int), ("clip_R_mode", C.c_int), ("clip_R", C.c_double),
         ("site_cell_scale", C.c_double), ("vert_cell_scale", C.c_double),
-        ("debug", C.c_int), ("reserved", C.c_int * 7),
+        ("debug", C.c_int), ("exclude_self_vertex", C.c_int), ("remesh_R_mode", C.c_int),
+        ("reserved", C.c_int * 5),
     ]
 
 

@@ -28,7 +28,8 @@ extern "C" {
  * main.cu:381-432 and the tier tables of main.cu:19-39. */
 typedef struct cvt_params {
     int K;              /* site neighbours per cell (main.cu:398 K_NEIGH); one of 4,6,8,12,16,20,24,32 */
-    int kproj;          /* vertex list length for reprojection (main.cu:394 KPROJ); 5 */
+    int kproj;          /* vertex list length for reprojection: 5 (main.cu:394 KPROJ) or 10
+                           (scripts/testfreeze.py:379 k_proj) */
     int freeze_mode;    /* 0 off (main.cu mode 0), 1 single-tier streak, 2 per-site tiers (modes 1/2) */
     int freeze_need;    /* mode 1: streak length (main.cu:407 freeze_monitor_iters) */
     double freeze_disp_rel;   /* displacement threshold, fraction of the mesh bbox diagonal (linear) */
@@ -41,7 +42,12 @@ typedef struct cvt_params {
     double site_cell_scale;  /* site grid cell size in units of the expected (K+2)-NN radius (0 = default, 1.3) */
     double vert_cell_scale;  /* vertex grid cell size in units of the expected (kproj+2)-NN radius (0 = default, 1.2) */
     int debug;          /* 1: record per-site intermediates of the last step (cvt_job_get_debug) */
-    int reserved[7];
+    int exclude_self_vertex; /* 1: a site's centroid -> mesh kproj-NN skips the mesh vertex whose
+                                index equals the site's (the CLI's mode 0: main.cu:728,
+                                near_project.cuh:72-74; meaningful when sites start at the vertices) */
+    int remesh_R_mode;  /* clip half-size of the remesh cells: 0 the Lloyd R (clip_R_mode), 1 the
+                           max bbox edge of the current sites (main.cu:899-909 R2) */
+    int reserved[5];
 } cvt_params;
 
 /* Per-iteration counters; replaces the printf/CSV counters of main.cu:809-834

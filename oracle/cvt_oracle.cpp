@@ -754,11 +754,14 @@ void orc_freeze(int n, int K, const double* S, const double* Snew, const int* kn
 //   out: nn_site (site->nearest vertex used for the frame), U, Vf, cent,
 //        knnV (centroid->mesh kproj lists), Snew, area, moment, counts[4]
 // params: K, kproj, R, freeze_mode (0 off, else apply), thresh2/streak per
-//        tier, has_prev, nan_policy (0 keep old site, 1 propagate NaN)
+//        tier, has_prev, nan_policy (0 keep old site, 1 propagate NaN),
+//        exclude_self (1: site i's centroid -> mesh kproj-NN skips mesh vertex
+//        i, knn_vertices_bruteforce_k<KPROJ, true> of main.cu:728 with
+//        near_project.cuh:72-74)
 void orc_lloyd_step(int nV, const double* V, int nF, const int* F, const double* N, const int* vf_off,
                     const int* vf_faces, int nS, const double* S, int K, int kproj, double R, int freeze_mode,
                     const double* thresh2_tier, const int* streak_tier, const unsigned char* tier, int has_prev,
-                    int nan_policy, int* knn, int* knnV, unsigned char* frozen, unsigned char* streak,
+                    int nan_policy, int exclude_self, int* knn, int* knnV, unsigned char* frozen, unsigned char* streak,
                     int* nn_site, double* U, double* Vf, double* cent, double* Snew, double* area, double* moment,
                     int* counts) {
     std::vector<int> prev(knn, knn + (long)nS * K);
@@ -789,7 +792,7 @@ void orc_lloyd_step(int nV, const double* V, int nF, const int* F, const double*
         D3 c = ld3(cent, i);
         bool fin = std::isfinite(c.x) && std::isfinite(c.y) && std::isfinite(c.z);
         if (!frozen[i]) {
-            if (fin) grid_knn(gv, c, -1, kproj, knnV + i * (long)kproj);
+            if (fin) grid_knn(gv, c, exclude_self && i < nV ? (int)i : -1, kproj, knnV + i * (long)kproj);
             else for (int t = 0; t < kproj; ++t) knnV[i * (long)kproj + t] = -1;
         }
         D3 p = project_point(c, knnV + i * (long)kproj, kproj, V, nV, F, nF, vf_off, vf_faces);

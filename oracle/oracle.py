@@ -61,7 +61,7 @@ def lib():
         L.orc_freeze.argtypes = [C.c_int, C.c_int, _dp, _dp, _ip, _ip, _up, _dp, _ip, C.c_int, C.c_int, _up, _up, _ip]
         L.orc_lloyd_step.argtypes = [
             C.c_int, _dp, C.c_int, _ip, _dp, _ip, _ip, C.c_int, _dp, C.c_int, C.c_int, C.c_double, C.c_int,
-            _dp, _ip, _up, C.c_int, C.c_int, _ip, _ip, _up, _up, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
+            _dp, _ip, _up, C.c_int, C.c_int, C.c_int, _ip, _ip, _up, _up, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         L.orc_remesh.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, C.c_double, _dp, C.c_double, _ip]
         L.orc_remesh.restype = C.c_int
         L.orc_point_mesh_d2.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp]
@@ -217,8 +217,10 @@ class LloydState:
 
 
 def lloyd_step(mesh: Mesh, st: LloydState, *, K: int, kproj: int = 5, R: float, freeze_mode: int = 0,
-               thresh2_tier=None, streak_tier=None, nan_policy: int = 0):
+               thresh2_tier=None, streak_tier=None, nan_policy: int = 0, exclude_self: bool = False):
     """One FP64 Lloyd iteration from ``st`` (modified in place, except S).
+    exclude_self: site i's centroid -> mesh kproj-NN skips mesh vertex i
+    (main.cu:728, near_project.cuh:72-74).
 
     Returns a dict of intermediates and the new sites (float64, NOT rounded).
     """
@@ -232,7 +234,7 @@ def lloyd_step(mesh: Mesh, st: LloydState, *, K: int, kproj: int = 5, R: float, 
     lib().orc_lloyd_step(
         len(mesh.V), mesh.V.reshape(-1), len(mesh.F), mesh.F.reshape(-1), mesh.N.reshape(-1), off, faces, n,
         _d(st.S).reshape(-1), K, kproj, float(R), int(freeze_mode), thresh2_tier, streak_tier, st.tier,
-        int(st.has_prev), int(nan_policy), st.knn.reshape(-1), st.knnV.reshape(-1), st.frozen, st.streak,
+        int(st.has_prev), int(nan_policy), int(bool(exclude_self)), st.knn.reshape(-1), st.knnV.reshape(-1), st.frozen, st.streak,
         out["nn_site"], out["U"].reshape(-1), out["W"].reshape(-1), out["cent"].reshape(-1),
         out["Snew"].reshape(-1), out["area"], out["moment"], out["counts"])
     st.has_prev = 1

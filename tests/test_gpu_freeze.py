@@ -26,7 +26,7 @@ pytestmark = pytest.mark.gpu
 TOL_REL_DIAG = 1e-5
 
 
-def _teacher_forced_step(orc, mesh, job, K, freeze_mode, thr2, need, tier=None):
+def _teacher_forced_step(orc, mesh, job, K, freeze_mode, thr2, need, tier=None, kproj=5, exclude_self=False):
     """One GPU step from the job's current state, checked stage by stage
     against orc.lloyd_step from the same state.  Returns the GPU's stats and
     the largest centroid / site errors (as fractions of the diagonal)."""
@@ -37,7 +37,7 @@ def _teacher_forced_step(orc, mesh, job, K, freeze_mode, thr2, need, tier=None):
     knn_prev = job.knn()
     knnv_prev = job.knnv()
     fz0, st0 = job.freeze_state()
-    st = orc.LloydState(S32, K)
+    st = orc.LloydState(S32, K, kproj=kproj)
     st.knn[:] = knn_prev
     st.knnV[:] = knnv_prev
     st.frozen[:] = fz0
@@ -46,7 +46,8 @@ def _teacher_forced_step(orc, mesh, job, K, freeze_mode, thr2, need, tier=None):
     if tier is not None:
         st.tier[:] = tier
     stats = job.step(1, stats=True)[0]
-    out = orc.lloyd_step(mesh, st, K=K, R=R, freeze_mode=freeze_mode, thresh2_tier=thr2, streak_tier=need)
+    out = orc.lloyd_step(mesh, st, K=K, kproj=kproj, R=R, freeze_mode=freeze_mode, thresh2_tier=thr2,
+                         streak_tier=need, exclude_self=exclude_self)
     knn_g = job.knn()
     knnv_g = job.knnv()
     dbg = job.debug()
@@ -70,7 +71,10 @@ def _teacher_forced_step(orc, mesh, job, K, freeze_mode, thr2, need, tier=None):
     act = ~frozen & np.all(np.isfinite(cent32), axis=1)
     knnv_o = knnv_prev.copy()
     knnv_o[~frozen] = -1
-    knnv_o[act] = orc.knn(mesh.V, cent32[act], 5, exclude_self=False)
+    if exclude_self:  # query i skips vertex i: search every site in order
+        knnv_o[act] = orc.knn(mesh.V, np.where(np.isfinite(cent32), cent32, 0.0), kproj, exclude_self=True)[act]
+    else:
+        knnv_o[act] = orc.knn(mesh.V, cent32[act], kproj, exclude_self=False)
     assert np.array_equal(knnv_g, knnv_o), f"vertex lists differ in {np.sum(np.any(knnv_g != knnv_o, 1))} rows"
     assert np.array_equal(out["knnV"][frozen], knnv_prev[frozen])
     # 5. projected sites: against the full oracle step, and against the

@@ -20,7 +20,11 @@ answers, brute force written independently (numpy), invariants and symmetry.
                                      saturation, <= boundary, absorbing frozen
   orc_point_mesh_d2                  brute == BVH; translated plane == t^2
   orc_lloyd_step                     energy decreases; fixed point; frozen
-                                     lists pinned
+                                     lists pinned; exclude_self lists == numpy
+                                     brute force with vertex i removed; kproj
+                                     10 lists == numpy brute force and the
+                                     projection == the global nearest point
+                                     (icosphere, independent brute force)
 """
 import numpy as np
 import pytest
@@ -342,6 +346,73 @@ def test_lloyd_energy_decreases_and_frozen_lists_pinned(orc):
     orc.lloyd_step(mesh, st, K=12, R=R, freeze_mode=1, thresh2_tier=np.zeros(6), streak_tier=np.full(6, 5, np.int32))
     assert np.array_equal(st.knn[::4], knn0[::4])
     assert np.array_equal(st.knnV[::4], knnv0[::4])
+
+
+def test_lloyd_exclude_self_lists(orc):
+    """Sites at the mesh vertices (the command line's start), exclude_self:
+    the centroid -> mesh lists are the independent brute force with vertex i
+    removed for site i (main.cu:728, near_project.cuh:72-74), so no list holds
+    its own site's vertex; without the switch most lists start with it."""
+    V, F = mg.icosphere(3)
+    mesh = orc.Mesh(V, F)
+    S = mesh.V + 1e-4 * np.random.default_rng(5).standard_normal(mesh.V.shape)
+    R = np.float32(mesh.bbox_max_edge)
+    st = orc.LloydState(S, 12)
+    out = orc.lloyd_step(mesh, st, K=12, R=R, exclude_self=True)
+    assert np.array_equal(st.knnV, np_knn(mesh.V, out["cent"], 5, True))
+    assert not (st.knnV == np.arange(len(S))[:, None]).any()
+    st0 = orc.LloydState(S, 12)
+    orc.lloyd_step(mesh, st0, K=12, R=R)
+    assert np.array_equal(st0.knnV, np_knn(mesh.V, out["cent"], 5, False))
+    assert (st0.knnV[:, 0] == np.arange(len(S))).mean() > 0.9
+
+
+def test_lloyd_kproj10_lists_and_projection(orc):
+    """kproj = 10 (scripts/testfreeze.py:379): the lists are the independent
+    brute force, and on a convex mesh the projection through the 10 nearest
+    vertices' faces is the global nearest point (distance from a numpy brute
+    force over every face)."""
+    V, F = mg.icosphere(3)
+    mesh = orc.Mesh(V, F)
+    S = mg.sample_surface(V, F, 300, seed=8)
+    st = orc.LloydState(S, 12, kproj=10)
+    out = orc.lloyd_step(mesh, st, K=12, kproj=10, R=np.float32(mesh.bbox_max_edge))
+    C = out["cent"]
+    assert np.array_equal(st.knnV, np_knn(mesh.V, C, 10, False))
+    d_proj = np.linalg.norm(out["Snew"] - C, axis=1)
+    d_all = np.sqrt(np_point_tri_min(C, mesh.V, mesh.F))
+    np.testing.assert_allclose(d_proj, d_all, rtol=1e-9, atol=1e-12)
+
+
+def np_point_tri_min(Q, V, F):
+    """Squared distance from each query to the nearest of all triangles, by
+    sampling-free brute force: the plane foot when it falls inside (barycentric
+    test), else the nearest point on the three edges (independent of the
+    oracle's region code)."""
+    A, B, Cc = V[F[:, 0]], V[F[:, 1]], V[F[:, 2]]
+    out = np.empty(len(Q))
+    for i, q in enumerate(Q):
+        n = np.cross(B - A, Cc - A)
+        nn = np.einsum("ij,ij->i", n, n)
+        t = np.einsum("ij,ij->i", q - A, n) / nn
+        foot = q - t[:, None] * n
+        def bary_in(P):
+            v0, v1, v2 = B - A, Cc - A, P - A
+            d00 = np.einsum("ij,ij->i", v0, v0); d01 = np.einsum("ij,ij->i", v0, v1)
+            d11 = np.einsum("ij,ij->i", v1, v1); d20 = np.einsum("ij,ij->i", v2, v0)
+            d21 = np.einsum("ij,ij->i", v2, v1)
+            den = d00 * d11 - d01 * d01
+            v = (d11 * d20 - d01 * d21) / den
+            w = (d00 * d21 - d01 * d20) / den
+            return (v >= 0) & (w >= 0) & (v + w <= 1)
+        best = np.where(bary_in(foot), np.einsum("ij,ij->i", q - foot, q - foot), np.inf)
+        for P0, P1 in ((A, B), (B, Cc), (Cc, A)):
+            e = P1 - P0
+            u = np.clip(np.einsum("ij,ij->i", q - P0, e) / np.einsum("ij,ij->i", e, e), 0.0, 1.0)
+            X = P0 + u[:, None] * e
+            best = np.minimum(best, np.einsum("ij,ij->i", q - X, q - X))
+        out[i] = best.min()
+    return out
 
 
 # --------------------------------------------------------------------- remesh
