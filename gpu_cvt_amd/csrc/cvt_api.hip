@@ -182,6 +182,7 @@ struct cvt_job {
     float4* cent = nullptr;   // centroid and cell flags per site
     int* vs_list = nullptr;   // FP64 pass: sites waiting for a wave-wide vertex search
     int* excl_vslot = nullptr;  // exclude_self_vertex: per site, the vertex-grid slot its kproj search skips
+    int* inv_dev = nullptr;     // caller's site id -> internal index (made on first use)
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
@@ -1003,6 +1004,99 @@ int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, do
     if (precise_ms) *precise_ms = pr / iters;
     if (iter_ms) *iter_ms = t / iters;
     return 0;
+}
+
+int cvt_job_profile_stages(cvt_job* j, int iters, cvt_stage_ms* out) {
+    if (!j || !out) return fail(CVT_EINVAL, "null argument");
+    if (iters <= 0) return fail(CVT_EINVAL, "iters must be > 0");
+    hipStream_t s = j->stream;
+    hipEvent_t ev[6];
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int it = 0; it < iters; ++it) {
+        HIPCHK(hipEventRecord(ev[0], s));
+        launch_batch_grid(j->tables, 0, s);
+        HIPCHK(hipEventRecord(ev[1], s));
+        launch_batch_lloyd_staged(j->tables, j->p.K, j->p.kproj, s, ev[2], ev[3]);
+        HIPCHK(hipEventRecord(ev[4], s));
+        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 2);
+        launch_batch_end(j->tables, s);
+        HIPCHK(hipEventRecord(ev[5], s));
+        HIPCHK(hipEventSynchronize(ev[5]));
+        float f[5] = {0, 0, 0, 0, 0}, tot = 0;
+        for (int k = 0; k < 5; ++k) HIPCHK(hipEventElapsedTime(&f[k], ev[k], ev[k + 1]));
+        HIPCHK(hipEventElapsedTime(&tot, ev[0], ev[5]));
+        for (int k = 0; k < 5; ++k) acc[k] += f[k];
+        acc[5] += tot;
+        j->cur ^= 1;
+    }
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    HIPCHK(hipGetLastError());
+    j->iters_done += iters;
+    std::memset(out, 0, sizeof *out);
+    out->grid = acc[0] / iters;
+    out->frame = acc[1] / iters;
+    out->cell = acc[2] / iters;
+    out->project = acc[3] / iters;
+    out->precise = acc[4] / iters;
+    out->iter = acc[5] / iters;
+    return 0;
+}
+
+// Triangle quality of P (device points) over host faces F (ids < n; map:
+// device id -> row of P, nullable) on stream s.
+static int quality_device(const float4* P, int n, const int* map, const int* F, int nF, hipStream_t s,
+                          cvt_quality* out) {
+    for (long k = 0; k < 3L * nF; ++k)
+        if (F[k] < 0 || F[k] >= n) return fail(CVT_EINVAL, "face %ld: vertex id %d out of range", k / 3, F[k]);
+    TempPool T;
+    int* dF = (int*)T.get(sizeof(int) * 3 * (size_t)std::max(nF, 1));
+    auto* part = (QualityPart*)T.get(sizeof(QualityPart) * kQualityBlocks);
+    auto* dout = (QualityOut*)T.get(sizeof(QualityOut));
+    if (!dF || !part || !dout) return fail(CVT_ENOMEM, "quality temporaries");
+    if (nF > 0) HIPCHK(hipMemcpyAsync(dF, F, sizeof(int) * 3 * (size_t)nF, hipMemcpyHostToDevice, s));
+    launch_quality(P, map, dF, nF, part, dout, s);
+    QualityOut h;
+    HIPCHK(hipMemcpyAsync(&h, dout, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    std::memset(out, 0, sizeof *out);
+    out->Qmin = h.Qmin;
+    out->Qavg = h.Qavg;
+    out->theta_min = h.theta_min;
+    out->theta_min_avg = h.theta_min_avg;
+    out->theta_lt_30_pct = h.lt30;
+    out->theta_gt_90_pct = h.gt90;
+    out->nF = nF;
+    return 0;
+}
+
+int cvt_mesh_quality(const double* V, int nV, const int* F, int nF, cvt_quality* out) {
+    if (!V || !F || !out || nV < 0 || nF < 0) return fail(CVT_EINVAL, "bad argument");
+    std::vector<float4> P((size_t)std::max(nV, 1));
+    for (int k = 0; k < nV; ++k)
+        P[k] = make_float4((float)V[3 * k], (float)V[3 * k + 1], (float)V[3 * k + 2], 0.f);
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    TempPool T;
+    auto* dP = (float4*)T.get(sizeof(float4) * P.size());
+    int rc = dP ? 0 : fail(CVT_ENOMEM, "quality points");
+    if (!rc && hipMemcpyAsync(dP, P.data(), sizeof(float4) * P.size(), hipMemcpyHostToDevice, s) != hipSuccess)
+        rc = fail(CVT_EHIP, "quality upload failed");
+    if (!rc) rc = quality_device(dP, nV, nullptr, F, nF, s, out);
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    return rc;
+}
+
+int cvt_job_quality(cvt_job* j, const int* F, int nF, cvt_quality* out) {
+    if (!j || !F || !out || nF < 0) return fail(CVT_EINVAL, "bad argument");
+    if (!j->inv_dev) {
+        int rc = j->alloc(&j->inv_dev, j->nS);
+        if (rc) return rc;
+        HIPCHK(hipMemcpy(j->inv_dev, j->inv.data(), sizeof(int) * j->nS, hipMemcpyHostToDevice));
+    }
+    HIPCHK(hipStreamSynchronize(j->stream));
+    return quality_device(j->S[j->cur], j->nS, j->inv_dev, F, nF, j->stream, out);
 }
 
 int cvt_job_get_sites(cvt_job* j, double* S) {

@@ -7,10 +7,27 @@
 // bbox edge; every 10th iteration (DUMP_STRIDE) writes iter_%03d.obj,
 // iter_%03d_sites.xyz and iter_%03d_frozen.txt and appends a row to
 // eval_iters.csv under experiments/output/{gpucvt|freeze|freeze_tiered}/<mesh>/;
-// the run appends a row to experiments/runs.csv.  The remeshed surface is the
-// library's own dual-triangle extraction (the reference's non-Geogram path,
-// main.cu:928-974), since the external vorpalite binary is not part of this
-// build.
+// the run appends a row to experiments/runs.csv.  Mode 0 drops each site's own
+// vertex from its centroid -> mesh list (main.cu:728) and every remesh clips
+// by the current sites' bbox (R2, main.cu:899-909), as the reference.
+//
+// The remeshed surface is the library's own dual-triangle extraction (the
+// reference's non-Geogram path, main.cu:928-974).  Beside every
+// iter_%03d_sites.xyz the Geogram hand-off iter_%03d_co3ne.sh holds the
+// reference's vorpalite command and options (main.cu:314-340); with the
+// environment variable GPU_CVT_VORPALITE naming a vorpalite executable it is
+// run (outside the timed steps, as the reference) into iter_%03d_co3ne.obj,
+// and that surface -- the reference's default output -- is the one measured
+// in eval_iters.csv and runs.csv.
+//
+// Quality (evaluate.cuh:388-462) and the one-sided Hausdorff distance are
+// computed on the device; the stage columns come from per-kernel timing
+// (cvt_job_profile_stages): knn_sites_ms is the site-grid rebuild,
+// knn_site_to_mesh_ms the frame kernel (nearest vertex and tangent frame, so
+// uv_from_mesh_ms is 0), centroids_ms the cell kernel (site KNN, cell and
+// centroid) plus the FP64 pass, project_ms the projection kernel (kproj
+// search, reprojection and freeze test, so knn_centroid_to_mesh_ms and
+// freeze_ms are 0).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -92,58 +109,29 @@ void compact(std::vector<double>& V, std::vector<int>& F) {
     V.swap(V2);
 }
 
-struct Quality {
-    float Qmin, Qavg, theta_min, theta_min_avg, lt30, gt90;
-};
-
-// Triangle angles and the shape quality q = (6/sqrt 3) A / (s e_max) in FP32
-// (eval_quality_angles_cpu, evaluate.cuh:388-462).
-Quality quality(const std::vector<float>& P, const std::vector<int>& F) {
-    auto sub = [&](int a, int b, float* o) {
-        for (int k = 0; k < 3; ++k) o[k] = P[3 * (size_t)a + k] - P[3 * (size_t)b + k];
-    };
-    auto dot = [](const float* u, const float* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
-    auto angle = [&](const float* u, const float* v) {
-        const float den = std::fmax(std::sqrt(dot(u, u)) * std::sqrt(dot(v, v)), 1e-30f);
-        float cs = dot(u, v) / den;
-        cs = std::fmin(1.f, std::fmax(-1.f, cs));
-        return std::acos(cs) * (180.0f / 3.14159265358979323846f);
-    };
-    Quality q{1e30f, 0.f, 1e30f, 0.f, 0.f, 0.f};
-    double qsum = 0, thsum = 0, lt = 0, gt = 0, cnt = 0;
-    const size_t nF = F.size() / 3;
-    for (size_t f = 0; f < nF; ++f) {
-        const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
-        float ab[3], ac[3], ba[3], bc[3], ca[3], cb[3];
-        sub(b, a, ab); sub(c, a, ac); sub(a, b, ba); sub(c, b, bc); sub(a, c, ca); sub(b, c, cb);
-        const float A = angle(ab, ac), B = angle(ba, bc), C = angle(ca, cb);
-        const float tmin = std::fmin(A, std::fmin(B, C));
-        q.theta_min = std::fmin(q.theta_min, tmin);
-        thsum += tmin;
-        lt += (A < 30.f) + (B < 30.f) + (C < 30.f);
-        gt += (A > 90.f) + (B > 90.f) + (C > 90.f);
-        cnt += 3;
-        const float l0 = std::sqrt(dot(ab, ab)), l1 = std::sqrt(dot(bc, bc)), l2 = std::sqrt(dot(ca, ca));
-        const float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2],
-                             ab[0] * ac[1] - ab[1] * ac[0]};
-        const float area = 0.5f * std::sqrt(dot(cr, cr));
-        const float s = 0.5f * (l0 + l1 + l2), e = std::fmax(l0, std::fmax(l1, l2));
-        const float qq = (6.0f / std::sqrt(3.0f)) * (area / std::fmax(s * e, 1e-30f));
-        q.Qmin = std::fmin(q.Qmin, qq);
-        qsum += qq;
-    }
-    q.Qavg = nF ? (float)(qsum / (double)nF) : 0.f;
-    q.theta_min_avg = nF ? (float)(thsum / (double)nF) : 0.f;
-    q.lt30 = cnt > 0 ? (float)(lt * 100.0 / cnt) : 0.f;
-    q.gt90 = cnt > 0 ? (float)(gt * 100.0 / cnt) : 0.f;
-    return q;
-}
-
 const char* kEvalHeader =
     "mesh,mode,iter,Qmin,Qavg,theta_min,theta_min_avg,theta_lt_30_pct,theta_gt_90_pct,dH,iter_remesh_ms,"
     "total_remesh_ms,freeze_cell_num,n_vertices,freeze_pct,knn_sites_ms,knn_site_to_mesh_ms,uv_from_mesh_ms,"
     "centroids_ms,knn_centroid_to_mesh_ms,project_ms,freeze_ms,count_same,count_low_disp,count_both,"
     "count_newly_frozen,blocked_by_same,blocked_by_low_disp\n";
+
+// The vorpalite command of run_vorpalite (main.cu:316-340): co3ne
+// reconstruction of the site cloud with the reference's options.  quote:
+// escape the executable's characters (false: wrap it in double quotes as
+// given, so that a shell parameter expansion in it still expands).
+std::string co3ne_command(const std::string& exe, const std::string& xyz, const std::string& obj, bool quote = true) {
+    auto q = [](const std::string& x) {
+        std::string r = "\"";
+        for (char c : x) {
+            if (c == '"' || c == '\\') r += '\\';
+            r += c;
+        }
+        return r + "\"";
+    };
+    return (quote ? q(exe) : "\"" + exe + "\"") + " " + q(xyz) + " " + q(obj) +
+           " co3ne=true pre=false remesh=false post=true co3ne:repair=true co3ne:radius=5%" +
+           " co3ne:nb_neighbors=30 log:quiet=true log:pretty=false";
+}
 
 int fail_msg(const char* what) {
     std::fprintf(stderr, "gpu_cvt: %s: %s\n", what, cvt_last_error());
@@ -188,6 +176,8 @@ int main(int argc, char** argv) {
     p.clip_R = R;
     p.freeze_disp_abs2 = freeze_disp;
     p.freeze_need = 5;
+    p.exclude_self_vertex = mode == 0 ? 1 : 0;  // knn_vertices_bruteforce_k<KPROJ, true> (main.cu:728)
+    p.remesh_R_mode = 1;                         // R2 from the current sites (main.cu:899-909)
     if (mode == 0) {
         p.freeze_mode = 0;
     } else {
@@ -220,15 +210,17 @@ int main(int argc, char** argv) {
     double total_ms = 0;
     int used_iters = 0, final_nf = 0;
     float converge = 0.f;
+    const char* vorpalite = std::getenv("GPU_CVT_VORPALITE");
     for (int iter = 0; iter < total_iter; ++iter) {
         std::printf("\n=== iter %d ===\n", iter);
-        double g = 0, l = 0, pr = 0, t = 0;
-        if (cvt_job_profile(job, 1, &g, &l, &pr, &t) != CVT_OK) return fail_msg("step");
+        cvt_stage_ms sm;
+        if (cvt_job_profile_stages(job, 1, &sm) != CVT_OK) return fail_msg("step");
         cvt_iter_stats st;
         if (cvt_job_stats(job, 1, &st) != CVT_OK) return fail_msg("stats");
-        total_ms += t;
-        std::printf("grid_ms %.3f lloyd_ms %.3f precise_ms %.3f iter_remesh_ms %.3f (total_remesh_ms %.3f)\n", g, l,
-                    pr, t, total_ms);
+        total_ms += sm.iter;
+        std::printf("knn_sites_ms %.3f\nknn_site_to_mesh_ms %.3f\ncentroids_ms %.3f\nproject_ms %.3f\n"
+                    "iter_remesh_ms %.3f (total_remesh_ms %.3f)\n",
+                    sm.grid, sm.frame, sm.cell + sm.precise, sm.project, sm.iter, total_ms);
         std::printf("same neighbors %d\nlow displacement %d\nboth %d\npass2 %d\n", st.count_same, st.count_low_disp,
                     st.count_both, st.count_newly_frozen);
         converge = (float)st.frozen_total / (float)nV;
@@ -244,9 +236,9 @@ int main(int argc, char** argv) {
         if (cvt_job_remesh(job, 1e-6, &Fr, &nFr) != CVT_OK) return fail_msg("remesh");
         std::vector<int> Fn(Fr, Fr + 3L * nFr);
         cvt_free(Fr);
-        final_nf = nFr;
         char path[1024];
         std::snprintf(path, sizeof path, "%s/iter_%03d_sites.xyz", out_dir.c_str(), iter);
+        const std::string xyz_path = path;
         {
             std::ofstream os(path);
             for (int i = 0; i < nV; ++i) os << Sf[3 * i] << " " << Sf[3 * i + 1] << " " << Sf[3 * i + 2] << "\n";
@@ -264,22 +256,67 @@ int main(int argc, char** argv) {
             std::ofstream os(path);
             for (int i = 0; i < nV; ++i) os << (int)frozen[i] << "\n";
         }
-        const Quality q = quality(Sf, Fn);
+        // the Geogram hand-off: the reference's command and options
+        std::snprintf(path, sizeof path, "%s/iter_%03d_co3ne.obj", out_dir.c_str(), iter);
+        const std::string co3ne_obj = path;
+        const std::string cmd = co3ne_command(vorpalite ? vorpalite : "vorpalite", xyz_path, co3ne_obj);
+        std::snprintf(path, sizeof path, "%s/iter_%03d_co3ne.sh", out_dir.c_str(), iter);
+        {
+            std::ofstream os(path);
+            namespace fs = std::filesystem;
+            os << "#!/bin/sh\ncd \"$(dirname \"$0\")\" && "
+               << co3ne_command("${VORPALITE:-vorpalite}", fs::path(xyz_path).filename().string(),
+                                fs::path(co3ne_obj).filename().string(), false)
+               << "\n";
+        }
+        cvt_quality q;
         double dH = 0;
-        if (cvt_job_hausdorff(job, 1e-6, &dH, nullptr) != CVT_OK) return fail_msg("hausdorff");
-        // stage columns: the FP32 pass (frame, cell and projection kernels,
-        // which do KNN, frame, cell, projection and freeze) is reported as
-        // centroids_ms, the site grid rebuild as knn_sites_ms
+        int nf_eval = nFr;
+        if (vorpalite) {
+            const int vret = std::system(cmd.c_str());
+            if (vret != 0) {
+                std::fprintf(stderr, "vorpalite failed (exit %d)\n", vret);
+                return 1;
+            }
+            std::vector<double> Vg;
+            std::vector<int> Fg;
+            float Rg = 0.f;
+            if (!load_obj(co3ne_obj, Vg, Fg, Rg)) return fail_msg("co3ne surface");
+            const int nVg = (int)(Vg.size() / 3);
+            nf_eval = (int)(Fg.size() / 3);
+            if (cvt_mesh_quality(Vg.data(), nVg, Fg.data(), nf_eval, &q) != CVT_OK) return fail_msg("quality");
+            // one-sided distance over the probes of evaluate.cuh:252-283
+            std::vector<double> probes(Vg);
+            for (int f = 0; f < nf_eval; ++f) {
+                const double* a = &Vg[3 * (size_t)Fg[3 * f]];
+                const double* b = &Vg[3 * (size_t)Fg[3 * f + 1]];
+                const double* c = &Vg[3 * (size_t)Fg[3 * f + 2]];
+                for (int k = 0; k < 3; ++k) probes.push_back((a[k] + b[k] + c[k]) / 3.0);
+                for (int k = 0; k < 3; ++k) probes.push_back(0.5 * (a[k] + b[k]));
+                for (int k = 0; k < 3; ++k) probes.push_back(0.5 * (b[k] + c[k]));
+                for (int k = 0; k < 3; ++k) probes.push_back(0.5 * (c[k] + a[k]));
+            }
+            std::vector<double> d2(probes.size() / 3);
+            if (!d2.empty() &&
+                cvt_job_point_mesh_d2(job, probes.data(), (int)d2.size(), d2.data()) != CVT_OK)
+                return fail_msg("hausdorff");
+            for (double v : d2) dH = std::max(dH, std::sqrt(v));
+            std::printf("iter %d co3ne surface: %d vertices, %d faces\n", iter, nVg, nf_eval);
+        } else {
+            if (cvt_job_quality(job, Fn.data(), nFr, &q) != CVT_OK) return fail_msg("quality");
+            if (cvt_job_hausdorff(job, 1e-6, &dH, nullptr) != CVT_OK) return fail_msg("hausdorff");
+        }
+        final_nf = nf_eval;
         const int blocked_same = std::max(0, st.count_low_disp - st.count_both);
         const int blocked_low = std::max(0, st.count_same - st.count_both);
         std::ofstream out(eval_csv, std::ios::app);
-        out << mesh_name << "," << root_dir << "," << iter << "," << q.Qmin << "," << q.Qavg << "," << q.theta_min << ","
-            << q.theta_min_avg << "," << q.lt30 << "," << q.gt90 << "," << (float)dH << "," << (float)t << ","
-            << (float)total_ms << "," << st.frozen_total << "," << nV << ","
-            << (nV > 0 ? 100.f * (float)st.frozen_total / (float)nV : 0.f) << "," << (float)g << "," << 0.f << ","
-            << 0.f << "," << (float)(l + pr) << "," << 0.f << "," << 0.f << "," << 0.f << "," << st.count_same << ","
-            << st.count_low_disp << "," << st.count_both << "," << st.count_newly_frozen << "," << blocked_same << ","
-            << blocked_low << "\n";
+        out << mesh_name << "," << root_dir << "," << iter << "," << q.Qmin << "," << q.Qavg << "," << q.theta_min
+            << "," << q.theta_min_avg << "," << q.theta_lt_30_pct << "," << q.theta_gt_90_pct << "," << (float)dH
+            << "," << (float)sm.iter << "," << (float)total_ms << "," << st.frozen_total << "," << nV << ","
+            << (nV > 0 ? 100.f * (float)st.frozen_total / (float)nV : 0.f) << "," << (float)sm.grid << ","
+            << (float)sm.frame << "," << 0.f << "," << (float)(sm.cell + sm.precise) << "," << 0.f << ","
+            << (float)sm.project << "," << 0.f << "," << st.count_same << "," << st.count_low_disp << ","
+            << st.count_both << "," << st.count_newly_frozen << "," << blocked_same << "," << blocked_low << "\n";
     }
     std::printf("\nTotal remeshing time (mesh rebuilding excluded): %.3f ms\n", total_ms);
     {

@@ -207,6 +207,24 @@ constexpr int kLloydFrameDone = 4;
 void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3,
                         const SideStream* side = nullptr);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
+// The FP32 pass kernel by kernel, with an event recorded after the frame and
+// after the cell kernel (stage timing).
+void launch_batch_lloyd_staged(const BatchTables& bt, int K, int kproj, hipStream_t s, hipEvent_t after_frame,
+                               hipEvent_t after_cell);
+// Triangle quality (eval_quality_angles_cpu, evaluate.cuh:388-462): per-block
+// partials, then one block's fixed-order reduction into *out.  P: points
+// (x, y, z, -); map (nullable): F's ids -> rows of P.
+struct QualityPart {
+    double qsum, tsum;
+    float qmin, tmin;
+    int lt30, gt90;
+};
+struct QualityOut {
+    float Qmin, Qavg, theta_min, theta_min_avg, lt30, gt90;
+};
+constexpr int kQualityBlocks = 512;
+void launch_quality(const float4* P, const int* map, const int* F, int nF, QualityPart* part, QualityOut* out,
+                    hipStream_t s);
 // R_dev (nullable): a device float that replaces the job's R as the clip half-size.
 void launch_remesh_cells(const JobDesc* desc, int nS, int K, int* knn_tmp, unsigned long long* keys,
                          float4* site_n, const float* R_dev, hipStream_t s);

@@ -803,6 +803,117 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
     }
 }
 
+void launch_batch_lloyd_staged(const BatchTables& bt, int K, int kproj, hipStream_t s, hipEvent_t after_frame,
+                               hipEvent_t after_cell) {
+    launch_frame(bt, kproj, 0, s);
+    (void)hipEventRecord(after_frame, s);
+    launch_lloyd_part(bt, K, kproj, 0, 0, s);
+    (void)hipEventRecord(after_cell, s);
+    launch_project(bt, kproj, s);
+}
+
+// ------------------------------------------------------------ triangle quality
+// Per triangle, in FP32 with every operation rounded on its own (no fused
+// multiply-add), in the reference's order of operations: the angle at each
+// corner from the two edge vectors leaving it, q = (6 / sqrt 3) * (A / max(s
+// e_max, 1e-30)) with A = |ab x ac| / 2, s the half perimeter, e_max the
+// longest edge.  Sums are FP64 as in the reference (their order differs: a
+// fixed tree here, face order there).
+__device__ __forceinline__ float quality_angle_deg(float ux, float uy, float uz, float vx, float vy, float vz) {
+#pragma clang fp contract(off)
+    const float du = sqrtf(ux * ux + uy * uy + uz * uz);
+    const float dv = sqrtf(vx * vx + vy * vy + vz * vz);
+    const float denom = fmaxf(du * dv, 1e-30f);
+    float cs = (ux * vx + uy * vy + uz * vz) / denom;
+    cs = fminf(1.f, fmaxf(-1.f, cs));
+    return acosf(cs) * (180.0f / 3.14159265358979323846f);
+}
+
+__global__ __launch_bounds__(256) void k_face_quality(const float4* __restrict__ P, const int* __restrict__ map,
+                                                      const int* __restrict__ F, int nF,
+                                                      QualityPart* __restrict__ part) {
+#pragma clang fp contract(off)
+    double qs = 0.0, ts = 0.0;
+    float qmin = 1e30f, tmin = 1e30f;
+    int lt = 0, gt = 0;
+    for (int f = blockIdx.x * 256 + (int)threadIdx.x; f < nF; f += gridDim.x * 256) {
+        int ia = F[3 * f], ib = F[3 * f + 1], ic = F[3 * f + 2];
+        if (map) { ia = map[ia]; ib = map[ib]; ic = map[ic]; }
+        const float4 a = P[ia], b = P[ib], c = P[ic];
+        const float abx = b.x - a.x, aby = b.y - a.y, abz = b.z - a.z;
+        const float acx = c.x - a.x, acy = c.y - a.y, acz = c.z - a.z;
+        const float bax = a.x - b.x, bay = a.y - b.y, baz = a.z - b.z;
+        const float bcx = c.x - b.x, bcy = c.y - b.y, bcz = c.z - b.z;
+        const float cax = a.x - c.x, cay = a.y - c.y, caz = a.z - c.z;
+        const float cbx = b.x - c.x, cby = b.y - c.y, cbz = b.z - c.z;
+        const float A = quality_angle_deg(abx, aby, abz, acx, acy, acz);
+        const float B = quality_angle_deg(bax, bay, baz, bcx, bcy, bcz);
+        const float C = quality_angle_deg(cax, cay, caz, cbx, cby, cbz);
+        const float t = fminf(A, fminf(B, C));
+        tmin = fminf(tmin, t);
+        ts += (double)t;
+        lt += (A < 30.f) + (B < 30.f) + (C < 30.f);
+        gt += (A > 90.f) + (B > 90.f) + (C > 90.f);
+        const float l0 = sqrtf(abx * abx + aby * aby + abz * abz);
+        const float l1 = sqrtf(bcx * bcx + bcy * bcy + bcz * bcz);
+        const float l2 = sqrtf(cax * cax + cay * cay + caz * caz);
+        const float crx = aby * acz - abz * acy, cry = abz * acx - abx * acz, crz = abx * acy - aby * acx;
+        const float area = 0.5f * sqrtf(crx * crx + cry * cry + crz * crz);
+        const float S = 0.5f * (l0 + l1 + l2);
+        const float E = fmaxf(l0, fmaxf(l1, l2));
+        const float denom = fmaxf(S * E, 1e-30f);
+        const float q = (6.0f / sqrtf(3.0f)) * (area / denom);
+        qmin = fminf(qmin, q);
+        qs += (double)q;
+    }
+    __shared__ QualityPart red[256];
+    red[threadIdx.x] = QualityPart{qs, ts, qmin, tmin, lt, gt};
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            QualityPart& x = red[threadIdx.x];
+            const QualityPart& y = red[threadIdx.x + w];
+            x.qsum += y.qsum;
+            x.tsum += y.tsum;
+            x.qmin = fminf(x.qmin, y.qmin);
+            x.tmin = fminf(x.tmin, y.tmin);
+            x.lt30 += y.lt30;
+            x.gt90 += y.gt90;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_quality_final(const QualityPart* __restrict__ part, int nb, int nF, QualityOut* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double qs = 0.0, ts = 0.0;
+    float qmin = 1e30f, tmin = 1e30f;
+    long lt = 0, gt = 0;
+    for (int b = 0; b < nb; ++b) {
+        qs += part[b].qsum;
+        ts += part[b].tsum;
+        qmin = fminf(qmin, part[b].qmin);
+        tmin = fminf(tmin, part[b].tmin);
+        lt += part[b].lt30;
+        gt += part[b].gt90;
+    }
+    const double cnt = 3.0 * nF;
+    out->Qmin = qmin;
+    out->Qavg = nF > 0 ? (float)(qs / (double)nF) : 0.f;
+    out->theta_min = tmin;
+    out->theta_min_avg = nF > 0 ? (float)(ts / (double)nF) : 0.f;
+    out->lt30 = cnt > 0.0 ? (float)((double)lt * 100.0 / cnt) : 0.f;
+    out->gt90 = cnt > 0.0 ? (float)((double)gt * 100.0 / cnt) : 0.f;
+}
+
+void launch_quality(const float4* P, const int* map, const int* F, int nF, QualityPart* part, QualityOut* out,
+                    hipStream_t s) {
+    const int nb = std::max(1, std::min(kQualityBlocks, nblk(nF, 256)));
+    hipLaunchKernelGGL(k_face_quality, dim3(nb), dim3(256), 0, s, P, map, F, nF, part);
+    hipLaunchKernelGGL(k_quality_final, dim3(1), dim3(64), 0, s, part, nb, nF, out);
+}
+
 void launch_batch_end(const BatchTables& bt, hipStream_t s) {
     hipLaunchKernelGGL(kb_end, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 }

@@ -30,6 +30,7 @@ EXPORTS = [
     "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
     "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
     "cvt_job_compute_tiers", "cvt_job_hausdorff_within", "cvt_bvh_point_d2_noalloc",
+    "cvt_job_profile_stages", "cvt_mesh_quality", "cvt_job_quality",
 ]
 
 
@@ -105,6 +106,20 @@ class ServerInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class StageMs(C.Structure):
+    _fields_ = [("grid", C.c_double), ("frame", C.c_double), ("cell", C.c_double), ("project", C.c_double),
+                ("precise", C.c_double), ("iter", C.c_double), ("reserved", C.c_double * 2)]
+
+
+class Quality(C.Structure):
+    _fields_ = [("Qmin", C.c_float), ("Qavg", C.c_float), ("theta_min", C.c_float), ("theta_min_avg", C.c_float),
+                ("theta_lt_30_pct", C.c_float), ("theta_gt_90_pct", C.c_float), ("nF", C.c_int),
+                ("reserved", C.c_int * 5)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class CVTError(RuntimeError):
     def __init__(self, fn: str, code: int, msg: str):
         super().__init__(f"{fn} failed ({STATUS.get(code, code)}): {msg}")
@@ -166,6 +181,9 @@ def lib():
     L.cvt_job_profile.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
+    L.cvt_job_profile_stages.argtypes = [C.c_void_p, C.c_int, C.POINTER(StageMs)]
+    L.cvt_mesh_quality.argtypes = [_dp, C.c_int, _ip, C.c_int, C.POINTER(Quality)]
+    L.cvt_job_quality.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(Quality)]
     L.cvt_job_hausdorff.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cvt_job_hausdorff_within.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int),
                                            C.POINTER(C.c_double), C.POINTER(C.c_int)]
@@ -292,6 +310,20 @@ class Job:
         _check("cvt_job_profile", lib().cvt_job_profile(self.handle, iters, C.byref(g), C.byref(l), C.byref(pr),
                                                         C.byref(t)))
         return {"grid_ms": g.value, "lloyd_ms": l.value, "precise_ms": pr.value, "iter_ms": t.value}
+
+    def profile_stages(self, iters: int) -> dict:
+        """Mean device ms per iteration of each kernel stage (grid, frame, cell,
+        project, precise) and of the whole iteration."""
+        o = StageMs()
+        _check("cvt_job_profile_stages", lib().cvt_job_profile_stages(self.handle, iters, C.byref(o)))
+        return {k: getattr(o, k) for k, _ in o._fields_ if k != "reserved"}
+
+    def quality(self, F) -> dict:
+        """Triangle quality (evaluate.cuh:388-462) of faces F over the current sites, on the device."""
+        F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+        o = Quality()
+        _check("cvt_job_quality", lib().cvt_job_quality(self.handle, F.reshape(-1), len(F), C.byref(o)))
+        return o.as_dict()
 
     # ----------------------------------------------------------------- state
     def sites(self) -> np.ndarray:
@@ -428,6 +460,15 @@ class BVH:
             self.close()
         except Exception:
             pass
+
+
+def mesh_quality(V, F) -> dict:
+    """Triangle quality (evaluate.cuh:388-462) of (V, F) on the device, V rounded to FP32."""
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+    o = Quality()
+    _check("cvt_mesh_quality", lib().cvt_mesh_quality(V.reshape(-1), len(V), F.reshape(-1), len(F), C.byref(o)))
+    return o.as_dict()
 
 
 def hausdorff(V1, F1, V2, F2, n_samples: int, seed: int, symmetric: bool = True) -> float:

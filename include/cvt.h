@@ -118,6 +118,20 @@ int cvt_job_step_timed(cvt_job* job, int iters, double* device_ms);
 int cvt_job_profile(cvt_job* job, int iters, double* grid_ms, double* lloyd_ms, double* precise_ms,
                     double* iter_ms);
 
+/* The same per kernel stage: mean device milliseconds per iteration of the
+ * site-grid rebuild, the frame kernel (site -> nearest mesh vertex, tangent
+ * frame: the reference's knn_site_to_mesh + uv_from_mesh), the cell kernel
+ * (site KNN, tangent-plane cell, centroid: knn_sites + centroids), the
+ * projection kernel (centroid -> mesh kproj-NN, reprojection, freeze test:
+ * knn_centroid_to_mesh + project + freeze), the FP64 pass and the whole
+ * iteration.  Replaces the stage columns of eval_iters.csv (main.cu:692-803,
+ * evaluate.cuh:468-550). */
+typedef struct cvt_stage_ms {
+    double grid, frame, cell, project, precise, iter;
+    double reserved[2];
+} cvt_stage_ms;
+int cvt_job_profile_stages(cvt_job* job, int iters, cvt_stage_ms* out);
+
 /* State access (host arrays).  knn rows are in (d^2, index) order; knnv are
  * the pinned/last vertex lists (nS*kproj); frozen/streak one byte per site. */
 int cvt_job_get_sites(cvt_job* job, double* S_out);
@@ -176,6 +190,22 @@ void cvt_free(void* p);
  * hausdorff_cand_to_ref_gpu (evaluate.cuh:359-383).  nF_out (nullable): the
  * remeshed triangle count. */
 int cvt_job_hausdorff(cvt_job* job, double eps, double* h_out, int* nF_out);
+
+/* Triangle quality of a surface, on the device: per triangle the three
+ * angles (degrees, clamped acos) and q = (6 / sqrt 3) A / (s e_max) in FP32,
+ * then the minima, means and the shares of angles below 30 / above 90
+ * degrees (percent).  Restates eval_quality_angles_cpu (evaluate.cuh:388-462).
+ * Triangles with an index outside the vertex range are rejected (CVT_EINVAL). */
+typedef struct cvt_quality {
+    float Qmin, Qavg, theta_min, theta_min_avg, theta_lt_30_pct, theta_gt_90_pct;
+    int nF;
+    int reserved[5];
+} cvt_quality;
+/* V: nV*3 doubles (rounded to FP32 on the device, as the reference's float3 sites). */
+int cvt_mesh_quality(const double* V, int nV, const int* F, int nF, cvt_quality* out);
+/* Over the job's current (FP32) sites, F indexing sites by the caller's ids
+ * (e.g. the faces of cvt_job_remesh). */
+int cvt_job_quality(cvt_job* job, const int* F, int nF, cvt_quality* out);
 
 /* Convergence test for the same one-sided distance: *within = 1 iff every
  * probe lies within tol of the input mesh (i.e. cvt_job_hausdorff <= tol), by

@@ -908,4 +908,52 @@ int orc_remesh(const double* S, int nS, const double* U, const double* Vv, const
     return m;
 }
 
+// Triangle quality in FP32, eval_quality_angles_cpu (evaluate.cuh:388-462)
+// restated: per triangle the corner angles from the edge vectors leaving each
+// corner (clamped acos, degrees), q = (6/sqrt 3) A / max(s e_max, 1e-30); FP64
+// sums in face order.  out: Qmin, Qavg, theta_min, theta_min_avg, % angles
+// < 30, % angles > 90.  P: nP*3 floats.  (Built without contraction: every
+// FP32 operation rounds on its own, as the reference's host code.)
+void orc_quality(const float* P, int nP, const int* F, int nF, float* out) {
+    auto sub = [&](int a, int b, float* o) {
+        for (int k = 0; k < 3; ++k) o[k] = P[3 * (long)a + k] - P[3 * (long)b + k];
+    };
+    auto dot = [](const float* u, const float* v) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; };
+    auto angle = [&](const float* u, const float* v) {
+        const float du = std::sqrt(dot(u, u)), dv = std::sqrt(dot(v, v));
+        const float den = std::fmax(du * dv, 1e-30f);
+        float cs = dot(u, v) / den;
+        cs = std::fmin(1.f, std::fmax(-1.f, cs));
+        return std::acos(cs) * (180.0f / 3.14159265358979323846f);
+    };
+    float qmin = 1e30f, tmin = 1e30f;
+    double qsum = 0, tsum = 0, lt = 0, gt = 0, cnt = 0;
+    for (int f = 0; f < nF; ++f) {
+        const int a = F[3 * f], b = F[3 * f + 1], c = F[3 * f + 2];
+        float ab[3], ac[3], ba[3], bc[3], ca[3], cb[3];
+        sub(b, a, ab); sub(c, a, ac); sub(a, b, ba); sub(c, b, bc); sub(a, c, ca); sub(b, c, cb);
+        const float A = angle(ab, ac), B = angle(ba, bc), C = angle(ca, cb);
+        const float t = std::fmin(A, std::fmin(B, C));
+        tmin = std::fmin(tmin, t);
+        tsum += t;
+        lt += (A < 30.f) + (B < 30.f) + (C < 30.f);
+        gt += (A > 90.f) + (B > 90.f) + (C > 90.f);
+        cnt += 3;
+        const float l0 = std::sqrt(dot(ab, ab)), l1 = std::sqrt(dot(bc, bc)), l2 = std::sqrt(dot(ca, ca));
+        const float cr[3] = {ab[1] * ac[2] - ab[2] * ac[1], ab[2] * ac[0] - ab[0] * ac[2], ab[0] * ac[1] - ab[1] * ac[0]};
+        const float area = 0.5f * std::sqrt(dot(cr, cr));
+        const float sp = 0.5f * (l0 + l1 + l2), e = std::fmax(l0, std::fmax(l1, l2));
+        const float q = (6.0f / std::sqrt(3.0f)) * (area / std::fmax(sp * e, 1e-30f));
+        qmin = std::fmin(qmin, q);
+        qsum += q;
+    }
+    (void)nP;
+    out[0] = qmin;
+    out[1] = nF ? (float)(qsum / (double)nF) : 0.f;
+    out[2] = tmin;
+    out[3] = nF ? (float)(tsum / (double)nF) : 0.f;
+    out[4] = cnt > 0 ? (float)(lt * 100.0 / cnt) : 0.f;
+    out[5] = cnt > 0 ? (float)(gt * 100.0 / cnt) : 0.f;
+}
+
 }  // extern "C"

@@ -64,6 +64,8 @@ def lib():
             _dp, _ip, _up, C.c_int, C.c_int, C.c_int, _ip, _ip, _up, _up, _ip, _dp, _dp, _dp, _dp, _dp, _dp, _ip]
         L.orc_remesh.argtypes = [_dp, C.c_int, _dp, _dp, _ip, C.c_int, C.c_double, _dp, C.c_double, _ip]
         L.orc_remesh.restype = C.c_int
+        L.orc_quality.argtypes = [np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS"), C.c_int, _ip,
+                                  C.c_int, np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")]
         L.orc_point_mesh_d2.argtypes = [C.c_int, _dp, C.c_int, _dp, C.c_int, _ip, C.c_int, _dp]
         L.orc_tiers.argtypes = [C.c_int, C.c_int, _ip, _ip, _dp, C.c_int, C.c_int, _up, _dp, _dp]
         _lib = L
@@ -250,6 +252,18 @@ def point_mesh_d2(P, V, F, use_bvh: bool = True):
     out = np.zeros(len(P))
     lib().orc_point_mesh_d2(len(P), P.reshape(-1), len(V), V.reshape(-1), len(F), F.reshape(-1), int(use_bvh), out)
     return out
+
+
+QUALITY_KEYS = ("Qmin", "Qavg", "theta_min", "theta_min_avg", "theta_lt_30_pct", "theta_gt_90_pct")
+
+
+def quality(P, F) -> dict:
+    """Triangle quality in FP32 (evaluate.cuh:388-462): P rounded to float32."""
+    P = np.ascontiguousarray(P, dtype=np.float32).reshape(-1, 3)
+    F = _i(F).reshape(-1, 3)
+    out = np.zeros(6, np.float32)
+    lib().orc_quality(P.reshape(-1), len(P), F.reshape(-1), len(F), out)
+    return dict(zip(QUALITY_KEYS, (float(x) for x in out)))
 
 
 def remesh(S, U, W, knn_, R: float, Nsite, eps: float = 1e-6):

@@ -18,6 +18,15 @@ EVAL_HEADER = ("mesh,mode,iter,Qmin,Qavg,theta_min,theta_min_avg,theta_lt_30_pct
                "count_low_disp,count_both,count_newly_frozen,blocked_by_same,blocked_by_low_disp")
 
 
+CO3NE_OPTS = ("co3ne=true pre=false remesh=false post=true co3ne:repair=true co3ne:radius=5% "
+              "co3ne:nb_neighbors=30 log:quiet=true log:pretty=false")
+
+
+def orc_quality(V, F):
+    from oracle import oracle as orc
+    return orc.quality(V, F)
+
+
 def _write_obj(path, V, F):
     with open(path, "w") as f:
         for v in V:
@@ -51,7 +60,8 @@ def test_cli_outputs_and_iterates(tmp_path, mode):
     Vf = np.asarray(V, np.float32)
     R = float((Vf.max(0) - Vf.min(0)).max())
     fd = np.float32(0.01) * np.float32(R) * np.float32(0.01) * np.float32(R)
-    kw = dict(K=32, clip_R_mode=1, clip_R=R, freeze_disp_abs2=float(fd), freeze_need=5)
+    kw = dict(K=32, clip_R_mode=1, clip_R=R, freeze_disp_abs2=float(fd), freeze_need=5,
+              exclude_self_vertex=int(mode == 0), remesh_R_mode=1)
     if mode:
         kw.update(freeze_mode=2, tier_streak=[5, 10, 15, 20, 25, 25] if mode == 1 else [5, 10, 15, 20, 25, 10],
                   tier_disp_abs2=[float(fd)] * 6)
@@ -65,3 +75,67 @@ def test_cli_outputs_and_iterates(tmp_path, mode):
     mesh_lines = (out / "iter_010.obj").read_text().split("\n")
     nf = sum(1 for ln in mesh_lines if ln.startswith("f "))
     assert nf > len(V)
+    # quality columns: the oracle's FP32 restatement on the written surface
+    Vo = np.array([[float(t) for t in ln.split()[1:4]] for ln in mesh_lines if ln.startswith("v ")])
+    Fo = np.array([[int(t) - 1 for t in ln.split()[1:4]] for ln in mesh_lines if ln.startswith("f ")])
+    q = orc_quality(Vo, Fo)
+    row = dict(zip(EVAL_HEADER.split(","), rows[1]))
+    for k in ("Qmin", "Qavg", "theta_min", "theta_min_avg"):
+        assert abs(float(row[k]) - q[k]) <= 2e-5 * abs(q[k]), (k, row[k], q[k])
+    for k in ("theta_lt_30_pct", "theta_gt_90_pct"):
+        assert abs(float(row[k]) - q[k]) <= 200.0 / (3 * nf) + 1e-4, (k, row[k], q[k])
+    # stage columns from per-kernel timing
+    for k in ("knn_sites_ms", "knn_site_to_mesh_ms", "centroids_ms", "project_ms", "iter_remesh_ms"):
+        assert float(row[k]) > 0, k
+    # the Geogram hand-off beside the sites
+    sh = (out / "iter_010_co3ne.sh").read_text()
+    assert '"iter_010_sites.xyz" "iter_010_co3ne.obj" ' + CO3NE_OPTS in sh
+
+
+FAKE_VORPALITE = """#!/bin/sh
+echo "$@" >> "$(dirname "$0")/args.log"
+head -4 "$1" | sed 's/^/v /' > "$2"
+printf 'f 1 2 3\\nf 1 4 2\\nf 1 3 4\\nf 2 4 3\\n' >> "$2"
+"""
+
+
+def test_cli_co3ne_handoff(tmp_path):
+    """With GPU_CVT_VORPALITE set, the command line runs it with the
+    reference's arguments (main.cu:316-340) on every dumped iteration and
+    measures the surface it returns, as the reference measures vorpalite's.
+    A stand-in executable records its arguments and returns a tetrahedron on
+    the first four sites."""
+    V, F = mg.displaced_sphere(10, seed=5, crease_seed=6)
+    obj = tmp_path / "m.obj"
+    _write_obj(obj, V, F)
+    fake = tmp_path / "vorpalite"
+    fake.write_text(FAKE_VORPALITE)
+    fake.chmod(0o755)
+    env = dict(os.environ, GPU_CVT_VORPALITE=str(fake))
+    print(f"[cli co3ne] nV={len(V)}", flush=True)
+    r = subprocess.run([EXE, str(obj), "1", "11"], capture_output=True, text=True, cwd=tmp_path, timeout=120,
+                       env=env)
+    assert r.returncode == 0, r.stderr[-2000:]
+    out = tmp_path / "experiments" / "output" / "freeze" / "m"
+    args = (tmp_path / "args.log").read_text().strip().split("\n")
+    assert len(args) == 2  # iterations 0 and 10
+    for it, a in zip((0, 10), args):
+        d = "experiments/output/freeze/m"
+        assert a == f"{d}/iter_{it:03d}_sites.xyz {d}/iter_{it:03d}_co3ne.obj {CO3NE_OPTS}", a
+    tet = (out / "iter_010_co3ne.obj").read_text().split("\n")
+    Vt = np.array([[float(t) for t in ln.split()[1:4]] for ln in tet if ln.startswith("v ")])
+    Ft = np.array([[int(t) - 1 for t in ln.split()[1:4]] for ln in tet if ln.startswith("f ")])
+    q = orc_quality(Vt, Ft)
+    lines = (out / "eval_iters.csv").read_text().strip().split("\n")
+    row = dict(zip(EVAL_HEADER.split(","), lines[2].split(",")))
+    for k in ("Qmin", "Qavg", "theta_min", "theta_min_avg"):
+        assert abs(float(row[k]) - q[k]) <= 2e-5 * abs(q[k]), (k, row[k], q[k])
+    # one-sided distance of the tetrahedron's probes to the input mesh
+    P = Vt[Ft]
+    probes = np.concatenate([Vt, P.mean(axis=1), 0.5 * (P[:, 0] + P[:, 1]), 0.5 * (P[:, 1] + P[:, 2]),
+                             0.5 * (P[:, 2] + P[:, 0])])
+    from oracle import oracle as orc
+    h = float(np.sqrt(orc.point_mesh_d2(probes, np.asarray(V, np.float64), np.asarray(F)).max()))
+    assert abs(float(row["dH"]) - h) <= 1e-4 * h + 1e-6
+    runs = (tmp_path / "experiments" / "runs.csv").read_text().strip().split("\n")
+    assert runs[1].split(",")[3] == "4"  # final_nf: the co3ne surface's faces

@@ -19,6 +19,11 @@ answers, brute force written independently (numpy), invariants and symmetry.
   orc_freeze                         hand-worked streak scenarios, 255
                                      saturation, <= boundary, absorbing frozen
   orc_point_mesh_d2                  brute == BVH; translated plane == t^2
+  orc_quality                        closed forms: equilateral (Q 1, 60 deg),
+                                     right isosceles (Q sqrt3 (sqrt2 - 1), 45),
+                                     obtuse isosceles (25/25/130 shares),
+                                     collinear (Q 0, 0/0/180); means of a
+                                     mixed set; rotation and scale invariance
   orc_lloyd_step                     energy decreases; fixed point; frozen
                                      lists pinned; exclude_self lists == numpy
                                      brute force with vertex i removed; kproj
@@ -524,3 +529,44 @@ def test_tiers_cone_has_no_split(orc):
     (t1, nv1, _), (t2, nv2, L2) = _tier_case([0.0, 0.0, 1.0], nb)
     assert abs(nv1 - (1 - np.cos(al))) < 1e-12
     assert t1 == 4 and t2 == 4 and L2 < 1e-6
+
+
+# -------------------------------------------------------------------- quality
+def _iso(apex_deg):
+    h = np.radians(apex_deg) / 2
+    return np.array([[0.0, 0.0, 0.0], [np.sin(h), -np.cos(h), 0.0], [-np.sin(h), -np.cos(h), 0.0]])
+
+
+def test_quality_closed_forms(orc):
+    eq = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, np.sqrt(3) / 2, 0.0]])
+    q = orc.quality(eq, [[0, 1, 2]])
+    assert abs(q["Qmin"] - 1) < 1e-6 and abs(q["theta_min"] - 60) < 1e-4
+    assert q["theta_lt_30_pct"] == 0 and q["theta_gt_90_pct"] == 0
+    ri = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    q = orc.quality(ri, [[0, 1, 2]])
+    assert abs(q["Qmin"] - np.sqrt(3) * (np.sqrt(2) - 1)) < 1e-6 and abs(q["theta_min"] - 45) < 1e-4
+    # apex 130 degrees: base angles 25, legs 1, base 2 sin 65
+    q = orc.quality(_iso(130.0), [[0, 1, 2]])
+    s65 = np.sin(np.radians(65.0))
+    q_cf = (6 / np.sqrt(3)) * (0.5 * np.sin(np.radians(130.0))) / ((1 + s65) * 2 * s65)
+    assert abs(q["Qmin"] - q_cf) < 1e-6 and abs(q["theta_min"] - 25) < 1e-4
+    assert abs(q["theta_lt_30_pct"] - 200 / 3) < 1e-4 and abs(q["theta_gt_90_pct"] - 100 / 3) < 1e-4
+    # collinear: no area, angles 0, 0, 180
+    q = orc.quality(np.array([[0.0, 0, 0], [1.0, 0, 0], [3.0, 0, 0]]), [[0, 1, 2]])
+    assert q["Qmin"] == 0 and q["theta_min"] == 0
+    assert abs(q["theta_lt_30_pct"] - 200 / 3) < 1e-4 and abs(q["theta_gt_90_pct"] - 100 / 3) < 1e-4
+
+
+def test_quality_means_and_invariance(orc):
+    P = np.concatenate([np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.5, np.sqrt(3) / 2, 0.0]]),
+                        np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]]) + 5.0])
+    q = orc.quality(P, [[0, 1, 2], [3, 4, 5]])
+    assert abs(q["Qavg"] - (1 + np.sqrt(3) * (np.sqrt(2) - 1)) / 2) < 1e-6
+    assert abs(q["theta_min_avg"] - (60 + 45) / 2) < 1e-4
+    assert abs(q["Qmin"] - np.sqrt(3) * (np.sqrt(2) - 1)) < 1e-6
+    V, F = mg.displaced_sphere(10, seed=2, crease_seed=3)
+    rng = np.random.default_rng(4)
+    Rm, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    q0, q1 = orc.quality(V, F), orc.quality(3.7 * V @ Rm.T + 1.0, F)
+    for k in ("Qmin", "Qavg", "theta_min", "theta_min_avg"):
+        assert abs(q0[k] - q1[k]) <= 2e-5 * max(1.0, abs(q0[k])), k
