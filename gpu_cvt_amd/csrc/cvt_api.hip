@@ -513,63 +513,6 @@ int check_finite(const double* a, long n) {
     return 1;
 }
 
-// f(t, begin, end) over [0, n) split into contiguous chunks on up to 8 host
-// threads (one below 64k items); chunk t is [n t / T, n (t + 1) / T).
-template <class Fn>
-void parallel_chunks(size_t n, Fn&& f) {
-    const int T = (int)std::max<size_t>(1, std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
-                                                             n / 65536 + 1}));
-    std::vector<std::thread> th;
-    for (int t = 1; t < T; ++t) th.emplace_back([&, t] { f(t, n * t / T, n * (t + 1) / T); });
-    f(0, 0, n / T);
-    for (auto& x : th) x.join();
-}
-
-// Sorts keys whose low `lo_bit` bits are an index in ascending order (as
-// built) by the bits above, stably, 8 bits per pass (LSD radix, the passes
-// split over up to 8 host threads): the same order as a full sort of the
-// unique keys, in O(n) passes instead of O(n log n) comparisons.
-void sort_indexed_keys(std::vector<unsigned long long>& k, int lo_bit) {
-    const size_t n = k.size();
-    std::vector<unsigned long long> tmp(n);
-    unsigned long long all = 0;
-    for (const unsigned long long x : k) all |= x;
-    const int T = (int)std::max<size_t>(1, std::min<size_t>({8, std::max(1u, std::thread::hardware_concurrency()),
-                                                             n / 65536 + 1}));
-    std::vector<size_t> cnt((size_t)T * 256);
-    auto chunk = [&](int t, size_t& b, size_t& e) { b = n * t / T; e = n * (t + 1) / T; };
-    auto run = [&](auto&& f) {
-        std::vector<std::thread> th;
-        for (int t = 1; t < T; ++t) th.emplace_back(f, t);
-        f(0);
-        for (auto& x : th) x.join();
-    };
-    for (int sh = lo_bit; sh < 64 && (all >> sh) != 0; sh += 8) {
-        std::fill(cnt.begin(), cnt.end(), 0);
-        run([&](int t) {
-            size_t b, e;
-            chunk(t, b, e);
-            size_t* c = &cnt[(size_t)t * 256];
-            for (size_t i = b; i < e; ++i) c[(k[i] >> sh) & 255]++;
-        });
-        // exclusive offsets in (digit, thread) order keep the pass stable
-        size_t run_sum = 0;
-        for (int d = 0; d < 256; ++d)
-            for (int t = 0; t < T; ++t) {
-                const size_t c = cnt[(size_t)t * 256 + d];
-                cnt[(size_t)t * 256 + d] = run_sum;
-                run_sum += c;
-            }
-        run([&](int t) {
-            size_t b, e;
-            chunk(t, b, e);
-            size_t* c = &cnt[(size_t)t * 256];
-            for (size_t i = b; i < e; ++i) tmp[c[(k[i] >> sh) & 255]++] = k[i];
-        });
-        k.swap(tmp);
-    }
-}
-
 }  // namespace
 
 extern "C" {
@@ -651,66 +594,33 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->diag = std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
     j->R = p.clip_R_mode == 1 ? p.clip_R : (double)(float)emax;  // main.cu:226 (FP32 max bbox edge)
     if (!(j->R > 0)) return bail(fail(CVT_EINVAL, "clip half-size R must be > 0"));
+    if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess || side_create(j->side))
+        return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
+    hipStream_t s = j->stream;
+    auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
+    TempPool build_tmp;  // lives until the set-up synchronises below
     // faces are stored in the Morton order of their centroids (face ids never
-    // leave the library), so the faces a query visits sit close in memory
-    std::vector<int> Fi((size_t)3 * nF);
+    // leave the library), so the faces a query visits sit close in memory;
+    // the order, the face table, the area and the longest edge are made on
+    // the device
+    if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
+        (rc = j->alloc(&j->F, (size_t)3 * nF)))
+        return bail(rc);
     {
-        std::vector<unsigned long long> key(nF);
-        auto spread = [](unsigned x) {
-            unsigned long long v = x & 0x3ff;
-            v = (v | (v << 16)) & 0x30000ffull;
-            v = (v | (v << 8)) & 0x300f00full;
-            v = (v | (v << 4)) & 0x30c30c3ull;
-            v = (v | (v << 2)) & 0x9249249ull;
-            return v;
-        };
-        parallel_chunks(nF, [&](int, size_t b, size_t e2) {
-            for (size_t f = b; f < e2; ++f) {
-                unsigned q[3];
-                for (int a = 0; a < 3; ++a) {
-                    const double c = ((double)(&V4[Fh[3 * f]].x)[a] + (&V4[Fh[3 * f + 1]].x)[a] + (&V4[Fh[3 * f + 2]].x)[a]) / 3.0;
-                    const double t = e[a] > 0 ? (c - j->lo[a]) / e[a] : 0.0;
-                    q[a] = (unsigned)std::min(1023.0, std::max(0.0, t * 1024.0));
-                }
-                key[f] = ((spread(q[0]) << 2 | spread(q[1]) << 1 | spread(q[2])) << 32) | (unsigned)f;
-            }
-        });
-        sort_indexed_keys(key, 32);
-        parallel_chunks(nF, [&](int, size_t b, size_t e2) {
-            for (size_t k = b; k < e2; ++k) {
-                const size_t f = (size_t)(key[k] & 0xffffffffu);
-                for (int c = 0; c < 3; ++c) Fi[3 * k + c] = Fh[3 * f + c];
-            }
-        });
+        int* Fin = (int*)build_tmp.get(sizeof(int) * 3 * (size_t)nF);
+        void* ftmp = build_tmp.get((size_t)face_order_temp_bytes(nF));
+        double* sums = (double*)build_tmp.get(2 * sizeof(double));
+        if (!Fin || !ftmp || !sums) return bail(fail(CVT_ENOMEM, "face order temporaries"));
+        if (up(j->V, V4.data(), sizeof(float4) * nV) || up(Fin, Fh, sizeof(int) * 3 * (size_t)nF))
+            return bail(fail(CVT_EHIP, "upload failed"));
+        launch_face_order(j->V, Fin, nF, j->lo, e, j->F, j->tri, sums, ftmp, s);
+        double hs[2] = {0, 0};
+        if (hipMemcpyAsync(hs, sums, sizeof hs, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return bail(fail(CVT_EHIP, "face order failed"));
+        j->area = hs[0] > 0 ? hs[0] : emax * emax;
+        j->max_edge = hs[1];
     }
-    std::vector<float4> tri((size_t)3 * nF);
-    std::vector<double> farea(nF), fedge(nF);
-    parallel_chunks(nF, [&](int, size_t b, size_t e2) {
-        for (size_t f = b; f < e2; ++f) {
-            const float4 A = V4[Fi[3 * f]], B = V4[Fi[3 * f + 1]], C = V4[Fi[3 * f + 2]];
-            for (int k = 0; k < 3; ++k) {
-                const float4 P = V4[Fi[3 * f + k]];
-                int id = Fi[3 * f + k];
-                float w;
-                std::memcpy(&w, &id, 4);
-                tri[3 * f + k] = make_float4(P.x, P.y, P.z, w);
-            }
-            double ux = (double)B.x - A.x, uy = (double)B.y - A.y, uz = (double)B.z - A.z;
-            double vx = (double)C.x - A.x, vy = (double)C.y - A.y, vz = (double)C.z - A.z;
-            double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
-            farea[f] = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
-            const double wx = (double)C.x - B.x, wy = (double)C.y - B.y, wz = (double)C.z - B.z;
-            fedge[f] = std::sqrt(std::max({ux * ux + uy * uy + uz * uz, vx * vx + vy * vy + vz * vz,
-                                           wx * wx + wy * wy + wz * wz}));
-        }
-    });
-    double area = 0, max_edge = 0;
-    for (int f = 0; f < nF; ++f) {
-        area += farea[f];  // in face order, as one thread would
-        max_edge = std::max(max_edge, fedge[f]);
-    }
-    j->area = area > 0 ? area : emax * emax;
-    j->max_edge = max_edge;
     // the vertex -> incident-face CSR (main.cu:259-285) is built on the device below
     // padded grid bbox
     double glo[3], ghi[3];
@@ -732,11 +642,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->sgp = make_grid(glo, ghi, ss * rK, 32L * nS + 4096);
     j->vg = make_grid(glo, ghi, vs * rV, 32L * nV + 4096);
 
-    if (hipStreamCreateWithFlags(&j->stream, hipStreamNonBlocking) != hipSuccess || side_create(j->side))
-        return bail(fail(CVT_EHIP, "hipStreamCreate failed"));
-    if ((rc = j->alloc(&j->V, nV)) || (rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->FU, nV)) ||
-        (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->tri, (size_t)3 * nF)) ||
-        (rc = j->alloc(&j->F, (size_t)3 * nF)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
+    if ((rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->FU, nV)) ||
+        (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
         (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.ncell)) ||
         (rc = alloc_grid(j, j->sgb, nS, j->sgp.ncell)) || (rc = j->alloc(&j->S[0], nS)) ||
         (rc = j->alloc(&j->S[1], nS)) || (rc = j->alloc(&j->knn, (size_t)nS * p.K)) ||
@@ -757,39 +664,23 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
             return bail(rc);
     }
     // internal numbering: sites stably sorted by their cell in the site grid
+    // (on the device; the caller's order enters through S[1], free until the
+    // first iteration)
     {
-        const GridP& g = j->sgp;
-        std::vector<unsigned long long> key(nS);
-        for (int i = 0; i < nS; ++i) {
-            const float x = (float)S0[3 * i], y = (float)S0[3 * i + 1], z = (float)S0[3 * i + 2];
-            auto cl = [](float v, float o, float ih, int d) {
-                const int c = (int)std::floor((v - o) * ih);
-                return c < 0 ? 0 : (c >= d ? d - 1 : c);
-            };
-            const long c = ((long)cl(z, g.oz, g.inv_h, g.dz) * g.dy + cl(y, g.oy, g.inv_h, g.dy)) * g.dx +
-                           cl(x, g.ox, g.inv_h, g.dx);
-            key[i] = ((unsigned long long)c << 31) | (unsigned)i;
-        }
-        sort_indexed_keys(key, 31);
+        std::vector<float4> S4(nS);
+        for (int i = 0; i < nS; ++i)
+            S4[i] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
+        void* stmp = build_tmp.get((size_t)site_order_temp_bytes(nS));
+        if (!stmp) return bail(fail(CVT_ENOMEM, "site order temporaries"));
+        if (up(j->S[1], S4.data(), sizeof(float4) * nS)) return bail(fail(CVT_EHIP, "upload failed"));
+        launch_site_order(j->S[1], nS, j->sgp, j->S[0], j->cid, stmp, s);
         j->perm.resize(nS);
         j->inv.resize(nS);
-        for (int k = 0; k < nS; ++k) {
-            j->perm[k] = (int)(key[k] & 0x7fffffff);
-            j->inv[j->perm[k]] = k;
-        }
+        if (hipMemcpyAsync(j->perm.data(), j->cid, sizeof(int) * nS, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return bail(fail(CVT_EHIP, "site order failed"));
+        for (int k = 0; k < nS; ++k) j->inv[j->perm[k]] = k;
     }
-    std::vector<float4> S4(nS);
-    for (int k = 0; k < nS; ++k) {
-        const int i = j->perm[k];
-        S4[k] = make_float4((float)S0[3 * i], (float)S0[3 * i + 1], (float)S0[3 * i + 2], 0.f);
-    }
-    std::vector<int>& Fv = Fi;
-    hipStream_t s = j->stream;
-    auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
-    if (up(j->V, V4.data(), sizeof(float4) * nV) || up(j->tri, tri.data(), sizeof(float4) * tri.size()) ||
-        up(j->F, Fv.data(), sizeof(int) * Fv.size()) ||
-        up(j->S[0], S4.data(), sizeof(float4) * nS) || up(j->cid, j->perm.data(), sizeof(int) * nS))
-        return bail(fail(CVT_EHIP, "upload failed"));
     launch_fill(j->knn, (long)nS * p.K, 0, s);
     launch_fill(j->knnV, (long)nS * p.kproj, -1, s);
     if (hipMemsetAsync(j->frozen, 0, nS, s) || hipMemsetAsync(j->streak, 0, nS, s) ||
@@ -804,7 +695,6 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatCopies * kStatStride, 0, s);
-    TempPool build_tmp;  // lives until the set-up synchronises below
     {
         void* csr_tmp = build_tmp.get((size_t)csr_temp_bytes(nV, nF));
         if (!csr_tmp) return bail(fail(CVT_ENOMEM, "CSR temporaries"));
@@ -1068,6 +958,48 @@ static int quality_device(const float4* P, int n, const int* map, const int* F, 
     out->theta_gt_90_pct = h.gt90;
     out->nF = nF;
     return 0;
+}
+
+int cvt_mesh_compact(const double* V, int nV, const int* F, int nF, double* V_out, int* F_out, int* nV_out) {
+    if (!V || !F || !V_out || !F_out || !nV_out || nV < 0 || nF < 0) return fail(CVT_EINVAL, "bad argument");
+    for (long k = 0; k < 3L * nF; ++k)
+        if (F[k] < 0 || F[k] >= nV) return fail(CVT_EINVAL, "face %ld: vertex id %d out of range", k / 3, F[k]);
+    if (nV == 0) {
+        *nV_out = 0;
+        return 0;
+    }
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc = 0;
+    {
+        TempPool T;
+        auto* dV = (double*)T.get(sizeof(double) * 3 * (size_t)nV);
+        auto* dV2 = (double*)T.get(sizeof(double) * 3 * (size_t)nV);
+        int* dF = (int*)T.get(sizeof(int) * 3 * (size_t)std::max(nF, 1));
+        int* used = (int*)T.get(sizeof(int) * (size_t)nV);
+        int* remap = (int*)T.get(sizeof(int) * ((size_t)nV + 1));
+        int* bsum = (int*)T.get(sizeof(int) * (size_t)scan_partials(nV));
+        if (!dV || !dV2 || !dF || !used || !remap || !bsum) rc = fail(CVT_ENOMEM, "compaction temporaries");
+        auto up = [&](void* d, const void* h, size_t n) { return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s); };
+        if (!rc && (up(dV, V, sizeof(double) * 3 * (size_t)nV) ||
+                    (nF > 0 && up(dF, F, sizeof(int) * 3 * (size_t)nF))))
+            rc = fail(CVT_EHIP, "compaction upload failed");
+        int kept = 0;
+        if (!rc) {
+            launch_mesh_compact(dV, nV, dF, nF, used, remap, bsum, dV2, s);
+            if (hipMemcpyAsync(&kept, remap + nV, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess)
+                rc = fail(CVT_EHIP, "compaction failed");
+        }
+        if (!rc && (hipMemcpyAsync(V_out, dV2, sizeof(double) * 3 * (size_t)kept, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                    (nF > 0 && hipMemcpyAsync(F_out, dF, sizeof(int) * 3 * (size_t)nF, hipMemcpyDeviceToHost, s) != hipSuccess) ||
+                    hipStreamSynchronize(s) != hipSuccess))
+            rc = fail(CVT_EHIP, "compaction download failed");
+        if (!rc) *nV_out = kept;
+        (void)hipStreamSynchronize(s);
+    }
+    (void)hipStreamDestroy(s);
+    return rc;
 }
 
 int cvt_mesh_quality(const double* V, int nV, const int* F, int nF, cvt_quality* out) {

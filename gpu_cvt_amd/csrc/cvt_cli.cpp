@@ -92,23 +92,6 @@ bool load_obj(const std::string& path, std::vector<double>& V, std::vector<int>&
     return true;
 }
 
-// Drop vertices no face references and renumber (compact_mesh, main.cu:240-268).
-void compact(std::vector<double>& V, std::vector<int>& F) {
-    const int nV = (int)(V.size() / 3);
-    std::vector<int> remap((size_t)nV, -1);
-    for (int v : F)
-        if (v >= 0 && v < nV) remap[(size_t)v] = 0;
-    int n = 0;
-    std::vector<double> V2;
-    for (int v = 0; v < nV; ++v)
-        if (remap[(size_t)v] == 0) {
-            remap[(size_t)v] = n++;
-            V2.insert(V2.end(), V.begin() + 3L * v, V.begin() + 3L * v + 3);
-        }
-    for (int& v : F) v = (v >= 0 && v < nV) ? remap[(size_t)v] : -1;
-    V.swap(V2);
-}
-
 const char* kEvalHeader =
     "mesh,mode,iter,Qmin,Qavg,theta_min,theta_min_avg,theta_lt_30_pct,theta_gt_90_pct,dH,iter_remesh_ms,"
     "total_remesh_ms,freeze_cell_num,n_vertices,freeze_pct,knn_sites_ms,knn_site_to_mesh_ms,uv_from_mesh_ms,"
@@ -161,7 +144,14 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "gpu_cvt: cannot read %s\n", input.c_str());
         return 1;
     }
-    compact(V, F);
+    {
+        // drop unreferenced vertices (compact_mesh, main.cu:240-268), on the device
+        int kept = 0;
+        if (cvt_mesh_compact(V.data(), (int)(V.size() / 3), F.data(), (int)(F.size() / 3), V.data(), F.data(), &kept) !=
+            CVT_OK)
+            return fail_msg("compact");
+        V.resize(3 * (size_t)kept);
+    }
     const int nV = (int)(V.size() / 3), nF = (int)(F.size() / 3);
     std::printf("V %d\nF %d\n", nV, nF);
     if (nV == 0 || nF == 0) return 1;

@@ -181,6 +181,25 @@ void launch_frames(const MeshDev& m, float4* FU, float4* FW, hipStream_t s);
 // (nV + 1) and vf_faces (3 nF); tmp holds csr_temp_bytes(nV, nF) bytes.
 long csr_temp_bytes(int nV, int nF);
 void launch_build_csr(const int* F, int nF, int nV, int* vf_off, int* vf_faces, void* tmp, hipStream_t s);
+// Job set-up on the device.  Faces in the Morton order of their centroids
+// (10 bits per axis over the bbox lo + [0, e]): F (3 nF, caller's ids) is
+// reordered into F_out, with the face table tri (corners with their vertex
+// ids as float bits), and the mesh area (FP64 sum, fixed order) and longest
+// edge into sums[0], sums[1].  tmp: face_order_temp_bytes(nF).
+long face_order_temp_bytes(int nF);
+void launch_face_order(const float4* V, const int* F, int nF, const double lo[3], const double e[3], int* F_out,
+                       float4* tri, double* sums, void* tmp, hipStream_t s);
+// Sites (float4, caller's order) stably sorted by their cell of grid g:
+// S_out the sorted sites, cid[k] the caller's index of S_out[k].
+// tmp: site_order_temp_bytes(nS).
+long site_order_temp_bytes(int nS);
+void launch_site_order(const float4* S, int nS, const GridP& g, float4* S_out, int* cid, void* tmp, hipStream_t s);
+// compact_mesh (main.cu:240-268) on the device: used[v] (nV ints) = some face
+// names v, remap (nV + 1 ints) = exclusive scan of used (remap[nV] = kept
+// count), F renumbered in place, the kept vertices (3 doubles each) gathered
+// into V_out.  bsum: scan_partials(nV) ints.
+void launch_mesh_compact(const double* V, int nV, int* F, int nF, int* used, int* remap, int* bsum, double* V_out,
+                         hipStream_t s);
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
 bool lloyd_supported(int K, int kproj);

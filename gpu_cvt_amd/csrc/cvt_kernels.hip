@@ -154,6 +154,184 @@ __global__ void k_vertex_normals(const float4* __restrict__ V, const int* __rest
     N[v] = make_float4(sx, sy, sz, 0.f);
 }
 
+// ------------------------------------------------------------- job set-up
+__device__ __forceinline__ unsigned long long spread10(unsigned x) {
+    unsigned long long v = x & 0x3ff;
+    v = (v | (v << 16)) & 0x30000ffull;
+    v = (v | (v << 8)) & 0x300f00full;
+    v = (v | (v << 4)) & 0x30c30c3ull;
+    v = (v | (v << 2)) & 0x9249249ull;
+    return v;
+}
+
+__global__ void k_face_keys(const float4* __restrict__ V, const int* __restrict__ F, int nF, double lx, double ly,
+                            double lz, double ex, double ey, double ez, unsigned long long* __restrict__ keys) {
+#pragma clang fp contract(off)
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nF) return;
+    const float4 A = V[F[3 * f]], B = V[F[3 * f + 1]], C = V[F[3 * f + 2]];
+    const double c[3] = {((double)A.x + B.x + C.x) / 3.0, ((double)A.y + B.y + C.y) / 3.0,
+                         ((double)A.z + B.z + C.z) / 3.0};
+    const double lo[3] = {lx, ly, lz}, e[3] = {ex, ey, ez};
+    unsigned q[3];
+    for (int a = 0; a < 3; ++a) {
+        const double t = e[a] > 0 ? (c[a] - lo[a]) / e[a] : 0.0;
+        q[a] = (unsigned)fmin(1023.0, fmax(0.0, t * 1024.0));
+    }
+    keys[f] = ((spread10(q[0]) << 2 | spread10(q[1]) << 1 | spread10(q[2])) << 32) | (unsigned)f;
+}
+
+// Per sorted face: the reordered corner ids, the face table, the area and the
+// longest edge (FP64 on the FP32 corners), the last two into per-block
+// partials (sum, max) in a fixed order.
+__global__ __launch_bounds__(256) void k_face_gather(const unsigned long long* __restrict__ keys,
+                                                     const int* __restrict__ F, const float4* __restrict__ V, int nF,
+                                                     int* __restrict__ F_out, float4* __restrict__ tri,
+                                                     double* __restrict__ part) {
+#pragma clang fp contract(off)
+    const int k = blockIdx.x * 256 + (int)threadIdx.x;
+    double area = 0.0, edge = 0.0;
+    if (k < nF) {
+        const int f = (int)(keys[k] & 0xffffffffu);
+        int id[3];
+        float4 P[3];
+        for (int c = 0; c < 3; ++c) {
+            id[c] = F[3 * f + c];
+            P[c] = V[id[c]];
+            F_out[3 * k + c] = id[c];
+            tri[3 * k + c] = make_float4(P[c].x, P[c].y, P[c].z, __int_as_float(id[c]));
+        }
+        const double ux = (double)P[1].x - P[0].x, uy = (double)P[1].y - P[0].y, uz = (double)P[1].z - P[0].z;
+        const double vx = (double)P[2].x - P[0].x, vy = (double)P[2].y - P[0].y, vz = (double)P[2].z - P[0].z;
+        const double cx = uy * vz - uz * vy, cy = uz * vx - ux * vz, cz = ux * vy - uy * vx;
+        area = 0.5 * sqrt(cx * cx + cy * cy + cz * cz);
+        const double wx = (double)P[2].x - P[1].x, wy = (double)P[2].y - P[1].y, wz = (double)P[2].z - P[1].z;
+        edge = sqrt(fmax(ux * ux + uy * uy + uz * uz, fmax(vx * vx + vy * vy + vz * vz, wx * wx + wy * wy + wz * wz)));
+    }
+    __shared__ double sa[256], se[256];
+    sa[threadIdx.x] = area;
+    se[threadIdx.x] = edge;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            se[threadIdx.x] = fmax(se[threadIdx.x], se[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sa[0];
+        part[2 * blockIdx.x + 1] = se[0];
+    }
+}
+
+__global__ void k_face_sums(const double* __restrict__ part, int nb, double* __restrict__ sums) {
+    if (threadIdx.x != 0) return;
+    double a = 0.0, e = 0.0;
+    for (int b = 0; b < nb; ++b) {
+        a += part[2 * b];
+        e = fmax(e, part[2 * b + 1]);
+    }
+    sums[0] = a;
+    sums[1] = e;
+}
+
+static int face_blocks(int nF) { return std::max(1, nblk(nF, 256)); }
+
+long face_order_temp_bytes(int nF) {
+    const long hw = radix_hist_words(nF);
+    auto r = [](long b) { return (b + 255) & ~255L; };
+    return 2 * r(8L * nF) + r(4 * (hw + 1)) + r(4L * scan_partials(hw)) + r(16L * face_blocks(nF)) + r(4L * nF);
+}
+
+void launch_face_order(const float4* V, const int* F, int nF, const double lo[3], const double e[3], int* F_out,
+                       float4* tri, double* sums, void* tmp, hipStream_t s) {
+    auto take = [&](long bytes) {
+        void* p = tmp;
+        tmp = (char*)tmp + ((bytes + 255) & ~255L);
+        return p;
+    };
+    const long hw = radix_hist_words(nF);
+    auto* keys = (unsigned long long*)take(8L * nF);
+    auto* alt = (unsigned long long*)take(8L * nF);
+    int* hist = (int*)take(4 * (hw + 1));
+    int* hsum = (int*)take(4L * scan_partials(hw));
+    auto* part = (double*)take(16L * face_blocks(nF));
+    hipLaunchKernelGGL(k_face_keys, dim3(face_blocks(nF)), dim3(256), 0, s, V, F, nF, lo[0], lo[1], lo[2], e[0], e[1],
+                       e[2], keys);
+    radix_sort_u64(keys, nF, 64, alt, hist, hsum, s, 32);
+    hipLaunchKernelGGL(k_face_gather, dim3(face_blocks(nF)), dim3(256), 0, s, keys, F, V, nF, F_out, tri, part);
+    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(64), 0, s, part, face_blocks(nF), sums);
+}
+
+__global__ void k_site_keys(const float4* __restrict__ S, int nS, GridP g, unsigned long long* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nS) return;
+    const float4 p = S[i];
+    int cx, cy, cz;
+    cell_of(g, p.x, p.y, p.z, cx, cy, cz);
+    keys[i] = ((unsigned long long)cell_index(g, cx, cy, cz) << 31) | (unsigned)i;
+}
+
+__global__ void k_site_gather(const unsigned long long* __restrict__ keys, const float4* __restrict__ S, int nS,
+                              float4* __restrict__ S_out, int* __restrict__ cid) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nS) return;
+    const int i = (int)(keys[k] & 0x7fffffffu);
+    S_out[k] = S[i];
+    cid[k] = i;
+}
+
+long site_order_temp_bytes(int nS) {
+    const long hw = radix_hist_words(nS);
+    auto r = [](long b) { return (b + 255) & ~255L; };
+    return 2 * r(8L * nS) + r(4 * (hw + 1)) + r(4L * scan_partials(hw));
+}
+
+void launch_site_order(const float4* S, int nS, const GridP& g, float4* S_out, int* cid, void* tmp, hipStream_t s) {
+    auto take = [&](long bytes) {
+        void* p = tmp;
+        tmp = (char*)tmp + ((bytes + 255) & ~255L);
+        return p;
+    };
+    const long hw = radix_hist_words(nS);
+    auto* keys = (unsigned long long*)take(8L * nS);
+    auto* alt = (unsigned long long*)take(8L * nS);
+    int* hist = (int*)take(4 * (hw + 1));
+    int* hsum = (int*)take(4L * scan_partials(hw));
+    hipLaunchKernelGGL(k_site_keys, dim3(nblk(nS, 256)), dim3(256), 0, s, S, nS, g, keys);
+    radix_sort_u64(keys, nS, 64, alt, hist, hsum, s, 31);
+    hipLaunchKernelGGL(k_site_gather, dim3(nblk(nS, 256)), dim3(256), 0, s, keys, S, nS, S_out, cid);
+}
+
+__global__ void k_mark_used(const int* __restrict__ F, long n3, int* __restrict__ used) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n3) used[F[e]] = 1;  // every writer stores the same value
+}
+
+__global__ void k_remap_faces(int* __restrict__ F, long n3, const int* __restrict__ remap) {
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n3) F[e] = remap[F[e]];
+}
+
+__global__ void k_gather_used(const double* __restrict__ V, int nV, const int* __restrict__ used,
+                              const int* __restrict__ remap, double* __restrict__ V_out) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV || !used[v]) return;
+    const int w = remap[v];
+    for (int c = 0; c < 3; ++c) V_out[3 * (long)w + c] = V[3 * (long)v + c];
+}
+
+void launch_mesh_compact(const double* V, int nV, int* F, int nF, int* used, int* remap, int* bsum, double* V_out,
+                         hipStream_t s) {
+    const long n3 = 3L * nF;
+    launch_fill(used, nV, 0, s);
+    if (n3 > 0) hipLaunchKernelGGL(k_mark_used, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, F, n3, used);
+    launch_scan(used, nV, remap, bsum, s);
+    if (n3 > 0) hipLaunchKernelGGL(k_remap_faces, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, s, F, n3, remap);
+    if (nV > 0) hipLaunchKernelGGL(k_gather_used, dim3(nblk(nV, 256)), dim3(256), 0, s, V, nV, used, remap, V_out);
+}
+
 // --------------------------------------------------------------- Lloyd step
 // FP64 vertex normal (same CSR order and operations as the oracle), for
 // frames whose axis switch |n_z| > 0.9 FP32 cannot decide.

@@ -30,7 +30,7 @@ EXPORTS = [
     "cvt_server_config_default", "cvt_server_create", "cvt_server_submit", "cvt_server_poll",
     "cvt_server_get_sites", "cvt_server_wait_idle", "cvt_server_info_get", "cvt_server_destroy",
     "cvt_job_compute_tiers", "cvt_job_hausdorff_within", "cvt_bvh_point_d2_noalloc",
-    "cvt_job_profile_stages", "cvt_mesh_quality", "cvt_job_quality",
+    "cvt_job_profile_stages", "cvt_mesh_quality", "cvt_job_quality", "cvt_mesh_compact",
 ]
 
 
@@ -183,6 +183,7 @@ def lib():
     L.cvt_job_stats.argtypes = [C.c_void_p, C.c_int, C.c_void_p]
     L.cvt_job_profile_stages.argtypes = [C.c_void_p, C.c_int, C.POINTER(StageMs)]
     L.cvt_mesh_quality.argtypes = [_dp, C.c_int, _ip, C.c_int, C.POINTER(Quality)]
+    L.cvt_mesh_compact.argtypes = [_dp, C.c_int, _ip, C.c_int, _dp, _ip, C.POINTER(C.c_int)]
     L.cvt_job_quality.argtypes = [C.c_void_p, _ip, C.c_int, C.POINTER(Quality)]
     L.cvt_job_hausdorff.argtypes = [C.c_void_p, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)]
     L.cvt_job_hausdorff_within.argtypes = [C.c_void_p, C.c_double, C.c_double, C.POINTER(C.c_int),
@@ -460,6 +461,19 @@ class BVH:
             self.close()
         except Exception:
             pass
+
+
+def mesh_compact(V, F):
+    """(V', F'): the vertices some face references, in their order, and the
+    faces renumbered (compact_mesh, main.cu:240-268), on the device."""
+    V = np.ascontiguousarray(V, dtype=np.float64).reshape(-1, 3)
+    F = np.ascontiguousarray(F, dtype=np.int32).reshape(-1, 3)
+    Vo = np.zeros_like(V)
+    Fo = np.zeros_like(F)
+    n = C.c_int()
+    _check("cvt_mesh_compact", lib().cvt_mesh_compact(V.reshape(-1), len(V), F.reshape(-1), len(F), Vo.reshape(-1),
+                                                      Fo.reshape(-1), C.byref(n)))
+    return Vo[:n.value].copy(), Fo
 
 
 def mesh_quality(V, F) -> dict:

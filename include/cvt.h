@@ -201,6 +201,13 @@ typedef struct cvt_quality {
     int nF;
     int reserved[5];
 } cvt_quality;
+/* Drops the vertices that no face references and renumbers the faces, on the
+ * device (compact_mesh, main.cu:240-268): V_out receives the kept vertices in
+ * their original order (*nV_out of them, at most nV), F_out the nF faces over
+ * the new ids.  V_out may be V and F_out may be F.  Face ids must lie in
+ * [0, nV) (CVT_EINVAL otherwise). */
+int cvt_mesh_compact(const double* V, int nV, const int* F, int nF, double* V_out, int* F_out, int* nV_out);
+
 /* V: nV*3 doubles (rounded to FP32 on the device, as the reference's float3 sites). */
 int cvt_mesh_quality(const double* V, int nV, const int* F, int nF, cvt_quality* out);
 /* Over the job's current (FP32) sites, F indexing sites by the caller's ids
