@@ -366,6 +366,9 @@ def main(argv=None) -> int:
     ap.add_argument("--vert-cell", type=float, default=0.0, help="vertex grid cell scale (0 = library default)")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="first run one plain device copy of MIB MiB (torch), to calibrate HBM counters")
+    ap.add_argument("--gather-calib", type=int, default=0, metavar="MIB",
+                    help="first run one gather of MIB MiB in 128-byte rows of random order (torch index_select; "
+                         "every row read once), to calibrate HBM counters on a gather")
     ap.add_argument("--no-extra", action="store_true",
                     help="plain mode: skip the freezing-on, K = 8, batched and served measurements")
     ap.add_argument("--extra-arrivals", type=int, default=400,
@@ -391,6 +394,18 @@ def main(argv=None) -> int:
         print(f"[bench] copy calibration: {args.copy_calib} MiB read + {args.copy_calib} MiB written",
               file=sys.stderr, flush=True)
         del src, dst
+    if args.gather_calib > 0:
+        import torch
+
+        rows = args.gather_calib * (1 << 20) // 128
+        src = torch.ones((rows, 32), dtype=torch.float32, device="cuda")
+        perm = torch.randperm(rows, device="cuda")
+        torch.cuda.synchronize()
+        dst = torch.index_select(src, 0, perm)
+        torch.cuda.synchronize()
+        print(f"[bench] gather calibration: {args.gather_calib} MiB read in {rows} random 128-byte rows "
+              f"(+ {8 * rows} index bytes) + {args.gather_calib} MiB written", file=sys.stderr, flush=True)
+        del src, dst, perm
 
     if args.mode == "batch":
         return bench_batch(args)

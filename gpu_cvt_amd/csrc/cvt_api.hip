@@ -807,6 +807,7 @@ static int read_stats(cvt_job* j, int first, int iters, cvt_iter_stats* stats) {
         std::memcpy(&o.max_disp, &w[10], 4);
         o.deferred_projections = w[11];
         o.fp64_cells = w[12];
+        o.far_walks = w[14];
     }
     return 0;
 }

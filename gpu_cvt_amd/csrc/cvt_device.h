@@ -621,8 +621,9 @@ __device__ __forceinline__ void grid_exact64(const GridP& g, const CVT_G int* __
 
 // K nearest (in exact (key64, id) order) of q as slots.  Returns 0 when FP32
 // proved the list, 1 when isolated near-tie pairs were ordered by FP64 keys,
-// 2 when the FP64 rescan was needed, -1 (only !PRECISE) when the query must go
-// to the FP64 pass.
+// 2 when the FP64 rescan was needed, 3 when a proven list came from the LBVH
+// walk (a query past two cells of the grid), -1 (only !PRECISE) when the query
+// must go to the FP64 pass.
 // tau: a provable upper bound of the (K+2)-th distance^2, or inf.
 // l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
 // lds (optional, with collect_tau finite): try grid_collect_topl first.
@@ -660,6 +661,7 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __rest
             }
         }
     }
+    const int far = limited ? 3 : 0;  // a successful result says it walked the LBVH
     l_out = T.s[L - 1] >= 0 ? (keyed ? worst_up : T.d[L - 1]) : __builtin_huge_valf();
     int pairs;
     const bool ok = resolve_topk<K, L>(pts, qx, qy, qz, T, pairs, cid, keyed ? kRelKey : kRel);
@@ -669,7 +671,7 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __rest
 #pragma unroll
         for (int t = 0; t < L; ++t) d_out[t] = T.s[t] >= 0 ? T.d[t] : __builtin_huge_valf();
     }
-    if (ok) return pairs > 0 ? 1 : 0;
+    if (ok) return far ? far : (pairs > 0 ? 1 : 0);
     if constexpr (!PRECISE) {
         return -1;
     } else {

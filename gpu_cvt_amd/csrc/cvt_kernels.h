@@ -6,7 +6,7 @@
 
 namespace cvt {
 
-constexpr int kStatWords = 14;  // words recorded per iteration (cvt_iter_stats maps them)
+constexpr int kStatWords = 15;  // words recorded per iteration (cvt_iter_stats maps them)
 // Stat word layout (cvt_iter_stats): 0 same, 1 low, 2 both, 3 newly frozen,
 // 4 frozen total, 5 exact rescans, 6 reranks, 7 cell failures, 8 non-finite,
 // 9 deferred to the precise pass (also that pass's work-list length), 10 the
@@ -22,6 +22,9 @@ constexpr int kStatFp64Cells = 12;
 // Word 13: front-list sites of the FP64 pass waiting for a wave-wide vertex
 // search (their list in vs_list).
 constexpr int kStatVsearch = 13;
+// Word 14: FP32 vertex searches that walked the vertex LBVH (a query past two
+// cells of the grid, out of line: the only FP32 code with a stack frame).
+constexpr int kStatFarWalks = 14;
 // The per-site counters (words 0-8) and the largest displacement (word 10, float
 // bits, atomic max) are accumulated per block into one of kStatCopies copies of
 // the stat words, each on its own 128-byte line (copy c at stats[c * kStatStride]),

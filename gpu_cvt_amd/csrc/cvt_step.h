@@ -134,6 +134,7 @@ __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, S
             const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1,
                                                  nullptr, 0.f, d1, nullptr, &a.m.vb);
             ok = k1 >= 0;
+            if (!PRECISE && k1 == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
             c.nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
             if (ok && !eval_only) {
                 if (!c.use_cache) gp = a.gaps[c.i];
@@ -434,6 +435,7 @@ __device__ __forceinline__ bool vertex_search(const StepArgs& a, int i, const fl
     const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, vertex_excl(a, i),
                                               tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
     if (!PRECISE && kk < 0) return false;
+    if (!PRECISE && kk == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
     int id[KP + 2];
 #pragma unroll
     for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;

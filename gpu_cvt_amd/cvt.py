@@ -52,7 +52,7 @@ class IterStats(C.Structure):
         ("count_newly_frozen", C.c_int), ("frozen_total", C.c_int), ("knn_exact_rescans", C.c_int),
         ("knn_reranks", C.c_int), ("cell_failures", C.c_int), ("nonfinite_sites", C.c_int),
         ("deferred_sites", C.c_int), ("max_disp", C.c_float), ("deferred_projections", C.c_int),
-        ("fp64_cells", C.c_int), ("reserved", C.c_int * 1),
+        ("fp64_cells", C.c_int), ("far_walks", C.c_int),
     ]
 
     def as_dict(self) -> dict:

@@ -66,7 +66,7 @@ typedef struct cvt_iter_stats {
     float max_disp;        /* largest |S' - S| of the iteration */
     int deferred_projections; /* of deferred_sites: nearest face not decidable in FP32 */
     int fp64_cells;        /* of deferred_sites: cells whose FP32 error allowance exceeded the tolerance */
-    int reserved[1];
+    int far_walks;         /* FP32 vertex searches past two grid cells, finished by the vertex LBVH */
 } cvt_iter_stats;
 
 typedef struct cvt_job_info {
