@@ -13,7 +13,10 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
-LIB = os.path.join(HERE, "libcvt_hip.so")
+# CVT_STAGE_CLOCKS=1: the profiling build with per-stage shader clocks
+# (-DCVT_STAGE_CLOCKS) into its own library, beside the product one
+STAGE_CLOCKS = os.environ.get("CVT_STAGE_CLOCKS", "0") == "1"
+LIB = os.path.join(HERE, "libcvt_hip_stageclk.so" if STAGE_CLOCKS else "libcvt_hip.so")
 CLI = os.path.join(HERE, "gpu_cvt")
 SOURCES = ["cvt_kernels.hip", "cvt_bvh.hip", "cvt_api.hip", "cvt_server.hip"]
 # cvt_lloyd.hip is compiled once per neighbour count K (-DCVT_K=k) and kproj (-DCVT_KP=p)
@@ -40,7 +43,7 @@ def k_values() -> list[int]:
 
 
 def build(verbose: bool = False) -> str:
-    objdir = os.path.join(HERE, "_obj")
+    objdir = os.path.join(HERE, "_obj_stageclk" if STAGE_CLOCKS else "_obj")
     os.makedirs(objdir, exist_ok=True)
     units = [(src, src.replace(".hip", ".o"), []) for src in SOURCES]
     units += [(PER_K_SOURCE, f"cvt_lloyd_k{k}.o", [f"-DCVT_K={k}"]) for k in k_values()]
@@ -50,7 +53,8 @@ def build(verbose: bool = False) -> str:
     def compile_one(unit) -> str:
         src, oname, defs = unit
         obj = os.path.join(objdir, oname)
-        cmd = [HIPCC, *FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
+        extra = ["-DCVT_STAGE_CLOCKS"] if STAGE_CLOCKS else []
+        cmd = [HIPCC, *FLAGS, *extra, *defs, "-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -70,6 +74,8 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    if STAGE_CLOCKS:
+        return LIB
     # the `gpu_cvt` command line (host C++ over the C ABI)
     cli_tmp = CLI + ".tmp"
     cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "cvt_cli.cpp"),

@@ -1032,6 +1032,27 @@ int cvt_job_quality(cvt_job* j, const int* F, int nF, cvt_quality* out) {
     return quality_device(j->S[j->cur], j->nS, j->inv_dev, F, nF, j->stream, out);
 }
 
+#ifdef CVT_STAGE_CLOCKS
+// Profiling build only (no declaration in cvt.h): the per-stage shader-clock
+// cycle sums of the job's last iteration, kStageCount of them (stage order as
+// at kStatClocks in cvt_kernels.h).
+int cvt_stage_clocks(cvt_job* j, unsigned long long* out) {
+    if (!j || !out) return fail(CVT_EINVAL, "null argument");
+    std::vector<int> h((size_t)kStatCopies * kStatStride);
+    HIPCHK(hipStreamSynchronize(j->stream));
+    HIPCHK(hipMemcpy(h.data(), j->stats, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < kStageCount; ++k) {
+        out[k] = 0;
+        for (int c = 0; c < kStatCopies; ++c) {
+            unsigned long long v;
+            std::memcpy(&v, &h[(size_t)c * kStatStride + kStatClocks + 2 * k], 8);
+            out[k] += v;
+        }
+    }
+    return 0;
+}
+#endif
+
 int cvt_job_get_sites(cvt_job* j, double* S) {
     if (!j || !S) return fail(CVT_EINVAL, "null argument");
     std::vector<float4> h(j->nS);

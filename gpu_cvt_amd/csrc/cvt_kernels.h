@@ -25,6 +25,27 @@ constexpr int kStatVsearch = 13;
 // Word 14: FP32 vertex searches that walked the vertex LBVH (a query past two
 // cells of the grid, out of line: the only FP32 code with a stack frame).
 constexpr int kStatFarWalks = 14;
+
+// Stage clocks (a profiling build, compiled with -DCVT_STAGE_CLOCKS): every
+// lane adds its shader-clock cycles per stage to 64-bit sums in words
+// kStatClocks.. of its block's stats copy (kStatCopies spread the atomics):
+// stage 0 the cell kernel's site KNN (seeds, collect, selection, order proof),
+// 1 its row write and dual points, 2 its dual hull walk and moments, 3 the
+// projection kernel's vertex list (pool cache or grid search), 4 its face
+// walk, new site and freeze step, 5 the frame kernel's nearest-vertex search
+// and frame.  The product build compiles none of it.
+constexpr int kStatClocks = 16;
+constexpr int kStageCount = 6;
+#ifdef CVT_STAGE_CLOCKS
+#define CVT_CLOCK(v) const unsigned long long v = clock64()
+#define CVT_STAGE_ADD(a, k, cycles)                                                                            \
+    atomicAdd(reinterpret_cast<unsigned long long*>((a).stats + (blockIdx.x % kStatCopies) * kStatStride + \
+                                                    kStatClocks) + (k),                                      \
+              (unsigned long long)(cycles))
+#else
+#define CVT_CLOCK(v)
+#define CVT_STAGE_ADD(a, k, cycles)
+#endif
 // The per-site counters (words 0-8) and the largest displacement (word 10, float
 // bits, atomic max) are accumulated per block into one of kStatCopies copies of
 // the stat words, each on its own 128-byte line (copy c at stats[c * kStatStride]),

@@ -225,15 +225,30 @@ __global__ __launch_bounds__(256) void k_face_gather(const unsigned long long* _
     }
 }
 
-__global__ void k_face_sums(const double* __restrict__ part, int nb, double* __restrict__ sums) {
-    if (threadIdx.x != 0) return;
+// One block: each thread folds the partials b = t, t + 256, ... in order, then
+// a fixed tree (the same sum on every run).
+__global__ __launch_bounds__(256) void k_face_sums(const double* __restrict__ part, int nb,
+                                                   double* __restrict__ sums) {
     double a = 0.0, e = 0.0;
-    for (int b = 0; b < nb; ++b) {
+    for (int b = (int)threadIdx.x; b < nb; b += 256) {
         a += part[2 * b];
         e = fmax(e, part[2 * b + 1]);
     }
-    sums[0] = a;
-    sums[1] = e;
+    __shared__ double sa[256], se[256];
+    sa[threadIdx.x] = a;
+    se[threadIdx.x] = e;
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            sa[threadIdx.x] += sa[threadIdx.x + w];
+            se[threadIdx.x] = fmax(se[threadIdx.x], se[threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        sums[0] = sa[0];
+        sums[1] = se[0];
+    }
 }
 
 static int face_blocks(int nF) { return std::max(1, nblk(nF, 256)); }
@@ -261,7 +276,7 @@ void launch_face_order(const float4* V, const int* F, int nF, const double lo[3]
                        e[2], keys);
     radix_sort_u64(keys, nF, 64, alt, hist, hsum, s, 32);
     hipLaunchKernelGGL(k_face_gather, dim3(face_blocks(nF)), dim3(256), 0, s, keys, F, V, nF, F_out, tri, part);
-    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(64), 0, s, part, face_blocks(nF), sums);
+    hipLaunchKernelGGL(k_face_sums, dim3(1), dim3(256), 0, s, part, face_blocks(nF), sums);
 }
 
 __global__ void k_site_keys(const float4* __restrict__ S, int nS, GridP g, unsigned long long* __restrict__ keys) {
@@ -384,7 +399,10 @@ __global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ desc
     const int i = br.blk * 256 + (int)threadIdx.x;
     if (i >= a.nS) return;
     SiteCtx c = site_ctx_by_index(a, eval_only != 0, i);
+    CVT_CLOCK(t_frame0);
     const bool ok = phase_frame<KP, false>(a, eval_only != 0, c);
+    CVT_CLOCK(t_frame1);
+    CVT_STAGE_ADD(a, 5, t_frame1 - t_frame0);
     a.fnn[i] = ok ? c.nn : kFrameDeferred;
 }
 
@@ -422,13 +440,18 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
 #pragma unroll
             for (int k = 0; k < KP; ++k) vl[k] = -1;
             bool ok = true;
+            CVT_CLOCK(t_list0);
             if (needs_list(c, cr)) {
                 float tauv;
                 ok = pool_lookup_with<KP>(a, cr.c3, vc, p0, p1, vl, tauv) ||
                      vertex_search<KP, false>(a, i, cr.c3, tauv, vl);
             }
+            CVT_CLOCK(t_list1);
+            CVT_STAGE_ADD(a, 3, t_list1 - t_list0);
             if (ok) {
                 project_finish<KP>(a, par, false, c, cr, vl, f);
+                CVT_CLOCK(t_face1);
+                CVT_STAGE_ADD(a, 4, t_face1 - t_list1);
             } else {
                 f.kind = cr.kind;
                 f.same = cr.same;

@@ -235,11 +235,15 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
             ctau = fminf(rc * rc, tau);
         }
         float lk;
+        CVT_CLOCK(t_knn0);
         r.kind = knn_exact<K, PRECISE, PRECISE ? 0 : kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t,
                                                                    tau, slots, lk, lds, ctau, nullptr, a.sg.cid);
         if (r.kind < 0) return false;
         if (!eval_only) a.rad2[c.i] = lk;
+        CVT_CLOCK(t_knn1);
+        if (!PRECISE) CVT_STAGE_ADD(a, 0, t_knn1 - t_knn0);
     }
+    CVT_CLOCK(t_dual0);
     const float3 u = c.u, w = c.w;
     CVT_G int* __restrict__ row = a.knn + (size_t)c.i * K;
     // the K neighbours: the proven row (sites by id) or the new list (grid
@@ -329,7 +333,11 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
         // value; 2 / |d| is largest for the nearest neighbour
         const float eta = 4.8e-7f * sqrtf(2.f * icmax);
         // near-collinear orientation tests are decided in FP64 inline (both passes)
+        CVT_CLOCK(t_dual1);
+        CVT_STAGE_ADD(a, 1, t_dual1 - t_dual0);
         const CellOut co = dual_hull_cell<K + 4>(px, py, eta);
+        CVT_CLOCK(t_hull1);
+        CVT_STAGE_ADD(a, 2, t_hull1 - t_dual1);
         r.cell_ok = co.ok;
         r.area = co.area;
         r.moment = co.moment;

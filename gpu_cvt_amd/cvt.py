@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libcvt_hip.so")
+# CVT_LIB selects another build of the library (the stage-clock profiling build)
+LIB_PATH = os.environ.get("CVT_LIB") or os.path.join(_HERE, "libcvt_hip.so")
 
 CVT_OK = 0
 STATUS = {-1: "EINVAL", -2: "EHIP", -3: "ENOMEM", -4: "ESTATE", -5: "ENONFINITE"}
