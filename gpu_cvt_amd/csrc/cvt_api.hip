@@ -175,8 +175,8 @@ struct cvt_job {
     int *has_prev = nullptr, *stats = nullptr, *hist = nullptr, *hist_count = nullptr, *defer_list = nullptr;
     int* seed_ok = nullptr;   // search seeds (rad2, dmax) valid
     float *rad2 = nullptr, *dmax = nullptr;  // per-site seed radius^2; max displacement per parity
-    float4 *fcache = nullptr, *vcache = nullptr, *gaps = nullptr;  // per-site vertex-search caches
-    int4* vpool = nullptr;  // candidate pool of the centroid vertex search
+    float4* vcache = nullptr;  // per-site vertex pool cache: centroid and bound
+    int4* vpool = nullptr;     // the pool (kPoolVecs words per site)
     int* fnn = nullptr;       // FP32-pass hand-offs: frame vertex per site,
     int* slot_of = nullptr;   // each site's slot in the current site grid
     float4* cent = nullptr;   // centroid and cell flags per site
@@ -428,9 +428,7 @@ JobDesc make_desc(cvt_job* j) {
     a.defer_list = to_g(j->defer_list);
     a.rad2 = to_g(j->rad2);
     a.seed_ok = to_g(j->seed_ok);
-    a.fcache = to_g(j->fcache);
     a.vcache = to_g(j->vcache);
-    a.gaps = to_g(j->gaps);
     a.vpool = to_g(j->vpool);
     a.fnn = to_g(j->fnn);
     a.slot_of = to_g(j->slot_of);
@@ -585,7 +583,9 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     for (int a = 0; a < 3; ++a) { j->lo[a] = 1e300; j->hi[a] = -1e300; }
     for (int v = 0; v < nV; ++v) {
         float x = (float)Vh[3 * v], y = (float)Vh[3 * v + 1], z = (float)Vh[3 * v + 2];
-        V4[v] = make_float4(x, y, z, 0.f);
+        float vid;  // the vertex id bits: V doubles as a point store with index ties
+        std::memcpy(&vid, &v, 4);
+        V4[v] = make_float4(x, y, z, vid);
         double c[3] = {x, y, z};
         for (int a = 0; a < 3; ++a) { j->lo[a] = std::min(j->lo[a], c[a]); j->hi[a] = std::max(j->hi[a], c[a]); }
     }
@@ -652,8 +652,8 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         (rc = j->alloc(&j->stats, kStatCopies * kStatStride)) || (rc = j->alloc(&j->hist, (size_t)kHistCap * kStatWords)) ||
         (rc = j->alloc(&j->hist_count, 1)) || (rc = j->alloc(&j->defer_list, nS)) ||
         (rc = j->alloc(&j->seed_ok, 1)) || (rc = j->alloc(&j->rad2, nS)) ||
-        (rc = j->alloc(&j->fcache, nS)) || (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) || (rc = j->alloc(&j->gaps, nS)) ||
-        (rc = j->alloc(&j->vpool, 2L * nS)) || (rc = j->alloc(&j->dmax, 2)) ||
+        (rc = j->alloc(&j->cid, nS)) || (rc = j->alloc(&j->vcache, nS)) ||
+        (rc = j->alloc(&j->vpool, (long)kPoolVecs * nS)) || (rc = j->alloc(&j->dmax, 2)) ||
         (rc = j->alloc(&j->fnn, nS)) || (rc = j->alloc(&j->slot_of, nS)) || (rc = j->alloc(&j->cent, nS)) ||
         (rc = j->alloc(&j->vs_list, nS)) ||
         (rc = j->alloc(&j->parity, 1)) || (rc = j->alloc(&j->desc, 1)))
@@ -689,9 +689,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill(j->has_prev, 1, 0, s);
     launch_fill(j->parity, 1, 0, s);
     launch_fill(j->seed_ok, 1, 0, s);
-    launch_fill((int*)j->fcache, 4L * nS, 0x7f800000, s);  // +inf: no cached search yet
-    launch_fill((int*)j->vcache, 4L * nS, 0x7f800000, s);
-    launch_fill((int*)j->gaps, 4L * nS, (int)0xbf800000, s);  // gaps -1: never reusable
+    launch_fill((int*)j->vcache, 4L * nS, 0x7f800000, s);  // +inf: no cached search yet
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatCopies * kStatStride, 0, s);
@@ -1035,18 +1033,23 @@ int cvt_job_quality(cvt_job* j, const int* F, int nF, cvt_quality* out) {
 #ifdef CVT_STAGE_CLOCKS
 // Profiling build only (no declaration in cvt.h): the per-stage shader-clock
 // cycle sums of the job's last iteration, kStageCount of them (stage order as
-// at kStatClocks in cvt_kernels.h).
+// at kStatClocks in cvt_kernels.h), then kStageCounters event counts.
 int cvt_stage_clocks(cvt_job* j, unsigned long long* out) {
     if (!j || !out) return fail(CVT_EINVAL, "null argument");
     std::vector<int> h((size_t)kStatCopies * kStatStride);
     HIPCHK(hipStreamSynchronize(j->stream));
     HIPCHK(hipMemcpy(h.data(), j->stats, sizeof(int) * h.size(), hipMemcpyDeviceToHost));
-    for (int k = 0; k < kStageCount; ++k) {
+    for (int k = 0; k < kStageCount + kStageCounters; ++k) {
         out[k] = 0;
         for (int c = 0; c < kStatCopies; ++c) {
-            unsigned long long v;
-            std::memcpy(&v, &h[(size_t)c * kStatStride + kStatClocks + 2 * k], 8);
-            out[k] += v;
+            const int* w = &h[(size_t)c * kStatStride];
+            if (k < kStageCount) {
+                unsigned long long v;
+                std::memcpy(&v, w + kStatClocks + 2 * k, 8);
+                out[k] += v;
+            } else {
+                out[k] += (unsigned)w[kStageCountWord + k - kStageCount];
+            }
         }
     }
     return 0;

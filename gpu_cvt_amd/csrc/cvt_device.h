@@ -626,13 +626,14 @@ __device__ __forceinline__ void grid_exact64(const GridP& g, const CVT_G int* __
 // must go to the FP64 pass.
 // tau: a provable upper bound of the (K+2)-th distance^2, or inf.
 // l_out: the FP32 (K+2)-th distance^2 (the next iteration's seed), or inf.
-// lds (optional, with collect_tau finite): try grid_collect_topl first.
+// lds (optional, with collect_tau finite): try grid_collect_topl first;
+// collected (optional) receives whether that collect proved the list.
 template <int K, bool PRECISE, int CAP = 0>
 __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __restrict__ start, const CVT_G float4* __restrict__ pts,
                                          float qx, float qy, float qz, int self_slot, float tau, int* __restrict__ slot_out,
                                          float& l_out, int* lds = nullptr, float collect_tau = 0.f,
                                          float* d_out = nullptr, const CVT_G int* __restrict__ cid = nullptr,
-                                         const PointBVH* pbvh = nullptr) {
+                                         const PointBVH* pbvh = nullptr, bool* collected = nullptr) {
     constexpr int L = K + 2;
     TopL<L> T;
     bool have = false;
@@ -642,6 +643,7 @@ __device__ __forceinline__ int knn_exact(const GridP& g, const CVT_G int* __rest
             have = grid_collect_topl<L, CAP>(g, start, pts, qx, qy, qz, self_slot, collect_tau, lds, T, worst_up);
     }
     const bool keyed = have;  // T.d holds truncated keys
+    if (collected) *collected = have;
     bool limited = false;
     if (!have) {
         // with a point LBVH, the ring search stops after two shells and the

@@ -27,25 +27,21 @@ constexpr int kStatVsearch = 13;
 constexpr int kStatFarWalks = 14;
 
 // Stage clocks (a profiling build, compiled with -DCVT_STAGE_CLOCKS): every
-// lane adds its shader-clock cycles per stage to 64-bit sums in words
+// lane's shader-clock cycles per stage go into 64-bit sums in words
 // kStatClocks.. of its block's stats copy (kStatCopies spread the atomics):
 // stage 0 the cell kernel's site KNN (seeds, collect, selection, order proof),
 // 1 its row write and dual points, 2 its dual hull walk and moments, 3 the
 // projection kernel's vertex list (pool cache or grid search), 4 its face
 // walk, new site and freeze step, 5 the frame kernel's nearest-vertex search
-// and frame.  The product build compiles none of it.
+// and frame.  32-bit event counts follow the clocks: 0 the projection
+// kernel's vertex lists, 1 those the pool cache could not prove (a grid
+// search), 2 the frame kernel's nearest-vertex searches (the gap cache did not
+// prove the vertex), 3 the cell kernel's site searches that the LDS collect
+// could not prove (the ring search ran).  The product build compiles none of it.
 constexpr int kStatClocks = 16;
 constexpr int kStageCount = 6;
-#ifdef CVT_STAGE_CLOCKS
-#define CVT_CLOCK(v) const unsigned long long v = clock64()
-#define CVT_STAGE_ADD(a, k, cycles)                                                                            \
-    atomicAdd(reinterpret_cast<unsigned long long*>((a).stats + (blockIdx.x % kStatCopies) * kStatStride + \
-                                                    kStatClocks) + (k),                                      \
-              (unsigned long long)(cycles))
-#else
-#define CVT_CLOCK(v)
-#define CVT_STAGE_ADD(a, k, cycles)
-#endif
+constexpr int kStageCounters = 4;
+constexpr int kStageCountWord = kStatClocks + 2 * kStageCount;
 // The per-site counters (words 0-8) and the largest displacement (word 10, float
 // bits, atomic max) are accumulated per block into one of kStatCopies copies of
 // the stat words, each on its own 128-byte line (copy c at stats[c * kStatStride]),
@@ -53,6 +49,32 @@ constexpr int kStageCount = 6;
 // end folds the copies into copy 0, which also holds the deferred-list lengths.
 constexpr int kStatCopies = 8;
 constexpr int kStatStride = 32;
+static_assert(kStageCountWord + kStageCounters <= kStatStride, "stage words fit one stat copy");
+#ifdef CVT_STAGE_CLOCKS
+// The shader clock is read per wave, so the lanes active at an add hold the
+// same cycle count: the wave's first active lane adds it once per active lane
+// (one atomic per wave; per-lane atomics on one address would serialise).
+__device__ __forceinline__ void stage_add(CVT_G int* stats, int k, unsigned long long v) {
+    const unsigned long long m = __ballot(1);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1)
+        atomicAdd(reinterpret_cast<CVT_G unsigned long long*>(stats + (blockIdx.x % kStatCopies) * kStatStride +
+                                                        kStatClocks) + k,
+                  v * (unsigned long long)__popcll(m));
+}
+// Event count: the wave's first active lane adds the number of active lanes with pred.
+__device__ __forceinline__ void stage_count(CVT_G int* stats, int k, bool pred) {
+    const unsigned long long m = __ballot(1), c = __ballot(pred);
+    if ((int)(threadIdx.x & 63) == __ffsll((long long)m) - 1 && c)
+        atomicAdd(stats + (blockIdx.x % kStatCopies) * kStatStride + kStageCountWord + k, (int)__popcll(c));
+}
+#define CVT_CLOCK(v) const unsigned long long v = clock64()
+#define CVT_STAGE_ADD(a, k, cycles) stage_add((a).stats, (k), (unsigned long long)(cycles))
+#define CVT_STAGE_COUNT(a, k, pred) stage_count((a).stats, (k), (pred))
+#else
+#define CVT_CLOCK(v)
+#define CVT_STAGE_ADD(a, k, cycles)
+#define CVT_STAGE_COUNT(a, k, pred)
+#endif
 // Blocks per job of the FP64 pass: one per 256 sites up to kPreciseMaxBlocks
 // (about two per CU), striding over the job's deferred list, so that a list
 // of any length is spread over the whole device.
@@ -78,8 +100,22 @@ __device__ __forceinline__ int xcd_block(int b, int nb8) {
 }
 
 // Static per-job mesh data on the device.
+// Vertex pool cache of the kproj = 5 search, per site: the kPoolIds nearest
+// mesh vertices (ids, nearest first) at the centroid of the site's last
+// search, then an upper bound of the (kPoolIds + 2)-th nearest distance there
+// (float bits: the next search's seed) and a lower bound of the 5th-nearest
+// (float bits: the frame's nearest-vertex proof), in kPoolVecs int4 words.
+// vcache holds that centroid and a lower bound of the (kPoolIds + 1)-th
+// nearest distance.  Other kproj values keep no pool.
+constexpr int kPoolKP = 5;
+constexpr int kPoolIds = 7;
+constexpr int kPoolSeedWord = kPoolIds;
+constexpr int kPoolFrameWord = kPoolIds + 1;
+constexpr int kPoolVecs = (kPoolIds + 2 + 3) / 4;
+constexpr int kPoolLookVecs = (kPoolIds + 1 + 3) / 4;  // the words a list lookup reads: ids and seed
+
 struct MeshDev {
-    const CVT_G float4* V;      // nV  (x, y, z, 0)
+    const CVT_G float4* V;      // nV  (x, y, z, vertex id bits)
     const CVT_G float4* N;      // nV  unit vertex normals (FP32)
     const CVT_G float4* FU;     // nV  tangent frame u of each vertex normal (site_normals.cuh:68-110)
     const CVT_G float4* FW;     // nV  tangent frame v
@@ -121,10 +157,9 @@ struct StepArgs {
     CVT_G int* defer_list;      // nS: sites handed from the FP32 pass to the precise pass
     CVT_G float* rad2;          // nS: last FP32 (K+2)-th neighbour distance^2 per site (search seed)
     CVT_G int* seed_ok;         // device flag: rad2 and dmax are valid seeds for this iteration
-    CVT_G float4* fcache;       // nS: (site position at its last vertex search, 3rd-nearest distance there; +inf: none)
-    CVT_G float4* vcache;       // nS: (centroid at its last kproj search, 8th-nearest distance there; +inf: none)
-    CVT_G int4* vpool;          // 2 nS: the 7 nearest vertices there, 9th-nearest distance (float bits)
-    CVT_G float4* gaps;         // nS: (1st/2nd-nearest gap at fcache, nearest vertex id bits, -, -)
+    CVT_G float4* vcache;       // nS: (centroid at its last kproj search, a lower bound of the
+                                //      (kPoolIds + 1)-th nearest vertex distance there; +inf: none)
+    CVT_G int4* vpool;          // kPoolVecs nS: the pool of that search (layout at kPoolIds)
     CVT_G float* dmax01;        // [2] largest site displacement per parity (float bits, atomic max)
     CVT_G int* fnn;             // nS: frame vertex of this iteration (hand-off of the FP32 pass; -2: deferred)
     CVT_G int* slot_of;         // nS: each site's slot in this iteration's site grid (sg.pts)

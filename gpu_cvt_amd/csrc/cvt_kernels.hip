@@ -422,11 +422,14 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
         // the hand-off and the pool cache entry in one round trip
         const float4 ce = a.cent[i];
         float4 vc = make_float4(0.f, 0.f, 0.f, __builtin_huge_valf());
-        int4 p0 = make_int4(0, 0, 0, 0), p1 = p0;
-        if (c.valid && !c.fz && c.use_cache) {
+        int4 pv[kPoolLookVecs];
+#pragma unroll
+        for (int v = 0; v < kPoolLookVecs; ++v) pv[v] = make_int4(0, 0, 0, 0);
+        if (KP == kPoolKP && c.valid && !c.fz && c.use_cache) {
             vc = a.vcache[i];
-            p0 = a.vpool[2 * (size_t)i];
-            p1 = a.vpool[2 * (size_t)i + 1];
+            const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * i;
+#pragma unroll
+            for (int v = 0; v < kPoolLookVecs; ++v) pv[v] = pw[v];
         }
         const int flags = c.valid ? __float_as_int(ce.w) : kCentDone;
         if (flags & kCentDone) {
@@ -443,8 +446,10 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
             CVT_CLOCK(t_list0);
             if (needs_list(c, cr)) {
                 float tauv;
-                ok = pool_lookup_with<KP>(a, cr.c3, vc, p0, p1, vl, tauv) ||
-                     vertex_search<KP, false>(a, i, cr.c3, tauv, vl);
+                const bool pooled = pool_lookup_with<KP>(a, cr.c3, vc, pv, vl, tauv);
+                ok = pooled || vertex_search<KP, false>(a, i, cr.c3, tauv, vl);
+                CVT_STAGE_COUNT(a, 0, true);
+                CVT_STAGE_COUNT(a, 1, !pooled);
             }
             CVT_CLOCK(t_list1);
             CVT_STAGE_ADD(a, 3, t_list1 - t_list0);

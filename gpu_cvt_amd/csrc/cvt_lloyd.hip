@@ -151,7 +151,7 @@ __global__ __launch_bounds__(256) void kb_lloyd_precise(const JobDesc* __restric
 template <int KP>
 __global__ __launch_bounds__(256) void kb_precise_vsearch(const JobDesc* __restrict__ descs,
                                                           const BlockRef* __restrict__ bt) {
-    static_assert(KP == 5, "the searches fill the vertex pool, which holds the kproj + 2 nearest");
+    static_assert(KP == kPoolKP, "the searches fill the vertex pool");
     __shared__ double vs_lds[4 * kVsLdsBytes / 8];
     const BlockRef br = bt[blockIdx.x / kWaveBlocks];
     if (br.job < 0) return;
@@ -166,18 +166,18 @@ __global__ __launch_bounds__(256) void kb_precise_vsearch(const JobDesc* __restr
         int ms;
         double mk;
         bool ok;
-        wave_bvh_topl<KP + 4, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, c4.x, c4.y, c4.z, lds, ms, mk, ok,
-                                                 vertex_excl(a, i));
-        int id[KP + 2];
+        wave_bvh_topl<kPoolIds + 2, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, c4.x, c4.y, c4.z, lds, ms, mk,
+                                                       ok, vertex_excl(a, i));
+        int id[kPoolIds];
 #pragma unroll
-        for (int k = 0; k < KP + 2; ++k) {
+        for (int k = 0; k < kPoolIds; ++k) {
             const int sl = __shfl(ms, k, 64);
             id[k] = sl >= 0 ? __float_as_int(a.m.vg_pts[sl].w) : -1;
         }
-        const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
+        const double k5 = __shfl(mk, 4, 64), kb = __shfl(mk, kPoolIds, 64), ks = __shfl(mk, kPoolIds + 1, 64);
         if (((int)threadIdx.x & 63) == 0) {
             if (ok) {
-                store_vertex_pool(a, i, c4.x, c4.y, c4.z, id, k8, k9);
+                store_vertex_pool(a, i, c4.x, c4.y, c4.z, id, k5, kb, ks);
             } else {
                 // beyond the wave's LDS: the per-lane search (rare)
                 int vl[KP];
@@ -214,8 +214,9 @@ __global__ __launch_bounds__(256) void kb_precise_finish(const JobDesc* __restri
             cr.ill = false;
             cr.kind = (flags >> kCentKindShift) & 3;
             cr.area = cr.moment = 0.f;
-            const int4 p0 = a.vpool[2 * (size_t)i];
-            int vl[KP] = {p0.x, p0.y, p0.z, p0.w, a.vpool[2 * (size_t)i + 1].x};
+            const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * i;
+            const int4 p0 = pw[0];
+            int vl[KP] = {p0.x, p0.y, p0.z, p0.w, pw[1].x};
             project_finish<KP, true>(a, par, false, c, cr, vl, f);
         }
         count_site(a, par, false, f, live);

@@ -94,14 +94,55 @@ __device__ __forceinline__ void set_frame(const StepArgs& a, SiteCtx& c) {
     }
 }
 
+// Word w of a site's pool (w a constant).
+template <int W>
+__device__ __forceinline__ int pool_word(const CVT_G int4* pw) {
+    const int4 v = pw[W / 4];
+    return W % 4 == 0 ? v.x : (W % 4 == 1 ? v.y : (W % 4 == 2 ? v.z : v.w));
+}
+
+// The nearest mesh vertex of q from the site's vertex pool cache (vc: the
+// centroid of its last kproj search and its bound; ids: the
+// pool's first four; b5: the 5th-nearest distance there, rounded down).
+// Every vertex but those four was at least b5 from vc.xyz, so at most
+// b5 - dl from q after a move by dl: when the nearest of the four by exact
+// (FP64 key, id) order lies closer than that, it is the nearest vertex
+// (returns true).  Otherwise tau1 receives the 3rd smallest of the four keys,
+// within which three vertices lie (the search's seed).
+__device__ __forceinline__ bool pool_nearest(const StepArgs& a, const float4& q, const float4& vc, const int4& ids,
+                                             float b5, int& nn, float& tau1) {
+    const float dl = sqrtf((q.x - vc.x) * (q.x - vc.x) + (q.y - vc.y) * (q.y - vc.y) + (q.z - vc.z) * (q.z - vc.z));
+    const int id[4] = {ids.x, ids.y, ids.z, ids.w};
+    double key[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
+        key[k] = id[k] >= 0 ? key64(q.x, q.y, q.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
+    }
+    double bk = key[0];
+    int bi = id[0] >= 0 ? id[0] : 0x7fffffff;
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        const int ik = id[k] >= 0 ? id[k] : 0x7fffffff;
+        const bool take = key[k] < bk || (key[k] == bk && ik < bi);
+        bk = take ? key[k] : bk;
+        bi = take ? ik : bi;
+    }
+    if (bk < __builtin_huge_val() && sqrtf((float)bk) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < b5 - dl) {
+        nn = bi;
+        return true;
+    }
+    // the 3rd smallest of four: the larger of (the smaller maximum of the two
+    // pairs, the larger minimum)
+    const double k3 = fmax(fmin(fmax(key[0], key[1]), fmax(key[2], key[3])), fmax(fmin(key[0], key[1]), fmin(key[2], key[3])));
+    tau1 = k3 < __builtin_huge_val() ? (float)k3 * (1.f + 1e-4f) : __builtin_huge_valf();
+    return false;
+}
+
 // ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110).
-// Search cache (per site): fcache = (position of the last vertex search, its
-// 3rd-nearest distance), gaps.x = gap between the 1st and 2nd nearest
-// distances there, gaps.y = that nearest vertex.  A point that moved by d
-// keeps its nearest vertex while 2d < gap (every other vertex gains at most d
-// on the nearest), so the search is skipped; otherwise the cached radius plus
-// d seeds it (triangle inequality).  A proven search refreshes the cache at
-// once (it is a valid proof for this position whatever the later phases do).
+// The nearest vertex comes from the pool cache of the site's last kproj search
+// when that proves it (pool_nearest; kproj = 5, and no vertex excluded from
+// the site's searches), else from the seeded grid search.
 // Returns false when the FP32 search could not be proven.
 template <int KP, bool PRECISE>
 __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, SiteCtx& c) {
@@ -112,37 +153,24 @@ __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, S
         const float4 q = c.q;
         float tau1 = inf;
         bool have = false;
-        float4 fc = make_float4(0.f, 0.f, 0.f, inf), gp = make_float4(-1.f, 0.f, -1.f, 0.f);
-        if (c.use_cache) {
-            fc = a.fcache[c.i];
-            gp = a.gaps[c.i];
-            if (fc.w < inf) {
-                const float dq = sqrtf((q.x - fc.x) * (q.x - fc.x) + (q.y - fc.y) * (q.y - fc.y) +
-                                       (q.z - fc.z) * (q.z - fc.z));
-                if (gp.x - 2.f * dq > 2e-5f * fc.w + 4.f * a.m.vg.eps_abs) {
-                    c.nn = __float_as_int(gp.y);
-                    have = true;
-                } else {
-                    const float rr = (fc.w + dq) * (1.f + 1e-4f);
-                    tau1 = rr * rr;
+        if constexpr (KP == kPoolKP) {
+            if (c.use_cache && !a.excl_vslot) {
+                const float4 vc = a.vcache[c.i];
+                if (vc.w < inf) {
+                    const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * c.i;
+                    have = pool_nearest(a, q, vc, pw[0], __int_as_float(pool_word<kPoolFrameWord>(pw)), c.nn, tau1);
                 }
             }
         }
+        if (!PRECISE) CVT_STAGE_COUNT(a, 2, !have);
         if (!have) {
             int s1[1];
-            float l1, d1[3];
+            float l1;
             const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1,
-                                                 nullptr, 0.f, d1, nullptr, &a.m.vb);
+                                                 nullptr, 0.f, nullptr, nullptr, &a.m.vb);
             ok = k1 >= 0;
             if (!PRECISE && k1 == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
             c.nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
-            if (ok && !eval_only) {
-                if (!c.use_cache) gp = a.gaps[c.i];
-                gp.x = c.nn >= 0 ? sqrtf(d1[1]) - sqrtf(d1[0]) : -1.f;
-                gp.y = __int_as_float(c.nn);
-                a.fcache[c.i] = make_float4(q.x, q.y, q.z, d1[2] < inf ? sqrtf(d1[2]) : inf);
-                a.gaps[c.i] = gp;
-            }
         }
     } else {
         c.nn = a.knnV[(size_t)c.i * KP];  // pinned at the freeze iteration
@@ -236,8 +264,11 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
         }
         float lk;
         CVT_CLOCK(t_knn0);
+        bool collected = false;
         r.kind = knn_exact<K, PRECISE, PRECISE ? 0 : kCollectCap>(a.sg.g, a.sg.start, a.sg.pts, q.x, q.y, q.z, t,
-                                                                   tau, slots, lk, lds, ctau, nullptr, a.sg.cid);
+                                                                   tau, slots, lk, lds, ctau, nullptr, a.sg.cid,
+                                                                   nullptr, &collected);
+        if (!PRECISE) CVT_STAGE_COUNT(a, 3, !collected);
         if (r.kind < 0) return false;
         if (!eval_only) a.rad2[c.i] = lk;
         CVT_CLOCK(t_knn1);
@@ -370,90 +401,151 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 // (the faces, the new site and the freeze state), run in one thread by
 // phase_project.
 
-// Vertex pool cache: vcache = (centroid at the last kproj search, the 8th-
-// nearest distance there), vpool = the 7 nearest vertices there and the 9th-
-// nearest distance (float bits).  After a move by d every vertex outside the
-// pool is at least (8th - d) away, so when the 5th nearest pooled vertex (exact
-// FP64 keys, index ties) is closer, the pool's first 5 are the list (returns
-// true); otherwise the 9th distance plus d seeds the search in tauv (triangle
-// inequality; +inf without a cache).
-// pool_lookup on a cache entry already loaded (vc: vcache, p0/p1: vpool).
-// The pool holds kproj + 2 = 7 vertices: with another kproj there is no
-// cache (every list is searched).
+// Vertex pool cache (layout at kPoolIds): after a move by d from the pool's
+// centroid, every vertex outside the pool is at least (bound - d) away, so
+// when the 5th nearest pooled vertex, in exact (FP64 key, id) order, is closer
+// than that, the pool's 5 nearest are the list (returns true).  The pooled
+// vertices are ranked by 32-bit keys as in grid_collect_topl (the FP32
+// distance bits with the low 4 bits replaced by the pool index, one
+// median-of-three per list entry); the order is exact when the first six
+// keys are pairwise apart by more than their error allowance.  Otherwise the
+// seed distance plus d seeds the search in tauv (triangle inequality; +inf
+// without a cache).  pv: the pool's first kPoolLookVecs int4 words.
 template <int KP>
-__device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3& c3, const float4& vc, const int4& p0,
-                                                 const int4& p1, int (&vl)[KP], float& tauv) {
+__device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3& c3, const float4& vc,
+                                                 const int4 (&pv)[kPoolLookVecs], int (&vl)[KP], float& tauv) {
     const float inf = __builtin_huge_valf();
     tauv = inf;
-    if constexpr (KP != 5) return false;
-    if (!(vc.w < inf)) return false;
-    const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
-                           (c3.z - vc.z) * (c3.z - vc.z));
-    int id[KP + 2] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z};
-    double key[KP + 2];
+    if constexpr (KP != kPoolKP) {
+        return false;
+    } else {
+        static_assert(kPoolIds <= 16 && kPoolIds > KP, "pool indices fit the 4 low key bits");
+        if (!(vc.w < inf)) return false;
+        const float dc = sqrtf((c3.x - vc.x) * (c3.x - vc.x) + (c3.y - vc.y) * (c3.y - vc.y) +
+                               (c3.z - vc.z) * (c3.z - vc.z));
+        int w[4 * kPoolLookVecs];
 #pragma unroll
-    for (int k = 0; k < KP + 2; ++k) {
-        const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
-        key[k] = id[k] >= 0 ? key64(c3.x, c3.y, c3.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
-        id[k] = id[k] >= 0 ? id[k] : 0x7fffffff;
-    }
-#pragma unroll
-    for (int p0i = 1; p0i < KP + 2; ++p0i) {  // insertion sort by (key, id)
-#pragma unroll
-        for (int k = p0i; k >= 1; --k) {
-            const bool sw = key[k] < key[k - 1] || (key[k] == key[k - 1] && id[k] < id[k - 1]);
-            const double tk = key[k];
-            const int tv = id[k];
-            key[k] = sw ? key[k - 1] : key[k];
-            id[k] = sw ? id[k - 1] : id[k];
-            key[k - 1] = sw ? tk : key[k - 1];
-            id[k - 1] = sw ? tv : id[k - 1];
+        for (int v = 0; v < kPoolLookVecs; ++v) {
+            w[4 * v] = pv[v].x; w[4 * v + 1] = pv[v].y; w[4 * v + 2] = pv[v].z; w[4 * v + 3] = pv[v].w;
         }
-    }
-    if (key[KP - 1] < __builtin_huge_val() &&
-        sqrtf((float)key[KP - 1]) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < vc.w - dc) {
+        constexpr int L = KP + 2;
+        unsigned key[L];
 #pragma unroll
-        for (int k = 0; k < KP; ++k) vl[k] = id[k];
-        return true;
+        for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < kPoolIds; ++k) {
+            // the position only (three registers per load in flight, not four)
+            const float3 p3 = *reinterpret_cast<const CVT_G float3*>(a.m.V + (w[k] >= 0 ? w[k] : 0));
+            const unsigned x =
+                w[k] >= 0 ? ((__float_as_uint(d2f(c3.x, c3.y, c3.z, make_float4(p3.x, p3.y, p3.z, 0.f))) & ~15u) |
+                             (unsigned)k)
+                          : 0xffffffffu;
+#pragma unroll
+            for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
+            key[0] = min(key[0], x);
+        }
+        // the seed, and whether the 5th lies within the bound, before the order
+        const float rr = (__int_as_float(w[kPoolSeedWord]) + dc) * (1.f + 1e-4f);
+        const bool inside = key[KP - 1] != 0xffffffffu &&
+                            sqrtf(__uint_as_float(key[KP - 1] | 15u) * (1.f + kRel)) * (1.f + 2e-5f) +
+                                    4.f * a.m.vg.eps_abs < vc.w - dc;
+        // exact order: consecutive entries up to the 6th provably apart (an
+        // ambiguous pair, rare, takes the search, which orders it by FP64 keys)
+        bool apart = inside;
+#pragma unroll
+        for (int t = 0; t < KP; ++t)
+            apart = apart && key[t + 1] != 0xffffffffu &&
+                    __uint_as_float(key[t + 1] & ~15u) > __uint_as_float(key[t] & ~15u) * (1.f + kRelKey);
+        if (apart) {
+#pragma unroll
+            for (int t = 0; t < KP; ++t) {
+                const unsigned kn = key[t] & 15u;
+                int sel = -1;
+#pragma unroll
+                for (int k = 0; k < kPoolIds; ++k) sel = kn == (unsigned)k ? w[k] : sel;
+                vl[t] = sel;
+            }
+            return true;
+        }
+        tauv = rr * rr;
+        return false;
     }
-    const float rr = (__int_as_float(p1.w) + dc) * (1.f + 1e-4f);
-    tauv = rr * rr;
-    return false;
 }
 
 template <int KP>
 __device__ __forceinline__ bool pool_lookup(const StepArgs& a, int i, const float3& c3, bool use_cache, int (&vl)[KP],
                                             float& tauv) {
     tauv = __builtin_huge_valf();
-    if (!use_cache) return false;
-    return pool_lookup_with<KP>(a, c3, a.vcache[i], a.vpool[2 * (size_t)i], a.vpool[2 * (size_t)i + 1], vl, tauv);
+    if (!use_cache || KP != kPoolKP) return false;
+    const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * i;
+    int4 pv[kPoolLookVecs];
+#pragma unroll
+    for (int v = 0; v < kPoolLookVecs; ++v) pv[v] = pw[v];
+    return pool_lookup_with<KP>(a, c3, a.vcache[i], pv, vl, tauv);
 }
 
 // The vertex-grid slot that site i's kproj search skips (-1: none).
 __device__ __forceinline__ int vertex_excl(const StepArgs& a, int i) { return a.excl_vslot ? a.excl_vslot[i] : -1; }
 
+// Writes site i's pool: the centroid, the ids, the bound (rounded down), the
+// seed (rounded up) and the frame bound (5th distance, rounded down).
+__device__ __forceinline__ void write_vertex_pool(const StepArgs& a, int i, float qx, float qy, float qz,
+                                                  const int (&id)[kPoolIds], float bound, float seed, float r5) {
+    a.vcache[i] = make_float4(qx, qy, qz, bound);
+    int w[4 * kPoolVecs];
+#pragma unroll
+    for (int k = 0; k < 4 * kPoolVecs; ++k) w[k] = k < kPoolIds ? id[k < kPoolIds ? k : 0] : 0;
+    w[kPoolSeedWord] = __float_as_int(seed);
+    w[kPoolFrameWord] = __float_as_int(r5);
+    CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * i;
+#pragma unroll
+    for (int v = 0; v < kPoolVecs; ++v) pw[v] = make_int4(w[4 * v], w[4 * v + 1], w[4 * v + 2], w[4 * v + 3]);
+}
+
+// The pool cache of site i after an exact search at (qx, qy, qz) from FP64
+// keys: kb of the (kPoolIds + 1)-th and k5 of the 5th nearest rounded down
+// (lower bounds: every vertex outside the pool, or past the first four, is at
+// least that far), ks of the (kPoolIds + 2)-th rounded up (it seeds the next
+// search from above).
+__device__ __forceinline__ void store_vertex_pool(const StepArgs& a, int i, float qx, float qy, float qz,
+                                                  const int (&id)[kPoolIds], double k5, double kb, double ks) {
+    const float inf = __builtin_huge_valf();
+    auto down = [&](double k) {
+        float r = k < 1e300 ? (float)sqrt(k) : inf;
+        if (r < inf && (double)r > sqrt(k)) r = __int_as_float(__float_as_int(r) - 1);
+        return r;
+    };
+    float rs = ks < 1e300 ? (float)sqrt(ks) : inf;
+    if (rs < inf && (double)rs < sqrt(ks)) rs = __int_as_float(__float_as_int(rs) + 1);
+    write_vertex_pool(a, i, qx, qy, qz, id, down(kb), rs, down(k5));
+}
+
 // The kproj nearest mesh vertices of c3 by the vertex grid (seed tauv), and
-// the pool cache refreshed.  Returns false when the FP32 search could not be
-// proven (never for PRECISE).
+// the pool cache refreshed (kproj = 5: the search takes the kPoolIds nearest,
+// and its FP32 list the two distances after them).
+// Returns false when the FP32 search could not be proven (never for PRECISE).
 template <int KP, bool PRECISE>
 __device__ __forceinline__ bool vertex_search(const StepArgs& a, int i, const float3& c3, float tauv, int (&vl)[KP]) {
     const float inf = __builtin_huge_valf();
-    int sl[KP + 2];
-    float lv, dv[KP + 4];
-    const int kk = knn_exact<KP + 2, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, vertex_excl(a, i),
-                                              tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
+    constexpr bool kPool = KP == kPoolKP;
+    constexpr int NS = kPool ? kPoolIds : KP;
+    int sl[NS];
+    float lv, dv[NS + 2];
+    const int kk = knn_exact<NS, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, c3.x, c3.y, c3.z, vertex_excl(a, i),
+                                          tauv, sl, lv, nullptr, 0.f, dv, nullptr, &a.m.vb);
     if (!PRECISE && kk < 0) return false;
     if (!PRECISE && kk == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
-    int id[KP + 2];
+    int id[NS];
 #pragma unroll
-    for (int k = 0; k < KP + 2; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
+    for (int k = 0; k < NS; ++k) id[k] = sl[k] >= 0 ? __float_as_int(a.m.vg_pts[sl[k]].w) : -1;
 #pragma unroll
     for (int k = 0; k < KP; ++k) vl[k] = id[k];
-    if constexpr (KP != 5) return true;  // no pool cache
-    a.vcache[i] = make_float4(c3.x, c3.y, c3.z, dv[KP + 2] < inf ? sqrtf(dv[KP + 2]) : inf);
-    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
-    a.vpool[2 * (size_t)i + 1] =
-        make_int4(id[4], id[5], id[6], __float_as_int(dv[KP + 3] < inf ? sqrtf(dv[KP + 3]) : inf));
+    if constexpr (kPool) {
+        // FP32 distances: each within a few roundings of the exact one, which
+        // the lookups' margins cover
+        auto r = [&](float d2) { return d2 < inf ? sqrtf(d2) : inf; };
+        write_vertex_pool(a, i, c3.x, c3.y, c3.z, id, r(dv[kPoolIds]), r(dv[kPoolIds + 1]), r(dv[4]));
+    }
     return true;
 }
 
@@ -629,36 +721,21 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
     r.nonfinite = c.live && !finite3(sp.x, sp.y, sp.z);
 }
 
+constexpr int kVsStack = 2048, kVsCand = 256;
+constexpr int kVsLdsBytes = 4 * kVsStack + 8 * kVsCand;  // per wave
+
 // The kproj nearest mesh vertices of c3 (and the pool cache refreshed) for
 // every lane of the wave with want set, one lane at a time by the whole wave
 // (wave_bvh_topl over the vertex LBVH: the FP64 pass's centroids are often far
 // off the surface, where a per-lane search is a long chain of dependent node
 // visits).  Every lane of the wave must call it.  Lanes whose search did not
 // fit the wave's LDS keep want set (the caller searches per lane).
-// The pool cache of site i after an exact search at (qx, qy, qz): its 7
-// nearest vertices, the 8th distance (rounded down: it bounds every vertex
-// outside the pool from below) and the 9th (rounded up: it seeds the next
-// search from above), from FP64 keys.
-__device__ __forceinline__ void store_vertex_pool(const StepArgs& a, int i, float qx, float qy, float qz,
-                                                  const int (&id)[7], double k8, double k9) {
-    const float inf = __builtin_huge_valf();
-    float r8 = k8 < 1e300 ? (float)sqrt(k8) : inf, r9 = k9 < 1e300 ? (float)sqrt(k9) : inf;
-    if (r8 < inf && (double)r8 > sqrt(k8)) r8 = __int_as_float(__float_as_int(r8) - 1);
-    if (r9 < inf && (double)r9 < sqrt(k9)) r9 = __int_as_float(__float_as_int(r9) + 1);
-    a.vcache[i] = make_float4(qx, qy, qz, r8);
-    a.vpool[2 * (size_t)i] = make_int4(id[0], id[1], id[2], id[3]);
-    a.vpool[2 * (size_t)i + 1] = make_int4(id[4], id[5], id[6], __float_as_int(r9));
-}
-
-constexpr int kVsStack = 2048, kVsCand = 256;
-constexpr int kVsLdsBytes = 4 * kVsStack + 8 * kVsCand;  // per wave
-
 template <int KP>
 __device__ __forceinline__ void wave_vertex_searches(const StepArgs& a, int i, float3 c3, bool& want, int (&vl)[KP],
                                                      char* lds) {
-    // with the pool (kproj 5): its 7, its 8th distance (bound) and 9th (seed)
-    constexpr bool kPool = KP == 5;
-    constexpr int L = kPool ? KP + 4 : KP;
+    // with the pool (kproj 5): its kPoolIds, then the bound and the seed distances
+    constexpr bool kPool = KP == kPoolKP;
+    constexpr int L = kPool ? kPoolIds + 2 : KP;
     const int lane = (int)threadIdx.x & 63;
     const int my_excl = vertex_excl(a, i);
     unsigned long long need = __ballot(want);
@@ -672,7 +749,7 @@ __device__ __forceinline__ void wave_vertex_searches(const StepArgs& a, int i, f
         bool ok;
         wave_bvh_topl<L, kVsStack, kVsCand>(a.m.vb, a.m.vg_pts, nullptr, qx, qy, qz, lds, ms, mk, ok, qe);
         if (!ok) continue;  // want stays set: lane l searches itself
-        constexpr int NI = kPool ? KP + 2 : KP;
+        constexpr int NI = kPool ? kPoolIds : KP;
         int id[NI];
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
@@ -685,8 +762,8 @@ __device__ __forceinline__ void wave_vertex_searches(const StepArgs& a, int i, f
             want = false;
         }
         if constexpr (kPool) {
-            const double k8 = __shfl(mk, KP + 2, 64), k9 = __shfl(mk, KP + 3, 64);
-            if (lane == 0) store_vertex_pool(a, qi, qx, qy, qz, id, k8, k9);
+            const double k5 = __shfl(mk, 4, 64), kb = __shfl(mk, kPoolIds, 64), ks = __shfl(mk, kPoolIds + 1, 64);
+            if (lane == 0) store_vertex_pool(a, qi, qx, qy, qz, id, k5, kb, ks);
         }
     }
 }
@@ -716,7 +793,7 @@ __device__ __forceinline__ void phase_project(const StepArgs& a, int par, bool e
             vertex_search<KP, true>(a, c.i, cr.c3, tauv, vl);
             want = false;
         }
-        if (post_search && KP == 5) {  // the post-pass searches fill the pool
+        if (post_search && KP == kPoolKP) {  // the post-pass searches fill the pool
             if (want) {
                 // without kCentDone: the projection kernel, which may still be
                 // running beside this pass, keeps skipping the site

@@ -49,15 +49,17 @@ def main(argv=None) -> int:
     V, F, S, K, _iters, _fz = mg.workload(args.workload)
     print(f"[stage_split] {args.workload}: nS={len(S)} nF={len(F)} K={K}", file=sys.stderr, flush=True)
     clk = np.zeros(len(STAGES))
+    counts = np.zeros(4)
     ms = {"cell": 0.0, "project": 0.0, "frame": 0.0, "grid": 0.0, "precise": 0.0, "iter": 0.0}
     with cvt.Job(V, F, S, K=K) as job:
         far = [r["far_walks"] for r in job.step(args.warmup, stats=True)]
         for _ in range(args.iters):
             st = job.profile_stages(1)
-            out = (C.c_ulonglong * len(STAGES))()
+            out = (C.c_ulonglong * (len(STAGES) + len(counts)))()
             if L.cvt_stage_clocks(job.handle, out) != 0:
                 raise RuntimeError(L.cvt_last_error().decode())
-            clk += np.array(out[:], dtype=np.float64)
+            clk += np.array(out[:len(STAGES)], dtype=np.float64)
+            counts += np.array(out[len(STAGES):], dtype=np.float64)
             for k in ms:
                 ms[k] += st[k] / args.iters
     print(f"# Stage split, {args.workload} (iterations {args.warmup + 1}-{args.warmup + args.iters}, "
@@ -73,6 +75,10 @@ def main(argv=None) -> int:
                   f"{1e3 * ms[kern] * share:.1f} |")
     print(f"\nOther stages per iteration: site-grid rebuild {1e3 * ms['grid']:.1f} us, FP64 pass "
           f"{1e3 * ms['precise']:.1f} us; whole iteration (kernel by kernel, with events) {1e3 * ms['iter']:.1f} us.")
+    n = counts / args.iters
+    print(f"\nPer iteration: projection kernel vertex lists {n[0]:.0f}, of which the pool cache could not prove "
+          f"{n[1]:.0f} ({100 * n[1] / max(n[0], 1):.1f} %: a grid search); frame kernel nearest-vertex searches "
+          f"(gap cache not proven) {n[2]:.0f}; cell kernel site searches the LDS collect did not prove {n[3]:.0f}.")
     print(f"\nFP32 vertex searches finished by the out-of-line LBVH walk (the FP32 kernels' only code with a "
           f"stack frame), iterations 1-{args.warmup}: {far}")
     return 0
