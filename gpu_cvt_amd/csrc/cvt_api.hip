@@ -156,6 +156,8 @@ struct cvt_job {
     // mesh
     float4 *V = nullptr, *N = nullptr, *FU = nullptr, *FW = nullptr, *tri = nullptr;
     int *F = nullptr, *vf_off = nullptr, *vf_faces = nullptr;
+    float4* ring = nullptr;  // fan rings (projection face walk)
+    int* ring_off = nullptr;
     GridP vg{};
     GridBufs vgb{};
     PointBVH vb{};  // LBVH over the vertex grid's points
@@ -402,6 +404,8 @@ JobDesc make_desc(cvt_job* j) {
     a.m.F = to_g(j->F);
     a.m.vf_off = to_g(j->vf_off);
     a.m.vf_faces = to_g(j->vf_faces);
+    a.m.ring = to_g(j->ring);
+    a.m.ring_off = to_g(j->ring_off);
     a.m.nV = j->nV;
     a.m.nF = j->nF;
     a.m.max_edge = (float)(j->max_edge * (1.0 + 1e-6));
@@ -643,7 +647,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     j->vg = make_grid(glo, ghi, vs * rV, 32L * nV + 4096);
 
     if ((rc = j->alloc(&j->N, nV)) || (rc = j->alloc(&j->FU, nV)) ||
-        (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->vf_off, nV + 1)) ||
+        (rc = j->alloc(&j->FW, nV)) || (rc = j->alloc(&j->vf_off, nV + 1)) || (rc = j->alloc(&j->ring_off, nV + 1)) ||
         (rc = j->alloc(&j->vf_faces, (size_t)3 * nF)) || (rc = alloc_grid(j, j->vgb, nV, j->vg.ncell)) ||
         (rc = alloc_grid(j, j->sgb, nS, j->sgp.ncell)) || (rc = j->alloc(&j->S[0], nS)) ||
         (rc = j->alloc(&j->S[1], nS)) || (rc = j->alloc(&j->knn, (size_t)nS * p.K)) ||
@@ -697,6 +701,19 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         void* csr_tmp = build_tmp.get((size_t)csr_temp_bytes(nV, nF));
         if (!csr_tmp) return bail(fail(CVT_ENOMEM, "CSR temporaries"));
         launch_build_csr(j->F, nF, nV, j->vf_off, j->vf_faces, csr_tmp, s);
+    }
+    {
+        // fan rings: entry counts and offsets, then the entries once their total is known
+        int* cnt = (int*)build_tmp.get(sizeof(int) * ((size_t)nV + 1));
+        int* bsum = (int*)build_tmp.get(sizeof(int) * (size_t)scan_partials(nV + 1));
+        if (!cnt || !bsum) return bail(fail(CVT_ENOMEM, "ring temporaries"));
+        launch_ring_count(j->vf_off, j->vf_faces, j->F, nV, cnt, j->ring_off, bsum, s);
+        int nring = 0;
+        if (hipMemcpyAsync(&nring, j->ring_off + nV, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            return bail(fail(CVT_EHIP, "ring size read back failed"));
+        if ((rc = j->alloc(&j->ring, (size_t)std::max(nring, 1)))) return bail(rc);
+        launch_ring_fill(j->vf_off, j->vf_faces, j->F, j->V, nV, j->ring_off, j->ring, s);
     }
     launch_normals(j->V, j->F, j->vf_off, j->vf_faces, nV, j->N, s);
     {

@@ -114,6 +114,8 @@ constexpr int kPoolFrameWord = kPoolIds + 1;
 constexpr int kPoolVecs = (kPoolIds + 2 + 3) / 4;
 constexpr int kPoolLookVecs = (kPoolIds + 1 + 3) / 4;  // the words a list lookup reads: ids and seed
 
+constexpr int kRingLast = (int)0x80000000;  // ring entry flag (id bits): the chain ends here
+
 struct MeshDev {
     const CVT_G float4* V;      // nV  (x, y, z, vertex id bits)
     const CVT_G float4* N;      // nV  unit vertex normals (FP32)
@@ -123,6 +125,11 @@ struct MeshDev {
     const CVT_G int* F;         // 3*nF  vertex ids
     const CVT_G int* vf_off;    // nV+1
     const CVT_G int* vf_faces;  // 3*nF
+    // Each vertex's fan as chains of corners (ring_off[v] .. ring_off[v+1]):
+    // (x, y, z, corner id bits, kRingLast on a chain's last corner); face k
+    // of a chain is (v, entry k, entry k + 1).  About 3 nF + nV entries.
+    const CVT_G float4* ring;
+    const CVT_G int* ring_off;  // nV+1
     int nV, nF;
     float max_edge;       // longest edge of the mesh (rounding allowance of the projection)
     GridP vg;             // static vertex grid (pts: x, y, z, vertex id bits)
@@ -261,6 +268,13 @@ void launch_mesh_compact(const double* V, int nV, int* F, int nF, int* used, int
                          hipStream_t s);
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s);
+// Fan rings (MeshDev::ring): per-vertex entry counts into cnt (nV ints) and
+// their exclusive scan into ring_off (nV + 1 ints; bsum: scan_partials(nV)
+// ints), then the entries.
+void launch_ring_count(const int* vf_off, const int* vf_faces, const int* F, int nV, int* cnt, int* ring_off,
+                       int* bsum, hipStream_t s);
+void launch_ring_fill(const int* vf_off, const int* vf_faces, const int* F, const float4* V, int nV,
+                      const int* ring_off, float4* ring, hipStream_t s);
 bool lloyd_supported(int K, int kproj);
 int scan_tile();
 // One Lloyd iteration of every job of a batch (all jobs share K and kproj).

@@ -920,6 +920,99 @@ void launch_build_csr(const int* F, int nF, int nV, int* vf_off, int* vf_faces, 
     launch_scan(cnt, nV, vf_off, bsum, s);
 }
 
+// ----------------------------------------------------------------- fan rings
+// Each vertex v's fan as chains of its faces' other corners: face (v, a, b),
+// with (a, b) the next two corners after v in the face's own order, follows a
+// face (v, a', b') in a chain when a = b'.  A chain lists its corners c0 c1
+// ... cm, face k being (v, ck, ck+1); a closed fan is one chain whose last
+// corner repeats its first, an open or non-manifold one several.  Fans of
+// more than kFanChainMax faces list every face as its own chain.
+constexpr int kFanChainMax = 32;
+
+// Calls emit(corner id, last) for the corners of v's chains in order; last:
+// the corner ends its chain (no face from it to the next corner).
+template <class Emit>
+__device__ void fan_chains(int v, const int* __restrict__ vf_off, const int* __restrict__ vf_faces,
+                           const int* __restrict__ F, Emit&& emit) {
+    const int o0 = vf_off[v], d = vf_off[v + 1] - o0;
+    auto corners = [&](int f, int& a, int& b) {
+        const int r = F[3 * f] == v ? 0 : (F[3 * f + 1] == v ? 1 : 2);
+        a = F[3 * f + (r + 1) % 3];
+        b = F[3 * f + (r + 2) % 3];
+    };
+    if (d > kFanChainMax) {
+        for (int k = 0; k < d; ++k) {
+            int a, b;
+            corners(vf_faces[o0 + k], a, b);
+            emit(a, false);
+            emit(b, true);
+        }
+        return;
+    }
+    int ea[kFanChainMax], eb[kFanChainMax];
+    for (int k = 0; k < d; ++k) corners(vf_faces[o0 + k], ea[k], eb[k]);
+    const unsigned full = d == 32 ? 0xffffffffu : (1u << d) - 1u;
+    unsigned used = 0;
+    while (used != full) {
+        // an open chain's first face (no unused face leads into it), else any unused face
+        int st = -1, first = -1;
+        for (int k = 0; k < d && st < 0; ++k) {
+            if (used >> k & 1u) continue;
+            if (first < 0) first = k;
+            bool led = false;
+            for (int j = 0; j < d; ++j) led = led || (!(used >> j & 1u) && j != k && eb[j] == ea[k]);
+            if (!led) st = k;
+        }
+        if (st < 0) st = first;
+        used |= 1u << st;
+        emit(ea[st], false);
+        int cur = eb[st];
+        for (;;) {
+            int nx = -1;
+            for (int j = 0; j < d && nx < 0; ++j)
+                if (!(used >> j & 1u) && ea[j] == cur) nx = j;
+            if (nx < 0) break;
+            used |= 1u << nx;
+            emit(cur, false);
+            cur = eb[nx];
+        }
+        emit(cur, true);
+    }
+}
+
+__global__ void k_ring_count(const int* __restrict__ vf_off, const int* __restrict__ vf_faces,
+                             const int* __restrict__ F, int nV, int* __restrict__ cnt) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    int n = 0;
+    fan_chains(v, vf_off, vf_faces, F, [&](int, bool) { ++n; });
+    cnt[v] = n;
+}
+
+__global__ void k_ring_fill(const int* __restrict__ vf_off, const int* __restrict__ vf_faces,
+                            const int* __restrict__ F, const float4* __restrict__ V, int nV,
+                            const int* __restrict__ ring_off, float4* __restrict__ ring) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nV) return;
+    int e = ring_off[v];
+    fan_chains(v, vf_off, vf_faces, F, [&](int c, bool last) {
+        const float4 p = V[c];
+        ring[e++] = make_float4(p.x, p.y, p.z, __int_as_float(c | (last ? kRingLast : 0)));
+    });
+}
+
+void launch_ring_count(const int* vf_off, const int* vf_faces, const int* F, int nV, int* cnt, int* ring_off,
+                       int* bsum, hipStream_t s) {
+    hipLaunchKernelGGL(k_ring_count, dim3(nblk(nV, 256)), dim3(256), 0, s, vf_off, vf_faces, F, nV, cnt);
+    launch_scan(cnt, nV, ring_off, bsum, s);
+}
+
+void launch_ring_fill(const int* vf_off, const int* vf_faces, const int* F, const float4* V, int nV,
+                      const int* ring_off, float4* ring, hipStream_t s) {
+    hipLaunchKernelGGL(k_ring_fill, dim3(nblk(nV, 256)), dim3(256), 0, s, vf_off, vf_faces, F, V, nV, ring_off,
+                       ring);
+}
+
 void launch_normals(const float4* V, const int* F, const int* vf_off, const int* vf_faces, int nV, float4* N,
                     hipStream_t s) {
     hipLaunchKernelGGL(k_vertex_normals, dim3(nblk(nV, 256)), dim3(256), 0, s, V, F, vf_off, vf_faces, nV, N);

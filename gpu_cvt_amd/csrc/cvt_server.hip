@@ -92,7 +92,7 @@ struct cvt_server {
 // exact figure replaces it once the job exists).
 static unsigned long long estimate_bytes(int nV, int nF, int nS, int K) {
     const double v = 16.0 * 2 * nV + 4.0 * (nV + 1) + 32.0 * nV;   // V, N, CSR offsets, vertex grid
-    const double f = 48.0 * nF + 12.0 * nF + 12.0 * nF;            // face table, F, CSR faces
+    const double f = 48.0 * nF + 12.0 * nF + 12.0 * nF + 16.0 * (3.0 * nF + nV) + 4.0 * nV;  // faces, F, CSR, rings
     const double s = nS * (16.0 * 2 + 4.0 * K + 20 + 3 + 4 + 4 + 16 + 4 * 2 + 4 + 4 * 32 + 20 + 32 + 4);
     return (unsigned long long)((v + f + s) * 1.25) + (1ull << 20);
 }

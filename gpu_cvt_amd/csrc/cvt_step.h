@@ -628,20 +628,22 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
             bool amb = false;
             const float tol2 = a.tol_abs * a.tol_abs;
             const float e3 = 3.f * a.m.max_edge;
+            // each list vertex's fan from its ring: face (v, e_j, e_j+1) for
+            // every entry e_j that does not end a chain; a face incident to an
+            // earlier list vertex was already visited
             int o0[KP], o1[KP];
+            float4 cv[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
-                o0[k] = ok ? a.m.vf_off[vl[k]] : 0;
-                o1[k] = ok ? a.m.vf_off[vl[k] + 1] : 0;
+                o0[k] = ok ? a.m.ring_off[vl[k]] : 0;
+                o1[k] = ok ? a.m.ring_off[vl[k] + 1] : 0;
+                cv[k] = a.m.V[ok ? vl[k] : 0];
             }
-            auto visit = [&](const float4& A, const float4& B, const float4& C, int k) {
+            auto visit = [&](const float4& A, const float4& B, const float4& C, int ib, int ic, int k) {
                 bool seen = false;
 #pragma unroll
-                for (int s2 = 0; s2 < KP - 1; ++s2) {
-                    const float vs = __int_as_float(vl[s2]);
-                    seen = seen || (s2 < k && ((A.w == vs) || (B.w == vs) || (C.w == vs)));
-                }
+                for (int s2 = 0; s2 < KP - 1; ++s2) seen = seen || (s2 < k && (ib == vl[s2] || ic == vl[s2]));
                 if (seen) return;
                 const float3 o = closest_offset_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
                                                     make_float3(C.x, C.y, C.z));
@@ -657,25 +659,21 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
                 }
                 if (d2 < best) { best = d2; bo = o; }
             };
-            // the fan bounds are loaded together and faces go kFaceBatch at a
-            // time, so that more loads are in flight per round trip
-            constexpr int kFaceBatch = 6;
+            // kRingBatch faces (kRingBatch + 1 entries) per round trip: a
+            // closed fan of valence six in one
+            constexpr int kRingBatch = 6;
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
-                for (int ii = o0[k]; ii < o1[k]; ii += kFaceBatch) {
-                    int f[kFaceBatch];
-                    float4 A[kFaceBatch], B[kFaceBatch], C[kFaceBatch];
+                for (int e = o0[k]; e + 1 < o1[k]; e += kRingBatch) {
+                    float4 E[kRingBatch + 1];
 #pragma unroll
-                    for (int b = 0; b < kFaceBatch; ++b) f[b] = a.m.vf_faces[min(ii + b, o1[k] - 1)];
+                    for (int b = 0; b <= kRingBatch; ++b) E[b] = a.m.ring[min(e + b, o1[k] - 1)];
 #pragma unroll
-                    for (int b = 0; b < kFaceBatch; ++b) {
-                        A[b] = a.m.tri[3 * f[b]];
-                        B[b] = a.m.tri[3 * f[b] + 1];
-                        C[b] = a.m.tri[3 * f[b] + 2];
+                    for (int b = 0; b < kRingBatch; ++b) {
+                        const int wb = __float_as_int(E[b].w);
+                        if (e + b + 1 < o1[k] && !(wb & kRingLast))
+                            visit(cv[k], E[b], E[b + 1], wb, __float_as_int(E[b + 1].w) & ~kRingLast, k);
                     }
-#pragma unroll
-                    for (int b = 0; b < kFaceBatch; ++b)
-                        if (ii + b < o1[k]) visit(A[b], B[b], C[b], k);
                 }
             }
             if (best < 1e30f) sp = make_float3(c3.x - bo.x, c3.y - bo.y, c3.z - bo.z);

@@ -51,6 +51,30 @@ def _case(name):
         V, F = mg.icosphere(4)
         S = mg.sample_surface(V, F, 1500, seed=20 + K)
         return V, F, S, K
+    if name == "hemisphere_open":  # an open surface: boundary fans are open chains; the lower
+        # half's vertices stay in V with no faces (empty fans)
+        V, F = mg.icosphere(4)
+        F = F[V[F].mean(axis=1)[:, 2] > 0.0]
+        S = mg.sample_surface(V, F, 1500, seed=31)
+        return V, F, S, 12
+    if name == "bowtie_nonmanifold":  # two spheres sharing one vertex (a fan of two cycles): a
+        # copy whose vertex q (normal about perpendicular to p's) lands on p;
+        # the copy's own q stays in V unused
+        V1, F1 = mg.icosphere(3)
+        p = int(np.argmax(V1[:, 0]))
+        q = int(np.argmin(np.abs(V1 @ V1[p])))
+        V2 = V1 - V1[q] + V1[p]
+        F2 = F1 + len(V1)
+        F2[F2 == q + len(V1)] = p
+        V, F = np.concatenate([V1, V2]), np.concatenate([F1, F2]).astype(np.int32)
+        S = mg.sample_surface(V, F, 1200, seed=32)
+        return V, F, S, 12
+    if name == "flipped_faces":  # every 7th face reversed: fans break into several chains
+        V, F = mg.icosphere(3)
+        F = F.copy()
+        F[::7] = F[::7, ::-1]
+        S = mg.sample_surface(V, F, 600, seed=33)
+        return V, F, S, 8
     if name == "dsphere_k32":
         V, F = mg.displaced_sphere(40, seed=4, crease_seed=5)
         S = mg.sample_surface(V, F, 6000, seed=6)
@@ -94,7 +118,8 @@ def _stage_check(orc, mesh, job, S32, K):
 
 
 @pytest.mark.parametrize("name", ["ico3_300", "ico5_2k_k20", "torus_20k_k24", "dsphere_k32", "cap_far",
-                                  "kcov_k12", "kcov_k16"])
+                                  "kcov_k12", "kcov_k16", "hemisphere_open", "bowtie_nonmanifold",
+                                  "flipped_faces"])
 def test_stages_teacher_forced(orc, name):
     V, F, S, K = _case(name)
     print(f"[parity] case={name} nV={len(V)} nF={len(F)} nS={len(S)} K={K}", flush=True)
