@@ -722,8 +722,11 @@ __device__ __forceinline__ int wave_exact_topl(const GridP& g, const CVT_G int* 
     int found = 0;
     for (int r = 1; r <= maxr; ++r) {
         const int x0 = max(cx - r, 0), x1 = min(cx + r, g.dx - 1);
-        // calls f(valid, slot) for every point of the block, 64 at a time
+        // calls f(valid, slot) for every point of the block, spread over the
+        // lanes: each row continues where the previous one stopped, so that
+        // short rows do not all fall on the first lanes
         auto for_block = [&](auto&& f) {
+            int carry = 0;  // points before this row, mod 64
             for (int dz = -r; dz <= r; ++dz) {
                 const int z = cz + dz;
                 if (z < 0 || z >= g.dz) continue;
@@ -732,10 +735,11 @@ __device__ __forceinline__ int wave_exact_topl(const GridP& g, const CVT_G int* 
                     if (y < 0 || y >= g.dy) continue;
                     const int row = (z * g.dy + y) * g.dx;
                     const int s0 = start[row + x0], s1 = start[row + x1 + 1];
-                    for (int base = s0; base < s1; base += 64) {
-                        const int sl = base + lane;
-                        f(sl < s1 && sl != self_slot, sl);
+                    for (int j = (lane - carry) & 63; s0 + j < s1; j += 64) {
+                        const int sl = s0 + j;
+                        f(sl != self_slot, sl);
                     }
+                    carry = (carry + (s1 - s0)) & 63;
                 }
             }
         };
