@@ -78,7 +78,7 @@ def main(argv=None) -> int:
     n = counts / args.iters
     print(f"\nPer iteration: projection kernel vertex lists {n[0]:.0f}, of which the pool cache could not prove "
           f"{n[1]:.0f} ({100 * n[1] / max(n[0], 1):.1f} %: a grid search); frame kernel nearest-vertex searches "
-          f"(gap cache not proven) {n[2]:.0f}; cell kernel site searches the LDS collect did not prove {n[3]:.0f}.")
+          f"(pool not proven) {n[2]:.0f}; cell kernel site searches the LDS collect did not prove {n[3]:.0f}.")
     print(f"\nFP32 vertex searches finished by the out-of-line LBVH walk (the FP32 kernels' only code with a "
           f"stack frame), iterations 1-{args.warmup}: {far}")
     return 0
