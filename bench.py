@@ -11,10 +11,14 @@ Prints one JSON line (rank 0).  Also measures, in the same run:
   * roofline: algorithmic bytes of one Lloyd iteration divided by the mean
     duration of the FP32 Lloyd pass -- the three kernels that do all of the
     per-site work, kb_frame + kb_cell + kb_project, launched back to back --
-    timed with HIP events on the job's stream;
-  * cpu_baseline: the FP64 CPU oracle's Lloyd step on this host, on a bounded
-    sample (a sub-sampled site set on the same mesh, or the whole job over as
-    many iterations as fill about half of --cpu-budget seconds).
+    timed with HIP events on the job's stream; beside it valu_frac, the pass's
+    VALU issue floor (from the committed SQ-counter profile) over the same time;
+  * cpu_baseline: an FP32 C++ Lloyd step and the FP64 CPU oracle's step on this
+    host's allowed CPUs, on a bounded sample (a sub-sampled site set on the
+    same mesh, or the whole job over as many iterations as fill about half of
+    --cpu-budget seconds);
+  * unless --no-extra: freezing on, K = 8, the batched form (256 jobs) and the
+    served form (a Poisson request trace) on the same GPU.
 Other modes: batch, serve, check (see --mode).
 """
 from __future__ import annotations
@@ -42,12 +46,12 @@ def algorithmic_bytes(nS: int, nV: int, nF: int, K: int) -> int:
 
 
 def traffic_from_profile(workload: str):
-    """HBM bytes per FP32 Lloyd pass (kb_frame + kb_cell + kb_project), as
-    recorded by the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this
-    workload (each counter scaled by its copy-kernel calibration factor);
-    (None, None) when no summary exists for the workload.  The counters cannot
-    be read from inside this process, so the figure comes from the profile run
-    of the same command."""
+    """Bytes past the L2 per FP32 Lloyd pass (kb_frame + kb_cell + kb_project),
+    as recorded by the committed rocprofv3 summary of this workload (reads from
+    the per-request-size L2 read counters, writes from WRITE_SIZE); (None,
+    None) when no summary exists for the workload.  The counters cannot be read
+    from inside this process, so the figure comes from the profile run of the
+    same command."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"lloyd_pass_{workload}_hbm_traffic.md")
     if not os.path.exists(path):
@@ -57,6 +61,22 @@ def traffic_from_profile(workload: str):
             if line.startswith("traffic_bytes_per_launch:"):
                 return int(line.split(":")[1].strip()), os.path.relpath(path, os.path.dirname(path) + "/..")
     return None, None
+
+
+def valu_issue_from_profile(workload: str):
+    """The VALU issue floor of the FP32 Lloyd pass in microseconds per launch
+    (VALU instructions per wave x 4 cycles x waves per SIMD at 2.4 GHz, summed
+    over kb_frame, kb_cell and kb_project), as recorded by the committed
+    rocprofv3 SQ-counter summary of this workload; None when there is none."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        f"lloyd_pass_{workload}_sq_counters.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            if line.startswith("valu_issue_us_per_launch:"):
+                return float(line.split(":")[1].strip())
+    return None
 
 
 def cpu_threads() -> tuple[int, int]:
@@ -453,6 +473,12 @@ def main(argv=None) -> int:
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                 "kernel": "FP32 Lloyd pass: kb_frame + kb_cell<24> + kb_project"}
+    # the roof that binds these kernels: their VALU issue floor (from the
+    # committed SQ-counter profile) against the same live FP32-pass time
+    valu_us = valu_issue_from_profile(args.workload)
+    if valu_us is not None:
+        roofline["valu_issue_us"] = valu_us
+        roofline["valu_frac"] = round(valu_us / (prof["lloyd_ms"] * 1e3), 4)
     whole_iter_gbs = alg / (ms_per_step * 1e-3) / 1e9
     print(f"[bench] timed {args.steps} steps: {ms:.3f} ms device ({wall * 1e3:.1f} ms wall); "
           f"profile {prof}; kernel {achieved:.1f} GB/s; whole iteration {whole_iter_gbs:.1f} GB/s",
