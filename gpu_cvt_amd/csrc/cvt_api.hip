@@ -188,7 +188,7 @@ struct cvt_job {
     int* parity = nullptr;    // device copy of `cur` (flipped by every iteration on the device)
     // this job as a batch of one: its descriptor and launch tables on the device
     JobDesc* desc = nullptr;
-    BlockRef *site_blocks = nullptr, *cell_blocks = nullptr, *tile_blocks = nullptr, *prec_blocks = nullptr;
+    BlockRef *site_blocks = nullptr, *tile_blocks = nullptr, *prec_blocks = nullptr;
     BatchTables tables{};
     // debug
     int* dbg_nn = nullptr;
@@ -480,13 +480,12 @@ JobDesc make_desc(cvt_job* j) {
 
 // Block tables of a set of jobs (job index k refers to descs[k]).
 struct HostTables {
-    std::vector<BlockRef> site, cell, tile, prec;
+    std::vector<BlockRef> site, tile, prec;
 };
 
 void add_job_blocks(HostTables& h, int k, int nS, int ncell1) {
     for (int b = 0; b < (nS + 255) / 256; ++b) h.site.push_back({k, b});
     for (int b = 0; b < precise_blocks(nS); ++b) h.prec.push_back({k, b});
-    for (int b = 0; b < (ncell1 + 1023) / 1024; ++b) h.cell.push_back({k, b});  // 1024 counters per block
     for (int b = 0; b < (ncell1 + scan_tile() - 1) / scan_tile(); ++b) h.tile.push_back({k, b});
 }
 
@@ -697,6 +696,7 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
     launch_fill((int*)j->dmax, 2, 0, s);
     launch_fill(j->hist_count, 1, 0, s);
     launch_fill(j->stats, kStatCopies * kStatStride, 0, s);
+    launch_fill(j->sgb.count, (long)j->sgp.ncell + 2, 0, s);  // each rebuild's scan clears them for the next
     {
         void* csr_tmp = build_tmp.get((size_t)csr_temp_bytes(nV, nF));
         if (!csr_tmp) return bail(fail(CVT_ENOMEM, "CSR temporaries"));
@@ -753,17 +753,15 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         HostTables ht;
         add_job_blocks(ht, 0, nS, j->sgp.ncell + 1);
         pad_site_blocks(ht);
-        if ((rc = j->alloc(&j->site_blocks, ht.site.size())) || (rc = j->alloc(&j->cell_blocks, ht.cell.size())) ||
-            (rc = j->alloc(&j->tile_blocks, ht.tile.size())) || (rc = j->alloc(&j->prec_blocks, ht.prec.size())))
+        if ((rc = j->alloc(&j->site_blocks, ht.site.size())) || (rc = j->alloc(&j->tile_blocks, ht.tile.size())) ||
+            (rc = j->alloc(&j->prec_blocks, ht.prec.size())))
             return bail(rc);
         if (up(j->desc, &dh, sizeof dh) || up(j->site_blocks, ht.site.data(), sizeof(BlockRef) * ht.site.size()) ||
-            up(j->cell_blocks, ht.cell.data(), sizeof(BlockRef) * ht.cell.size()) ||
             up(j->tile_blocks, ht.tile.data(), sizeof(BlockRef) * ht.tile.size()) ||
             up(j->prec_blocks, ht.prec.data(), sizeof(BlockRef) * ht.prec.size()))
             return bail(fail(CVT_EHIP, "upload failed"));
-        j->tables = BatchTables{j->desc, 1, j->site_blocks, (int)ht.site.size(), j->cell_blocks,
-                                (int)ht.cell.size(), j->tile_blocks, (int)ht.tile.size(), j->prec_blocks,
-                                (int)ht.prec.size()};
+        j->tables = BatchTables{j->desc,          1, j->site_blocks, (int)ht.site.size(), j->tile_blocks,
+                                (int)ht.tile.size(), j->prec_blocks, (int)ht.prec.size()};
     }
     hipError_t e2 = hipStreamSynchronize(s);
     if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup: %s", hipGetErrorString(e2)));
@@ -1570,18 +1568,15 @@ int cvt_batch_create(cvt_job* const* jobs, int njobs, cvt_batch** out) {
     };
     auto* dd = (JobDesc*)dalloc(sizeof(JobDesc) * njobs);
     auto* sb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.site.size());
-    auto* cb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.cell.size());
     auto* tb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.tile.size());
     auto* pb = (BlockRef*)dalloc(sizeof(BlockRef) * ht.prec.size());
-    if (!dd || !sb || !cb || !tb || !pb) return bail(fail(CVT_ENOMEM, "batch tables"));
+    if (!dd || !sb || !tb || !pb) return bail(fail(CVT_ENOMEM, "batch tables"));
     if (hipMemcpy(dd, descs.data(), sizeof(JobDesc) * njobs, hipMemcpyHostToDevice) ||
         hipMemcpy(sb, ht.site.data(), sizeof(BlockRef) * ht.site.size(), hipMemcpyHostToDevice) ||
-        hipMemcpy(cb, ht.cell.data(), sizeof(BlockRef) * ht.cell.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(tb, ht.tile.data(), sizeof(BlockRef) * ht.tile.size(), hipMemcpyHostToDevice) ||
         hipMemcpy(pb, ht.prec.data(), sizeof(BlockRef) * ht.prec.size(), hipMemcpyHostToDevice))
         return bail(fail(CVT_EHIP, "batch table upload failed"));
-    b->tables = BatchTables{dd, njobs, sb, (int)ht.site.size(), cb, (int)ht.cell.size(), tb, (int)ht.tile.size(),
-                            pb, (int)ht.prec.size()};
+    b->tables = BatchTables{dd, njobs, sb, (int)ht.site.size(), tb, (int)ht.tile.size(), pb, (int)ht.prec.size()};
     hipGraph_t g = nullptr;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
         return bail(fail(CVT_EHIP, "capture begin failed"));
@@ -1689,8 +1684,9 @@ struct TableRuns {
     }
 };
 
-// Tables: site, counter (cell), scan tile, FP64 pass, and the job slots.
-enum { kTabSite, kTabCell, kTabTile, kTabPrec, kTabSlot, kNumTab };
+// Tables: site, scan tile, FP64 pass (the block tables), and the job slots.
+enum { kTabSite, kTabTile, kTabPrec, kTabSlot, kNumTab };
+constexpr int kNumBlockTab = kTabSlot;
 
 struct SlotBatch {
     int K = 0, kproj = 0;
@@ -1699,13 +1695,13 @@ struct SlotBatch {
     DevArena* arena = current_arena();
     std::vector<cvt_job*> slots;  // nullptr: empty
     struct Place {
-        int slot, run[4], len[4];
+        int slot, run[kNumBlockTab], len[kNumBlockTab];
     };
     std::map<cvt_job*, Place> place;
-    TableRuns tab[4];
+    TableRuns tab[kNumBlockTab];
     // device copies (capacity-sized)
     JobDesc* d_descs = nullptr;
-    BlockRef* d_tab[4] = {nullptr, nullptr, nullptr, nullptr};
+    BlockRef* d_tab[kNumBlockTab] = {nullptr, nullptr, nullptr};
     std::vector<std::pair<void*, size_t>> dev;
     // the captured graph (kept: its kernel nodes are edited through it)
     hipGraph_t tmpl = nullptr;
@@ -1717,7 +1713,7 @@ struct SlotBatch {
         int table, mult;
     };
     std::vector<KNode> knodes;
-    int launched[kNumTab] = {0, 0, 0, 0, 0};  // extents the graph's grids cover now
+    int launched[kNumTab] = {0, 0, 0, 0};  // extents the graph's grids cover now
     SlotBatchCounters n{};
 
     int cap(int t) const { return t == kTabSlot ? (int)slots.size() : (int)tab[t].mirror.size(); }
@@ -1751,14 +1747,13 @@ struct SlotBatch {
 // spans by the grid's size, so the capacities are kept pairwise distinct by
 // their residues: site = 0 (mod 16) (the Lloyd launches also need a multiple
 // of 8), FP64 pass = 3 (mod 4) (its wave kernels launch kWaveBlocks = 4 times
-// that, = 12 (mod 16)), counters = 1 (mod 4), scan tiles = 2 (mod 4), slots
-// = 4 (mod 16).  The per-slot kernels of 64-thread blocks launch at most
+// that, = 12 (mod 16)), scan tiles = 2 (mod 4), slots = 4 (mod 16).  The per-slot kernels of 64-thread blocks launch at most
 // kSlotMaxJobs / 64 = 256 blocks, below every table's smallest capacity.
 static_assert(kWaveBlocks == 4, "capacity residues assume 4 wave blocks per entry");
 constexpr int kSlotMaxJobs = 16384;
-constexpr int kCapFloor[kNumTab] = {512, 260, 260, 260, 4};
-constexpr int kCapMod[kNumTab] = {16, 4, 4, 4, 16};
-constexpr int kCapRes[kNumTab] = {0, 1, 2, 3, 4};
+constexpr int kCapFloor[kNumTab] = {512, 260, 260, 4};
+constexpr int kCapMod[kNumTab] = {16, 4, 4, 16};
+constexpr int kCapRes[kNumTab] = {0, 2, 3, 4};
 static int cap_fit(int t, long want) {
     long c = std::max<long>(want, kCapFloor[t]);
     c += ((kCapRes[t] - c % kCapMod[t]) % kCapMod[t] + kCapMod[t]) % kCapMod[t];
@@ -1770,17 +1765,15 @@ static int cap_fit(int t, long want) {
 static void caps_initial(int jobs0, int site_blocks0, int cap[kNumTab]) {
     const long sb = site_blocks0 > 0 ? site_blocks0 : 65536;
     cap[kTabSite] = cap_fit(kTabSite, sb);
-    cap[kTabCell] = cap_fit(kTabCell, sb / 4);
     cap[kTabTile] = cap_fit(kTabTile, sb / 16);
     cap[kTabPrec] = cap_fit(kTabPrec, std::min<long>(sb, kPreciseMaxBlocks * 64L));
     cap[kTabSlot] = cap_fit(kTabSlot, std::min(jobs0 > 0 ? jobs0 : 1028, kSlotMaxJobs - 15));
 }
 
-// Block counts of a job in the four tables (as add_job_blocks lays them out).
-static void job_block_counts(cvt_job* j, int len[4]) {
+// Block counts of a job in the block tables (as add_job_blocks lays them out).
+static void job_block_counts(cvt_job* j, int len[kNumBlockTab]) {
     const int ncell1 = j->sgp.ncell + 1;
     len[kTabSite] = (j->nS + 255) / 256;
-    len[kTabCell] = (ncell1 + 1023) / 1024;
     len[kTabTile] = (ncell1 + scan_tile() - 1) / scan_tile();
     len[kTabPrec] = precise_blocks(j->nS);
 }
@@ -1827,7 +1820,7 @@ static int slot_batch_upload(SlotBatch* b) {
     for (int k = 0; k < nslot; ++k)
         if (b->slots[k]) descs[k] = make_desc(b->slots[k]);
     HIPCHK(hipMemcpyAsync(b->d_descs, descs.data(), sizeof(JobDesc) * nslot, hipMemcpyHostToDevice, b->stream));
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < kNumBlockTab; ++t)
         HIPCHK(hipMemcpyAsync(b->d_tab[t], b->tab[t].mirror.data(), sizeof(BlockRef) * b->cap(t),
                               hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
@@ -1842,14 +1835,13 @@ static int slot_batch_capture(SlotBatch* b, bool growth) {
     b->drop_graph();
     b->free_dev();
     b->d_descs = (JobDesc*)b->dalloc(sizeof(JobDesc) * b->cap(kTabSlot));
-    for (int t = 0; t < 4; ++t) b->d_tab[t] = (BlockRef*)b->dalloc(sizeof(BlockRef) * b->cap(t));
-    if (!b->d_descs || !b->d_tab[0] || !b->d_tab[1] || !b->d_tab[2] || !b->d_tab[3])
+    for (int t = 0; t < kNumBlockTab; ++t) b->d_tab[t] = (BlockRef*)b->dalloc(sizeof(BlockRef) * b->cap(t));
+    if (!b->d_descs || !b->d_tab[0] || !b->d_tab[1] || !b->d_tab[2])
         return fail(CVT_ENOMEM, "slot batch tables");
     int rc = slot_batch_upload(b);
     if (rc) return rc;
-    const BatchTables bt{b->d_descs,          b->cap(kTabSlot), b->d_tab[kTabSite], b->cap(kTabSite),
-                         b->d_tab[kTabCell],  b->cap(kTabCell), b->d_tab[kTabTile], b->cap(kTabTile),
-                         b->d_tab[kTabPrec],  b->cap(kTabPrec)};
+    const BatchTables bt{b->d_descs,         b->cap(kTabSlot), b->d_tab[kTabSite], b->cap(kTabSite),
+                         b->d_tab[kTabTile], b->cap(kTabTile), b->d_tab[kTabPrec], b->cap(kTabPrec)};
     HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     launch_batch_iteration(bt, b->K, b->kproj, b->stream, &b->side);
     HIPCHK(hipStreamEndCapture(b->stream, &b->tmpl));
@@ -1888,14 +1880,14 @@ static void slot_batch_layout(SlotBatch* b, const int cap[kNumTab]) {
     for (cvt_job* j : b->slots)
         if (j) members.push_back(j);
     b->slots.assign(std::max<int>(cap[kTabSlot], (int)members.size()), nullptr);
-    for (int t = 0; t < 4; ++t) b->tab[t].reset(cap[t]);
+    for (int t = 0; t < kNumBlockTab; ++t) b->tab[t].reset(cap[t]);
     b->place.clear();
     for (size_t k = 0; k < members.size(); ++k) {
         cvt_job* j = members[k];
         SlotBatch::Place pl{(int)k, {}, {}};
         job_block_counts(j, pl.len);
         b->slots[k] = j;
-        for (int t = 0; t < 4; ++t) {
+        for (int t = 0; t < kNumBlockTab; ++t) {
             pl.run[t] = b->tab[t].alloc(pl.len[t]);
             for (int q = 0; q < pl.len[t]; ++q) b->tab[t].mirror[pl.run[t] + q] = BlockRef{(int)k, q};
         }
@@ -1936,11 +1928,11 @@ int slot_batch_add(SlotBatch* b, cvt_job* j) {
     if (!j || j->p.K != b->K || j->p.kproj != b->kproj) return fail(CVT_EINVAL, "job does not fit the slot batch");
     if (b->place.count(j)) return fail(CVT_EINVAL, "job already in the slot batch");
     HIPCHK(hipStreamSynchronize(j->stream));
-    int len[4];
+    int len[kNumBlockTab];
     job_block_counts(j, len);
     // room to spare in every table (else grow), contiguous (else compact)
-    long need[kNumTab] = {0, 0, 0, 0, (long)b->place.size() + 1};
-    for (int t = 0; t < 4; ++t) need[t] = (long)b->tab[t].used + len[t];
+    long need[kNumTab] = {0, 0, 0, (long)b->place.size() + 1};
+    for (int t = 0; t < kNumBlockTab; ++t) need[t] = (long)b->tab[t].used + len[t];
     need[kTabSite] += 8;  // the site extent rounds up to a multiple of 8
     bool grow = false;
     for (int t = 0; t < kNumTab; ++t) grow |= need[t] > b->cap(t);
@@ -1959,7 +1951,7 @@ int slot_batch_add(SlotBatch* b, cvt_job* j) {
     while (b->slots[slot]) ++slot;
     SlotBatch::Place pl{slot, {}, {}};
     bool fits = true;
-    for (int t = 0; t < 4; ++t) {
+    for (int t = 0; t < kNumBlockTab; ++t) {
         pl.len[t] = len[t];
         pl.run[t] = fits ? b->tab[t].alloc(len[t]) : -1;
         fits &= pl.run[t] >= 0;
@@ -1967,7 +1959,7 @@ int slot_batch_add(SlotBatch* b, cvt_job* j) {
     if (!fits) {
         // no contiguous room (there is room in total): compact, the new job
         // in its free slot
-        for (int t = 0; t < 4; ++t)
+        for (int t = 0; t < kNumBlockTab; ++t)
             if (pl.run[t] >= 0) b->tab[t].release(pl.run[t], len[t]);
         b->slots[slot] = j;
         int rc = slot_batch_compact(b);
@@ -1975,13 +1967,13 @@ int slot_batch_add(SlotBatch* b, cvt_job* j) {
     }
     b->slots[slot] = j;
     b->place[j] = pl;
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < kNumBlockTab; ++t)
         for (int q = 0; q < len[t]; ++q) b->tab[t].mirror[pl.run[t] + q] = BlockRef{slot, q};
     if (!b->graph) return slot_batch_capture(b, false);
     // in place: the descriptor and the new block references, then the grids
     const JobDesc d = make_desc(j);
     HIPCHK(hipMemcpyAsync(b->d_descs + slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream));
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < kNumBlockTab; ++t)
         HIPCHK(hipMemcpyAsync(b->d_tab[t] + pl.run[t], &b->tab[t].mirror[pl.run[t]], sizeof(BlockRef) * len[t],
                               hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));  // the host sources must outlive the copies
@@ -1994,7 +1986,7 @@ int slot_batch_remove(SlotBatch* b, cvt_job* j) {
     const SlotBatch::Place pl = it->second;
     b->place.erase(it);
     b->slots[pl.slot] = nullptr;
-    for (int t = 0; t < 4; ++t) b->tab[t].release(pl.run[t], pl.len[t]);
+    for (int t = 0; t < kNumBlockTab; ++t) b->tab[t].release(pl.run[t], pl.len[t]);
     if (!b->graph) return 0;
     // holes below the extent are launched as empty blocks: compact when the
     // site table's extent is over twice what it holds
@@ -2002,7 +1994,7 @@ int slot_batch_remove(SlotBatch* b, cvt_job* j) {
     JobDesc d;
     std::memset((void*)&d, 0, sizeof d);
     HIPCHK(hipMemcpyAsync(b->d_descs + pl.slot, &d, sizeof d, hipMemcpyHostToDevice, b->stream));
-    for (int t = 0; t < 4; ++t)
+    for (int t = 0; t < kNumBlockTab; ++t)
         HIPCHK(hipMemcpyAsync(b->d_tab[t] + pl.run[t], &b->tab[t].mirror[pl.run[t]], sizeof(BlockRef) * pl.len[t],
                               hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));

@@ -226,8 +226,6 @@ struct BatchTables {
     int njobs;
     const BlockRef* site_blocks;   // 256 sites per block, padded to a multiple of 8
     int n_site_blocks;
-    const BlockRef* cell_blocks;   // 1024 count entries per block (256 threads x 4)
-    int n_cell_blocks;
     const BlockRef* tile_blocks;   // scan tiles of kScanTile entries
     int n_tile_blocks;
     const BlockRef* prec_blocks;   // precise_blocks(nS) per job (the FP64 pass)

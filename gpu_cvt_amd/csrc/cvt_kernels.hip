@@ -1,7 +1,7 @@
 // cvt_kernels.hip -- gfx950 kernels of the surface Lloyd iteration.
 //
 // One iteration (the body of the reference's main.cu:669-881) is:
-//   site grid rebuild   : kb_begin, kb_fill_count, kb_grid_count, kb_scan_* x3, kb_grid_scatter
+//   site grid rebuild   : kb_grid_count, kb_scan_reduce, kb_scan_down, kb_grid_scatter
 //   per-site step       : kb_frame, kb_cell (cvt_lloyd.hip, one object per K),
 //                         kb_project, then kb_lloyd_precise (cvt_lloyd.hip)
 //   iteration epilogue  : kb_end (stats history, parity flip)
@@ -480,19 +480,6 @@ __global__ void kb_begin(const JobDesc* __restrict__ descs, int njobs) {
     a.dmax01[*a.parity] = 0.f;
 }
 
-__global__ void kb_fill_count(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
-    const BlockRef br = bt[blockIdx.x];
-    if (br.job < 0) return;
-    const JobDesc& d = descs[br.job];
-    // four counters per thread (one int4 where the run is whole)
-    const int c = 4 * (br.blk * 256 + (int)threadIdx.x);
-    if (c + 4 <= d.ncell1) {
-        *(CVT_G int4*)(d.count + c) = make_int4(0, 0, 0, 0);
-    } else {
-        for (int k = c; k < d.ncell1; ++k) d.count[k] = 0;
-    }
-}
-
 __global__ void kb_grid_count(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt) {
     const BlockRef br = bt[blockIdx.x];
     if (br.job < 0) return;
@@ -551,27 +538,20 @@ __global__ __launch_bounds__(256) void kb_scan_reduce(const JobDesc* __restrict_
     if (threadIdx.x == 0) d.bsum[br.blk] = total;
 }
 
-__global__ __launch_bounds__(256) void kb_scan_bsum(const JobDesc* __restrict__ descs) {
-    const JobDesc& d = descs[blockIdx.x];
-    if (d.ncell1 == 0) return;  // empty slot
-    const int nb = (d.ncell1 + kScanTile - 1) / kScanTile;
-    int carry = 0;
-    for (int c = 0; c < nb; c += 256) {
-        const int j = c + threadIdx.x;
-        const int v = (j < nb) ? d.bsum[j] : 0;
-        int total;
-        const int ex = block_excl_scan(v, total);
-        if (j < nb) d.bsum[j] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) d.start[d.ncell1] = carry;
-}
-
+// The tile's cell starts from its counts: the tile's offset is the sum of
+// the tile sums before it (kb_scan_reduce's partials, summed by every block
+// itself); the tile that holds the last cell also writes the total.  The
+// counts are cleared as they are read, ready for the next rebuild's atomics
+// (a new job's counts start cleared).
 __global__ __launch_bounds__(256) void kb_scan_down(const JobDesc* __restrict__ descs,
                                                     const BlockRef* __restrict__ bt) {
     const BlockRef br = bt[blockIdx.x];
     if (br.job < 0) return;
     const JobDesc& d = descs[br.job];
+    int pre = 0;
+    for (int j = (int)threadIdx.x; j < br.blk; j += 256) pre += d.bsum[j];
+    int offset;
+    block_excl_scan(pre, offset);
     const int base = br.blk * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
     int s = 0;
@@ -582,18 +562,22 @@ __global__ __launch_bounds__(256) void kb_scan_down(const JobDesc* __restrict__ 
         for (int k = 0; k < kScanItems / 4; ++k) {
             const int4 q = *(const CVT_G int4*)(d.count + base + 4 * k);
             v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
+            *(CVT_G int4*)(d.count + base + 4 * k) = make_int4(0, 0, 0, 0);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             const int j = base + k;
             v[k] = (j < d.ncell1) ? d.count[j] : 0;
+            if (j < d.ncell1) d.count[j] = 0;
         }
     }
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) s += v[k];
     int total;
-    int run = block_excl_scan(s, total) + d.bsum[br.blk];
+    int run = block_excl_scan(s, total) + offset;
+    const int last = (d.ncell1 - 1) / kScanTile;
+    if (br.blk == last && threadIdx.x == 0) d.start[d.ncell1] = offset + total;
     if (full) {
 #pragma unroll
         for (int k = 0; k < kScanItems / 4; ++k) {
@@ -645,6 +629,12 @@ __global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
     a.stats[kStatMaxDisp] = (int)mx;
     a.stats[kStatDeferred] += a.stats[kStatDeferredB];
     for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
+#ifndef CVT_STAGE_CLOCKS
+    // the next iteration's counters start cleared (kb_begin's work; the
+    // profiling build keeps its stage words until they are read, so there
+    // every iteration starts with kb_begin)
+    for (int k = 0; k < kStatCopies * kStatStride; ++k) a.stats[k] = 0;
+#endif
     *a.hist_count = c + 1;
     *a.has_prev = 1;
     *a.seed_ok = 1;
@@ -1036,10 +1026,8 @@ bool lloyd_supported(int K, int kproj) {
 }
 
 static void launch_grid_rebuild(const BatchTables& bt, hipStream_t s) {
-    hipLaunchKernelGGL(kb_fill_count, dim3(bt.n_cell_blocks), dim3(256), 0, s, bt.descs, bt.cell_blocks);
     hipLaunchKernelGGL(kb_grid_count, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
     hipLaunchKernelGGL(kb_scan_reduce, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
-    hipLaunchKernelGGL(kb_scan_bsum, dim3(bt.njobs), dim3(256), 0, s, bt.descs);
     hipLaunchKernelGGL(kb_scan_down, dim3(bt.n_tile_blocks), dim3(256), 0, s, bt.descs, bt.tile_blocks);
     hipLaunchKernelGGL(kb_grid_scatter, dim3(bt.n_site_blocks), dim3(256), 0, s, bt.descs, bt.site_blocks);
 }
@@ -1219,13 +1207,16 @@ void launch_batch_end(const BatchTables& bt, hipStream_t s) {
 
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s, const SideStream* side) {
     if (side && side->s) {
-        // the frame phase (sites by index, static vertex grid; it touches none
-        // of what kb_begin clears) beside the stats reset and site grid rebuild
+        // the frame phase (sites by index, static vertex grid and the vertex
+        // pool) beside the site grid rebuild; the previous iteration's kb_end
+        // cleared the counters
         (void)hipEventRecord(side->ffork, s);
         (void)hipStreamWaitEvent(side->s, side->ffork, 0);
         launch_frame(bt, kproj, 0, side->s);
         (void)hipEventRecord(side->fjoin, side->s);
+#ifdef CVT_STAGE_CLOCKS
         hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+#endif
         launch_grid_rebuild(bt, s);
         (void)hipStreamWaitEvent(s, side->fjoin, 0);
         launch_batch_lloyd(bt, K, kproj, 0, s, 3 | kLloydFrameDone, side);
@@ -1237,10 +1228,12 @@ void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t
 }
 
 void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s) {
-    // eval_only: the stats words are reused for the deferred count only
+    // eval_only: the stats words are reused for the deferred count only, and
+    // cleared again after it for the next step
     hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
     launch_batch_grid(bt, 1, s);
     launch_batch_lloyd(bt, K, kproj, 1, s);
+    hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 }
 
 }  // namespace cvt
