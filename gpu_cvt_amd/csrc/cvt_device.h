@@ -385,52 +385,67 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
     // (row CAP is a spill row) and cnt advances only for a point within tau, so
     // rows below cnt hold the candidates.  The four loads of a round may run
     // up to 3 points past the row's end (into the next cells, or the store's
-    // padding); those points are not taken.
+    // padding); those points are not taken.  Each round issues the next
+    // round's loads before it tests its own points (after the stream's end
+    // the next round reads points 0-3, which every store holds).
     int cnt = 0;
     int s = nr > 0 ? rlo[0] : 0, e = nr > 0 ? rhi[0] : 0;
     int ns = rlo[256], ne = rhi[256];  // next row's bounds (meaningful while kn < nr)
     int kn = 1;
     int tag = -s;  // (row << 12) - first slot of the row: entry = tag + slot
-    while (s < e) {
-        const CVT_G float4* __restrict__ ps = pts + s;
+    {
         float4 p[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) p[b] = ps[b];
-        const int lim = e - s;  // >= 1
+        for (int b = 0; b < 4; ++b) p[b] = pts[s + b];
+        while (s < e) {
+            const int cs = s, ctag = tag;
+            const int lim = e - s;  // >= 1
+            s += 4;
+            if (s >= e) {  // next row (read one switch ahead), then read the one after
+                const bool more = kn < nr;
+                s = more ? ns : 0;
+                e = more ? ne : 0;
+                tag = (kn << 12) - s;
+                ++kn;
+                const int kc = min(kn, 8);
+                ns = rlo[kc * 256];
+                ne = rhi[kc * 256];
+            }
+            float4 pn[4];
 #pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const bool acc = (b == 0 || b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (s + b != self_slot);
-            cand[min(cnt, CAP) * 256] = (unsigned short)(tag + s + b);
-            cnt += acc ? 1 : 0;
-        }
-        s += 4;
-        if (s >= e) {  // next row (read one switch ahead), then read the one after
-            const bool more = kn < nr;
-            s = more ? ns : 0;
-            e = more ? ne : 0;
-            tag = (kn << 12) - s;
-            ++kn;
-            const int kc = min(kn, 8);
-            ns = rlo[kc * 256];
-            ne = rhi[kc * 256];
+            for (int b = 0; b < 4; ++b) pn[b] = pts[s + b];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bool acc = (b == 0 || b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (cs + b != self_slot);
+                cand[min(cnt, CAP) * 256] = (unsigned short)(ctag + cs + b);
+                cnt += acc ? 1 : 0;
+            }
+#pragma unroll
+            for (int b = 0; b < 4; ++b) p[b] = pn[b];
         }
     }
     if (cnt < L || cnt > CAP) return false;
     unsigned key[L];
 #pragma unroll
     for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
-    // four candidates per round trip; past cnt the key is ~0u, whose
-    // insertion leaves the list unchanged
+    // four candidates per round trip, the next four gathered while these are
+    // inserted (entries past cnt repeat the last candidate); past cnt the key
+    // is ~0u, whose insertion leaves the list unchanged
     constexpr int kB = 4;
-    for (int k = 0; k < cnt; k += kB) {
+    auto gather = [&](int k0, float4 (&p)[kB]) {
         int sl[kB];
-        float4 p[kB];
 #pragma unroll
-        for (int b = 0; b < kB; ++b) sl[b] = cand[min(k + b, cnt - 1) * 256];
+        for (int b = 0; b < kB; ++b) sl[b] = cand[min(k0 + b, cnt - 1) * 256];
 #pragma unroll
         for (int b = 0; b < kB; ++b) sl[b] = rlo[(sl[b] >> 12) * 256] + (sl[b] & 4095);
 #pragma unroll
         for (int b = 0; b < kB; ++b) p[b] = pts[sl[b]];
+    };
+    float4 p[kB];
+    gather(0, p);
+    for (int k = 0; k < cnt; k += kB) {
+        float4 pn[kB];
+        gather(k + kB, pn);
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
             const unsigned x = k + b < cnt ? ((__float_as_uint(d2f(qx, qy, qz, p[b])) & ~63u) | (unsigned)(k + b))
@@ -439,6 +454,8 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
             for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
             key[0] = min(key[0], x);
         }
+#pragma unroll
+        for (int b = 0; b < kB; ++b) p[b] = pn[b];
     }
 #pragma unroll
     for (int t = 0; t < L; ++t) {
