@@ -442,11 +442,12 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
             int vl[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) vl[k] = -1;
-            bool ok = true;
+            bool ok = true, pooled = false;
+            FanBounds<KP> fb;
             CVT_CLOCK(t_list0);
             if (needs_list(c, cr)) {
                 float tauv;
-                const bool pooled = pool_lookup_with<KP>(a, cr.c3, vc, pv, vl, tauv);
+                pooled = pool_lookup_with<KP>(a, cr.c3, vc, pv, vl, tauv, &fb);
                 ok = pooled || vertex_search<KP, false>(a, i, cr.c3, tauv, vl);
                 CVT_STAGE_COUNT(a, 0, true);
                 CVT_STAGE_COUNT(a, 1, !pooled);
@@ -454,7 +455,7 @@ __global__ __launch_bounds__(256) void kb_project(const JobDesc* __restrict__ de
             CVT_CLOCK(t_list1);
             CVT_STAGE_ADD(a, 3, t_list1 - t_list0);
             if (ok) {
-                project_finish<KP>(a, par, false, c, cr, vl, f);
+                project_finish<KP>(a, par, false, c, cr, vl, f, &fb, pooled);
                 CVT_CLOCK(t_face1);
                 CVT_STAGE_ADD(a, 4, t_face1 - t_list1);
             } else {

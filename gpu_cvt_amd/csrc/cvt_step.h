@@ -401,6 +401,13 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 // (the faces, the new site and the freeze state), run in one thread by
 // phase_project.
 
+// The list vertices' fan bounds (ring_off[v], ring_off[v + 1]), where the
+// projection's face walk starts.
+template <int KP>
+struct FanBounds {
+    int o0[KP], o1[KP];
+};
+
 // Vertex pool cache (layout at kPoolIds): after a move by d from the pool's
 // centroid, every vertex outside the pool is at least (bound - d) away, so
 // when the 5th nearest pooled vertex, in exact (FP64 key, id) order, is closer
@@ -410,10 +417,14 @@ __device__ __forceinline__ void write_debug(const StepArgs& a, const SiteCtx& c,
 // median-of-three per list entry); the order is exact when the first six
 // keys are pairwise apart by more than their error allowance.  Otherwise the
 // seed distance plus d seeds the search in tauv (triangle inequality; +inf
-// without a cache).  pv: the pool's first kPoolLookVecs int4 words.
+// without a cache).  pv: the pool's first kPoolLookVecs int4 words.  fb
+// (optional) receives the list's fan bounds when the lookup succeeds: the
+// bounds of every pooled vertex are loaded beside the positions, so the face
+// walk's first loads do not wait for a round trip after the list is known.
 template <int KP>
 __device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3& c3, const float4& vc,
-                                                 const int4 (&pv)[kPoolLookVecs], int (&vl)[KP], float& tauv) {
+                                                 const int4 (&pv)[kPoolLookVecs], int (&vl)[KP], float& tauv,
+                                                 FanBounds<KP>* fb = nullptr) {
     const float inf = __builtin_huge_valf();
     tauv = inf;
     if constexpr (KP != kPoolKP) {
@@ -432,10 +443,21 @@ __device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3
         unsigned key[L];
 #pragma unroll
         for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
+        float3 pp[kPoolIds];
+        int r0[kPoolIds], r1[kPoolIds];
 #pragma unroll
         for (int k = 0; k < kPoolIds; ++k) {
             // the position only (three registers per load in flight, not four)
-            const float3 p3 = *reinterpret_cast<const CVT_G float3*>(a.m.V + (w[k] >= 0 ? w[k] : 0));
+            const int v = w[k] >= 0 ? w[k] : 0;
+            pp[k] = *reinterpret_cast<const CVT_G float3*>(a.m.V + v);
+            if (fb) {
+                r0[k] = a.m.ring_off[v];
+                r1[k] = a.m.ring_off[v + 1];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kPoolIds; ++k) {
+            const float3 p3 = pp[k];
             const unsigned x =
                 w[k] >= 0 ? ((__float_as_uint(d2f(c3.x, c3.y, c3.z, make_float4(p3.x, p3.y, p3.z, 0.f))) & ~15u) |
                              (unsigned)k)
@@ -464,6 +486,16 @@ __device__ __forceinline__ bool pool_lookup_with(const StepArgs& a, const float3
 #pragma unroll
                 for (int k = 0; k < kPoolIds; ++k) sel = kn == (unsigned)k ? w[k] : sel;
                 vl[t] = sel;
+                if (fb) {
+                    int o0 = r0[0], o1 = r1[0];
+#pragma unroll
+                    for (int k = 1; k < kPoolIds; ++k) {
+                        o0 = kn == (unsigned)k ? r0[k] : o0;
+                        o1 = kn == (unsigned)k ? r1[k] : o1;
+                    }
+                    fb->o0[t] = o0;
+                    fb->o1[t] = o1;
+                }
             }
             return true;
         }
@@ -595,9 +627,12 @@ __device__ __forceinline__ float3 nearest_on_faces64(const StepArgs& a, float3 c
 // axis of its faces, whose choice FP32 cannot make), the walk is repeated in
 // FP64 after the FP32 one, so the choice is the FP64 one.  PRECISE: the FP64
 // walk alone.
+// pre (optional, FP32 walk): the list's fan bounds, taken when use_pre is set
+// (already loaded).
 template <int KP, bool PRECISE = false>
 __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool eval_only, const SiteCtx& c,
-                                               const CellRes& cr, int (&vl)[KP], SiteFlags& r) {
+                                               const CellRes& cr, int (&vl)[KP], SiteFlags& r,
+                                               const FanBounds<KP>* pre = nullptr, bool use_pre = false) {
     const int has_prev = eval_only ? 0 : *a.has_prev;
     CVT_G float4* __restrict__ Snew = a.S01[par ^ 1];
     const float4 q = c.q;
@@ -631,21 +666,28 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
             // each list vertex's fan from its ring: face (v, e_j, e_j+1) for
             // every entry e_j that does not end a chain; a face incident to an
             // earlier list vertex was already visited
-            int o0[KP], o1[KP];
-            float4 cv[KP];
+            FanBounds<KP> fb;
+            float3 cv[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
-                o0[k] = ok ? a.m.ring_off[vl[k]] : 0;
-                o1[k] = ok ? a.m.ring_off[vl[k] + 1] : 0;
-                cv[k] = a.m.V[ok ? vl[k] : 0];
+                if (pre && use_pre) {
+                    fb.o0[k] = pre->o0[k];
+                    fb.o1[k] = pre->o1[k];
+                } else {
+                    fb.o0[k] = ok ? a.m.ring_off[vl[k]] : 0;
+                    fb.o1[k] = ok ? a.m.ring_off[vl[k] + 1] : 0;
+                }
+                cv[k] = *reinterpret_cast<const CVT_G float3*>(a.m.V + (ok ? vl[k] : 0));
             }
-            auto visit = [&](const float4& A, const float4& B, const float4& C, int ib, int ic, int k) {
+            const int (&o0)[KP] = fb.o0;
+            const int (&o1)[KP] = fb.o1;
+            auto visit = [&](const float3& A, const float4& B, const float4& C, int ib, int ic, int k) {
                 bool seen = false;
 #pragma unroll
                 for (int s2 = 0; s2 < KP - 1; ++s2) seen = seen || (s2 < k && (ib == vl[s2] || ic == vl[s2]));
                 if (seen) return;
-                const float3 o = closest_offset_tri(c3, make_float3(A.x, A.y, A.z), make_float3(B.x, B.y, B.z),
+                const float3 o = closest_offset_tri(c3, A, make_float3(B.x, B.y, B.z),
                                                     make_float3(C.x, C.y, C.z));
                 const float d2 = o.x * o.x + o.y * o.y + o.z * o.z;
                 const float mx = fmaxf(d2, best), dd = fabsf(d2 - best);
