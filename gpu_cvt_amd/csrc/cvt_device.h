@@ -387,19 +387,17 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
     // up to 3 points past the row's end (into the next cells, or the store's
     // padding); those points are not taken.  Each round issues the next
     // round's loads before it tests its own points (after the stream's end
-    // the next round reads points 0-3, which every store holds).
+    // the next round reads points 0-3, which every store holds); two register
+    // sets alternate, so a round's loads are first waited for a round later.
     int cnt = 0;
     int s = nr > 0 ? rlo[0] : 0, e = nr > 0 ? rhi[0] : 0;
     int ns = rlo[256], ne = rhi[256];  // next row's bounds (meaningful while kn < nr)
     int kn = 1;
     int tag = -s;  // (row << 12) - first slot of the row: entry = tag + slot
     {
-        float4 p[4];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) p[b] = pts[s + b];
-        while (s < e) {
+        auto round = [&](const float4 (&p)[4], float4 (&pn)[4]) {
             const int cs = s, ctag = tag;
-            const int lim = e - s;  // >= 1
+            const int lim = e - s;  // >= 1 before the stream's end, 0 after
             s += 4;
             if (s >= e) {  // next row (read one switch ahead), then read the one after
                 const bool more = kn < nr;
@@ -411,17 +409,23 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
                 ns = rlo[kc * 256];
                 ne = rhi[kc * 256];
             }
-            float4 pn[4];
 #pragma unroll
             for (int b = 0; b < 4; ++b) pn[b] = pts[s + b];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const bool acc = (b == 0 || b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (cs + b != self_slot);
+                const bool acc = (b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (cs + b != self_slot);
                 cand[min(cnt, CAP) * 256] = (unsigned short)(ctag + cs + b);
                 cnt += acc ? 1 : 0;
             }
+        };
+        // rounds in pairs: a round after the stream's end (s = e = 0) takes
+        // no point, reads points 0-3 and writes the spare entry at cnt
+        float4 pa[4], pb[4];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) p[b] = pn[b];
+        for (int b = 0; b < 4; ++b) pa[b] = pts[s + b];
+        while (s < e) {
+            round(pa, pb);
+            round(pb, pa);
         }
     }
     if (cnt < L || cnt > CAP) return false;
@@ -441,21 +445,28 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
 #pragma unroll
         for (int b = 0; b < kB; ++b) p[b] = pts[sl[b]];
     };
-    float4 p[kB];
-    gather(0, p);
-    for (int k = 0; k < cnt; k += kB) {
-        float4 pn[kB];
-        gather(k + kB, pn);
+    auto select = [&](int k, const float4 (&p)[kB]) {
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            const unsigned x = k + b < cnt ? ((__float_as_uint(d2f(qx, qy, qz, p[b])) & ~63u) | (unsigned)(k + b))
-                                           : 0xffffffffu;
+            // branch-free: all ones past cnt
+            const unsigned x = (__float_as_uint(d2f(qx, qy, qz, p[b])) & ~63u) | (unsigned)(k + b) |
+                               (k + b < cnt ? 0u : 0xffffffffu);
 #pragma unroll
             for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
             key[0] = min(key[0], x);
         }
-#pragma unroll
-        for (int b = 0; b < kB; ++b) p[b] = pn[b];
+    };
+    // two register sets alternate, as in the collect, in pairs of rounds with
+    // no exit between them (a round past cnt inserts nothing)
+    float4 pa[kB], pb[kB];
+    gather(0, pa);
+    for (int k = 0; k < cnt; k += 2 * kB) {
+        gather(k + kB, pb);
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads ahead of the insertions
+        select(k, pa);
+        gather(k + 2 * kB, pa);
+        __builtin_amdgcn_sched_barrier(0);
+        select(k + kB, pb);
     }
 #pragma unroll
     for (int t = 0; t < L; ++t) {

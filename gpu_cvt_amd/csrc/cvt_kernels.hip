@@ -1072,14 +1072,15 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
         if (!(which & kLloydFrameDone)) launch_frame(bt, kproj, eval_only, s);
         launch_lloyd_part(bt, K, kproj, eval_only, 0, s);
         const bool fork = side && side->s && (which & 2);
+        if (fork) (void)hipEventRecord(side->fork, s);
+        // the projection is captured before the side branch, so a graph
+        // executor that gives each chain of its depth-first walk its own queue
+        // keeps it on the cell kernel's queue
+        if (!eval_only) launch_project(bt, kproj, s);
         if (fork) {
-            (void)hipEventRecord(side->fork, s);
             (void)hipStreamWaitEvent(side->s, side->fork, 0);
             launch_lloyd_part(bt, K, kproj, eval_only, 1, side->s);
             (void)hipEventRecord(side->join, side->s);
-        }
-        if (!eval_only) launch_project(bt, kproj, s);
-        if (fork) {
             (void)hipStreamWaitEvent(s, side->join, 0);
             if (!eval_only) launch_lloyd_part(bt, K, kproj, eval_only, 2, s);
             return;
@@ -1210,15 +1211,19 @@ void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t
     if (side && side->s) {
         // the frame phase (sites by index, static vertex grid and the vertex
         // pool) beside the site grid rebuild; the previous iteration's kb_end
-        // cleared the counters
+        // cleared the counters.  The main chain (grid rebuild, cell,
+        // projection, end) is captured first: a graph executor that walks the
+        // nodes depth first in capture order and gives each chain its own
+        // queue then runs the whole chain on one queue, and only the side
+        // branches wait across queues.
         (void)hipEventRecord(side->ffork, s);
-        (void)hipStreamWaitEvent(side->s, side->ffork, 0);
-        launch_frame(bt, kproj, 0, side->s);
-        (void)hipEventRecord(side->fjoin, side->s);
 #ifdef CVT_STAGE_CLOCKS
         hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 #endif
         launch_grid_rebuild(bt, s);
+        (void)hipStreamWaitEvent(side->s, side->ffork, 0);
+        launch_frame(bt, kproj, 0, side->s);
+        (void)hipEventRecord(side->fjoin, side->s);
         (void)hipStreamWaitEvent(s, side->fjoin, 0);
         launch_batch_lloyd(bt, K, kproj, 0, s, 3 | kLloydFrameDone, side);
     } else {
