@@ -9,8 +9,9 @@ before the timed region.
 
 Prints one JSON line (rank 0).  Also measures, in the same run:
   * roofline: algorithmic bytes of one Lloyd iteration divided by the mean
-    duration of the FP32 Lloyd pass -- the three kernels that do all of the
-    per-site work, kb_frame + kb_cell + kb_project, launched back to back --
+    duration of the FP32 Lloyd pass -- the two kernels that do all of the
+    per-site work, kb_cell + kb_project, launched back to back (the frame
+    vertex of each new site is found by the projection kernel) --
     timed with HIP events on the job's stream; beside it valu_frac, the pass's
     VALU issue floor (from the committed SQ-counter profile) over the same time;
   * cpu_baseline: an FP32 C++ Lloyd step and the FP64 CPU oracle's step on this
@@ -46,7 +47,7 @@ def algorithmic_bytes(nS: int, nV: int, nF: int, K: int) -> int:
 
 
 def traffic_from_profile(workload: str):
-    """Bytes past the L2 per FP32 Lloyd pass (kb_frame + kb_cell + kb_project),
+    """Bytes past the L2 per FP32 Lloyd pass (kb_cell + kb_project),
     as recorded by the committed rocprofv3 summary of this workload (reads from
     the per-request-size L2 read counters, writes from WRITE_SIZE); (None,
     None) when no summary exists for the workload.  The counters cannot be read
@@ -66,7 +67,7 @@ def traffic_from_profile(workload: str):
 def valu_issue_from_profile(workload: str):
     """The VALU issue floor of the FP32 Lloyd pass in microseconds per launch
     (VALU instructions per wave x 4 cycles x waves per SIMD at 2.4 GHz, summed
-    over kb_frame, kb_cell and kb_project), as recorded by the committed
+    over kb_cell and kb_project), as recorded by the committed
     rocprofv3 SQ-counter summary of this workload; None when there is none."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                         f"lloyd_pass_{workload}_sq_counters.md")
@@ -472,7 +473,7 @@ def main(argv=None) -> int:
     traffic, traffic_src = traffic_from_profile(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
-                "kernel": "FP32 Lloyd pass: kb_frame + kb_cell<24> + kb_project"}
+                "kernel": "FP32 Lloyd pass: kb_cell<24> + kb_project"}
     # the roof that binds these kernels: their VALU issue floor (from the
     # committed SQ-counter profile) against the same live FP32-pass time
     valu_us = valu_issue_from_profile(args.workload)

@@ -129,17 +129,13 @@ namespace {
 static int side_create(SideStream& sd) {
     if (hipStreamCreateWithFlags(&sd.s, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&sd.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sd.join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sd.ffork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&sd.fjoin, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&sd.join, hipEventDisableTiming) != hipSuccess)
         return -1;
     return 0;
 }
 static void side_destroy(SideStream& sd) {
     if (sd.fork) (void)hipEventDestroy(sd.fork);
     if (sd.join) (void)hipEventDestroy(sd.join);
-    if (sd.ffork) (void)hipEventDestroy(sd.ffork);
-    if (sd.fjoin) (void)hipEventDestroy(sd.fjoin);
     if (sd.s) (void)hipStreamDestroy(sd.s);
     sd = SideStream{};
 }
@@ -763,6 +759,9 @@ int cvt_job_create(const double* Vh, int nV, const int* Fh, int nF, const double
         j->tables = BatchTables{j->desc,          1, j->site_blocks, (int)ht.site.size(), j->tile_blocks,
                                 (int)ht.tile.size(), j->prec_blocks, (int)ht.prec.size()};
     }
+    // the first iteration's frame vertices (every later iteration's come from
+    // the projection of the one before)
+    launch_batch_frame(j->tables, p.kproj, s);
     hipError_t e2 = hipStreamSynchronize(s);
     if (e2 != hipSuccess) return bail(fail(CVT_EHIP, "setup: %s", hipGetErrorString(e2)));
     e2 = hipGetLastError();
@@ -882,7 +881,9 @@ int cvt_job_profile(cvt_job* j, int iters, double* grid_ms, double* lloyd_ms, do
         HIPCHK(hipEventRecord(ev[0], s));
         launch_batch_grid(j->tables, 0, s);
         HIPCHK(hipEventRecord(ev[1], s));
-        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 1);
+        // the FP32 pass as the graph runs it: the frame vertices came from
+        // the previous iteration's projection (or the set-up)
+        launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 1 | kLloydFrameDone);
         HIPCHK(hipEventRecord(ev[2], s));
         launch_batch_lloyd(j->tables, j->p.K, j->p.kproj, 0, s, 2);
         HIPCHK(hipEventRecord(ev[3], s));
@@ -1175,6 +1176,9 @@ int cvt_job_set_state(cvt_job* j, const double* S, const int* knn_prev, const in
         int zero = 0;
         HIPCHK(hipMemcpy(j->seed_ok, &zero, sizeof(int), hipMemcpyHostToDevice));
     }
+    // the next iteration's frame vertices, for the state just set
+    launch_batch_frame(j->tables, j->p.kproj, j->stream);
+    HIPCHK(hipGetLastError());
     return 0;
 }
 

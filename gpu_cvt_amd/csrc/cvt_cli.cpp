@@ -23,11 +23,11 @@
 // Quality (evaluate.cuh:388-462) and the one-sided Hausdorff distance are
 // computed on the device; the stage columns come from per-kernel timing
 // (cvt_job_profile_stages): knn_sites_ms is the site-grid rebuild,
-// knn_site_to_mesh_ms the frame kernel (nearest vertex and tangent frame, so
-// uv_from_mesh_ms is 0), centroids_ms the cell kernel (site KNN, cell and
-// centroid) plus the FP64 pass, project_ms the projection kernel (kproj
-// search, reprojection and freeze test, so knn_centroid_to_mesh_ms and
-// freeze_ms are 0).
+// centroids_ms the cell kernel (site KNN, tangent frame, cell and centroid)
+// plus the FP64 pass, project_ms the projection kernel (kproj search,
+// reprojection, freeze test and the new site's nearest vertex, the next
+// iteration's frame source), so knn_site_to_mesh_ms, uv_from_mesh_ms,
+// knn_centroid_to_mesh_ms and freeze_ms are 0.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -210,7 +210,7 @@ int main(int argc, char** argv) {
         total_ms += sm.iter;
         std::printf("knn_sites_ms %.3f\nknn_site_to_mesh_ms %.3f\ncentroids_ms %.3f\nproject_ms %.3f\n"
                     "iter_remesh_ms %.3f (total_remesh_ms %.3f)\n",
-                    sm.grid, sm.frame, sm.cell + sm.precise, sm.project, sm.iter, total_ms);
+                    sm.grid, 0.0, sm.cell + sm.precise, sm.project, sm.iter, total_ms);
         std::printf("same neighbors %d\nlow displacement %d\nboth %d\npass2 %d\n", st.count_same, st.count_low_disp,
                     st.count_both, st.count_newly_frozen);
         converge = (float)st.frozen_total / (float)nV;
@@ -304,7 +304,7 @@ int main(int argc, char** argv) {
             << "," << q.theta_min_avg << "," << q.theta_lt_30_pct << "," << q.theta_gt_90_pct << "," << (float)dH
             << "," << (float)sm.iter << "," << (float)total_ms << "," << st.frozen_total << "," << nV << ","
             << (nV > 0 ? 100.f * (float)st.frozen_total / (float)nV : 0.f) << "," << (float)sm.grid << ","
-            << (float)sm.frame << "," << 0.f << "," << (float)(sm.cell + sm.precise) << "," << 0.f << ","
+            << 0.f << "," << 0.f << "," << (float)(sm.cell + sm.precise) << "," << 0.f << ","
             << (float)sm.project << "," << 0.f << "," << st.count_same << "," << st.count_low_disp << ","
             << st.count_both << "," << st.count_newly_frozen << "," << blocked_same << "," << blocked_low << "\n";
     }

@@ -497,11 +497,10 @@ __device__ __forceinline__ bool key_less(const CVT_G float4* __restrict__ pts, f
 
 // Exact K nearest under (key64, id) from an FP32 top-(K+2) list.  Adjacent
 // entries closer than the relative error bound rel (kRel for FP32 distances,
-// kRelKey for truncated keys) are "ambiguous"; isolated
-// ambiguous pairs (including the pair at the K-th boundary) are ordered by
-// FP64 keys here.  Returns false (caller defers the query to the FP64 pass)
-// when three or more entries are mutually ambiguous or the (K+2)-th entry is
-// not provably beyond the K-th.
+// kRelKey for truncated keys) are "ambiguous"; ambiguous pairs (including the
+// pair at the K-th boundary) and runs of them are ordered by FP64 keys here.
+// Returns false (caller defers the query to the FP64 pass) when the (K+2)-th
+// entry is not provably beyond the K-th.
 template <int K, int L>
 __device__ __forceinline__ bool resolve_topk(const CVT_G float4* __restrict__ pts, float qx, float qy, float qz, TopL<L>& T,
                                              int& n_pairs, const CVT_G int* __restrict__ cid, float rel = kRel) {
@@ -516,7 +515,28 @@ __device__ __forceinline__ bool resolve_topk(const CVT_G float4* __restrict__ pt
     bool chain = false;
 #pragma unroll
     for (int t = 0; t + 1 < K; ++t) chain = chain || (amb[t] && amb[t + 1]);
-    if (chain) return false;
+    if (chain) {
+        // a run of three or more mutually ambiguous entries (rare): adjacent
+        // exchanges by FP64 (key, tie) order, only across the ambiguous
+        // pairs, repeated until none happens.  The runs' members stay inside
+        // their runs, whose boundaries FP32 already orders, so this sorts
+        // each run exactly (a run of m entries takes at most m passes).
+        bool swapped = true;
+        for (int pass = 0; swapped && pass <= K; ++pass) {
+            swapped = false;
+#pragma unroll
+            for (int t = 0; t < K; ++t) {
+                if (amb[t] && key_less(pts, qx, qy, qz, T.s[t + 1], T.s[t], cid)) {
+                    const int ts = T.s[t]; T.s[t] = T.s[t + 1]; T.s[t + 1] = ts;
+                    const float td = T.d[t]; T.d[t] = T.d[t + 1]; T.d[t + 1] = td;
+                    swapped = true;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < K; ++t) n_pairs += amb[t] ? 1 : 0;
+        return true;
+    }
 #pragma unroll
     for (int t = 0; t < K; ++t) {
         if (amb[t]) {

@@ -32,12 +32,14 @@ constexpr int kStatFarWalks = 14;
 // stage 0 the cell kernel's site KNN (seeds, collect, selection, order proof),
 // 1 its row write and dual points, 2 its dual hull walk and moments, 3 the
 // projection kernel's vertex list (pool cache or grid search), 4 its face
-// walk, new site and freeze step, 5 the frame kernel's nearest-vertex search
-// and frame.  32-bit event counts follow the clocks: 0 the projection
-// kernel's vertex lists, 1 those the pool cache could not prove (a grid
-// search), 2 the frame kernel's nearest-vertex searches (the gap cache did not
-// prove the vertex), 3 the cell kernel's site searches that the LDS collect
-// could not prove (the ring search ran).  The product build compiles none of it.
+// walk, new site, freeze step and the new site's nearest vertex, 5 the frame
+// kernel's nearest-vertex search and frame (run at set-up only).  32-bit event
+// counts follow the clocks: 0 the projection kernel's vertex lists, 1 those
+// the pool cache could not prove (a grid search), 2 nearest-vertex searches
+// that the pool cache did not prove (new sites whose projection list did not
+// prove it, or the frame kernel's), 3 the cell kernel's site searches that the
+// LDS collect could not prove (the ring search ran).  The product build
+// compiles none of it.
 constexpr int kStatClocks = 16;
 constexpr int kStageCount = 6;
 constexpr int kStageCounters = 4;
@@ -282,10 +284,15 @@ int scan_tile();
 struct SideStream {
     hipStream_t s = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;    // precise pass beside the projection
-    hipEvent_t ffork = nullptr, fjoin = nullptr;  // frame phase beside the grid rebuild
 };
+// One iteration over a batch.  With a side stream (the graph-captured form) it
+// takes the frame vertices from the previous iteration's projection, which
+// computes them for the new sites: after the sites or the freeze state are set
+// from the host, launch_batch_frame must run first.
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s,
                             const SideStream* side = nullptr);
+// The frame phase alone (every site's frame vertex for its current position).
+void launch_batch_frame(const BatchTables& bt, int kproj, hipStream_t s);
 // The fused step alone over a batch, without state writes (energy / debug evaluation).
 void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s);
 // Pieces of an iteration, for per-stage timing.
@@ -297,8 +304,10 @@ constexpr int kLloydFrameDone = 4;
 void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, hipStream_t s, int which = 3,
                         const SideStream* side = nullptr);
 void launch_batch_end(const BatchTables& bt, hipStream_t s);
-// The FP32 pass kernel by kernel, with an event recorded after the frame and
-// after the cell kernel (stage timing).
+// The FP32 pass kernel by kernel as the graph runs it (the cell and the
+// projection kernels; the frame vertices come from the projection before),
+// with events recorded before the cell kernel (after_frame: the frame stage is
+// empty) and after it (stage timing).
 void launch_batch_lloyd_staged(const BatchTables& bt, int K, int kproj, hipStream_t s, hipEvent_t after_frame,
                                hipEvent_t after_cell);
 // Triangle quality (eval_quality_angles_cpu, evaluate.cuh:388-462): per-block

@@ -387,9 +387,12 @@ __global__ void k_vertex_frames(MeshDev m, float4* __restrict__ FU, float4* __re
 // FP32 pass, phase 1 over all sites of all jobs of a batch, in internal site
 // order (cell-sorted when the job was made, so still spatially coherent; the
 // block table is walked XCD-contiguously).  It reads the sites themselves,
-// not the site grid, so it can run beside the grid rebuild; a site it cannot
-// prove (fnn = kFrameDeferred) is put on the precise pass's list by kb_cell,
-// which knows its grid slot.
+// not the site grid.  The graph-launched iteration does without it: there the
+// projection of the iteration before computes the frame vertex of every new
+// site, so this runs when a job is made, after its state is set from the
+// host, and in the evaluation and per-stage paths.  A site it cannot prove
+// (fnn = kFrameDeferred) is put on the precise pass's list by kb_cell, which
+// knows its grid slot.
 template <int KP>
 __global__ __launch_bounds__(256) void kb_frame(const JobDesc* __restrict__ descs, const BlockRef* __restrict__ bt,
                                                 int eval_only) {
@@ -1094,7 +1097,8 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
 
 void launch_batch_lloyd_staged(const BatchTables& bt, int K, int kproj, hipStream_t s, hipEvent_t after_frame,
                                hipEvent_t after_cell) {
-    launch_frame(bt, kproj, 0, s);
+    // no frame kernel: the previous iteration's projection (or the set-up)
+    // found the frame vertices, as in the graph-launched iteration
     (void)hipEventRecord(after_frame, s);
     launch_lloyd_part(bt, K, kproj, 0, 0, s);
     (void)hipEventRecord(after_cell, s);
@@ -1207,24 +1211,21 @@ void launch_batch_end(const BatchTables& bt, hipStream_t s) {
     hipLaunchKernelGGL(kb_end, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 }
 
+void launch_batch_frame(const BatchTables& bt, int kproj, hipStream_t s) { launch_frame(bt, kproj, 0, s); }
+
 void launch_batch_iteration(const BatchTables& bt, int K, int kproj, hipStream_t s, const SideStream* side) {
     if (side && side->s) {
-        // the frame phase (sites by index, static vertex grid and the vertex
-        // pool) beside the site grid rebuild; the previous iteration's kb_end
-        // cleared the counters.  The main chain (grid rebuild, cell,
-        // projection, end) is captured first: a graph executor that walks the
-        // nodes depth first in capture order and gives each chain its own
-        // queue then runs the whole chain on one queue, and only the side
-        // branches wait across queues.
-        (void)hipEventRecord(side->ffork, s);
+        // The frame vertices come from the previous iteration's projection
+        // (or from launch_batch_frame after the sites were set), and the
+        // previous iteration's kb_end cleared the counters.  The main chain
+        // (grid rebuild, cell, projection, end) is captured first: a graph
+        // executor that walks the nodes depth first in capture order and
+        // gives each chain its own queue then runs the whole chain on one
+        // queue, and only the FP64 pass's side branch waits across queues.
 #ifdef CVT_STAGE_CLOCKS
         hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
 #endif
         launch_grid_rebuild(bt, s);
-        (void)hipStreamWaitEvent(side->s, side->ffork, 0);
-        launch_frame(bt, kproj, 0, side->s);
-        (void)hipEventRecord(side->fjoin, side->s);
-        (void)hipStreamWaitEvent(s, side->fjoin, 0);
         launch_batch_lloyd(bt, K, kproj, 0, s, 3 | kLloydFrameDone, side);
     } else {
         launch_batch_grid(bt, 0, s);
@@ -1240,6 +1241,9 @@ void launch_batch_eval(const BatchTables& bt, int K, int kproj, hipStream_t s) {
     launch_batch_grid(bt, 1, s);
     launch_batch_lloyd(bt, K, kproj, 1, s);
     hipLaunchKernelGGL(kb_begin, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+    // the evaluation's frame vertices treat frozen sites as free: the next
+    // iteration's are restored
+    launch_frame(bt, kproj, 0, s);
 }
 
 }  // namespace cvt

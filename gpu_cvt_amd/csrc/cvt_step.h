@@ -116,7 +116,7 @@ __device__ __forceinline__ bool pool_nearest(const StepArgs& a, const float4& q,
     double key[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float4 pv = a.m.V[id[k] >= 0 ? id[k] : 0];
+        const float3 pv = *reinterpret_cast<const CVT_G float3*>(a.m.V + (id[k] >= 0 ? id[k] : 0));
         key[k] = id[k] >= 0 ? key64(q.x, q.y, q.z, pv.x, pv.y, pv.z) : __builtin_huge_val();
     }
     double bk = key[0];
@@ -139,42 +139,45 @@ __device__ __forceinline__ bool pool_nearest(const StepArgs& a, const float4& q,
     return false;
 }
 
+
 // ---- 1. frame from the nearest mesh vertex's normal (site_normals.cuh:68-110).
 // The nearest vertex comes from the pool cache of the site's last kproj search
 // when that proves it (pool_nearest; kproj = 5, and no vertex excluded from
 // the site's searches), else from the seeded grid search.
 // Returns false when the FP32 search could not be proven.
+// The nearest mesh vertex nn of an active site i at q.  Returns false when
+// the FP32 search could not be proven.
+template <int KP, bool PRECISE>
+__device__ __forceinline__ bool frame_vertex(const StepArgs& a, int i, const float4& q, bool use_cache, int& nn) {
+    const float inf = __builtin_huge_valf();
+    float tau1 = inf;
+    bool have = false;
+    if constexpr (KP == kPoolKP) {
+        if (use_cache && !a.excl_vslot) {
+            const float4 vc = a.vcache[i];
+            if (vc.w < inf) {
+                const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * i;
+                have = pool_nearest(a, q, vc, pw[0], __int_as_float(pool_word<kPoolFrameWord>(pw)), nn, tau1);
+            }
+        }
+    }
+    if (!PRECISE) CVT_STAGE_COUNT(a, 2, !have);
+    if (have) return true;
+    int s1[1];
+    float l1;
+    const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1, nullptr,
+                                         0.f, nullptr, nullptr, &a.m.vb);
+    if (!PRECISE && k1 == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
+    nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
+    return k1 >= 0;
+}
+
 template <int KP, bool PRECISE>
 __device__ __forceinline__ bool phase_frame(const StepArgs& a, bool eval_only, SiteCtx& c) {
     if (!c.valid) return true;
-    const float inf = __builtin_huge_valf();
     bool ok = true;
-    if (!c.fz) {
-        const float4 q = c.q;
-        float tau1 = inf;
-        bool have = false;
-        if constexpr (KP == kPoolKP) {
-            if (c.use_cache && !a.excl_vslot) {
-                const float4 vc = a.vcache[c.i];
-                if (vc.w < inf) {
-                    const CVT_G int4* pw = a.vpool + (size_t)kPoolVecs * c.i;
-                    have = pool_nearest(a, q, vc, pw[0], __int_as_float(pool_word<kPoolFrameWord>(pw)), c.nn, tau1);
-                }
-            }
-        }
-        if (!PRECISE) CVT_STAGE_COUNT(a, 2, !have);
-        if (!have) {
-            int s1[1];
-            float l1;
-            const int k1 = knn_exact<1, PRECISE>(a.m.vg, a.m.vg_start, a.m.vg_pts, q.x, q.y, q.z, -1, tau1, s1, l1,
-                                                 nullptr, 0.f, nullptr, nullptr, &a.m.vb);
-            ok = k1 >= 0;
-            if (!PRECISE && k1 == 3) atomicAdd(&a.stats[kStatFarWalks], 1);
-            c.nn = s1[0] >= 0 ? __float_as_int(a.m.vg_pts[s1[0]].w) : -1;
-        }
-    } else {
-        c.nn = a.knnV[(size_t)c.i * KP];  // pinned at the freeze iteration
-    }
+    if (!c.fz) ok = frame_vertex<KP, PRECISE>(a, c.i, c.q, c.use_cache, c.nn);
+    else c.nn = a.knnV[(size_t)c.i * KP];  // pinned at the freeze iteration
     set_frame(a, c);
     return ok;
 }
@@ -640,6 +643,8 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
     const int i = c.i;
     r.kind = cr.kind;
     float3 sp = c3;
+    bool fresh = false;  // a finite new site was written
+    float3 cv[KP];       // FP32 walk: the list vertices' positions
     if (c.valid && !eval_only) {
         if (!c.fz) {
             CVT_G int* row = a.knnV + (size_t)i * KP;
@@ -667,7 +672,6 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
             // every entry e_j that does not end a chain; a face incident to an
             // earlier list vertex was already visited
             FanBounds<KP> fb;
-            float3 cv[KP];
 #pragma unroll
             for (int k = 0; k < KP; ++k) {
                 const bool ok = (unsigned)vl[k] < (unsigned)a.m.nV;
@@ -728,6 +732,7 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
     r.frozen_after = c.valid && c.fz;
     if (c.valid && !eval_only) {
         Snew[i] = make_float4(sp.x, sp.y, sp.z, 0.f);
+        fresh = finite3(sp.x, sp.y, sp.z);
         r.disp = sqrtf((sp.x - q.x) * (sp.x - q.x) + (sp.y - q.y) * (sp.y - q.y) + (sp.z - q.z) * (sp.z - q.z));
         if (has_prev) {
             const int tr = a.freeze_mode == 2 ? min((int)a.tier[i], 5) : 0;
@@ -756,6 +761,47 @@ __device__ __forceinline__ void project_finish(const StepArgs& a, int par, bool 
     } else if (c.live && !c.valid && !eval_only) {
         // non-finite site: keep it (it is excluded from every neighbour list)
         Snew[i] = make_float4(q.x, q.y, q.z, 0.f);
+    }
+    if (c.live && !eval_only) {
+        // the frame vertex of the next iteration, at the new site: what that
+        // iteration's frame phase would find (with the pool cache, which the
+        // iteration end validates), found here while the site's pool is at
+        // hand; a frozen site takes the first vertex of its pinned list
+        int nn = -1;
+        bool ok = true;
+        if (fresh) {
+            bool have = r.frozen_after;
+            if (have) nn = vl[0];
+            if constexpr (!PRECISE) {
+                // Every vertex outside the list (the kproj nearest of c3, in
+                // exact order) lies at least d_kproj from c3, so at least
+                // d_kproj - |sp - c3| from the new site: the nearest list
+                // vertex, by exact (FP64 key, id) order at sp, is the nearest
+                // vertex when it lies closer than that (a list with a vertex
+                // left out of the search proves nothing)
+                if (!have && !a.excl_vslot && (unsigned)vl[KP - 1] < (unsigned)a.m.nV) {
+                    double bk = __builtin_huge_val();
+                    int bi = 0x7fffffff;
+#pragma unroll
+                    for (int k = 0; k < KP; ++k) {
+                        const double kk = key64(sp.x, sp.y, sp.z, cv[k].x, cv[k].y, cv[k].z);
+                        const bool take = kk < bk || (kk == bk && vl[k] < bi);
+                        bk = take ? kk : bk;
+                        bi = take ? vl[k] : bi;
+                    }
+                    const float dk = sqrtf(d2f(c3.x, c3.y, c3.z, make_float4(cv[KP - 1].x, cv[KP - 1].y,
+                                                                              cv[KP - 1].z, 0.f))) * (1.f - 1e-6f);
+                    const float dl = sqrtf((sp.x - c3.x) * (sp.x - c3.x) + (sp.y - c3.y) * (sp.y - c3.y) +
+                                           (sp.z - c3.z) * (sp.z - c3.z)) * (1.f + 1e-6f);
+                    if (sqrtf((float)bk) * (1.f + 2e-5f) + 4.f * a.m.vg.eps_abs < dk - dl) {
+                        nn = bi;
+                        have = true;
+                    }
+                }
+            }
+            if (!have) ok = frame_vertex<KP, PRECISE>(a, i, make_float4(sp.x, sp.y, sp.z, 0.f), true, nn);
+        }
+        a.fnn[i] = ok ? nn : kFrameDeferred;
     }
     r.cell_fail = c.valid && !cr.cell_ok;
     r.nonfinite = c.live && !finite3(sp.x, sp.y, sp.z);

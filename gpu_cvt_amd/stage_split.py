@@ -27,8 +27,8 @@ STAGES = [  # (stage, kernel) in the order of the clock sums
     ("KNN row write and dual points", "cell"),
     ("dual hull walk and cell moments", "cell"),
     ("vertex list: pool cache or grid search", "project"),
-    ("faces: nearest point, new site, freeze step", "project"),
-    ("nearest vertex and tangent frame", "frame"),
+    ("faces: nearest point, new site, freeze step, its nearest vertex", "project"),
+    ("nearest vertex and tangent frame (frame kernel: set-up only)", "frame"),
 ]
 
 
@@ -66,7 +66,7 @@ def main(argv=None) -> int:
           f"mean of {args.iters})\n")
     print("| kernel | stage | share of kernel cycles | us per iteration |")
     print("|---|---|---|---|")
-    for kern in ("frame", "cell", "project"):
+    for kern in ("cell", "project"):
         idx = [i for i, (_, k) in enumerate(STAGES) if k == kern]
         tot = clk[idx].sum()
         for i in idx:
@@ -77,8 +77,8 @@ def main(argv=None) -> int:
           f"{1e3 * ms['precise']:.1f} us; whole iteration (kernel by kernel, with events) {1e3 * ms['iter']:.1f} us.")
     n = counts / args.iters
     print(f"\nPer iteration: projection kernel vertex lists {n[0]:.0f}, of which the pool cache could not prove "
-          f"{n[1]:.0f} ({100 * n[1] / max(n[0], 1):.1f} %: a grid search); frame kernel nearest-vertex searches "
-          f"(pool not proven) {n[2]:.0f}; cell kernel site searches the LDS collect did not prove {n[3]:.0f}.")
+          f"{n[1]:.0f} ({100 * n[1] / max(n[0], 1):.1f} %: a grid search); new-site nearest-vertex searches "
+          f"(neither the projection's list nor the pool proved it) {n[2]:.0f}; cell kernel site searches the LDS collect did not prove {n[3]:.0f}.")
     print(f"\nFP32 vertex searches finished by the out-of-line LBVH walk (the FP32 kernels' only code with a "
           f"stack frame), iterations 1-{args.warmup}: {far}")
     return 0

@@ -87,9 +87,12 @@ def test_cli_outputs_and_iterates(tmp_path, mode):
         assert abs(float(row[k]) - q[k]) <= rel * abs(q[k]), (k, row[k], q[k])
     for k in ("theta_lt_30_pct", "theta_gt_90_pct"):
         assert abs(float(row[k]) - q[k]) <= 200.0 / (3 * nf) + 1e-4, (k, row[k], q[k])
-    # stage columns from per-kernel timing
-    for k in ("knn_sites_ms", "knn_site_to_mesh_ms", "centroids_ms", "project_ms", "iter_remesh_ms"):
+    # stage columns from per-kernel timing; the frame, vertex-list and freeze
+    # stages run inside the cell and projection kernels
+    for k in ("knn_sites_ms", "centroids_ms", "project_ms", "iter_remesh_ms"):
         assert float(row[k]) > 0, k
+    for k in ("knn_site_to_mesh_ms", "uv_from_mesh_ms", "knn_centroid_to_mesh_ms", "freeze_ms"):
+        assert float(row[k]) == 0, k
     # the Geogram hand-off beside the sites
     sh = (out / "iter_010_co3ne.sh").read_text()
     assert '"iter_010_sites.xyz" "iter_010_co3ne.obj" ' + CO3NE_OPTS in sh
