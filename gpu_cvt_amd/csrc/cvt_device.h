@@ -143,6 +143,65 @@ __device__ __forceinline__ void scan_run(const CVT_G float4* __restrict__ pts, i
     }
 }
 
+// The points of up to 9 rows of cells (slot ranges [r0[k], r1[k])) into T, as
+// one stream of four points per round trip with the next round's loads in
+// flight (two register sets alternate), so a search waits on about a quarter
+// of the round trips of a row-by-row scan.  A round never spans two rows; its
+// loads past a row's end read the next cells or the store's padding
+// (kPtsPad), and those points are not taken.
+template <int L>
+__device__ __forceinline__ void scan_rows(const CVT_G float4* __restrict__ pts, const int (&r0)[9], const int (&r1)[9],
+                                          float qx, float qy, float qz, int self_slot, TopL<L>& T) {
+    // the rows not opened yet, as a queue shifted by one per row opened (all
+    // indices static, so the queue stays in registers)
+    int q0[9], q1[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        q0[k] = r0[k];
+        q1[k] = r1[k];
+    }
+    int s = 0, e = 0;
+    // the next non-empty row, or s = e = 0 after the last
+    auto open = [&]() {
+        s = 0;
+        e = 0;
+        for (int n = 0; n < 9 && e <= s; ++n) {
+            s = q0[0];
+            e = q1[0];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                q0[k] = q0[k + 1];
+                q1[k] = q1[k + 1];
+            }
+            q0[8] = 0;
+            q1[8] = 0;
+        }
+        if (e <= s) s = e = 0;
+    };
+    auto round = [&](const float4 (&p)[4], float4 (&pn)[4]) {
+        const int cs = s, lim = e - s;  // lim <= 0 after the stream's end
+        s += 4;
+        if (s >= e) open();
+#pragma unroll
+        for (int b = 0; b < 4; ++b) pn[b] = pts[s + b];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            if (b < lim && cs + b != self_slot) {
+                const float d = d2f(qx, qy, qz, p[b]);
+                if (d < T.worst()) T.insert(d, cs + b);
+            }
+        }
+    };
+    open();
+    float4 pa[4], pb[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b) pa[b] = pts[s + b];
+    while (s < e) {
+        round(pa, pb);
+        round(pb, pa);
+    }
+}
+
 // x-cells of the 3x3x3 block that can hold a point within sqrt(tau): the
 // neighbouring cell on a side is dropped when its x-gap alone exceeds it.
 __device__ __forceinline__ void block_xrange(const GridP& g, float qx, int cx, float tau, int& x0, int& x1) {
@@ -275,8 +334,7 @@ __device__ __forceinline__ bool grid_topl(const GridP& g, const CVT_G int* __res
             r0[k] = keep ? start[row + x0] : 0;
             r1[k] = keep ? start[row + x1 + 1] : 0;
         }
-#pragma unroll
-        for (int k = 0; k < 9; ++k) scan_run<L>(pts, r0[k], r1[k], qx, qy, qz, self_slot, T);
+        scan_rows<L>(pts, r0, r1, qx, qy, qz, self_slot, T);
     }
     const int maxr = max(g.dx, max(g.dy, g.dz));
     for (int r = 1; r <= maxr; ++r) {

@@ -616,33 +616,49 @@ __global__ void kb_grid_scatter(const JobDesc* __restrict__ descs, const BlockRe
 }
 
 // End of an iteration: stats history, flags, buffer parity.
-__global__ void kb_end(const JobDesc* __restrict__ descs, int njobs) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void kb_end(const JobDesc* __restrict__ descs, int njobs) {
+    // one wave per job, lane k folding stat word k
+    const int j = blockIdx.x;
     if (j >= njobs) return;
     const StepArgs& a = descs[j].a;
     if (a.nS == 0) return;  // empty slot
+    const int k = (int)threadIdx.x;
     const int c = *a.hist_count;
     const int slot = c < kHistCap ? c : kHistCap - 1;
-    // fold the per-block stat copies into copy 0
-    unsigned mx = (unsigned)a.stats[kStatMaxDisp];
-    for (int c = 1; c < kStatCopies; ++c) {
-        for (int k = 0; k < 9; ++k) a.stats[k] += a.stats[c * kStatStride + k];
-        mx = max(mx, (unsigned)a.stats[c * kStatStride + kStatMaxDisp]);
+    // word k of the totals: the per-block copies folded (sums, and the largest
+    // displacement), the two deferred lists together
+    int v = 0;
+    if (k < kStatWords) {
+        v = a.stats[k];
+        if (k < 9) {
+#pragma unroll
+            for (int q = 1; q < kStatCopies; ++q) v += a.stats[q * kStatStride + k];
+        } else if (k == kStatMaxDisp) {
+            unsigned mx = (unsigned)v;
+#pragma unroll
+            for (int q = 1; q < kStatCopies; ++q) mx = max(mx, (unsigned)a.stats[q * kStatStride + k]);
+            v = (int)mx;
+            a.dmax01[*a.parity] = __uint_as_float(mx);
+        } else if (k == kStatDeferred) {
+            v += a.stats[kStatDeferredB];
+        }
+        a.hist[slot * kStatWords + k] = v;
     }
-    a.dmax01[*a.parity] = __uint_as_float(mx);
-    a.stats[kStatMaxDisp] = (int)mx;
-    a.stats[kStatDeferred] += a.stats[kStatDeferredB];
-    for (int k = 0; k < kStatWords; ++k) a.hist[slot * kStatWords + k] = a.stats[k];
+    __syncthreads();  // every word read before any is cleared, the parity before its flip
 #ifndef CVT_STAGE_CLOCKS
-    // the next iteration's counters start cleared (kb_begin's work; the
-    // profiling build keeps its stage words until they are read, so there
-    // every iteration starts with kb_begin)
-    for (int k = 0; k < kStatCopies * kStatStride; ++k) a.stats[k] = 0;
+    // the next iteration's counters start cleared (kb_begin's work)
+    for (int w = k; w < kStatCopies * kStatStride; w += 64) a.stats[w] = 0;
+#else
+    // the profiling build keeps its stage words until they are read (every
+    // iteration there starts with kb_begin); copy 0 holds the totals
+    if (k < kStatWords) a.stats[k] = v;
 #endif
-    *a.hist_count = c + 1;
-    *a.has_prev = 1;
-    *a.seed_ok = 1;
-    *a.parity ^= 1;
+    if (k == 0) {
+        *a.hist_count = c + 1;
+        *a.has_prev = 1;
+        *a.seed_ok = 1;
+        *a.parity ^= 1;
+    }
 }
 
 // Nearest mesh vertex of every site (the normal source of the freeze tiers).
@@ -1208,7 +1224,7 @@ void launch_quality(const float4* P, const int* map, const int* F, int nF, Quali
 }
 
 void launch_batch_end(const BatchTables& bt, hipStream_t s) {
-    hipLaunchKernelGGL(kb_end, dim3(nblk(bt.njobs, 64)), dim3(64), 0, s, bt.descs, bt.njobs);
+    hipLaunchKernelGGL(kb_end, dim3(bt.njobs), dim3(64), 0, s, bt.descs, bt.njobs);
 }
 
 void launch_batch_frame(const BatchTables& bt, int kproj, hipStream_t s) { launch_frame(bt, kproj, 0, s); }
