@@ -471,7 +471,7 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
             for (int b = 0; b < 4; ++b) pn[b] = pts[s + b];
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                const bool acc = (b < lim) & (d2f(qx, qy, qz, p[b]) < tau) & (cs + b != self_slot);
+                const bool acc = (b < lim) & (d2f(qx, qy, qz, p[b]) < tau);
                 cand[min(cnt, CAP) * 256] = (unsigned short)(ctag + cs + b);
                 cnt += acc ? 1 : 0;
             }
@@ -486,7 +486,9 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
             round(pb, pa);
         }
     }
-    if (cnt < L || cnt > CAP) return false;
+    // the query's own point (self_slot: distance 0, in the centre cell) is
+    // collected like any other and dropped by the selection
+    if (cnt - (self_slot >= 0 ? 1 : 0) < L || cnt > CAP) return false;
     unsigned key[L];
 #pragma unroll
     for (int t = 0; t < L; ++t) key[t] = 0xffffffffu;
@@ -494,6 +496,7 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
     // inserted (entries past cnt repeat the last candidate); past cnt the key
     // is ~0u, whose insertion leaves the list unchanged
     constexpr int kB = 4;
+    // p[b].w receives whether candidate k0 + b is the query's own point
     auto gather = [&](int k0, float4 (&p)[kB]) {
         int sl[kB];
 #pragma unroll
@@ -501,14 +504,17 @@ __device__ __forceinline__ bool grid_collect_topl(const GridP& g, const CVT_G in
 #pragma unroll
         for (int b = 0; b < kB; ++b) sl[b] = rlo[(sl[b] >> 12) * 256] + (sl[b] & 4095);
 #pragma unroll
-        for (int b = 0; b < kB; ++b) p[b] = pts[sl[b]];
+        for (int b = 0; b < kB; ++b) {
+            p[b] = pts[sl[b]];
+            p[b].w = __int_as_float(sl[b] == self_slot ? -1 : 0);
+        }
     };
     auto select = [&](int k, const float4 (&p)[kB]) {
 #pragma unroll
         for (int b = 0; b < kB; ++b) {
-            // branch-free: all ones past cnt
+            // branch-free: all ones past cnt and for the query's own point
             const unsigned x = (__float_as_uint(d2f(qx, qy, qz, p[b])) & ~63u) | (unsigned)(k + b) |
-                               (k + b < cnt ? 0u : 0xffffffffu);
+                               (k + b < cnt ? 0u : 0xffffffffu) | (unsigned)__float_as_int(p[b].w);
 #pragma unroll
             for (int t = L - 1; t >= 1; --t) key[t] = umed3(key[t - 1], x, key[t]);
             key[0] = min(key[0], x);

@@ -566,14 +566,15 @@ __global__ __launch_bounds__(256) void kb_scan_down(const JobDesc* __restrict__ 
         for (int k = 0; k < kScanItems / 4; ++k) {
             const int4 q = *(const CVT_G int4*)(d.count + base + 4 * k);
             v[4 * k] = q.x; v[4 * k + 1] = q.y; v[4 * k + 2] = q.z; v[4 * k + 3] = q.w;
-            *(CVT_G int4*)(d.count + base + 4 * k) = make_int4(0, 0, 0, 0);
+            // most cells of the dense grid are empty: only counts that are set are cleared
+            if (q.x | q.y | q.z | q.w) *(CVT_G int4*)(d.count + base + 4 * k) = make_int4(0, 0, 0, 0);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < kScanItems; ++k) {
             const int j = base + k;
             v[k] = (j < d.ncell1) ? d.count[j] : 0;
-            if (j < d.ncell1) d.count[j] = 0;
+            if (v[k]) d.count[j] = 0;
         }
     }
 #pragma unroll
