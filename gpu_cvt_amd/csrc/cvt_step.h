@@ -258,11 +258,14 @@ __device__ __forceinline__ bool phase_cell(const StepArgs& a, int par, bool eval
             const float rp = sqrtf(a.rad2[c.i]);
             const float r0 = rp + 2.f * a.dmax01[par ^ 1];
             tau = r0 * r0 * (1.f + 1e-5f);
-            // collection radius: a guess (no proof needed), from this site's own move
+            // collection radius: a guess (no proof needed), from this site's own
+            // move; the annulus beyond the previous (K+2)-th distance holds
+            // about (K+2) x 4 % more points at 1.02, a margin too thin below K = 24
             const float4 sp = a.S01[par ^ 1][c.i];
             const float di = sqrtf((q.x - sp.x) * (q.x - sp.x) + (q.y - sp.y) * (q.y - sp.y) +
                                    (q.z - sp.z) * (q.z - sp.z));
-            const float rc = 1.05f * rp + 2.5f * di;
+            constexpr float kGrow = 1.f + 0.5f / (K + 2) > 1.02f ? 1.f + 0.5f / (K + 2) : 1.02f;
+            const float rc = kGrow * rp + 2.5f * di;
             ctau = fminf(rc * rc, tau);
         }
         float lk;
