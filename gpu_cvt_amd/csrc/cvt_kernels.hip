@@ -1101,8 +1101,10 @@ void launch_batch_lloyd(const BatchTables& bt, int K, int kproj, int eval_only, 
             (void)hipStreamWaitEvent(side->s, side->fork, 0);
             launch_lloyd_part(bt, K, kproj, eval_only, 1, side->s);
             (void)hipEventRecord(side->join, side->s);
-            (void)hipStreamWaitEvent(s, side->join, 0);
+            // the projection's deferrals (disjoint from the cell phase's) go
+            // first, so the join's cross-queue wait overlaps them
             if (!eval_only) launch_lloyd_part(bt, K, kproj, eval_only, 2, s);
+            (void)hipStreamWaitEvent(s, side->join, 0);
             return;
         }
     }
