@@ -2,8 +2,11 @@
 //
 // One iteration (the body of the reference's main.cu:669-881) is:
 //   site grid rebuild   : kb_grid_count, kb_scan_reduce, kb_scan_down, kb_grid_scatter
-//   per-site step       : kb_frame, kb_cell (cvt_lloyd.hip, one object per K),
-//                         kb_project, then kb_lloyd_precise (cvt_lloyd.hip)
+//   per-site step       : kb_cell (cvt_lloyd.hip, one object per K), kb_project
+//                         (which also finds each new site's frame vertex for the
+//                         next iteration), then kb_lloyd_precise (cvt_lloyd.hip);
+//                         kb_frame only at set-up, after a state is set and in
+//                         evaluations
 //   iteration epilogue  : kb_end (stats history, parity flip)
 // all captured in one HIP graph.  This file holds everything that does not
 // depend on K, plus the K dispatch of the per-K launchers.

@@ -23,11 +23,14 @@ constexpr int kCollectCap = 48;  // LDS candidates per thread in the FP32 pass
 // One site of one Lloyd iteration (main.cu:669-881 restated per site), in
 // three phases: the frame (nearest mesh vertex), the cell (site KNN, dual
 // hull, centroid), and the projection (centroid -> mesh kproj-NN, nearest
-// point on the incident faces, freeze state).  The FP32 pass runs each phase
-// as its own kernel, so the light vertex-search phases are not held to the
-// occupancy of the register-heavy cell phase; the hand-offs go through
-// memory (fnn: the frame vertex; cent: the centroid and the cell phase's
-// flags).  The FP64-resolving pass runs all three in one thread.
+// point on the incident faces, freeze state).  The FP32 pass runs the cell
+// and projection phases as kernels of their own, so the light vertex-search
+// phase is not held to the occupancy of the register-heavy cell phase; the
+// projection also finds the new site's nearest vertex, the next iteration's
+// frame (the frame phase's own kernel runs only when sites are set from
+// outside).  The hand-offs go through memory (fnn: the frame vertex; cent:
+// the centroid and the cell phase's flags).  The FP64-resolving pass runs all
+// three phases in one thread.
 // !PRECISE: pure FP32; a phase reports a deferral (and the site is handed to
 // the precise pass) when an FP32 comparison cannot be proven.
 struct SiteCtx {
