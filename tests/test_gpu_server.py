@@ -87,9 +87,12 @@ def test_server_batch_tables_grow_and_compact():
     bit-identical to its solo run."""
     Vb, Fb = mg.displaced_sphere(128, seed=41, crease_seed=42)
     V, F = mg.icosphere(3)
-    reqs = [(Vb, Fb, mg.sample_surface(Vb, Fb, 60_000, seed=43), dict(), 40)]
+    # iteration counts with margin on both sides of the race: the large job
+    # must still run when the five small jobs join (it has done >= 10 of its
+    # 200 when they are submitted) and retire before they do (400)
+    reqs = [(Vb, Fb, mg.sample_surface(Vb, Fb, 60_000, seed=43), dict(), 200)]
     for k in range(15):
-        reqs.append((V, F, mg.sample_surface(V, F, 300 + 40 * k, seed=500 + k), dict(), 80 if k < 5 else 30))
+        reqs.append((V, F, mg.sample_surface(V, F, 300 + 40 * k, seed=500 + k), dict(), 400 if k < 5 else 30))
 
     def big_running(srv, tickets):
         t_end = time.time() + 120
