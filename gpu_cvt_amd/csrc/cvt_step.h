@@ -208,7 +208,7 @@ __device__ __forceinline__ void load_row(const CVT_G int* __restrict__ row, int 
 }
 
 // Writes the row; with compare, first reads the old one and returns whether
-// they are equal.
+// they are equal (an equal row is not written again).
 template <int K>
 __device__ __forceinline__ bool store_row(CVT_G int* __restrict__ row, const int (&v)[K], bool compare) {
     bool same = true;
@@ -217,6 +217,7 @@ __device__ __forceinline__ bool store_row(CVT_G int* __restrict__ row, const int
         load_row<K>(row, old);
 #pragma unroll
         for (int k = 0; k < K; ++k) same &= old[k] == v[k];
+        if (same) return true;  // the row already holds the list
     }
     if constexpr (K % 4 == 0) {
         CVT_G int4* r4 = reinterpret_cast<CVT_G int4*>(row);
